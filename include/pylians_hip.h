@@ -1,0 +1,180 @@
+/*
+ * pylians_hip.h -- C ABI of libpylians_hip.so, the MI355X (gfx950) density-field
+ * and power-spectrum path.
+ *
+ * All entry points are extern "C", take plain pointers and sizes, return an int
+ * status (PYL_OK = 0, negative on error; pyl_last_error() gives a thread-local
+ * message), and never abort the process.  Host pointers are the default
+ * (the library stages them through device memory); PYL_PTR_DEVICE marks
+ * device-resident inputs/outputs.  A context (pyl_ctx) owns one HIP stream,
+ * grow-only scratch memory within an HBM budget, and resident FFT and
+ * mode-count plans.  One context per host thread.
+ *
+ * Reference interfaces replaced (library/ paths are in the reference tree):
+ *   pyl_ma            MAS_library.MA                library/MAS_library/MAS_library.pyx:57-112
+ *                     and its kernels NGP..PCSW      MAS_library.pyx:123-545
+ *   NGP/CIC/TSC/PCS   the C kernels, same signature  library/MAS_library/MAS_c.h:3-10 (MAS_c.c:8-266)
+ *   pyl_fft_r2c       FFT3Dr_f                       library/Pk_library/Pk_library.pyx:117-130
+ *   pyl_pk            Pk (F=1) and XPk (F>=2)        Pk_library.pyx:263-420, :529-793
+ *   pyl_pk_from_fourier  the mode loops alone        Pk_library.pyx:311-378, :623-732
+ *   pyl_mode_counts   Nmodes/k sums of those loops and check_number_modes
+ *                                                    Pk_library.pyx:87-99, :331-378
+ *   pyl_mas_correction  MAS_correction               Pk_library.pyx:83-84
+ *   pyl_pos_redshift_space  pos_redshift_space       library/redshift_space_library/
+ *                                                    redshift_space_library.pyx:29-46
+ */
+#ifndef PYLIANS_HIP_H
+#define PYLIANS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PYL_OK        0
+#define PYL_EINVAL   -1   /* bad argument (the reference's sys.exit()/ValueError cases) */
+#define PYL_EHIP     -2   /* HIP runtime error */
+#define PYL_ENOMEM   -3   /* device allocation failed */
+#define PYL_EFFT     -4   /* rocFFT error */
+#define PYL_EMODES   -5   /* check_number_modes failed */
+#define PYL_EBUDGET  -6   /* request exceeds the context's HBM budget */
+
+/* ---- mass-assignment schemes (Pk_library.pyx:72-78 MAS_function) ----- */
+#define PYL_MAS_NONE 0
+#define PYL_NGP      1
+#define PYL_CIC      2
+#define PYL_TSC      3
+#define PYL_PCS      4
+
+/* ---- flags ------------------------------------------------------------ */
+#define PYL_PTR_DEVICE     0x0001  /* array arguments are device pointers */
+#define PYL_MA_ATOMIC      0x0002  /* MA: plain per-particle global float atomics (the
+                                      non-deterministic "plain GPU form") */
+#define PYL_MA_CYTHON_2D   0x0004  /* MA, axes==2: MA()'s expand_dims semantics, i.e. each
+                                      deposit counted F times (MAS_library.pyx:84-110) */
+#define PYL_MA_RENORM_2D   0x0008  /* with PYL_MA_CYTHON_2D: divide the whole grid by F */
+#define PYL_MA_OVERWRITE   0x0010  /* MA: grid = deposit instead of grid += deposit */
+#define PYL_NO_SYNC        0x0020  /* device pointers: return without synchronising the stream */
+#define PYL_PK_DESTROY_INPUT 0x0040 /* Pk, device delta: the FFT may overwrite delta */
+#define PYL_PK_NO_PHASE    0x0080  /* Pk: skip the Pkphase accumulator */
+
+/* ---- context ---------------------------------------------------------- */
+typedef struct pyl_ctx pyl_ctx;
+
+const char *pyl_last_error(void);
+int pyl_version(void);
+/* hbm_budget: bytes of device scratch the context may hold (0 = 90% of free memory). */
+int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget);
+int pyl_ctx_destroy(pyl_ctx *ctx);
+/* Use an external HIP stream (hipStream_t) for all work; NULL restores the own stream. */
+int pyl_ctx_set_stream(pyl_ctx *ctx, void *hip_stream);
+int pyl_ctx_synchronize(pyl_ctx *ctx);
+/* Release scratch, FFT plans and mode-count plans. */
+int pyl_ctx_trim(pyl_ctx *ctx);
+/* Bytes of device memory the context currently holds. */
+int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out);
+/* Per-stage device times (ms, HIP events on the context stream) of the last call.
+ * Stage ids are PYL_STAGE_*; enable=1 turns event recording on. */
+#define PYL_STAGE_MA_COUNT   0
+#define PYL_STAGE_MA_SCAN    1
+#define PYL_STAGE_MA_SCATTER 2
+#define PYL_STAGE_MA_DEPOSIT 3
+#define PYL_STAGE_MA_ATOMIC  4
+#define PYL_STAGE_PK_PREP    5
+#define PYL_STAGE_PK_FFT     6
+#define PYL_STAGE_PK_BIN     7
+#define PYL_STAGE_COUNT      8
+int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable);
+int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */);
+
+/* ---- mass assignment (MAS_library.pyx:57-545) --------------------------
+ * pos     (n, axes) float32, C order (x,y[,z] per particle)
+ * W       (n,) float32 or NULL
+ * grid    (dims,dims[,dims]) float32, C order; grid += deposit (or = with OVERWRITE)
+ * mas     PYL_NGP..PYL_PCS
+ * Default algorithm: particles binned by tile with a device counting sort, each
+ * tile summed in LDS with exact (order-independent) accumulation, then written
+ * once -- bitwise reproducible.  PYL_MA_ATOMIC selects per-particle global atomics.
+ */
+int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
+           float *grid, int dims, float BoxSize, int mas, int flags);
+
+/* Reference C kernels, same names and signatures as MAS_c.h:3-10 (host pointers,
+ * threads ignored, += semantics, native 2D as in MAS_c.c:28-34).  They run on
+ * a process-wide context on device 0.  On error they print pyl_last_error() to
+ * stderr and leave number unchanged. */
+void NGP(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads);
+void CIC(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads);
+void TSC(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads);
+void PCS(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads);
+
+/* ---- FFT (Pk_library.pyx:117-130) -------------------------------------
+ * Forward, unnormalised r2c of a (dims,dims,dims) float32 cube into a
+ * (dims,dims,dims/2+1) complex64 array (interleaved re,im).  delta is not
+ * modified (unless PYL_PK_DESTROY_INPUT with device pointers). */
+int pyl_fft_r2c(pyl_ctx *ctx, const float *delta, float *delta_k, int dims, int flags);
+
+/* ---- power spectra (Pk_library.pyx:263-420, :529-793) -------------------
+ * Output lengths for a dims^3 grid (F fields, S = F + F(F-1)/2 spectra):
+ *   n1d = dims/2            (1D bins, DC dropped)
+ *   n2d = (dims/2+1)*(int(sqrt(2)*(dims/2))+1)   (2D bins, all kept)
+ *   n3d = int(sqrt(3)*(dims/2))                    (3D bins, DC dropped)
+ * Result arrays (float64, caller-allocated, any may be NULL to skip):
+ *   k1D[n1d] Nmodes1D[n1d] Pk1D[n1d*S]            Pk1D[i*S+s]
+ *   kpar[n2d] kper[n2d] Nmodes2D[n2d] Pk2D[n2d*S] Pk2D[i*S+s]
+ *   k3D[n3d] Nmodes3D[n3d] Pk3D[n3d*3*S]          Pk3D[(i*3+l)*S+s], l = 0,2,4 multipoles
+ *   Pkphase[n3d]                                   (F==1 only)
+ * Spectrum s < F is the auto-spectrum of field s; s >= F are the cross spectra
+ * of pairs i<j in lexicographic order (XPk's index_X).
+ */
+typedef struct {
+    int64_t n1d, n2d, n3d;
+    int nspectra;
+} pyl_pk_sizes;
+int pyl_pk_get_sizes(int dims, int nfields, pyl_pk_sizes *out);
+
+typedef struct {
+    double *k1D, *Nmodes1D, *Pk1D;
+    double *kpar, *kper, *Nmodes2D, *Pk2D;
+    double *k3D, *Nmodes3D, *Pk3D, *Pkphase;
+} pyl_pk_out;
+
+/* deltas: nfields pointers to (dims,dims,dims) float32 fields.  mas: nfields MAS
+ * ids (PYL_MAS_NONE = no deconvolution).  Optional prologue, applied to every
+ * field before the FFT exactly as "delta /= scale; delta -= shift" in float32:
+ * pass scale=1, shift=0 for none.  axis: line of sight, 0..2. */
+int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas,
+           int dims, double BoxSize, int axis, float scale, float shift, int flags,
+           pyl_pk_out *out);
+
+/* Same binning pass on fields that are already in Fourier space:
+ * delta_k[f] is (dims,dims,dims/2+1) complex64 (interleaved). */
+int pyl_pk_from_fourier(pyl_ctx *ctx, const float *const *delta_k, int nfields,
+                        const int *mas, int dims, double BoxSize, int axis, int flags,
+                        pyl_pk_out *out);
+
+/* Host-only: exact mode counts of the reference loop for (dims, axis), with the DC
+ * bins kept: N1D[dims/2+1], N2D[n2d], N3D[n3d+1], and the k sums k1Dsum[dims/2+1]
+ * (sum of |k_par|) and k3Dsum[n3d+1] (sum of |k|), in units of kF. */
+int pyl_mode_counts(int dims, int axis, double *N1D, double *N2D, double *N3D,
+                    double *k1Dsum, double *k3Dsum);
+
+/* Host-only: MAS_correction(x, p) = 1 if x == 0 else (x/sin x)^p. */
+double pyl_mas_correction(double x, int p);
+
+/* ---- redshift-space shift (redshift_space_library.pyx:29-46) -----------
+ * pos/vel (n,3) float32; pos[:,axis] += vel[:,axis]*(1+z)/H, then periodic wrap. */
+int pyl_pos_redshift_space(pyl_ctx *ctx, float *pos, const float *vel, int64_t n,
+                           float BoxSize, float Hubble, float redshift, int axis, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYLIANS_HIP_H */

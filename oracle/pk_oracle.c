@@ -1,0 +1,144 @@
+/*
+ * ORACLE -- test infrastructure only.  Nothing in the product path may link,
+ * load or call this file; only tests/, __graft_entry__.smoke() and the CPU
+ * baseline leg of bench.py use it, and only as the checker.
+ *
+ * CPU restatement of the reference power-spectrum mode loops:
+ *   library/Pk_library/Pk_library.pyx:83-84    MAS_correction
+ *   library/Pk_library/Pk_library.pyx:311-378  Pk  mode loop (serial)
+ *   library/Pk_library/Pk_library.pyx:623-732  XPk mode loop (serial)
+ * The loops take the half-complex FFT output (dims, dims, dims/2+1) and fill
+ * the raw per-bin sums; normalisation (Pk_library.pyx:384-418, 735-791) is in
+ * oracle/oracle.py.  The float32 variant keeps the reference's types: float
+ * MAS_factor, complex64 scaling, double |delta_k|^2 and double accumulators.
+ * The float64 variant keeps everything in double (the FP64 oracle).
+ *
+ * Pinned by tests/test_oracle_pk.py: mode counts against the closed form of
+ * check_number_modes and brute-force Hermitian enumeration, single plane waves
+ * (closed-form bin value), modes on the line of sight (Legendre factors 1),
+ * the MAS window closed form, and XPk([d,d]) / XPk([d,-d]) identities.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static double mas_correction(double x, int p)
+{
+    return (x == 0.0) ? 1.0 : pow(x / sin(x), (double)p);
+}
+
+/* Shared per-mode geometry, Pk_library.pyx:313-348. Returns 0 if skipped. */
+typedef struct {
+    double k, mu2;
+    int k_index, k_par, k_per;
+} mode_geom;
+
+static int geom(int kx, int ky, int kz, int dims, int middle, int axis, mode_geom *g)
+{
+    if (kz == 0 || (kz == middle && dims % 2 == 0)) {
+        if (kx < 0) return 0;
+        else if (kx == 0 || (kx == middle && dims % 2 == 0)) {
+            if (ky < 0) return 0;
+        }
+    }
+    g->k = sqrt((double)(kx * kx + ky * ky + kz * kz));
+    g->k_index = (int)g->k;
+    if (axis == 0) { g->k_par = kx; g->k_per = (int)sqrt((double)(ky * ky + kz * kz)); }
+    else if (axis == 1) { g->k_par = ky; g->k_per = (int)sqrt((double)(kx * kx + kz * kz)); }
+    else { g->k_par = kz; g->k_per = (int)sqrt((double)(kx * kx + ky * ky)); }
+    {
+        double mu = (g->k == 0.0) ? 0.0 : g->k_par / g->k;
+        g->mu2 = mu * mu;
+    }
+    if (g->k_par < 0) g->k_par = -g->k_par;
+    return 1;
+}
+
+/*
+ * Pk mode loop.  dk: interleaved (re,im) of F complex fields, each
+ * (dims,dims,middle+1); is_f64 selects complex128 input and double MAS factor.
+ * Outputs for field f (auto) or pair p (cross, i<j lexicographic):
+ *   Pk1D[(middle+1)*S], Pk2D[nb2*S], Pk3D[(kmax+1)*3*S], Pkphase[kmax+1] (F==1 only)
+ * where S = F + F(F-1)/2 spectra (autos first, then crosses).
+ * Counters (shared): k1D, N1D [middle+1], N2D [nb2], k3D, N3D [kmax+1].
+ */
+void ora_pk_loop(const void *dk, int is_f64, int F, int dims, int axis,
+                 const int *mas_index,
+                 double *k1D, double *Pk1D, double *N1D,
+                 double *Pk2D, double *N2D,
+                 double *k3D, double *Pk3D, double *Pkphase, double *N3D)
+{
+    int middle = dims / 2;
+    int kmax_par = middle;
+    int kmax_per = (int)sqrt((double)(middle * middle + middle * middle));
+    int kmax = (int)sqrt((double)(3 * middle * middle));
+    long nb2 = (long)(kmax_par + 1) * (kmax_per + 1);
+    int S = F + F * (F - 1) / 2;
+    long zlen = middle + 1;
+    double prefact = M_PI / dims;
+    double *corr = (double *)malloc(sizeof(double) * 3 * F);
+    double *re = (double *)malloc(sizeof(double) * F);
+    double *im = (double *)malloc(sizeof(double) * F);
+    int kxx, kyy, kzz, f, i, j, s;
+    (void)kmax;
+
+    for (kxx = 0; kxx < dims; kxx++) {
+        int kx = (kxx > middle) ? kxx - dims : kxx;
+        for (f = 0; f < F; f++) corr[3 * f + 0] = mas_correction(prefact * kx, mas_index[f]);
+        for (kyy = 0; kyy < dims; kyy++) {
+            int ky = (kyy > middle) ? kyy - dims : kyy;
+            for (f = 0; f < F; f++) corr[3 * f + 1] = mas_correction(prefact * ky, mas_index[f]);
+            for (kzz = 0; kzz < middle + 1; kzz++) {
+                int kz = kzz;
+                mode_geom g;
+                long index_2D;
+                for (f = 0; f < F; f++) corr[3 * f + 2] = mas_correction(prefact * kz, mas_index[f]);
+                if (!geom(kx, ky, kz, dims, middle, axis, &g)) continue;
+
+                for (f = 0; f < F; f++) {
+                    long e = (((long)kxx * dims + kyy) * zlen + kzz);
+                    if (is_f64) {
+                        const double *d = (const double *)dk + 2 * ((long)f * dims * dims * zlen + e);
+                        double mf = corr[3 * f + 0] * corr[3 * f + 1] * corr[3 * f + 2];
+                        re[f] = d[0] * mf; im[f] = d[1] * mf;
+                    } else {
+                        const float *d = (const float *)dk + 2 * ((long)f * dims * dims * zlen + e);
+                        float mf = (float)(corr[3 * f + 0] * corr[3 * f + 1] * corr[3 * f + 2]);
+                        float r = d[0] * mf, m = d[1] * mf;   /* complex64 * float */
+                        re[f] = (double)r; im[f] = (double)m;
+                    }
+                }
+                index_2D = (long)(kmax_par + 1) * g.k_per + g.k_par;
+                if (g.k <= middle) { k1D[g.k_par] += g.k_par; N1D[g.k_par] += 1.0; }
+                N2D[index_2D] += 1.0;
+                k3D[g.k_index] += g.k;
+                N3D[g.k_index] += 1.0;
+
+                s = 0;
+                for (i = 0; i < F; i++) {
+                    for (j = i; j < F; j++) {
+                        double d2;
+                        int sp;
+                        if (j == i) { d2 = re[i] * re[i] + im[i] * im[i]; sp = i; }
+                        else { d2 = re[i] * re[j] + im[i] * im[j]; sp = F + s++; }
+                        if (g.k <= middle) Pk1D[(long)g.k_par * S + sp] += d2;
+                        Pk2D[index_2D * S + sp] += d2;
+                        Pk3D[((long)g.k_index * 3 + 0) * S + sp] += d2;
+                        Pk3D[((long)g.k_index * 3 + 1) * S + sp] += (d2 * (3.0 * g.mu2 - 1.0) / 2.0);
+                        Pk3D[((long)g.k_index * 3 + 2) * S + sp] +=
+                            (d2 * (35.0 * g.mu2 * g.mu2 - 30.0 * g.mu2 + 3.0) / 8.0);
+                        if (F == 1 && Pkphase) {
+                            double phase = atan2(re[0], sqrt(d2));
+                            Pkphase[g.k_index] += phase * phase;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    (void)nb2;
+    free(corr); free(re); free(im);
+}
+
+/* MAS_correction exported for the closed-form tests (Pk_library.pyx:83-84). */
+double ora_mas_correction(double x, int p) { return mas_correction(x, p); }

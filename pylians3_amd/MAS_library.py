@@ -1,0 +1,246 @@
+"""Drop-in MAS_library on MI355X (gfx950).
+
+Same names, arguments, in-place ``+=`` semantics and error behaviour as the
+reference module (library/MAS_library/MAS_library.pyx):
+
+    MA(pos, number, BoxSize, MAS='CIC', W=None, verbose=False, renormalize_2D=True)
+        MAS_library.pyx:57-112
+    NGP/CIC/TSC/PCS(pos, number, BoxSize), NGPW/CICW/TSCW/PCSW(pos, number, BoxSize, W)
+        MAS_library.pyx:123-545 (3D kernels; a (d,d,1) grid with 2D positions gives the
+        expand_dims behaviour of MA)
+    NGPc3D ... PCSWc2D(pos, number, [W,] BoxSize, threads)
+        MAS_library.pyx:1305-1389 (the C kernels, MAS_c.c:8-266: native 2D, C-contiguous)
+    FLOAT_type()                                   MAS_library.pyx:11-13
+
+Arrays may be numpy arrays (host; staged through device memory) or contiguous
+torch tensors on the GPU (device-resident; no copies).  All arithmetic runs in
+libpylians_hip.so; there is no CPU path.
+"""
+import sys
+import time
+
+import numpy as np
+
+from . import _native as N
+
+_MAS_IDS = {'NGP': N.NGP, 'CIC': N.CIC, 'TSC': N.TSC, 'PCS': N.PCS}
+
+
+def FLOAT_type():
+    return np.float32
+
+
+def _check_f32(a, ndim, name):
+    """Cython typed-memoryview checks: float32 dtype and dimensionality (ValueError)."""
+    if N.is_device_array(a):
+        import torch
+        if a.dtype != torch.float32:
+            raise ValueError("Buffer dtype mismatch, expected 'float32_t' but got '%s'" % a.dtype)
+        if a.dim() != ndim:
+            raise ValueError('Buffer has wrong number of dimensions (expected %d, got %d)'
+                             % (ndim, a.dim()))
+        return
+    if not isinstance(a, np.ndarray):
+        raise TypeError('%s must be a numpy array or a GPU torch tensor' % name)
+    if a.dtype != np.float32:
+        raise ValueError("Buffer dtype mismatch, expected 'float32_t' but got '%s'"
+                         % ('double' if a.dtype == np.float64 else a.dtype.name))
+    if a.ndim != ndim:
+        raise ValueError('Buffer has wrong number of dimensions (expected %d, got %d)'
+                         % (ndim, a.ndim))
+
+
+def _deposit(pos, number, BoxSize, mas, W, flags, require_contig=False, ctx=None):
+    """Run pyl_ma on host or device arrays.  number is updated in place."""
+    ctx = ctx or N.default_context()
+    axes = pos.shape[1]
+    dims = number.shape[0]
+    n = pos.shape[0]
+    dev = N.is_device_array(pos) or N.is_device_array(number)
+    if dev:
+        if not (N.is_device_array(pos) and N.is_device_array(number)
+                and (W is None or N.is_device_array(W))):
+            raise ValueError('pos, number and W must all be on the GPU or all on the host')
+        with N.torch_stream(ctx, True):
+            N.check(N.lib().pyl_ma(ctx.handle, N.device_ptr(pos), n, axes,
+                                   None if W is None else N.device_ptr(W), N.device_ptr(number),
+                                   dims, float(BoxSize), mas, flags | N.PTR_DEVICE))
+        return
+    if require_contig:
+        for a, nm in ((pos, 'pos'), (number, 'number'), (W, 'W')):
+            if a is not None and not a.flags['C_CONTIGUOUS']:
+                raise ValueError('ndarray is not C-contiguous')
+    p = np.ascontiguousarray(pos)
+    w = None if W is None else np.ascontiguousarray(W)
+    g = number if number.flags['C_CONTIGUOUS'] else np.ascontiguousarray(number)
+    N.check(N.lib().pyl_ma(ctx.handle, p.ctypes.data, n, axes,
+                           None if w is None else w.ctypes.data, g.ctypes.data, dims,
+                           float(BoxSize), mas, flags))
+    if g is not number:
+        number[...] = g
+
+
+def MA(pos, number, BoxSize, MAS='CIC', W=None, verbose=False, renormalize_2D=True):
+    """Mass assignment of particles (optionally weighted) onto a 2D or 3D grid, += in place.
+
+    MAS_library.pyx:57-112.  pos (N,2|3) float32, number (d,d[,d]) float32,
+    W (N,) float32 or None.
+    """
+    coord, coord_aux = pos.shape[1], number.ndim
+    if coord != coord_aux:
+        print('pos have %d dimensions and the density %d!!!' % (coord, coord_aux))
+        sys.exit()
+    if verbose:
+        if W is None:
+            print('\nUsing %s mass assignment scheme' % MAS)
+        else:
+            print('\nUsing %s mass assignment scheme with weights' % MAS)
+    start = time.time()
+    if coord not in (2, 3):       # the reference dispatches only on coord 2 and 3
+        return
+    if MAS not in _MAS_IDS:
+        print('option not valid!!!')
+        sys.exit()
+    _check_f32(pos, 2, 'pos')
+    _check_f32(number, coord, 'number')
+    if W is not None:
+        _check_f32(W, 1, 'W')
+    flags = 0
+    if coord == 2:
+        flags |= N.MA_CYTHON_2D
+        if renormalize_2D:
+            flags |= N.MA_RENORM_2D
+    _deposit(pos, number, BoxSize, _MAS_IDS[MAS], W, flags)
+    if verbose:
+        print('Time taken = %.3f seconds\n' % (time.time() - start))
+
+
+def _kernel3d(mas, pos, number, BoxSize, W=None):
+    # The Cython kernels take a 3D grid; with 2D positions on a (d,d,1) grid the
+    # unused axis keeps weight 1 and index 0, so every deposit repeats F times.
+    _check_f32(pos, 2, 'pos')
+    _check_f32(number, 3, 'number')
+    if W is not None:
+        _check_f32(W, 1, 'W')
+    if pos.shape[1] == 2:
+        if number.shape[2] != 1:
+            raise ValueError('2D positions need a (d,d,1) grid')
+        g = number.reshape(number.shape[0], number.shape[1]) if not N.is_device_array(number) \
+            else number.view(number.shape[0], number.shape[1])
+        _deposit(pos, g, BoxSize, mas, W, N.MA_CYTHON_2D)
+        return
+    _deposit(pos, number, BoxSize, mas, W, 0)
+
+
+def NGP(pos, number, BoxSize):
+    _kernel3d(N.NGP, pos, number, BoxSize)
+
+
+def CIC(pos, number, BoxSize):
+    _kernel3d(N.CIC, pos, number, BoxSize)
+
+
+def TSC(pos, number, BoxSize):
+    _kernel3d(N.TSC, pos, number, BoxSize)
+
+
+def PCS(pos, number, BoxSize):
+    _kernel3d(N.PCS, pos, number, BoxSize)
+
+
+def NGPW(pos, number, BoxSize, W):
+    _kernel3d(N.NGP, pos, number, BoxSize, W)
+
+
+def CICW(pos, number, BoxSize, W):
+    _kernel3d(N.CIC, pos, number, BoxSize, W)
+
+
+def TSCW(pos, number, BoxSize, W):
+    _kernel3d(N.TSC, pos, number, BoxSize, W)
+
+
+def PCSW(pos, number, BoxSize, W):
+    _kernel3d(N.PCS, pos, number, BoxSize, W)
+
+
+def _ckernel(mas, pos, number, BoxSize, W, ndim):
+    # MAS_library.pyx:1305-1389: FLOAT[:,::1] memoryviews (C-contiguous), native 2D
+    _check_f32(pos, 2, 'pos')
+    _check_f32(number, ndim, 'number')
+    if W is not None:
+        _check_f32(W, 1, 'W')
+    _deposit(pos, number, BoxSize, mas, W, 0, require_contig=True)
+
+
+def NGPc3D(pos, number, BoxSize, threads):
+    _ckernel(N.NGP, pos, number, BoxSize, None, 3)
+
+
+def NGPWc3D(pos, number, W, BoxSize, threads):
+    _ckernel(N.NGP, pos, number, BoxSize, W, 3)
+
+
+def NGPc2D(pos, number, BoxSize, threads):
+    _ckernel(N.NGP, pos, number, BoxSize, None, 2)
+
+
+def NGPWc2D(pos, number, W, BoxSize, threads):
+    _ckernel(N.NGP, pos, number, BoxSize, W, 2)
+
+
+def CICc3D(pos, number, BoxSize, threads):
+    _ckernel(N.CIC, pos, number, BoxSize, None, 3)
+
+
+def CICWc3D(pos, number, W, BoxSize, threads):
+    _ckernel(N.CIC, pos, number, BoxSize, W, 3)
+
+
+def CICc2D(pos, number, BoxSize, threads):
+    _ckernel(N.CIC, pos, number, BoxSize, None, 2)
+
+
+def CICWc2D(pos, number, W, BoxSize, threads):
+    _ckernel(N.CIC, pos, number, BoxSize, W, 2)
+
+
+def TSCc3D(pos, number, BoxSize, threads):
+    _ckernel(N.TSC, pos, number, BoxSize, None, 3)
+
+
+def TSCWc3D(pos, number, W, BoxSize, threads):
+    _ckernel(N.TSC, pos, number, BoxSize, W, 3)
+
+
+def TSCc2D(pos, number, BoxSize, threads):
+    _ckernel(N.TSC, pos, number, BoxSize, None, 2)
+
+
+def TSCWc2D(pos, number, W, BoxSize, threads):
+    _ckernel(N.TSC, pos, number, BoxSize, W, 2)
+
+
+def PCSc3D(pos, number, BoxSize, threads):
+    _ckernel(N.PCS, pos, number, BoxSize, None, 3)
+
+
+def PCSWc3D(pos, number, W, BoxSize, threads):
+    _ckernel(N.PCS, pos, number, BoxSize, W, 3)
+
+
+def PCSc2D(pos, number, BoxSize, threads):
+    _ckernel(N.PCS, pos, number, BoxSize, None, 2)
+
+
+def PCSWc2D(pos, number, W, BoxSize, threads):
+    _ckernel(N.PCS, pos, number, BoxSize, W, 2)
+
+
+def MA_atomic(pos, number, BoxSize, MAS='CIC', W=None):
+    """The plain GPU form: per-particle global float atomics (not bitwise reproducible).
+    Same semantics as MA for 3D grids; used as the comparison baseline."""
+    _check_f32(pos, 2, 'pos')
+    _check_f32(number, pos.shape[1], 'number')
+    flags = N.MA_ATOMIC | (N.MA_CYTHON_2D | N.MA_RENORM_2D if pos.shape[1] == 2 else 0)
+    _deposit(pos, number, BoxSize, _MAS_IDS[MAS], W, flags)

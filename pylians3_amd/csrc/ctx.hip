@@ -1,0 +1,261 @@
+// Context, error reporting, scratch memory, stage timers and resident rocFFT plans.
+#include "pyl_internal.h"
+
+#include <cstring>
+#include <mutex>
+
+namespace pyl {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+size_t held_bytes(const Ctx *c) {
+    size_t s = 0;
+    for (int i = 0; i < BUF_COUNT_; ++i) s += c->bufs[i].cap;
+    for (auto &kv : c->fft_plans) s += kv.second.work_bytes;
+    return s;
+}
+
+// Grow-only scratch: reallocates only when the request exceeds the capacity.
+int ensure(Ctx *c, BufId id, size_t bytes) {
+    DevBuf &b = c->bufs[id];
+    if (bytes <= b.cap) return PYL_OK;
+    size_t need = bytes + (bytes >> 4);  // 6% headroom so slowly growing requests do not thrash
+    size_t others = held_bytes(c) - b.cap;
+    PYL_CHECK(others + need <= c->budget || others + bytes <= c->budget, PYL_EBUDGET,
+              "request of %zu bytes exceeds the context HBM budget (%zu held, %zu budget)",
+              bytes, others, c->budget);
+    if (others + need > c->budget) need = bytes;
+    if (b.ptr) {
+        PYL_HIP(hipStreamSynchronize(c->stream));
+        PYL_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, need);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        set_error("hipMalloc(%zu) failed: %s", need, hipGetErrorString(e));
+        return PYL_ENOMEM;
+    }
+    b.cap = need;
+    return PYL_OK;
+}
+
+// A stage's time spans its first begin and its last end within one API call.
+void reset_stages(Ctx *c) {
+    for (int s = 0; s < PYL_STAGE_COUNT; ++s) c->ev_used[s] = false;
+}
+
+int stage_begin(Ctx *c, int stage) {
+    if (!c->timing || c->ev_used[stage]) return PYL_OK;
+    PYL_HIP(hipEventRecord(c->ev[stage][0], c->stream));
+    c->ev_used[stage] = true;
+    return PYL_OK;
+}
+
+int stage_end(Ctx *c, int stage) {
+    if (!c->timing) return PYL_OK;
+    PYL_HIP(hipEventRecord(c->ev[stage][1], c->stream));
+    return PYL_OK;
+}
+
+int finish(Ctx *c, int flags) {
+    if ((flags & PYL_PTR_DEVICE) && (flags & PYL_NO_SYNC)) return PYL_OK;
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    return PYL_OK;
+}
+
+static void destroy_plan(FFTPlan &p) {
+    if (p.info) rocfft_execution_info_destroy(p.info);
+    if (p.plan) rocfft_plan_destroy(p.plan);
+    if (p.work) (void)hipFree(p.work);
+    p = FFTPlan();
+}
+
+static std::once_flag g_rocfft_once;
+
+// Resident rocFFT plan: forward, single precision, out-of-place r2c of `batch`
+// contiguous dims^3 cubes.  Created once per (dims, batch) and kept.
+int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out) {
+    auto key = std::make_pair(dims, batch);
+    auto it = c->fft_plans.find(key);
+    if (it != c->fft_plans.end()) {
+        *out = &it->second;
+        return PYL_OK;
+    }
+    std::call_once(g_rocfft_once, [] { rocfft_setup(); });
+    FFTPlan p;
+    p.dims = dims;
+    p.batch = batch;
+    size_t lengths[3] = {(size_t)dims, (size_t)dims, (size_t)dims};  // fastest first
+    rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace,
+                                          rocfft_transform_type_real_forward,
+                                          rocfft_precision_single, 3, lengths, (size_t)batch,
+                                          nullptr);
+    PYL_CHECK(st == rocfft_status_success, PYL_EFFT, "rocfft_plan_create(%d^3) failed (%d)",
+              dims, (int)st);
+    st = rocfft_plan_get_work_buffer_size(p.plan, &p.work_bytes);
+    if (st != rocfft_status_success) {
+        destroy_plan(p);
+        set_error("rocfft_plan_get_work_buffer_size failed (%d)", (int)st);
+        return PYL_EFFT;
+    }
+    st = rocfft_execution_info_create(&p.info);
+    if (st != rocfft_status_success) {
+        destroy_plan(p);
+        set_error("rocfft_execution_info_create failed (%d)", (int)st);
+        return PYL_EFFT;
+    }
+    if (p.work_bytes) {
+        if (held_bytes(c) + p.work_bytes > c->budget) {
+            size_t wb = p.work_bytes;
+            destroy_plan(p);
+            set_error("rocFFT work buffer of %zu bytes exceeds the HBM budget", wb);
+            return PYL_EBUDGET;
+        }
+        hipError_t e = hipMalloc(&p.work, p.work_bytes);
+        if (e != hipSuccess) {
+            destroy_plan(p);
+            set_error("hipMalloc(rocFFT work) failed: %s", hipGetErrorString(e));
+            return PYL_ENOMEM;
+        }
+        rocfft_execution_info_set_work_buffer(p.info, p.work, p.work_bytes);
+    }
+    auto ins = c->fft_plans.emplace(key, p);
+    *out = &ins.first->second;
+    return PYL_OK;
+}
+
+int run_fft(Ctx *c, FFTPlan *p, void *in, void *out) {
+    rocfft_status st = rocfft_execution_info_set_stream(p->info, c->stream);
+    PYL_CHECK(st == rocfft_status_success, PYL_EFFT, "rocfft set_stream failed (%d)", (int)st);
+    void *ib[1] = {in};
+    void *ob[1] = {out};
+    st = rocfft_execute(p->plan, ib, ob, p->info);
+    PYL_CHECK(st == rocfft_status_success, PYL_EFFT, "rocfft_execute failed (%d)", (int)st);
+    return PYL_OK;
+}
+
+}  // namespace pyl
+
+using namespace pyl;
+
+extern "C" {
+
+const char *pyl_last_error(void) { return g_err.c_str(); }
+
+int pyl_version(void) { return 100; }
+
+int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
+    PYL_CHECK(out != nullptr, PYL_EINVAL, "pyl_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    PYL_HIP(hipGetDeviceCount(&ndev));
+    PYL_CHECK(device >= 0 && device < ndev, PYL_EINVAL, "device %d not present (%d devices)",
+              device, ndev);
+    PYL_HIP(hipSetDevice(device));
+    std::unique_ptr<pyl_ctx> h(new pyl_ctx());
+    Ctx *c = &h->c;
+    c->device = device;
+    PYL_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+    if (hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        PYL_HIP(hipMemGetInfo(&fr, &tot));
+        hbm_budget = (size_t)(0.9 * (double)fr);
+    }
+    c->budget = hbm_budget;
+    for (int s = 0; s < PYL_STAGE_COUNT; ++s) {
+        PYL_HIP(hipEventCreate(&c->ev[s][0]));
+        PYL_HIP(hipEventCreate(&c->ev[s][1]));
+    }
+    c->ev_ready = true;
+    *out = h.release();
+    return PYL_OK;
+}
+
+int pyl_ctx_trim(pyl_ctx *ctx) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipSetDevice(c->device));
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < BUF_COUNT_; ++i) {
+        if (c->bufs[i].ptr) PYL_HIP(hipFree(c->bufs[i].ptr));
+        c->bufs[i] = DevBuf();
+    }
+    for (auto &kv : c->fft_plans) destroy_plan(kv.second);
+    c->fft_plans.clear();
+    c->mode_plans.clear();
+    for (auto &kv : c->bin_plans) {
+        (void)hipFree(kv.second.d_rows);
+        (void)hipFree(kv.second.d_items);
+        (void)hipFree(kv.second.d_key_items);
+    }
+    c->bin_plans.clear();
+    return PYL_OK;
+}
+
+int pyl_ctx_destroy(pyl_ctx *ctx) {
+    if (!ctx) return PYL_OK;
+    int s = pyl_ctx_trim(ctx);
+    Ctx *c = &ctx->c;
+    if (c->ev_ready)
+        for (int i = 0; i < PYL_STAGE_COUNT; ++i) {
+            (void)hipEventDestroy(c->ev[i][0]);
+            (void)hipEventDestroy(c->ev[i][1]);
+        }
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete ctx;
+    return s;
+}
+
+int pyl_ctx_set_stream(pyl_ctx *ctx, void *hip_stream) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    ctx->c.stream = hip_stream ? (hipStream_t)hip_stream : ctx->c.own_stream;
+    return PYL_OK;
+}
+
+int pyl_ctx_synchronize(pyl_ctx *ctx) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    PYL_HIP(hipSetDevice(ctx->c.device));
+    PYL_HIP(hipStreamSynchronize(ctx->c.stream));
+    return PYL_OK;
+}
+
+int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out) {
+    PYL_CHECK(ctx && bytes_out, PYL_EINVAL, "NULL argument");
+    *bytes_out = held_bytes(&ctx->c);
+    return PYL_OK;
+}
+
+int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    ctx->c.timing = enable != 0;
+    for (int s = 0; s < PYL_STAGE_COUNT; ++s) ctx->c.ev_used[s] = false;
+    return PYL_OK;
+}
+
+int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out) {
+    PYL_CHECK(ctx && ms_out, PYL_EINVAL, "NULL argument");
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < PYL_STAGE_COUNT; ++s) {
+        ms_out[s] = 0.0;
+        if (!c->ev_used[s]) continue;
+        float ms = 0.f;
+        PYL_HIP(hipEventElapsedTime(&ms, c->ev[s][0], c->ev[s][1]));
+        ms_out[s] = ms;
+    }
+    return PYL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,692 @@
+// Mass assignment (NGP/CIC/TSC/PCS, optional weights, 2D/3D) on gfx950.
+//
+// Reference semantics: library/MAS_library/MAS_library.pyx:57-545 (MA and the
+// per-scheme kernels) and library/MAS_library/MAS_c.c:8-266 (C kernels).  The
+// per-axis footprint below reproduces the reference's float/double expression
+// types exactly (file compiled with -ffp-contract=off).
+//
+// Two algorithms:
+//  * atomic: one thread per particle, one global float atomic per footprint
+//    node (the plain GPU form; summation order is not reproducible).
+//  * tiled (default): the grid is cut into tiles; a counting sort places a
+//    copy of each particle in every tile its footprint overlaps; one workgroup
+//    per tile sums the footprints of its records into an LDS accumulator and
+//    writes each node of its tile exactly once.  Contributions are rounded to
+//    an integer multiple of a per-tile power-of-two quantum 2^q chosen so
+//    that every partial sum is an integer below 2^53: the float64 LDS adds are
+//    then exact, so the result does not depend on the order of the atomics and
+//    is bitwise reproducible.  The quantum is ~2^-52 of the tile's weight bound
+//    (count x max|W|), far below float32 resolution.
+#include "pyl_internal.h"
+
+namespace pyl {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ int wrapi(int i, int d) {
+    int r = i % d;
+    return r < 0 ? r + d : r;
+}
+
+// (base + j) mod d for base in [0,d) and 0 <= j < 4; the second test only
+// fires on grids smaller than the footprint (d < 4).
+__device__ __forceinline__ int node_add(int base, int j, int d) {
+    int r = base + j;
+    if (r >= d) r -= d;
+    if (r >= d) r %= d;
+    return r;
+}
+
+// Per-axis footprint.  Writes the first node (wrapped into [0,dims)) and the F
+// weights; node j of the footprint is (base + j) mod dims.
+template <int MAS>
+__device__ __forceinline__ int axis_foot(float x, float inv, int dims, float *C) {
+    if constexpr (MAS == PYL_NGP) {
+        // MAS_library.pyx:288-289: <int>(pos*inv + 0.5), the 0.5 is a double
+        C[0] = 1.0f;
+        return wrapi((int)((double)(x * inv) + 0.5), dims);
+    } else if constexpr (MAS == PYL_CIC) {
+        // MAS_library.pyx:151-156: u = dist - <int>dist, d = 1.0 - u
+        float dist = x * inv;
+        int i = (int)dist;
+        float u = dist - (float)i;
+        C[1] = u;
+        C[0] = 1.0f - u;
+        return wrapi(i, dims);
+    } else if constexpr (MAS == PYL_TSC) {
+        // MAS_library.pyx:392-399
+        float dist = x * inv;
+        int minimum = (int)floor((double)dist - 1.5);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float diff = fabsf((float)(minimum + j + 1) - dist);
+            float c;
+            if (diff < 0.5f) {
+                float d2 = diff * diff;          // float product, then 0.75 - it
+                c = 0.75f - d2;
+            } else if (diff < 1.5f) {
+                float a = 1.5f - diff;           // exact
+                c = 0.5f * (a * a);              // == (float)(0.5*(1.5-d)*(1.5-d))
+            } else {
+                c = 0.0f;
+            }
+            C[j] = c;
+        }
+        return wrapi(minimum + 1, dims);
+    } else {
+        // PCS, MAS_library.pyx:485-492 (double polynomial, rounded to float)
+        float dist = x * inv;
+        int minimum = (int)floor((double)dist - 2.0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float diff = fabsf((float)(minimum + j + 1) - dist);
+            double dd = (double)diff;
+            float c;
+            if (diff < 1.0f)
+                c = (float)((4.0 - 6.0 * dd * dd + 3.0 * dd * dd * dd) / 6.0);
+            else if (diff < 2.0f)
+                c = (float)((2.0 - dd) * (2.0 - dd) * (2.0 - dd) / 6.0);
+            else
+                c = 0.0f;
+            C[j] = c;
+        }
+        return wrapi(minimum + 1, dims);
+    }
+}
+
+// Geometry of the tiled deposit.  2D grids are 3D grids with one z plane.
+struct TileGeom {
+    int dims;          // nodes per axis (x, y and, for 3D, z)
+    int dz;            // nodes along z: dims (3D) or 1 (2D)
+    int T[3];          // tile extent per axis
+    int nt[3];         // tiles per axis
+    long long ntiles;
+};
+
+// Distinct tiles touched along one axis by the F consecutive nodes starting at
+// base (mod d).  Returns their count (1..3) in order.
+template <int F>
+__device__ __forceinline__ int axis_tiles(int base, int d, int T, int *t) {
+    int n = 0;
+    int last = -1;
+#pragma unroll
+    for (int j = 0; j < F; ++j) {
+        int node = node_add(base, j, d);
+        int ti = node / T;
+        if (ti != last) {
+            bool seen = false;
+            for (int k = 0; k < n; ++k) seen |= (t[k] == ti);
+            if (!seen) t[n++] = ti;
+            last = ti;
+        }
+    }
+    return n;
+}
+
+// Wave-aggregated atomic add of 1 per active lane on counters[key].  Returns
+// each lane's slot (old value + rank among same-key lanes).  Called by the full
+// wave (converged); inactive lanes pass active=false.
+__device__ __forceinline__ unsigned long long agg_inc(unsigned *counters, long long key,
+                                                      bool active, bool want_slot) {
+    const int lane = __lane_id();
+    unsigned long long pend = __ballot(active);
+    unsigned long long slot = 0;
+    while (pend) {
+        int leader = __ffsll((unsigned long long)pend) - 1;
+        long long lk = __shfl(key, leader);
+        unsigned long long grp = __ballot(active && key == lk) & pend;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lk], (unsigned)__popcll(grp));
+        if (want_slot) {
+            base = __shfl(base, leader);
+            if ((grp >> lane) & 1ull)
+                slot = (unsigned long long)base + __popcll(grp & ((1ull << lane) - 1ull));
+        }
+        pend &= ~grp;
+    }
+    return slot;
+}
+
+template <int MAS>
+__device__ __forceinline__ void particle_tiles(const float *pos, long long i, int axes,
+                                               float inv, const TileGeom &g, int tx[3],
+                                               int ty[3], int tz[3], int &nx, int &ny,
+                                               int &nz) {
+    constexpr int F = MAS;
+    float C[4];
+    int bx = axis_foot<MAS>(pos[(long long)axes * i + 0], inv, g.dims, C);
+    int by = axis_foot<MAS>(pos[(long long)axes * i + 1], inv, g.dims, C);
+    nx = axis_tiles<F>(bx, g.dims, g.T[0], tx);
+    ny = axis_tiles<F>(by, g.dims, g.T[1], ty);
+    if (axes == 3) {
+        int bz = axis_foot<MAS>(pos[(long long)axes * i + 2], inv, g.dims, C);
+        nz = axis_tiles<F>(bz, g.dims, g.T[2], tz);
+    } else {
+        nz = 1;
+        tz[0] = 0;
+    }
+}
+
+// Pass 1: per-tile record counts and the global max |W| (as float bits).
+template <int MAS>
+__global__ void __launch_bounds__(kBlock)
+tile_count_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
+                  int axes, float inv, TileGeom g, unsigned *__restrict__ counts,
+                  unsigned *__restrict__ wmax_bits) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned wloc = 0;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
+        long long i = base + threadIdx.x;
+        bool valid = i < n;
+        int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
+        if (valid) {
+            particle_tiles<MAS>(pos, i, axes, inv, g, tx, ty, tz, nx, ny, nz);
+            if (W) wloc = max(wloc, __float_as_uint(fabsf(W[i])));
+        }
+        int nc = valid ? nx * ny * nz : 0;
+        for (int j = 0;; ++j) {
+            bool act = j < nc;
+            if (!__any(act)) break;
+            long long key = 0;
+            if (act) {
+                int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
+                key = ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
+            }
+            agg_inc(counts, key, act, false);
+        }
+    }
+    if (W) {
+        for (int off = 32; off > 0; off >>= 1) wloc = max(wloc, (unsigned)__shfl_xor((int)wloc, off));
+        if (__lane_id() == 0) atomicMax(wmax_bits, wloc);
+    }
+}
+
+// Pass 3: write one record (x, y, z, w) per (particle, overlapped tile).
+template <int MAS>
+__global__ void __launch_bounds__(kBlock)
+tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
+                    int axes, float inv, TileGeom g, const unsigned long long *__restrict__ offsets,
+                    unsigned *__restrict__ cursors, float4 *__restrict__ records) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
+        long long i = base + threadIdx.x;
+        bool valid = i < n;
+        int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
+        float4 rec = make_float4(0.f, 0.f, 0.f, 1.f);
+        if (valid) {
+            particle_tiles<MAS>(pos, i, axes, inv, g, tx, ty, tz, nx, ny, nz);
+            rec.x = pos[(long long)axes * i + 0];
+            rec.y = pos[(long long)axes * i + 1];
+            rec.z = (axes == 3) ? pos[(long long)axes * i + 2] : 0.f;
+            rec.w = W ? W[i] : 1.0f;
+        }
+        int nc = valid ? nx * ny * nz : 0;
+        for (int j = 0;; ++j) {
+            bool act = j < nc;
+            if (!__any(act)) break;
+            long long key = 0;
+            if (act) {
+                int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
+                key = ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
+            }
+            unsigned long long slot = agg_inc(cursors, key, act, true);
+            if (act) records[offsets[key] + slot] = rec;
+        }
+    }
+}
+
+// Exclusive scan of per-tile counts into 64-bit offsets (offsets[ntiles] = total).
+constexpr int kScanBlock = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ unsigned long long block_exclusive_scan(unsigned long long v,
+                                                                   unsigned long long *sh,
+                                                                   unsigned long long &total) {
+    // Hillis-Steele over the block in LDS (1024 entries).
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        unsigned long long a = (t >= off) ? sh[t - off] : 0ull;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    total = sh[kScanBlock - 1];
+    unsigned long long incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kScanBlock)
+scan_partial_kernel(const unsigned *__restrict__ in, long long n,
+                    unsigned long long *__restrict__ block_sums) {
+    __shared__ unsigned long long sh[kScanBlock];
+    long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    unsigned long long s = 0;
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += in[base + k];
+    unsigned long long total;
+    block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanBlock)
+scan_blocksums_kernel(unsigned long long *__restrict__ block_sums, int nblocks) {
+    // Single block: nblocks <= kScanTile.
+    __shared__ unsigned long long sh[kScanBlock];
+    int t = threadIdx.x;
+    unsigned long long v[kScanItems];
+    unsigned long long s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        int idx = t * kScanItems + k;
+        v[k] = idx < nblocks ? block_sums[idx] : 0ull;
+        s += v[k];
+    }
+    unsigned long long total;
+    unsigned long long ex = block_exclusive_scan(s, sh, total);
+    for (int k = 0; k < kScanItems; ++k) {
+        int idx = t * kScanItems + k;
+        if (idx < nblocks) block_sums[idx] = ex;
+        ex += v[k];
+    }
+}
+
+__global__ void __launch_bounds__(kScanBlock)
+scan_final_kernel(const unsigned *__restrict__ in, long long n,
+                  const unsigned long long *__restrict__ block_prefix,
+                  unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long sh[kScanBlock];
+    long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    unsigned long long v[kScanItems];
+    unsigned long long s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0ull;
+        s += v[k];
+    }
+    unsigned long long total;
+    unsigned long long ex = block_exclusive_scan(s, sh, total) + block_prefix[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) out[n] = ex;
+}
+
+// Pass 4: one workgroup per tile.  LDS accumulator of T0*T1*T2 doubles holding
+// integer multiples of 2^q (exact sums), then each node written once.
+template <int MAS, int AXES>
+__global__ void __launch_bounds__(kBlock)
+tile_deposit_kernel(const float4 *__restrict__ records,
+                    const unsigned long long *__restrict__ offsets, float inv, TileGeom g,
+                    const unsigned *__restrict__ wmax_bits, float weight_factor,
+                    float *__restrict__ grid, int overwrite) {
+    constexpr int F = MAS;
+    extern __shared__ double acc[];
+    const long long tile = blockIdx.x;
+    const int tz = (int)(tile % g.nt[2]);
+    const int ty = (int)((tile / g.nt[2]) % g.nt[1]);
+    const int tx = (int)(tile / ((long long)g.nt[2] * g.nt[1]));
+    const int ox = tx * g.T[0], oy = ty * g.T[1], oz = tz * g.T[2];
+    const int ex = min(g.T[0], g.dims - ox);
+    const int ey = min(g.T[1], g.dims - oy);
+    const int ez = (AXES == 3) ? min(g.T[2], g.dims - oz) : 1;
+    const int ncell = ex * ey * ez;
+    const unsigned long long r0 = offsets[tile], r1 = offsets[tile + 1];
+
+    if (r0 == r1) {
+        if (overwrite) {
+            for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+                int lz = c % ez, ly = (c / ez) % ey, lx = c / (ez * ey);
+                long long node = AXES == 3
+                    ? ((long long)(ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz)
+                    : (long long)(ox + lx) * g.dims + (oy + ly);
+                grid[node] = 0.0f;
+            }
+        }
+        return;
+    }
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) acc[c] = 0.0;
+
+    // Quantum 2^q: every |contribution| <= weight_factor*max|W| (B-spline weights
+    // are <= 1), so sum |c| <= count*wmax*factor < 2^e and 2^e/2^q = 2^52.
+    const double wmax = (double)__uint_as_float(*wmax_bits) * (double)weight_factor;
+    const double bound = (double)(r1 - r0) * wmax;
+    int e = 0;
+    if (bound > 0.0 && bound <= 1.0e300) frexp(bound, &e);
+    else e = -1000;
+    int q = e - 52;
+    if (q < -1020) q = -1020;
+    const double to_q = ldexp(1.0, -q);
+    const double from_q = ldexp(1.0, q);
+    __syncthreads();
+
+    for (unsigned long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        float4 rec = records[r];
+        float Cx[4], Cy[4], Cz[4];
+        int bx = axis_foot<MAS>(rec.x, inv, g.dims, Cx);
+        int by = axis_foot<MAS>(rec.y, inv, g.dims, Cy);
+        int bz = 0;
+        if (AXES == 3) bz = axis_foot<MAS>(rec.z, inv, g.dims, Cz);
+        const float w = rec.w * weight_factor;
+        const bool weighted = w != 1.0f;
+#pragma unroll
+        for (int l = 0; l < F; ++l) {
+            const int nx = node_add(bx, l, g.dims);
+            int lx = nx - ox;
+            if (lx < 0 || lx >= ex) continue;
+#pragma unroll
+            for (int m = 0; m < F; ++m) {
+                const int ny = node_add(by, m, g.dims);
+                int ly = ny - oy;
+                if (ly < 0 || ly >= ey) continue;
+                float cxy = Cx[l] * Cy[m];
+                if (AXES == 3) {
+#pragma unroll
+                    for (int k = 0; k < F; ++k) {
+                        const int nz = node_add(bz, k, g.dims);
+                        int lz = nz - oz;
+                        if (lz < 0 || lz >= ez) continue;
+                        float c = cxy * Cz[k];
+                        if (weighted) c = c * w;
+                        double qv = rint((double)c * to_q);
+                        atomicAdd(&acc[(lx * ey + ly) * ez + lz], qv);
+                    }
+                } else {
+                    float c = cxy;          // the reference multiplies by C[2][n] = 1.0
+                    if (weighted) c = c * w;
+                    double qv = rint((double)c * to_q);
+                    atomicAdd(&acc[lx * ey + ly], qv);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        int lz = c % ez, ly = (c / ez) % ey, lx = c / (ez * ey);
+        long long node = AXES == 3
+            ? ((long long)(ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz)
+            : (long long)(ox + lx) * g.dims + (oy + ly);
+        float v = (float)(acc[c] * from_q);
+        grid[node] = overwrite ? v : grid[node] + v;
+    }
+}
+
+// Plain form: one thread per particle, one global float atomic per node.
+template <int MAS, int AXES>
+__global__ void __launch_bounds__(kBlock)
+atomic_deposit_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
+                      float inv, int dims, float weight_factor, float *__restrict__ grid) {
+    constexpr int F = MAS;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float Cx[4], Cy[4], Cz[4];
+        int bx = axis_foot<MAS>(pos[(long long)AXES * i + 0], inv, dims, Cx);
+        int by = axis_foot<MAS>(pos[(long long)AXES * i + 1], inv, dims, Cy);
+        int bz = 0;
+        if (AXES == 3) bz = axis_foot<MAS>(pos[(long long)AXES * i + 2], inv, dims, Cz);
+        float w = (W ? W[i] : 1.0f) * weight_factor;
+        const bool weighted = w != 1.0f;
+#pragma unroll
+        for (int l = 0; l < F; ++l) {
+            const int nx = node_add(bx, l, dims);
+#pragma unroll
+            for (int m = 0; m < F; ++m) {
+                const int ny = node_add(by, m, dims);
+                float cxy = Cx[l] * Cy[m];
+                if (AXES == 3) {
+#pragma unroll
+                    for (int k = 0; k < F; ++k) {
+                        const int nz = node_add(bz, k, dims);
+                        float c = cxy * Cz[k];
+                        if (weighted) c = c * w;
+                        unsafeAtomicAdd(&grid[((long long)nx * dims + ny) * dims + nz], c);
+                    }
+                } else {
+                    float c = cxy;
+                    if (weighted) c = c * w;
+                    unsafeAtomicAdd(&grid[(long long)nx * dims + ny], c);
+                }
+            }
+        }
+    }
+}
+
+__global__ void scale_grid_kernel(float *__restrict__ grid, long long n, float divisor) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        grid[i] = grid[i] / divisor;
+}
+
+int grid_blocks(long long n, int block) {
+    long long b = (n + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > 65536) b = 65536;
+    return (int)b;
+}
+
+TileGeom make_geom(int dims, int axes) {
+    TileGeom g{};
+    g.dims = dims;
+    g.dz = axes == 3 ? dims : 1;
+    if (axes == 3) {
+        g.T[0] = g.T[1] = g.T[2] = dims < 16 ? dims : 16;
+    } else {
+        g.T[0] = g.T[1] = dims < 64 ? dims : 64;
+        g.T[2] = 1;
+    }
+    g.nt[0] = ceil_div(dims, g.T[0]);
+    g.nt[1] = ceil_div(dims, g.T[1]);
+    g.nt[2] = axes == 3 ? ceil_div(dims, g.T[2]) : 1;
+    g.ntiles = (long long)g.nt[0] * g.nt[1] * g.nt[2];
+    return g;
+}
+
+template <int MAS>
+int run_tiled(Ctx *c, const float *pos, long long n, int axes, const float *W, float *grid,
+              int dims, float inv, float weight_factor, int overwrite) {
+    TileGeom g = make_geom(dims, axes);
+    const long long nt = g.ntiles;
+    const int nscan_blocks = (int)((nt + kScanTile - 1) / kScanTile);
+    PYL_CHECK(nscan_blocks <= kScanTile, PYL_EINVAL, "too many tiles (%lld)", nt);
+    // tile buffer: counts[nt] | cursors[nt] | wmax (1) | pad | offsets[nt+1] | block sums
+    size_t cnt_bytes = ((size_t)nt * 2 + 4) * sizeof(unsigned);
+    cnt_bytes = (cnt_bytes + 255) & ~(size_t)255;
+    size_t off_bytes = ((size_t)nt + 1) * sizeof(unsigned long long);
+    size_t bs_bytes = (size_t)nscan_blocks * sizeof(unsigned long long);
+    PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes));
+    char *tb = (char *)c->bufs[BUF_TILE_CNT].ptr;
+    unsigned *counts = (unsigned *)tb;
+    unsigned *cursors = counts + nt;
+    unsigned *wmax = cursors + nt;
+    unsigned long long *offsets = (unsigned long long *)(tb + cnt_bytes);
+    unsigned long long *bsums = offsets + nt + 1;
+
+    PYL_HIP(hipMemsetAsync(counts, 0, ((size_t)nt * 2 + 4) * sizeof(unsigned), c->stream));
+    if (!W) {
+        unsigned one = 0x3f800000u;  // 1.0f
+        PYL_HIP(hipMemcpyAsync(wmax, &one, sizeof(one), hipMemcpyHostToDevice, c->stream));
+    }
+    const int nb = grid_blocks(n, kBlock);
+    if (n > 0) {
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
+        tile_count_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, counts, wmax);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
+    }
+    PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
+    scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, nt, bsums);
+    scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
+    scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, nt, bsums, offsets);
+    PYL_HIP(hipGetLastError());
+    PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
+    unsigned long long total = 0;
+    PYL_HIP(hipMemcpyAsync(&total, offsets + nt, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    PYL_TRY(ensure(c, BUF_RECORDS, (size_t)(total ? total : 1) * sizeof(float4)));
+    float4 *records = (float4 *)c->bufs[BUF_RECORDS].ptr;
+    if (n > 0) {
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
+        tile_scatter_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, offsets,
+                                                               cursors, records);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
+    }
+    const size_t lds = (size_t)g.T[0] * g.T[1] * g.T[2] * sizeof(double);
+    PYL_TRY(stage_begin(c, PYL_STAGE_MA_DEPOSIT));
+    if (axes == 3)
+        tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kBlock, lds, c->stream>>>(
+            records, offsets, inv, g, wmax, weight_factor, grid, overwrite);
+    else
+        tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kBlock, lds, c->stream>>>(
+            records, offsets, inv, g, wmax, weight_factor, grid, overwrite);
+    PYL_HIP(hipGetLastError());
+    PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));
+    return PYL_OK;
+}
+
+template <int MAS>
+int run_atomic(Ctx *c, const float *pos, long long n, int axes, const float *W, float *grid,
+               int dims, float inv, float weight_factor, int overwrite) {
+    long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
+    if (overwrite) PYL_HIP(hipMemsetAsync(grid, 0, (size_t)ncell * sizeof(float), c->stream));
+    if (n == 0) return PYL_OK;
+    PYL_TRY(stage_begin(c, PYL_STAGE_MA_ATOMIC));
+    const int nb = grid_blocks(n, kBlock);
+    if (axes == 3)
+        atomic_deposit_kernel<MAS, 3><<<nb, kBlock, 0, c->stream>>>(pos, W, n, inv, dims,
+                                                                   weight_factor, grid);
+    else
+        atomic_deposit_kernel<MAS, 2><<<nb, kBlock, 0, c->stream>>>(pos, W, n, inv, dims,
+                                                                   weight_factor, grid);
+    PYL_HIP(hipGetLastError());
+    PYL_TRY(stage_end(c, PYL_STAGE_MA_ATOMIC));
+    return PYL_OK;
+}
+
+}  // namespace
+
+// Device-pointer MA used by the C ABI and by the batched driver.
+int ma_device(Ctx *c, const float *pos, long long n, int axes, const float *W, float *grid,
+              int dims, float BoxSize, int mas, int flags) {
+    // MAS_library.pyx:135 inv_cell_size = dims/BoxSize in float (MAS_c.c:19 computes it
+    // in double and stores it to float, which rounds to the same value).
+    const float inv = (float)dims / BoxSize;
+    float weight_factor = 1.0f;
+    const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
+    const int overwrite = (flags & PYL_MA_OVERWRITE) ? 1 : 0;
+    if (axes == 2 && (flags & PYL_MA_CYTHON_2D) && mas != PYL_NGP) {
+        // MA() runs the 3D kernel on a (d,d,1) view, so each deposit lands F times
+        // in the same cell; with renormalize_2D the whole grid is then divided by F
+        // (MAS_library.pyx:84-110).  Equivalent: grid/F + deposit, or grid + F*deposit.
+        if (flags & PYL_MA_RENORM_2D) {
+            if (!overwrite) {
+                scale_grid_kernel<<<grid_blocks(ncell, kBlock), kBlock, 0, c->stream>>>(
+                    grid, ncell, (float)mas);
+                PYL_HIP(hipGetLastError());
+            }
+        } else {
+            weight_factor = (float)mas;
+        }
+    }
+    const bool atomic = (flags & PYL_MA_ATOMIC) != 0;
+    switch (mas) {
+#define PYL_MA_CASE(M)                                                                    \
+    case M:                                                                               \
+        return atomic ? run_atomic<M>(c, pos, n, axes, W, grid, dims, inv, weight_factor,  \
+                                      overwrite)                                          \
+                      : run_tiled<M>(c, pos, n, axes, W, grid, dims, inv, weight_factor,   \
+                                     overwrite);
+        PYL_MA_CASE(PYL_NGP)
+        PYL_MA_CASE(PYL_CIC)
+        PYL_MA_CASE(PYL_TSC)
+        PYL_MA_CASE(PYL_PCS)
+#undef PYL_MA_CASE
+    }
+    set_error("unknown MAS id %d", mas);
+    return PYL_EINVAL;
+}
+
+}  // namespace pyl
+
+using namespace pyl;
+
+extern "C" int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
+                      float *grid, int dims, float BoxSize, int mas, int flags) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    Ctx *c = &ctx->c;
+    PYL_CHECK(axes == 2 || axes == 3, PYL_EINVAL, "pos must have 2 or 3 columns (got %d)", axes);
+    PYL_CHECK(dims >= 1, PYL_EINVAL, "dims must be positive (got %d)", dims);
+    PYL_CHECK(mas >= PYL_NGP && mas <= PYL_PCS, PYL_EINVAL, "option not valid (MAS id %d)", mas);
+    PYL_CHECK(n >= 0, PYL_EINVAL, "negative particle count");
+    PYL_CHECK(grid != nullptr && (pos != nullptr || n == 0), PYL_EINVAL, "NULL array");
+    PYL_CHECK(BoxSize > 0.0f, PYL_EINVAL, "BoxSize must be positive");
+    const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
+    PYL_CHECK(ncell < (1ll << 34), PYL_EINVAL, "grid too large");
+    PYL_HIP(hipSetDevice(c->device));
+    reset_stages(c);
+    if (flags & PYL_PTR_DEVICE) {
+        PYL_TRY(ma_device(c, pos, n, axes, W, grid, dims, BoxSize, mas, flags));
+        return finish(c, flags);
+    }
+    // Host pointers: stage through device scratch.
+    PYL_TRY(ensure(c, BUF_POS, (size_t)(n ? n : 1) * axes * sizeof(float)));
+    PYL_TRY(ensure(c, BUF_GRID, (size_t)ncell * sizeof(float)));
+    if (W) PYL_TRY(ensure(c, BUF_W, (size_t)(n ? n : 1) * sizeof(float)));
+    float *dpos = (float *)c->bufs[BUF_POS].ptr;
+    float *dgrid = (float *)c->bufs[BUF_GRID].ptr;
+    float *dW = W ? (float *)c->bufs[BUF_W].ptr : nullptr;
+    if (n) PYL_HIP(hipMemcpyAsync(dpos, pos, (size_t)n * axes * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (W && n) PYL_HIP(hipMemcpyAsync(dW, W, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (!(flags & PYL_MA_OVERWRITE))
+        PYL_HIP(hipMemcpyAsync(dgrid, grid, (size_t)ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    PYL_TRY(ma_device(c, dpos, n, axes, dW, dgrid, dims, BoxSize, mas, flags | PYL_PTR_DEVICE));
+    PYL_HIP(hipMemcpyAsync(grid, dgrid, (size_t)ncell * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    return PYL_OK;
+}
+
+// ---- MAS_c.h:3-10 compatible entry points --------------------------------
+namespace {
+pyl_ctx *default_ctx() {
+    static pyl_ctx *ctx = nullptr;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        if (pyl_ctx_create(&ctx, 0, 0) != PYL_OK) ctx = nullptr;
+    });
+    return ctx;
+}
+
+void mas_c_entry(int mas, float *pos, float *number, float *W, long particles, int dims,
+                 int axes, float BoxSize) {
+    pyl_ctx *ctx = default_ctx();
+    int s = ctx ? pyl_ma(ctx, pos, particles, axes, W, number, dims, BoxSize, mas, 0)
+                : PYL_EHIP;
+    if (s != PYL_OK) fprintf(stderr, "pylians_hip: %s\n", pyl_last_error());
+}
+}  // namespace
+
+extern "C" {
+void NGP(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads) {
+    (void)threads;
+    mas_c_entry(PYL_NGP, pos, number, W, particles, dims, axes, BoxSize);
+}
+void CIC(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads) {
+    (void)threads;
+    mas_c_entry(PYL_CIC, pos, number, W, particles, dims, axes, BoxSize);
+}
+void TSC(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads) {
+    (void)threads;
+    mas_c_entry(PYL_TSC, pos, number, W, particles, dims, axes, BoxSize);
+}
+void PCS(float *pos, float *number, float *W, long particles, int dims, int axes,
+         float BoxSize, int threads) {
+    (void)threads;
+    mas_c_entry(PYL_PCS, pos, number, W, particles, dims, axes, BoxSize);
+}
+}

@@ -1,0 +1,931 @@
+// Power spectra on gfx950: rocFFT r2c + one fused binning pass.
+//
+// Reference semantics: library/Pk_library/Pk_library.pyx:263-420 (Pk) and
+// :529-793 (XPk).  Per mode of the half-complex FFT output the binning pass
+// does, in one read of delta_k: the half-space skip rule (:324-327), integer
+// bin assignment (k_index = isqrt(k^2), k_per = isqrt(k_perp^2), exact for the
+// reference's (int)sqrt of integer squares), MAS deconvolution with a float
+// factor equal to the double product of the per-axis corrections (:351-352),
+// |delta_k|^2 in double (:355-357), mu-Legendre weights for l = 0, 2, 4 and the
+// phase (:358), and the 1D / 2D / 3D accumulations (:361-378).
+//
+// Work decomposition.  Rows (kxx, kyy) of the half-complex cube are grouped by
+// a key that fixes the 2D bins a row can reach: |kx| (axis 0), |ky| (axis 1)
+// or k_per = isqrt(kx^2+ky^2) (axis 2).  A work item is up to R rows with one
+// key; one workgroup per item accumulates into LDS: the 3D bins, one "line" of
+// 2D bins (all k_per at fixed k_par for axes 0/1, all k_par at fixed k_per for
+// axis 2) and the 1D bins.  Each lane handles runs of 4 consecutive kz and
+// merges equal bins in registers before its LDS atomics.  Items write their
+// LDS partials with plain stores; a reduce kernel sums them in item order, so
+// the result is deterministic and no global atomics are used.
+//
+// Mode counts and k sums do not depend on the data: they are computed exactly
+// on the host once per (dims, axis) from Hermitian-pair counting and cached.
+#include "pyl_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+namespace pyl {
+
+// ------------------------------------------------------------------ mode plan
+// Independent modes = one of each conjugate pair plus the self-conjugate modes,
+// so per bin: N_indep = (N_all + N_self) / 2, where N_all counts the full
+// dims^3 cube and N_self the modes with every component in {0, dims/2}.
+// Every binned quantity depends only on (|kx|, |ky|, |kz|), so the cube is
+// walked over absolute values with multiplicities.
+void compute_mode_plan(int dims, int axis, ModePlan *mp) {
+    const int m = dims / 2;
+    const bool even = dims % 2 == 0;
+    const int kmax_par = m;
+    const int kmax_per = (int)std::sqrt((double)(m * m + m * m));
+    const int kmax = (int)std::sqrt((double)(3 * m * m));
+    const long long nb2 = (long long)(kmax_par + 1) * (kmax_per + 1);
+    const long long nk2 = 3LL * m * m + 1;
+    auto mult = [&](int a) -> long long { return (a == 0 || (even && a == m)) ? 1 : 2; };
+    auto isq = [](long long v) -> int {
+        long long s = (long long)std::sqrt((double)v);
+        while (s * s > v) --s;
+        while ((s + 1) * (s + 1) <= v) ++s;
+        return (int)s;
+    };
+    std::vector<int> isqrt_tab(2LL * m * m + 1);
+    for (long long v = 0; v < (long long)isqrt_tab.size(); ++v) isqrt_tab[v] = isq(v);
+
+    unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    if (m < 32) nth = 1;
+    struct Part {
+        std::vector<long long> c3, c2, c1;
+    };
+    std::vector<Part> parts(nth);
+    auto work = [&](unsigned t) {
+        Part &P = parts[t];
+        P.c3.assign(nk2, 0);
+        P.c2.assign(nb2, 0);
+        P.c1.assign(m + 1, 0);
+        for (int a = (int)t; a <= m; a += (int)nth) {
+            const long long ma = mult(a);
+            for (int b = 0; b <= m; ++b) {
+                const long long mab = ma * mult(b);
+                for (int c = 0; c <= m; ++c) {
+                    const long long w = mab * mult(c);
+                    const long long k2 = (long long)a * a + (long long)b * b + (long long)c * c;
+                    int kpar, kper;
+                    if (axis == 0) { kpar = a; kper = isqrt_tab[b * b + c * c]; }
+                    else if (axis == 1) { kpar = b; kper = isqrt_tab[a * a + c * c]; }
+                    else { kpar = c; kper = isqrt_tab[a * a + b * b]; }
+                    P.c3[k2] += w;
+                    P.c2[(long long)(kmax_par + 1) * kper + kpar] += w;
+                    if (k2 <= (long long)m * m) P.c1[kpar] += w;
+                }
+            }
+        }
+    };
+    if (nth == 1) work(0);
+    else {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; ++t) th.emplace_back(work, t);
+        for (auto &x : th) x.join();
+    }
+    std::vector<long long> c3(nk2, 0), c2(nb2, 0), c1(m + 1, 0);
+    for (auto &P : parts) {
+        for (long long i = 0; i < nk2; ++i) c3[i] += P.c3[i];
+        for (long long i = 0; i < nb2; ++i) c2[i] += P.c2[i];
+        for (int i = 0; i <= m; ++i) c1[i] += P.c1[i];
+    }
+    // self-conjugate modes: components in {0} (odd) or {0, m} (even)
+    std::vector<int> vals = even ? std::vector<int>{0, m} : std::vector<int>{0};
+    if (even && m == 0) vals = {0};
+    for (int a : vals)
+        for (int b : vals)
+            for (int c : vals) {
+                const long long k2 = (long long)a * a + (long long)b * b + (long long)c * c;
+                int kpar, kper;
+                if (axis == 0) { kpar = a; kper = isq((long long)b * b + (long long)c * c); }
+                else if (axis == 1) { kpar = b; kper = isq((long long)a * a + (long long)c * c); }
+                else { kpar = c; kper = isq((long long)a * a + (long long)b * b); }
+                c3[k2] += 1;
+                c2[(long long)(kmax_par + 1) * kper + kpar] += 1;
+                if (k2 <= (long long)m * m) c1[kpar] += 1;
+            }
+    mp->dims = dims;
+    mp->axis = axis;
+    mp->N3D.assign(kmax + 1, 0.0);
+    mp->k3D.assign(kmax + 1, 0.0);
+    for (long long k2 = 0; k2 < nk2; ++k2) {
+        long long n = c3[k2] / 2;  // (N_all + N_self) / 2, always an integer
+        if (!n) continue;
+        int b = isq(k2);
+        mp->N3D[b] += (double)n;
+        mp->k3D[b] += (double)n * std::sqrt((double)k2);
+    }
+    mp->N2D.resize(nb2);
+    for (long long i = 0; i < nb2; ++i) mp->N2D[i] = (double)(c2[i] / 2);
+    mp->N1D.resize(m + 1);
+    mp->k1D.resize(m + 1);
+    for (int i = 0; i <= m; ++i) {
+        mp->N1D[i] = (double)(c1[i] / 2);
+        mp->k1D[i] = (double)i * mp->N1D[i];
+    }
+}
+
+const ModePlan *get_mode_plan(Ctx *c, int dims, int axis) {
+    auto key = std::make_pair(dims, axis);
+    auto it = c->mode_plans.find(key);
+    if (it != c->mode_plans.end()) return &it->second;
+    ModePlan mp;
+    compute_mode_plan(dims, axis, &mp);
+    return &c->mode_plans.emplace(key, std::move(mp)).first->second;
+}
+
+namespace {
+
+constexpr int kRun = 4;          // consecutive kz per lane
+constexpr int kBinBlock = 512;
+
+struct PkDims {
+    int dims, m, zlen, zpad, even, kmax, kmax_par, kmax_per;
+    long long nb2;
+};
+
+PkDims pk_dims(int dims) {
+    PkDims p;
+    p.dims = dims;
+    p.m = dims / 2;
+    p.zlen = p.m + 1;
+    p.zpad = (p.zlen + kRun - 1) / kRun * kRun;
+    p.even = dims % 2 == 0;
+    p.kmax_par = p.m;
+    p.kmax_per = (int)std::sqrt((double)(p.m * p.m + p.m * p.m));
+    p.kmax = (int)std::sqrt((double)(3 * p.m * p.m));
+    p.nb2 = (long long)(p.kmax_par + 1) * (p.kmax_per + 1);
+    return p;
+}
+
+int row_key(int kxx, int kyy, int dims, int axis) {
+    const int m = dims / 2;
+    int kx = kxx > m ? kxx - dims : kxx;
+    int ky = kyy > m ? kyy - dims : kyy;
+    if (axis == 0) return std::abs(kx);
+    if (axis == 1) return std::abs(ky);
+    long long v = (long long)kx * kx + (long long)ky * ky;
+    long long s = (long long)std::sqrt((double)v);
+    while (s * s > v) --s;
+    while ((s + 1) * (s + 1) <= v) ++s;
+    return (int)s;
+}
+
+int get_bin_plan(Ctx *c, int dims, int axis, BinPlan **out) {
+    auto key = std::make_pair(dims, axis);
+    auto it = c->bin_plans.find(key);
+    if (it != c->bin_plans.end()) {
+        *out = &it->second;
+        return PYL_OK;
+    }
+    // rows per item: aim for >= ~2048 items on large grids
+    const long long nrows_total = (long long)dims * dims;
+    const int R = (int)std::max(4LL, std::min(256LL, nrows_total / 2048));
+    PkDims p = pk_dims(dims);
+    const int nkeys = (axis == 2) ? p.kmax_per + 1 : p.m + 1;
+    std::vector<std::vector<int>> by_key(nkeys);
+    for (int kxx = 0; kxx < dims; ++kxx)
+        for (int kyy = 0; kyy < dims; ++kyy)
+            by_key[row_key(kxx, kyy, dims, axis)].push_back(kxx * dims + kyy);
+    std::vector<int> rows;
+    rows.reserve(nrows_total);
+    std::vector<int4> items;
+    std::vector<int> key_items(nkeys + 1, 0);
+    for (int k = 0; k < nkeys; ++k) {
+        key_items[k] = (int)items.size();
+        const auto &v = by_key[k];
+        for (size_t s = 0; s < v.size(); s += R) {
+            int n = (int)std::min<size_t>(R, v.size() - s);
+            items.push_back(make_int4((int)rows.size() + (int)s, n, k, 0));
+        }
+        rows.insert(rows.end(), v.begin(), v.end());
+    }
+    key_items[nkeys] = (int)items.size();
+    BinPlan bp;
+    bp.dims = dims;
+    bp.axis = axis;
+    bp.R = R;
+    bp.nitems = (int)items.size();
+    bp.nkeys = nkeys;
+    auto fail = [&](hipError_t e) {
+        (void)hipFree(bp.d_rows);
+        (void)hipFree(bp.d_items);
+        (void)hipFree(bp.d_key_items);
+        set_error("binning plan upload failed: %s", hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? PYL_ENOMEM : PYL_EHIP;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&bp.d_rows, rows.size() * sizeof(int))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&bp.d_items, items.size() * sizeof(int4))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&bp.d_key_items, key_items.size() * sizeof(int))) != hipSuccess) return fail(e);
+    if ((e = hipMemcpy(bp.d_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return fail(e);
+    if ((e = hipMemcpy(bp.d_items, items.data(), items.size() * sizeof(int4),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e);
+    if ((e = hipMemcpy(bp.d_key_items, key_items.data(), key_items.size() * sizeof(int),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e);
+    *out = &c->bin_plans.emplace(key, bp).first->second;
+    return PYL_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+// Spectra computed by one launch: NFL = 1 -> {auto a}; NFL = 2 -> {auto a, auto b, cross ab}.
+template <int NFL> struct Spec { static constexpr int S = NFL == 1 ? 1 : 3; };
+
+struct BinArgs {
+    const float2 *dk[2];     // half-complex fields
+    const double *corr[2];   // MAS correction tables [dims] per field
+    const int *rows;
+    const int4 *items;
+    double *partials;        // [nitems][P]
+    int dims, m, zlen, zpad, even, kmax, L2, L1;
+    int P, off2, off1;       // partial layout: acc3 at 0, line2 at off2, line1 at off1
+    int rmax;                // max rows per item (LDS row table size)
+};
+
+__device__ __forceinline__ int isqrt_i(int v) {
+    int s = (int)sqrtf((float)v);
+    if (s * s > v) --s;
+    if ((s + 1) * (s + 1) <= v) ++s;
+    return s;
+}
+
+template <int AXIS, int NFL, bool PHASE>
+__global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
+    constexpr int S = Spec<NFL>::S;
+    constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);   // per 3D bin: P0,P2,P4 per spectrum [+phase]
+    extern __shared__ double lds[];
+    double *acc3 = lds;                               // [(kmax+1)*NA3]
+    double *line2 = acc3 + (a.kmax + 1) * NA3;        // [L2*S]
+    double *line1 = line2 + a.L2 * S;                 // [L1*S]
+    double *corrz = line1 + a.L1 * S;                 // [zlen*NFL]
+    double *rowc = corrz + a.zlen * NFL;              // [rmax*NFL]
+    int *rowi = (int *)(rowc + a.rmax * NFL);         // [rmax*3]: kx, ky, linear row
+
+    const int4 item = a.items[blockIdx.x];
+    const int nrows = item.y;
+    const int key = item.z;
+    const int tid = threadIdx.x;
+    const int n3 = (a.kmax + 1) * NA3;
+    for (int i = tid; i < n3 + (a.L2 + a.L1) * S; i += blockDim.x) lds[i] = 0.0;
+    for (int i = tid; i < a.zlen * NFL; i += blockDim.x) {
+        int f = i / a.zlen, kz = i - f * a.zlen;
+        corrz[i] = a.corr[f][kz];
+    }
+    for (int r = tid; r < nrows; r += blockDim.x) {
+        int lin = a.rows[item.x + r];
+        int kxx = lin / a.dims, kyy = lin - kxx * a.dims;
+        int kx = kxx > a.m ? kxx - a.dims : kxx;
+        int ky = kyy > a.m ? kyy - a.dims : kyy;
+        rowi[3 * r + 0] = kx;
+        rowi[3 * r + 1] = ky;
+        rowi[3 * r + 2] = lin;
+        for (int f = 0; f < NFL; ++f) rowc[r * NFL + f] = a.corr[f][kxx] * a.corr[f][kyy];
+    }
+    __syncthreads();
+
+    const int mm = a.m * a.m;
+    double p1[S];                                      // 1D accumulator (axes 0/1: k_par = key)
+#pragma unroll
+    for (int s = 0; s < S; ++s) p1[s] = 0.0;
+
+    // Lane runs start at flat index tid*kRun and advance by blockDim*kRun over the
+    // item's rows x zpad (zpad is a multiple of kRun, so a run never crosses a row).
+    const int step = blockDim.x * kRun;
+    const int dr = step / a.zpad, dc = step - dr * a.zpad;
+    int r = (tid * kRun) / a.zpad;
+    int kz0 = tid * kRun - r * a.zpad;
+    for (; r < nrows; r += dr, kz0 += dc) {
+        if (kz0 >= a.zpad) {
+            kz0 -= a.zpad;
+            ++r;
+            if (r >= nrows) break;
+        }
+        const int kx = rowi[3 * r + 0], ky = rowi[3 * r + 1];
+        const long long rowoff = (long long)rowi[3 * r + 2] * a.zlen;
+        const int kxy2 = kx * kx + ky * ky;
+        const bool kx_neg = kx < 0;
+        const bool kx_edge = (kx == 0) || (a.even && kx == a.m);
+        const bool ky_neg = ky < 0;
+        double c01[NFL];
+#pragma unroll
+        for (int f = 0; f < NFL; ++f) c01[f] = rowc[r * NFL + f];
+
+        float2 v[NFL][kRun];
+#pragma unroll
+        for (int f = 0; f < NFL; ++f)
+#pragma unroll
+            for (int s = 0; s < kRun; ++s)
+                v[f][s] = (kz0 + s < a.zlen) ? a.dk[f][rowoff + kz0 + s] : make_float2(0.f, 0.f);
+
+        int cur3 = -1, cur2 = -1;
+        double r3[NA3], r2[S];
+#pragma unroll
+        for (int i = 0; i < NA3; ++i) r3[i] = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) r2[s] = 0.0;
+
+#pragma unroll
+        for (int s = 0; s < kRun; ++s) {
+            const int kz = kz0 + s;
+            bool ok = kz < a.zlen;
+            if (kz == 0 || (a.even && kz == a.m)) {     // Pk_library.pyx:324-327
+                if (kx_neg) ok = false;
+                else if (kx_edge && ky_neg) ok = false;
+            }
+            if (!ok) continue;
+            const int k2 = kxy2 + kz * kz;
+            const int kidx = isqrt_i(k2);
+            int kpar, kper;
+            if (AXIS == 0) { kpar = kx < 0 ? -kx : kx; kper = isqrt_i(ky * ky + kz * kz); }
+            else if (AXIS == 1) { kpar = ky < 0 ? -ky : ky; kper = isqrt_i(kx * kx + kz * kz); }
+            else { kpar = kz; kper = key; }
+            const float mu2 = (k2 == 0) ? 0.0f : __fdividef((float)(kpar * kpar), (float)k2);
+            const float L2 = 1.5f * mu2 - 0.5f;
+            const float L4 = (35.0f * mu2 * mu2 - 30.0f * mu2 + 3.0f) * 0.125f;
+
+            double re[NFL], im[NFL];
+#pragma unroll
+            for (int f = 0; f < NFL; ++f) {
+                const float mf = (float)(c01[f] * corrz[f * a.zlen + kz]);
+                const float xr = v[f][s].x * mf, xi = v[f][s].y * mf;   // complex64 * float
+                re[f] = (double)xr;
+                im[f] = (double)xi;
+            }
+            double d2[S];
+            d2[0] = re[0] * re[0] + im[0] * im[0];
+            if (NFL == 2) {
+                d2[1] = re[1] * re[1] + im[1] * im[1];
+                d2[2] = re[0] * re[1] + im[0] * im[1];
+            }
+            // 3D: merge equal bins in registers, flush on change
+            if (kidx != cur3) {
+                if (cur3 >= 0) {
+#pragma unroll
+                    for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[cur3 * NA3 + i], r3[i]);
+                }
+                cur3 = kidx;
+#pragma unroll
+                for (int i = 0; i < NA3; ++i) r3[i] = 0.0;
+            }
+#pragma unroll
+            for (int sp = 0; sp < S; ++sp) {
+                r3[3 * sp + 0] += d2[sp];
+                r3[3 * sp + 1] += d2[sp] * (double)L2;
+                r3[3 * sp + 2] += d2[sp] * (double)L4;
+            }
+            if (PHASE) {
+                const float ph = atan2f((float)re[0], sqrtf((float)d2[0]));
+                r3[NA3 - 1] += (double)(ph * ph);
+            }
+            // 2D line: axes 0/1 index k_per (slowly varying), axis 2 index kz
+            const int li = (AXIS == 2) ? kz : kper;
+            if (AXIS == 2) {
+#pragma unroll
+                for (int sp = 0; sp < S; ++sp) atomicAdd(&line2[li * S + sp], d2[sp]);
+            } else {
+                if (li != cur2) {
+                    if (cur2 >= 0) {
+#pragma unroll
+                        for (int sp = 0; sp < S; ++sp) atomicAdd(&line2[cur2 * S + sp], r2[sp]);
+                    }
+                    cur2 = li;
+#pragma unroll
+                    for (int sp = 0; sp < S; ++sp) r2[sp] = 0.0;
+                }
+#pragma unroll
+                for (int sp = 0; sp < S; ++sp) r2[sp] += d2[sp];
+            }
+            // 1D: modes with k <= middle
+            if (k2 <= mm) {
+                if (AXIS == 2) {
+#pragma unroll
+                    for (int sp = 0; sp < S; ++sp) atomicAdd(&line1[kz * S + sp], d2[sp]);
+                } else {
+#pragma unroll
+                    for (int sp = 0; sp < S; ++sp) p1[sp] += d2[sp];
+                }
+            }
+        }
+        if (cur3 >= 0) {
+#pragma unroll
+            for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[cur3 * NA3 + i], r3[i]);
+        }
+        if (AXIS != 2 && cur2 >= 0) {
+#pragma unroll
+            for (int sp = 0; sp < S; ++sp) atomicAdd(&line2[cur2 * S + sp], r2[sp]);
+        }
+    }
+    if (AXIS != 2) {
+#pragma unroll
+        for (int sp = 0; sp < S; ++sp) {
+            double x = p1[sp];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if ((tid & 63) == 0) atomicAdd(&line1[sp], x);
+        }
+    }
+    __syncthreads();
+    double *out = a.partials + (long long)blockIdx.x * a.P;
+    for (int i = tid; i < n3; i += blockDim.x) out[i] = acc3[i];
+    for (int i = tid; i < a.L2 * S; i += blockDim.x) out[a.off2 + i] = line2[i];
+    for (int i = tid; i < a.L1 * S; i += blockDim.x) out[a.off1 + i] = line1[i];
+}
+
+// Sum item partials in item order (deterministic).  Output layout:
+//   out3[(kmax+1)*NA3], out2[nb2*S] (reference 2D index), out1[(m+1)*S]
+struct ReduceArgs {
+    const double *partials;
+    const int *key_items;
+    double *out3, *out2, *out1;
+    int nitems, nkeys, n3, L2, L1, S, P, off2, off1, axis, kmax_par;
+};
+
+__global__ void pk_reduce3_kernel(ReduceArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n3) return;
+    double s = 0.0;
+    for (int it = 0; it < a.nitems; ++it) s += a.partials[(long long)it * a.P + i];
+    a.out3[i] = s;
+}
+
+// One thread per (key, line index, spectrum): sums the key's items.
+__global__ void pk_reduce2_kernel(ReduceArgs a) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long n = (long long)a.nkeys * a.L2 * a.S;
+    if (i >= n) return;
+    int sp = (int)(i % a.S);
+    int li = (int)((i / a.S) % a.L2);
+    int key = (int)(i / ((long long)a.S * a.L2));
+    double s = 0.0;
+    for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
+        s += a.partials[(long long)it * a.P + a.off2 + li * a.S + sp];
+    // reference 2D index: (kmax_par+1)*k_per + k_par
+    long long idx2 = (a.axis == 2) ? (long long)(a.kmax_par + 1) * key + li
+                                   : (long long)(a.kmax_par + 1) * li + key;
+    a.out2[idx2 * a.S + sp] = s;
+}
+
+__global__ void pk_reduce1_kernel(ReduceArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.axis == 2) {
+        if (i >= a.L1 * a.S) return;   // L1 = m+1 bins (k_par = kz), all items
+        double s = 0.0;
+        for (int it = 0; it < a.nitems; ++it) s += a.partials[(long long)it * a.P + a.off1 + i];
+        a.out1[i] = s;
+    } else {
+        if (i >= a.nkeys * a.S) return;  // bin = key = k_par
+        int key = i / a.S, sp = i % a.S;
+        double s = 0.0;
+        for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
+            s += a.partials[(long long)it * a.P + a.off1 + sp];
+        a.out1[key * a.S + sp] = s;
+    }
+}
+
+// FFT prologue: out = (in / scale) - shift in float32 (the reference caller's
+// "delta /= mean; delta -= 1.0", Pk_snapshot.py:88), or a plain copy.
+__global__ void prologue_kernel(const float *__restrict__ in, float *__restrict__ out,
+                                long long n, float scale, float shift, int apply) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float x = in[i];
+        if (apply) {
+            x = x / scale;
+            x = x - shift;
+        }
+        out[i] = x;
+    }
+}
+
+__global__ void rsd_kernel(float *__restrict__ pos, const float *__restrict__ vel, long long n,
+                           float BoxSize, float factor, int axis) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float p = pos[3 * i + axis];
+        float v = vel[3 * i + axis];
+        float dv = v * factor;
+        p = p + dv;
+        // redshift_space_library.pyx:40-46.  The "while <0" loop is bounded here:
+        // after 64 box lengths the remainder is folded with fmodf.
+        int guard = 0;
+        while (p < 0.0f && guard < 64) { p += BoxSize; ++guard; }
+        if (p < 0.0f) p = fmodf(p, BoxSize) + BoxSize;
+        if (p > BoxSize) p = fmodf(p, BoxSize);
+        pos[3 * i + axis] = p;
+    }
+}
+
+template <typename K>
+int set_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        PYL_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)bytes));
+    return PYL_OK;
+}
+
+// Host-side MAS tables: corr[kk] = MAS_correction(pi/dims * k(kk), p) with the
+// reference's libm calls (Pk_library.pyx:83-84, prefact :310).
+void mas_table(int dims, int p, double *out) {
+    const int m = dims / 2;
+    const double prefact = M_PI / dims;
+    for (int kk = 0; kk < dims; ++kk) {
+        int k = kk > m ? kk - dims : kk;
+        double x = prefact * k;
+        out[kk] = (x == 0.0) ? 1.0 : std::pow(x / std::sin(x), (double)p);
+    }
+}
+
+// Run the binning of NFL fields (device half-complex arrays) -> reduced sums
+// appended into host arrays of spectra [s_off, s_off+S).
+struct HostAcc {
+    int S_total;
+    std::vector<double> P3;     // [(kmax+1)*3*S_total] l-major per bin: (bin*3+l)*S_total+s
+    std::vector<double> phase;  // [kmax+1]
+    std::vector<double> P2;     // [nb2*S_total]
+    std::vector<double> P1;     // [(m+1)*S_total]
+};
+
+template <int AXIS, int NFL, bool PHASE>
+int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
+               const double *const *corr, HostAcc &acc, const int *spec_out) {
+    constexpr int S = Spec<NFL>::S;
+    constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);
+    const int L2 = (AXIS == 2) ? p.zlen : p.kmax_per + 1;
+    const int L1 = (AXIS == 2) ? p.zlen : 1;
+    const int n3 = (p.kmax + 1) * NA3;
+    const int P = n3 + (L2 + L1) * S;
+    const int off2 = n3, off1 = n3 + L2 * S;
+    const int rmax = bp->R;
+    size_t lds = sizeof(double) * ((size_t)P + (size_t)p.zlen * NFL + (size_t)rmax * NFL) +
+                 sizeof(int) * 3 * (size_t)rmax;
+    PYL_CHECK(lds <= 160 * 1024, PYL_EINVAL, "grid too large for the binning kernel (%zu B LDS)", lds);
+    // scratch: partials + reduced outputs
+    const size_t part_bytes = sizeof(double) * (size_t)bp->nitems * P;
+    const size_t out3 = (size_t)n3, out2 = (size_t)p.nb2 * S, out1 = (size_t)(p.m + 1) * S;
+    PYL_TRY(ensure(c, BUF_BINS, part_bytes + sizeof(double) * (out3 + out2 + out1)));
+    double *partials = (double *)c->bufs[BUF_BINS].ptr;
+    double *d3 = partials + (size_t)bp->nitems * P;
+    double *d2 = d3 + out3;
+    double *d1 = d2 + out2;
+    PYL_HIP(hipMemsetAsync(d2, 0, sizeof(double) * out2, c->stream));
+
+    BinArgs a{};
+    for (int f = 0; f < NFL; ++f) {
+        a.dk[f] = dk[f];
+        a.corr[f] = corr[f];
+    }
+    a.rows = bp->d_rows;
+    a.items = bp->d_items;
+    a.partials = partials;
+    a.dims = p.dims; a.m = p.m; a.zlen = p.zlen; a.zpad = p.zpad; a.even = p.even;
+    a.kmax = p.kmax; a.L2 = L2; a.L1 = L1; a.P = P; a.off2 = off2; a.off1 = off1; a.rmax = rmax;
+    auto kern = pk_bin_kernel<AXIS, NFL, PHASE>;
+    PYL_TRY(set_lds(kern, lds));
+    PYL_TRY(stage_begin(c, PYL_STAGE_PK_BIN));
+    kern<<<bp->nitems, kBinBlock, lds, c->stream>>>(a);
+    PYL_HIP(hipGetLastError());
+    ReduceArgs r{};
+    r.partials = partials; r.key_items = bp->d_key_items; r.out3 = d3; r.out2 = d2; r.out1 = d1;
+    r.nitems = bp->nitems; r.nkeys = bp->nkeys; r.n3 = n3; r.L2 = L2; r.L1 = L1; r.S = S;
+    r.P = P; r.off2 = off2; r.off1 = off1; r.axis = AXIS; r.kmax_par = p.kmax_par;
+    pk_reduce3_kernel<<<ceil_div(n3, 256), 256, 0, c->stream>>>(r);
+    long long n2 = (long long)bp->nkeys * L2 * S;
+    pk_reduce2_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, c->stream>>>(r);
+    int n1 = (AXIS == 2) ? L1 * S : bp->nkeys * S;
+    pk_reduce1_kernel<<<ceil_div(n1, 256), 256, 0, c->stream>>>(r);
+    PYL_HIP(hipGetLastError());
+    PYL_TRY(stage_end(c, PYL_STAGE_PK_BIN));
+    std::vector<double> h3(out3), h2(out2), h1(out1);
+    PYL_HIP(hipMemcpyAsync(h3.data(), d3, sizeof(double) * out3, hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipMemcpyAsync(h2.data(), d2, sizeof(double) * out2, hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipMemcpyAsync(h1.data(), d1, sizeof(double) * out1, hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    // scatter into the caller's spectrum columns
+    const int ST = acc.S_total;
+    for (int b = 0; b <= p.kmax; ++b) {
+        for (int sp = 0; sp < S; ++sp) {
+            int so = spec_out[sp];
+            if (so < 0) continue;
+            for (int l = 0; l < 3; ++l)
+                acc.P3[((size_t)b * 3 + l) * ST + so] = h3[(size_t)b * NA3 + 3 * sp + l];
+        }
+        if (PHASE) acc.phase[b] = h3[(size_t)b * NA3 + NA3 - 1];
+    }
+    for (long long i = 0; i < p.nb2; ++i)
+        for (int sp = 0; sp < S; ++sp)
+            if (spec_out[sp] >= 0) acc.P2[(size_t)i * ST + spec_out[sp]] = h2[(size_t)i * S + sp];
+    for (int i = 0; i <= p.m; ++i)
+        for (int sp = 0; sp < S; ++sp)
+            if (spec_out[sp] >= 0) acc.P1[(size_t)i * ST + spec_out[sp]] = h1[(size_t)i * S + sp];
+    return PYL_OK;
+}
+
+template <int NFL, bool PHASE>
+int launch_bin_axis(Ctx *c, int axis, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
+                    const double *const *corr, HostAcc &acc, const int *spec_out) {
+    if (axis == 0) return launch_bin<0, NFL, PHASE>(c, bp, p, dk, corr, acc, spec_out);
+    if (axis == 1) return launch_bin<1, NFL, PHASE>(c, bp, p, dk, corr, acc, spec_out);
+    return launch_bin<2, NFL, PHASE>(c, bp, p, dk, corr, acc, spec_out);
+}
+
+// Normalisation, Pk_library.pyx:384-418 (and :735-791 for XPk).
+int finalize(const ModePlan *mp, const PkDims &p, double BoxSize, const HostAcc &acc,
+             bool want_phase, pyl_pk_out *out) {
+    const int S = acc.S_total;
+    const double kF = 2.0 * M_PI / BoxSize;
+    const double kN = p.m * kF;
+    const double fact = std::pow(BoxSize / ((double)p.dims * (double)p.dims), 3.0);
+    double sumN = 0.0;
+    for (double v : mp->N3D) sumN += v;
+    const long long own = (p.dims % 2 == 1) ? 1 : 8;
+    const long long d3 = (long long)p.dims * p.dims * p.dims;
+    const long long expected = (d3 - own) / 2 + own;
+    PYL_CHECK((long long)sumN == expected, PYL_EMODES,
+              "WARNING: Not all modes counted (counted %lld, expected %lld)", (long long)sumN,
+              expected);
+    // 1D: drop the DC bin
+    for (int i = 1; i <= p.m; ++i) {
+        const double N = mp->N1D[i];
+        const double k1 = (mp->k1D[i] / N) * kF;
+        if (out->k1D) out->k1D[i - 1] = k1;
+        if (out->Nmodes1D) out->Nmodes1D[i - 1] = N;
+        const double kmaxper = std::sqrt(kN * kN - k1 * k1);
+        if (out->Pk1D)
+            for (int s = 0; s < S; ++s) {
+                double v = acc.P1[(size_t)i * S + s] * fact;
+                v = v * (M_PI * (kmaxper * kmaxper) / N) / ((2.0 * M_PI) * (2.0 * M_PI));
+                out->Pk1D[(size_t)(i - 1) * S + s] = v;
+            }
+    }
+    // 2D: keep every bin, bin centres
+    for (int kpar = 0; kpar <= p.kmax_par; ++kpar)
+        for (int kper = 0; kper <= p.kmax_per; ++kper) {
+            const long long idx = (long long)(p.kmax_par + 1) * kper + kpar;
+            if (out->kpar) out->kpar[idx] = 0.5 * (kpar + kpar + 1) * kF;
+            if (out->kper) out->kper[idx] = 0.5 * (kper + kper + 1) * kF;
+        }
+    for (long long i = 0; i < p.nb2; ++i) {
+        const double N = mp->N2D[i];
+        if (out->Nmodes2D) out->Nmodes2D[i] = N;
+        if (out->Pk2D)
+            for (int s = 0; s < S; ++s)
+                out->Pk2D[(size_t)i * S + s] = acc.P2[(size_t)i * S + s] * fact / N;
+    }
+    // 3D: drop the DC bin, multipoles times (2l+1)
+    for (int b = 1; b <= p.kmax; ++b) {
+        const double N = mp->N3D[b];
+        if (out->k3D) out->k3D[b - 1] = (mp->k3D[b] / N) * kF;
+        if (out->Nmodes3D) out->Nmodes3D[b - 1] = N;
+        if (out->Pk3D)
+            for (int s = 0; s < S; ++s) {
+                const size_t o = (size_t)(b - 1) * 3 * S;
+                out->Pk3D[o + 0 * S + s] = (acc.P3[((size_t)b * 3 + 0) * S + s] / N) * fact;
+                out->Pk3D[o + 1 * S + s] = (acc.P3[((size_t)b * 3 + 1) * S + s] * 5.0 / N) * fact;
+                out->Pk3D[o + 2 * S + s] = (acc.P3[((size_t)b * 3 + 2) * S + s] * 9.0 / N) * fact;
+            }
+        if (want_phase && out->Pkphase) out->Pkphase[b - 1] = (acc.phase[b] / N) * fact;
+    }
+    return PYL_OK;
+}
+
+}  // namespace
+
+int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch) {
+    FFTPlan *plan = nullptr;
+    PYL_TRY(get_fft_plan(c, dims, batch, &plan));
+    PYL_TRY(stage_begin(c, PYL_STAGE_PK_FFT));
+    PYL_TRY(run_fft(c, plan, in, out));
+    PYL_TRY(stage_end(c, PYL_STAGE_PK_FFT));
+    return PYL_OK;
+}
+
+// Binning of F device half-complex fields -> finalized output.
+int pk_bin_device(Ctx *c, const float2 *const *dk, int F, const int *mas, int dims,
+                  double BoxSize, int axis, int flags, pyl_pk_out *out) {
+    const PkDims p = pk_dims(dims);
+    const ModePlan *mp = get_mode_plan(c, dims, axis);
+    BinPlan *bp = nullptr;
+    PYL_TRY(get_bin_plan(c, dims, axis, &bp));
+    // MAS tables (one per field, host libm, uploaded)
+    std::vector<double> tabs((size_t)F * dims);
+    for (int f = 0; f < F; ++f) mas_table(dims, mas[f], &tabs[(size_t)f * dims]);
+    PYL_TRY(ensure(c, BUF_MAS_TAB, tabs.size() * sizeof(double)));
+    double *dtab = (double *)c->bufs[BUF_MAS_TAB].ptr;
+    PYL_HIP(hipMemcpyAsync(dtab, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    const int S = F + F * (F - 1) / 2;
+    HostAcc acc;
+    acc.S_total = S;
+    acc.P3.assign((size_t)(p.kmax + 1) * 3 * S, 0.0);
+    acc.phase.assign(p.kmax + 1, 0.0);
+    acc.P2.assign((size_t)p.nb2 * S, 0.0);
+    acc.P1.assign((size_t)(p.m + 1) * S, 0.0);
+    const bool phase = (F == 1) && !(flags & PYL_PK_NO_PHASE);
+    if (F == 1) {
+        const float2 *f0[1] = {dk[0]};
+        const double *c0[1] = {dtab};
+        int so[1] = {0};
+        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, c0, acc, so)));
+        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, c0, acc, so)));
+    } else {
+        // pairs i<j in lexicographic order; each launch yields auto i, auto j, cross ij
+        std::vector<bool> have(F, false);
+        int xi = 0;
+        for (int i = 0; i < F; ++i)
+            for (int j = i + 1; j < F; ++j, ++xi) {
+                const float2 *fp[2] = {dk[i], dk[j]};
+                const double *cp[2] = {dtab + (size_t)i * dims, dtab + (size_t)j * dims};
+                int so[3] = {have[i] ? -1 : i, have[j] ? -1 : j, F + xi};
+                have[i] = have[j] = true;
+                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, cp, acc, so)));
+            }
+    }
+    return finalize(mp, p, BoxSize, acc, phase, out);
+}
+
+}  // namespace pyl
+
+using namespace pyl;
+
+extern "C" {
+
+double pyl_mas_correction(double x, int p) {
+    return (x == 0.0) ? 1.0 : std::pow(x / std::sin(x), (double)p);
+}
+
+int pyl_pk_get_sizes(int dims, int nfields, pyl_pk_sizes *out) {
+    PYL_CHECK(out && dims >= 2 && nfields >= 1, PYL_EINVAL, "bad arguments");
+    PkDims p = pk_dims(dims);
+    out->n1d = p.m;
+    out->n2d = p.nb2;
+    out->n3d = p.kmax;
+    out->nspectra = nfields + nfields * (nfields - 1) / 2;
+    return PYL_OK;
+}
+
+int pyl_mode_counts(int dims, int axis, double *N1D, double *N2D, double *N3D, double *k1Dsum,
+                    double *k3Dsum) {
+    PYL_CHECK(dims >= 2 && axis >= 0 && axis <= 2, PYL_EINVAL, "bad dims/axis");
+    ModePlan mp;
+    compute_mode_plan(dims, axis, &mp);
+    if (N1D) std::copy(mp.N1D.begin(), mp.N1D.end(), N1D);
+    if (N2D) std::copy(mp.N2D.begin(), mp.N2D.end(), N2D);
+    if (N3D) std::copy(mp.N3D.begin(), mp.N3D.end(), N3D);
+    if (k1Dsum) std::copy(mp.k1D.begin(), mp.k1D.end(), k1Dsum);
+    if (k3Dsum) std::copy(mp.k3D.begin(), mp.k3D.end(), k3Dsum);
+    return PYL_OK;
+}
+
+int pyl_fft_r2c(pyl_ctx *ctx, const float *delta, float *delta_k, int dims, int flags) {
+    PYL_CHECK(ctx && delta && delta_k && dims >= 2, PYL_EINVAL, "bad arguments");
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipSetDevice(c->device));
+    reset_stages(c);
+    const long long n = (long long)dims * dims * dims;
+    const long long nk = (long long)dims * dims * (dims / 2 + 1);
+    PYL_TRY(ensure(c, BUF_FFT_IN, (size_t)n * sizeof(float)));
+    float *in = (float *)c->bufs[BUF_FFT_IN].ptr;
+    float2 *outp;
+    if (flags & PYL_PTR_DEVICE) {
+        if (flags & PYL_PK_DESTROY_INPUT) in = (float *)delta;
+        else PYL_HIP(hipMemcpyAsync(in, delta, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        outp = (float2 *)delta_k;
+    } else {
+        PYL_TRY(ensure(c, BUF_FFT_OUT, (size_t)nk * sizeof(float2)));
+        outp = (float2 *)c->bufs[BUF_FFT_OUT].ptr;
+        PYL_HIP(hipMemcpyAsync(in, delta, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    PYL_TRY(fft_device(c, in, outp, dims, 1));
+    if (!(flags & PYL_PTR_DEVICE))
+        PYL_HIP(hipMemcpyAsync(delta_k, outp, nk * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    return finish(c, flags);
+}
+
+int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas, int dims,
+           double BoxSize, int axis, float scale, float shift, int flags, pyl_pk_out *out) {
+    PYL_CHECK(ctx && deltas && mas && out, PYL_EINVAL, "NULL argument");
+    PYL_CHECK(nfields >= 1, PYL_EINVAL, "need at least one field");
+    PYL_CHECK(dims >= 2, PYL_EINVAL, "dims must be >= 2");
+    PYL_CHECK(axis >= 0 && axis <= 2, PYL_EINVAL, "axis must be 0, 1 or 2");
+    PYL_CHECK(BoxSize > 0.0, PYL_EINVAL, "BoxSize must be positive");
+    for (int f = 0; f < nfields; ++f) {
+        PYL_CHECK(deltas[f] != nullptr, PYL_EINVAL, "field %d is NULL", f);
+        PYL_CHECK(mas[f] >= 0 && mas[f] <= 4, PYL_EINVAL, "bad MAS id %d", mas[f]);
+    }
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipSetDevice(c->device));
+    reset_stages(c);
+    const long long n = (long long)dims * dims * dims;
+    const long long nk = (long long)dims * dims * (dims / 2 + 1);
+    const bool apply = !(scale == 1.0f && shift == 0.0f);
+    const bool dev = (flags & PYL_PTR_DEVICE) != 0;
+    const bool destroy = dev && (flags & PYL_PK_DESTROY_INPUT);
+    PYL_TRY(ensure(c, BUF_FFT_OUT, (size_t)nk * nfields * sizeof(float2)));
+    float2 *dk = (float2 *)c->bufs[BUF_FFT_OUT].ptr;
+    std::vector<const float2 *> dks(nfields);
+    const int pb = 256;
+    const int nb = (int)std::min<long long>((n + pb - 1) / pb, 8192);
+    if (destroy) {
+        for (int f = 0; f < nfields; ++f) {
+            float *x = (float *)deltas[f];
+            if (apply) {
+                PYL_TRY(stage_begin(c, PYL_STAGE_PK_PREP));
+                prologue_kernel<<<nb, pb, 0, c->stream>>>(x, x, n, scale, shift, 1);
+                PYL_HIP(hipGetLastError());
+                PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
+            }
+            PYL_TRY(fft_device(c, x, dk + (size_t)f * nk, dims, 1));
+            dks[f] = dk + (size_t)f * nk;
+        }
+    } else {
+        PYL_TRY(ensure(c, BUF_FFT_IN, (size_t)n * nfields * sizeof(float)));
+        float *in = (float *)c->bufs[BUF_FFT_IN].ptr;
+        PYL_TRY(stage_begin(c, PYL_STAGE_PK_PREP));
+        for (int f = 0; f < nfields; ++f) {
+            float *dst = in + (size_t)f * n;
+            if (dev) {
+                if (apply) {
+                    prologue_kernel<<<nb, pb, 0, c->stream>>>(deltas[f], dst, n, scale, shift, 1);
+                    PYL_HIP(hipGetLastError());
+                } else {
+                    PYL_HIP(hipMemcpyAsync(dst, deltas[f], n * sizeof(float), hipMemcpyDeviceToDevice,
+                                           c->stream));
+                }
+            } else {
+                PYL_HIP(hipMemcpyAsync(dst, deltas[f], n * sizeof(float), hipMemcpyHostToDevice,
+                                       c->stream));
+                if (apply) {
+                    prologue_kernel<<<nb, pb, 0, c->stream>>>(dst, dst, n, scale, shift, 1);
+                    PYL_HIP(hipGetLastError());
+                }
+            }
+        }
+        PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
+        PYL_TRY(fft_device(c, in, dk, dims, nfields));
+        for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nk;
+    }
+    PYL_TRY(pk_bin_device(c, dks.data(), nfields, mas, dims, BoxSize, axis, flags, out));
+    return finish(c, flags);
+}
+
+int pyl_pk_from_fourier(pyl_ctx *ctx, const float *const *delta_k, int nfields, const int *mas,
+                        int dims, double BoxSize, int axis, int flags, pyl_pk_out *out) {
+    PYL_CHECK(ctx && delta_k && mas && out, PYL_EINVAL, "NULL argument");
+    PYL_CHECK(nfields >= 1 && dims >= 2 && axis >= 0 && axis <= 2 && BoxSize > 0.0, PYL_EINVAL,
+              "bad arguments");
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipSetDevice(c->device));
+    reset_stages(c);
+    const long long nk = (long long)dims * dims * (dims / 2 + 1);
+    std::vector<const float2 *> dks(nfields);
+    if (flags & PYL_PTR_DEVICE) {
+        for (int f = 0; f < nfields; ++f) dks[f] = (const float2 *)delta_k[f];
+    } else {
+        PYL_TRY(ensure(c, BUF_FFT_OUT, (size_t)nk * nfields * sizeof(float2)));
+        float2 *dk = (float2 *)c->bufs[BUF_FFT_OUT].ptr;
+        for (int f = 0; f < nfields; ++f) {
+            PYL_HIP(hipMemcpyAsync(dk + (size_t)f * nk, delta_k[f], nk * sizeof(float2),
+                                   hipMemcpyHostToDevice, c->stream));
+            dks[f] = dk + (size_t)f * nk;
+        }
+    }
+    PYL_TRY(pk_bin_device(c, dks.data(), nfields, mas, dims, BoxSize, axis, flags, out));
+    return finish(c, flags);
+}
+
+int pyl_pos_redshift_space(pyl_ctx *ctx, float *pos, const float *vel, int64_t n, float BoxSize,
+                           float Hubble, float redshift, int axis, int flags) {
+    PYL_CHECK(ctx && pos && vel, PYL_EINVAL, "NULL argument");
+    PYL_CHECK(axis >= 0 && axis <= 2, PYL_EINVAL, "axis must be 0, 1 or 2");
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipSetDevice(c->device));
+    if (n <= 0) return PYL_OK;
+    const float factor = (float)((1.0 + (double)redshift) / (double)Hubble);
+    float *dpos = pos;
+    const float *dvel = vel;
+    if (!(flags & PYL_PTR_DEVICE)) {
+        PYL_TRY(ensure(c, BUF_POS, (size_t)n * 3 * sizeof(float)));
+        PYL_TRY(ensure(c, BUF_VEL, (size_t)n * 3 * sizeof(float)));
+        dpos = (float *)c->bufs[BUF_POS].ptr;
+        dvel = (const float *)c->bufs[BUF_VEL].ptr;
+        PYL_HIP(hipMemcpyAsync(dpos, pos, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        PYL_HIP(hipMemcpyAsync((void *)dvel, vel, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice,
+                               c->stream));
+    }
+    const int nb = (int)std::min<long long>((n + 255) / 256, 65536);
+    rsd_kernel<<<nb, 256, 0, c->stream>>>(dpos, dvel, n, BoxSize, factor, axis);
+    PYL_HIP(hipGetLastError());
+    if (!(flags & PYL_PTR_DEVICE))
+        PYL_HIP(hipMemcpyAsync(pos, dpos, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return finish(c, flags);
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// Internal declarations shared by the libpylians_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/pylians_hip.h"
+
+namespace pyl {
+
+// ---- error handling: thread-local message, int status codes --------------
+void set_error(const char *fmt, ...);
+
+#define PYL_HIP(expr)                                                              \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            ::pyl::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,            \
+                             hipGetErrorString(_e));                               \
+            return (_e == hipErrorOutOfMemory) ? PYL_ENOMEM : PYL_EHIP;            \
+        }                                                                          \
+    } while (0)
+
+#define PYL_CHECK(cond, code, ...)                                                 \
+    do {                                                                           \
+        if (!(cond)) {                                                             \
+            ::pyl::set_error(__VA_ARGS__);                                         \
+            return (code);                                                         \
+        }                                                                          \
+    } while (0)
+
+#define PYL_TRY(expr)                                                              \
+    do {                                                                           \
+        int _s = (expr);                                                           \
+        if (_s != PYL_OK) return _s;                                               \
+    } while (0)
+
+// ---- grow-only device buffer owned by a context ---------------------------
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+};
+
+// ---- resident plans --------------------------------------------------------
+struct FFTPlan {
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    void *work = nullptr;
+    size_t work_bytes = 0;
+    int dims = 0;
+    int batch = 0;
+};
+
+// Exact mode counts of the reference loop for one (dims, axis), DC bins kept.
+struct ModePlan {
+    int dims = 0, axis = 0;
+    std::vector<double> N1D, N2D, N3D, k1D, k3D;
+};
+
+// Rows of the half-complex cube grouped by binning key and cut into work items
+// (device-resident).
+struct BinPlan {
+    int dims = 0, axis = 0, R = 0;
+    int nitems = 0, nkeys = 0;
+    int *d_rows = nullptr;       // kxx*dims + kyy, grouped by key
+    int4 *d_items = nullptr;     // (first row, nrows, key, 0)
+    int *d_key_items = nullptr;  // [nkeys+1] item range of each key
+};
+
+enum BufId {
+    BUF_POS = 0,     // staged positions (host-pointer path)
+    BUF_W,           // staged weights
+    BUF_GRID,        // staged grid
+    BUF_TILE_CNT,    // per-tile counters / cursors / offsets / weight bounds
+    BUF_RECORDS,     // tile-sorted particle records
+    BUF_SCAN_TMP,
+    BUF_FFT_IN,      // real FFT input (copy of delta)
+    BUF_FFT_OUT,     // half-complex FFT output
+    BUF_BINS,        // per-bin accumulators
+    BUF_MAS_TAB,     // MAS correction tables
+    BUF_VEL,
+    BUF_COUNT_
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;
+    DevBuf bufs[BUF_COUNT_];
+    std::map<std::pair<int, int>, FFTPlan> fft_plans;      // (dims, batch)
+    std::map<std::pair<int, int>, ModePlan> mode_plans;    // (dims, axis)
+    std::map<std::pair<int, int>, BinPlan> bin_plans;      // (dims, axis)
+    bool timing = false;
+    hipEvent_t ev[PYL_STAGE_COUNT][2] = {};
+    bool ev_used[PYL_STAGE_COUNT] = {};
+    bool ev_ready = false;
+};
+
+size_t held_bytes(const Ctx *c);
+int ensure(Ctx *c, BufId id, size_t bytes);
+void reset_stages(Ctx *c);
+int stage_begin(Ctx *c, int stage);
+int stage_end(Ctx *c, int stage);
+int finish(Ctx *c, int flags);   // synchronise unless PYL_NO_SYNC on device pointers
+
+// FFT plan (resident, one per (dims, batch))
+int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out);
+int run_fft(Ctx *c, FFTPlan *p, void *in, void *out);
+
+// Mode-count plan (host computed, cached)
+const ModePlan *get_mode_plan(Ctx *c, int dims, int axis);
+void compute_mode_plan(int dims, int axis, ModePlan *mp);
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace pyl
+
+struct pyl_ctx {
+    pyl::Ctx c;
+};

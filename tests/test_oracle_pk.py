@@ -1,0 +1,144 @@
+"""Pins the power-spectrum oracle (oracle/pk_oracle.c + oracle/oracle.py).
+
+Mode counts against the closed form of check_number_modes (Pk_library.pyx:87-99)
+and a brute-force Hermitian enumeration of all dims^3 modes; the FFT against a
+direct DFT, plane waves and Parseval; bin values against single plane waves
+with closed-form amplitudes; the Legendre weights with modes on the line of
+sight; the MAS window against x/sin(x) powers; XPk against auto-spectrum
+identities; the float32 path against the float64 oracle.
+"""
+import numpy as np
+import pytest
+
+
+@pytest.mark.parametrize('dims', [4, 5, 6, 8, 9, 16])
+@pytest.mark.parametrize('axis', [0, 1, 2])
+def test_mode_counts_brute_force(oracle, dims, axis):
+    x = np.random.default_rng(dims).standard_normal((dims,) * 3).astype(np.float32)
+    raw = oracle.pk(x, 50.0, axis=axis).raw
+    N1, N2, N3 = oracle.brute_mode_counts(dims, axis)
+    np.testing.assert_array_equal(raw['N3D'], N3)
+    np.testing.assert_array_equal(raw['N2D'], N2)
+    np.testing.assert_array_equal(raw['N1D'], N1)
+    assert raw['N3D'].sum() == oracle.expected_modes(dims)
+    # k1D sums |k_par| over 1D modes
+    np.testing.assert_array_equal(raw['k1D'], np.arange(dims // 2 + 1) * N1)
+
+
+def test_fft_direct_dft_and_parseval(oracle):
+    rng = np.random.default_rng(1)
+    for d in (4, 5, 6):
+        x = rng.standard_normal((d, d, d)).astype(np.float32)
+        ref = oracle.direct_dft_r2c(x.astype(np.float64))
+        got = oracle.fft_f32(x)
+        assert got.dtype == np.complex64
+        np.testing.assert_allclose(got, ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+    x = rng.standard_normal((16, 16, 16))
+    X = np.fft.fftn(x)
+    assert abs(np.sum(np.abs(X) ** 2) / 16 ** 3 - np.sum(x ** 2)) < 1e-9 * np.sum(x ** 2)
+
+
+def plane_wave(d, n, A=1.0):
+    i = np.arange(d)
+    X, Y, Z = np.meshgrid(i, i, i, indexing='ij')
+    return (A * np.cos(2 * np.pi * (n[0] * X + n[1] * Y + n[2] * Z) / d)).astype(np.float64)
+
+
+@pytest.mark.parametrize('n,axis', [((3, 0, 0), 0), ((0, 2, 5), 2), ((1, 2, 2), 1),
+                                    ((0, 0, 4), 2), ((2, -3, 1), 0)])
+def test_plane_wave_closed_form(oracle, n, axis):
+    d, L, A = 16, 200.0, 0.7
+    x = plane_wave(d, n, A)
+    r = oracle.pk(x.astype(np.float32), L, axis=axis, MAS='None', f64=True,
+                  dk=oracle.fft_f64(x))
+    k2 = n[0] ** 2 + n[1] ** 2 + n[2] ** 2
+    kidx = int(np.floor(np.sqrt(k2)))
+    fact = (L / d ** 2) ** 3
+    N3 = r.Nmodes3D[kidx - 1]
+    P0 = (A * d ** 3 / 2) ** 2 * fact / N3
+    expect = np.zeros_like(r.Pk[:, 0])
+    expect[kidx - 1] = P0
+    np.testing.assert_allclose(r.Pk[:, 0], expect, rtol=1e-10, atol=1e-10 * P0)
+    mu2 = n[axis] ** 2 / k2
+    np.testing.assert_allclose(r.Pk[kidx - 1, 1], 5 * P0 * (3 * mu2 - 1) / 2, rtol=1e-10, atol=1e-10 * P0)
+    np.testing.assert_allclose(r.Pk[kidx - 1, 2], 9 * P0 * (35 * mu2 ** 2 - 30 * mu2 + 3) / 8,
+                               rtol=1e-10, atol=1e-10 * P0)
+    np.testing.assert_allclose(r.k3D[kidx - 1], r.raw['k3D'][kidx] / N3 * 2 * np.pi / L, rtol=1e-14)
+    # 2D: the single mode lands in its (k_par, k_per) bin, divided by that bin's count
+    kmax_par = d // 2
+    kpar = abs(n[axis])
+    kper = int(np.floor(np.sqrt(k2 - n[axis] ** 2)))
+    i2 = (kmax_par + 1) * kper + kpar
+    P2 = np.zeros_like(r.Pk2D)
+    P2[i2] = (A * d ** 3 / 2) ** 2 * fact / r.Nmodes2D[i2]
+    np.testing.assert_allclose(r.Pk2D, P2, rtol=1e-10, atol=1e-10 * P2.max())
+    # 1D (k <= middle): Pk1D = |dk|^2 fact * (pi kmaxper^2 / N) / (2 pi)^2
+    if k2 <= (d // 2) ** 2:
+        kF = 2 * np.pi / L
+        kN = d // 2 * kF
+        N1 = r.Nmodes1D[kpar - 1] if kpar > 0 else None
+        if N1 is not None:
+            kmaxper = np.sqrt(kN ** 2 - (kpar * kF) ** 2)
+            v = (A * d ** 3 / 2) ** 2 * fact * (np.pi * kmaxper ** 2 / N1) / (2 * np.pi) ** 2
+            np.testing.assert_allclose(r.Pk1D[kpar - 1], v, rtol=1e-10)
+
+
+def test_line_of_sight_mode_legendre(oracle):
+    # mu^2 = 1: Pk[:,1] = 5 Pk[:,0], Pk[:,2] = 9 Pk[:,0] in that bin
+    for axis in (0, 1, 2):
+        n = [0, 0, 0]
+        n[axis] = 3
+        x = plane_wave(16, n)
+        r = oracle.pk(x.astype(np.float32), 100.0, axis=axis, MAS='None', f64=True,
+                      dk=oracle.fft_f64(x))
+        np.testing.assert_allclose(r.Pk[2, 1], 5 * r.Pk[2, 0], rtol=1e-12)
+        np.testing.assert_allclose(r.Pk[2, 2], 9 * r.Pk[2, 0], rtol=1e-12)
+
+
+def test_mas_correction_closed_form(oracle):
+    for x in (0.1, 0.5, 1.2, -0.7, np.pi / 2):
+        for p in (0, 1, 2, 3, 4):
+            assert oracle.mas_correction(x, p) == pytest.approx((x / np.sin(x)) ** p, rel=1e-15)
+    assert oracle.mas_correction(0.0, 3) == 1.0
+
+
+def test_mas_deconvolution_single_mode(oracle):
+    # Pk(MAS=m) / Pk(MAS='None') for one mode equals the squared float MAS factor
+    d, n = 16, (2, 3, 1)
+    x = plane_wave(d, n).astype(np.float32)
+    dk = oracle.fft_f32(x)
+    a = oracle.pk(x, 100.0, axis=2, MAS='None', dk=dk)
+    for mas, p in (('CIC', 2), ('TSC', 3), ('PCS', 4)):
+        b = oracle.pk(x, 100.0, axis=2, MAS=mas, dk=dk)
+        f = np.float32(np.prod([oracle.mas_correction(np.pi / d * k, p) for k in n]))
+        k = int(np.sqrt(sum(v * v for v in n)))
+        np.testing.assert_allclose(b.Pk[k - 1, 0] / a.Pk[k - 1, 0], float(f) ** 2, rtol=2e-7)
+
+
+def test_xpk_identities(oracle):
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((16, 16, 16)).astype(np.float32)
+    for axis in (0, 2):
+        r = oracle.xpk([x, x], 300.0, axis=axis, MAS=['CIC', 'CIC'])
+        np.testing.assert_allclose(r.XPk[:, :, 0], r.Pk[:, :, 0], rtol=1e-12)
+        np.testing.assert_allclose(r.PkX2D[:, 0], r.Pk2D[:, 0], rtol=1e-12)
+        np.testing.assert_allclose(r.PkX1D[:, 0], r.Pk1D[:, 0], rtol=1e-12)
+        r2 = oracle.xpk([x, -x], 300.0, axis=axis, MAS=['CIC', 'CIC'])
+        np.testing.assert_allclose(r2.XPk[:, :, 0], -r2.Pk[:, :, 0], rtol=1e-12)
+        r3 = oracle.xpk([x, np.float32(2.0) * x], 300.0, axis=axis, MAS=['CIC', 'CIC'])
+        np.testing.assert_allclose(r3.XPk[:, :, 0], 2 * r3.Pk[:, :, 0], rtol=1e-6)
+        single = oracle.pk(x, 300.0, axis=axis, MAS='CIC')
+        np.testing.assert_allclose(r.Pk[:, :, 0], single.Pk, rtol=1e-10)
+
+
+def test_f32_path_close_to_fp64_oracle(oracle):
+    rng = np.random.default_rng(6)
+    pos = (rng.random((40000, 3)) * 500).astype(np.float32)
+    g = np.zeros((32,) * 3, np.float32)
+    oracle.ma_f32(pos, g, 500.0, 'CIC')
+    delta = g / np.float32(g.mean()) - np.float32(1.0)
+    a = oracle.pk(delta, 500.0, axis=0, MAS='CIC')
+    g64 = oracle.ma_f64(pos, 32, 500.0, 'CIC')
+    b = oracle.pk(g64 / g64.mean() - 1.0, 500.0, axis=0, MAS='CIC', f64=True)
+    np.testing.assert_allclose(a.Pk[:, 0], b.Pk[:, 0], rtol=1e-4)
+    np.testing.assert_array_equal(a.Nmodes3D, b.Nmodes3D)
