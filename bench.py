@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Benchmark of the density-field -> power-spectrum hot path on one MI355X.
+
+Workload (clustered field, 512^3): a Gaussian random field with
+P(k) = A k^-1.5 exp(-(k/0.5)^2) (sigma_8 = 0.8) in a 1000 Mpc/h box, a 512^3
+particle lattice moved by its Zel'dovich displacement plus a redshift-space
+shift along axis 0 (f = 0.5); 134,217,728 float32 positions resident in HBM.
+
+One step = the reference caller's chain (Pk_snapshot.py:77-89):
+    grid = MA(pos, TSC)            (grid written, not accumulated: it starts at zero)
+    delta = grid / mean - 1        (fused into the FFT input pass)
+    Pk(delta, axis=0, MAS='TSC')   (rocFFT r2c + fused binning + finalisation)
+value = particles / step time (Mparticles/s).  The plain GPU form (per-particle
+global atomics) and a CPU baseline are timed in the same run.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print('[bench] ' + msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--grid', type=int, default=512, help='grid and lattice size (512 = config 2)')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-plain', action='store_true', help='skip the plain GPU (atomic) leg')
+    ap.add_argument('--cpu-seconds', type=float, default=20.0, help='target CPU sample time')
+    return ap.parse_args()
+
+
+class Pipeline:
+    """Device-resident MA + Pk step through the C ABI."""
+
+    def __init__(self, N, ctx, pos, dims, box, mas, axis, atomic=False):
+        import torch
+        self.N, self.ctx = N, ctx
+        self.pos, self.dims, self.box, self.mas, self.axis = pos, dims, box, mas, axis
+        self.np = pos.shape[0]
+        self.grid = torch.empty((dims,) * 3, dtype=torch.float32, device=pos.device)
+        self.mean = float(self.np) / dims ** 3
+        self.out, self.arrs, _ = N.alloc_pk_out(dims, 1)
+        self.flags_ma = N.PTR_DEVICE | N.MA_OVERWRITE | N.NO_SYNC | (N.MA_ATOMIC if atomic else 0)
+        self.gp = ctypes.c_void_p(self.grid.data_ptr())
+        self.ptrs = (ctypes.c_void_p * 1)(self.grid.data_ptr())
+        self.masid = (ctypes.c_int * 1)(mas)
+        self.stage_sum = {}
+        self.nsteps = 0
+
+    def step(self):
+        N, L = self.N, self.N.lib()
+        N.check(L.pyl_ma(self.ctx.handle, ctypes.c_void_p(self.pos.data_ptr()), self.np, 3, None,
+                         self.gp, self.dims, self.box, self.mas, self.flags_ma))
+        t_ma = self.ctx.stage_times()
+        N.check(L.pyl_pk(self.ctx.handle, self.ptrs, 1, self.masid, self.dims, self.box, self.axis,
+                         self.mean, 1.0, N.PTR_DEVICE | N.PK_DESTROY_INPUT, ctypes.byref(self.out)))
+        t_pk = self.ctx.stage_times()
+        for k, v in list(t_ma.items()) + list(t_pk.items()):
+            if v > 0:
+                self.stage_sum[k] = self.stage_sum.get(k, 0.0) + v
+        self.nsteps += 1
+
+    def reset(self):
+        self.stage_sum, self.nsteps = {}, 0
+
+    def stage_avg(self):
+        return {k: v / max(self.nsteps, 1) for k, v in self.stage_sum.items()}
+
+
+def time_steps(pipe, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        pipe.step()
+    torch.cuda.synchronize()
+    pipe.reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def cpu_baseline(pos_host, grid_host, dims, box, mas_name, axis, target_s):
+    """The CPU path on this host's cores: the reference's MAS_c.c (OpenMP build from
+    oracle/_ref) on a particle sample scaled to the full count, then the FFT and
+    the reference binning loop (oracle restatement, serial as in Pk_library.pyx)."""
+    import numpy as np
+    from oracle import oracle as O
+    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get('OMP_NUM_THREADS', '16') or 16))
+    n_total = pos_host.shape[0]
+    ns = min(n_total, 1 << 22)
+    parts = {}
+    g = np.zeros((dims,) * 3, np.float32)
+    if O.ref_available('omp'):
+        kind_ma = 'reference MAS_c.c (OpenMP build, %d threads)' % cores
+        run = lambda p, gg: O.ma_reference(p, gg, box, mas_name, None, flavour='omp', threads=cores)
+    else:
+        kind_ma = 'oracle restatement (serial)'
+        run = lambda p, gg: O.ma_f32(p, gg, box, mas_name)
+    # grow the sample until it takes ~1/3 of the budget
+    while True:
+        t0 = time.perf_counter()
+        run(pos_host[:ns], g)
+        dt = time.perf_counter() - t0
+        if dt > target_s / 3 or ns == n_total:
+            break
+        ns = min(n_total, ns * 4)
+    t_ma = dt * n_total / ns
+    parts['ma_s_scaled'] = t_ma
+    delta = grid_host / np.float32(grid_host.mean()) - np.float32(1.0)
+    t0 = time.perf_counter()
+    dk = O.fft_f32(delta, workers=cores)
+    t_fft = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.raw_bins([dk], dims, axis, [mas_name])
+    t_bin = time.perf_counter() - t0
+    total = t_ma + t_fft + t_bin
+    return {
+        'value': round(n_total / total / 1e6, 3),
+        'unit': 'Mparticles/s',
+        'cores': cores,
+        'kind': 'port' if 'oracle' in kind_ma else 'reference',
+        'sample': ('MA: %s on %d of %d particles (%.2f s), scaled linearly; FFT: scipy.fft.rfftn '
+                   'float32, %d workers, full %d^3 (%.2f s); binning: reference mode loop restated '
+                   'in C, serial, full grid (%.2f s)' % (kind_ma, ns, n_total, dt, cores, dims, t_fft,
+                                                         t_bin)),
+        'step_s': round(total, 3),
+    }
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get('WORLD_SIZE', '1'))
+    if args.gpus != 1 or world != 1:
+        print('bench.py: this build runs on one GPU only; no multi-GPU path exists '
+              '(requested --gpus %d, WORLD_SIZE %d)' % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from pylians3_amd import _native as N, workloads
+
+    dims, box, axis = args.grid, 1000.0, 0
+    mas = N.TSC
+    log('generating %d^3 Zel\'dovich + RSD particles on the GPU' % dims)
+    pos = workloads.zeldovich(dims, box, 2, f_rsd=0.5, rsd_axis=0)
+    torch.cuda.synchronize()
+    n = pos.shape[0]
+    ctx = N.Context(0)
+    ctx.enable_timing(True)
+
+    log('tiled MA + Pk: warmup %d, steps %d' % (args.warmup, args.steps))
+    pipe = Pipeline(N, ctx, pos, dims, box, mas, axis)
+    ms = time_steps(pipe, args.steps, args.warmup)
+    st = pipe.stage_avg()
+    log('step %.3f ms  stages %s' % (ms, {k: round(v, 3) for k, v in st.items()}))
+    grid_host = None
+    if not args.no_cpu:
+        # one more MA to keep the density grid for the CPU leg
+        N.check(N.lib().pyl_ma(ctx.handle, ctypes.c_void_p(pos.data_ptr()), n, 3, None,
+                               ctypes.c_void_p(pipe.grid.data_ptr()), dims, box, mas,
+                               N.PTR_DEVICE | N.MA_OVERWRITE))
+        grid_host = pipe.grid.cpu().numpy()
+
+    plain = None
+    if not args.no_plain:
+        log('plain GPU form (atomic MA) + Pk')
+        pp = Pipeline(N, ctx, pos, dims, box, mas, axis, atomic=True)
+        ms_p = time_steps(pp, args.steps, max(1, args.warmup // 2))
+        sp = pp.stage_avg()
+        plain = {'ms_per_step': round(ms_p, 4), 'mparticles_per_s': round(n / ms_p / 1e3, 2),
+                 'ma_ms': round(sp.get('ma_atomic', 0.0), 4)}
+        del pp
+
+    ma_ms = sum(st.get(k, 0.0) for k in ('ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit'))
+    pk_ms = sum(st.get(k, 0.0) for k in ('pk_prep', 'pk_fft', 'pk_bin'))
+    nk = dims * dims * (dims // 2 + 1)
+    B_ma = 12.0 * n + 4.0 * dims ** 3            # positions once + grid written once
+    B_pk = 4.0 * dims ** 3 + 16.0 * nk           # real grid, half-complex written + read once
+    B_bin = 8.0 * nk                             # binning pass: half-complex read once
+    stage_bytes = {'ma': (B_ma, ma_ms), 'pk': (B_pk, pk_ms), 'pk_bin': (B_bin, st.get('pk_bin', 0.0))}
+    stages = {k: {'ms': round(t, 4), 'algorithmic_GB': round(b / 1e9, 4),
+                  'GBps': round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None}
+              for k, (b, t) in stage_bytes.items()}
+    kernels = {k: round(v, 4) for k, v in st.items()}
+    # dominant kernel: the longest single stage
+    dom = max(st.items(), key=lambda kv: kv[1])[0]
+    dom_bytes = {'ma_deposit': None, 'pk_bin': B_bin}.get(dom)
+    roofline = None
+    if dom_bytes is None:
+        # account the whole MA stage (its kernels share the MA algorithmic bytes)
+        ach = B_ma / (ma_ms * 1e-3) / 1e9 if dom.startswith('ma') else B_pk / (pk_ms * 1e-3) / 1e9
+        roofline = {'bound': 'hbm', 'kernel': dom, 'scope': 'ma stage' if dom.startswith('ma')
+                    else 'pk stage', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
+                    'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None}
+    else:
+        ach = dom_bytes / (st[dom] * 1e-3) / 1e9
+        roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
+                    'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None}
+
+    cpu = None
+    if not args.no_cpu:
+        log('CPU baseline leg')
+        pos_host = pos.cpu().numpy()
+        cpu = cpu_baseline(pos_host, grid_host, dims, box, 'TSC', axis, args.cpu_seconds)
+
+    value = n / (ms * 1e-3) / 1e6
+    line = {
+        'metric': 'Mparticles/s (MA) + Pk ms at 512^3 TSC vs plain GPU; p50/p99 request latency',
+        'value': round(value, 2),
+        'unit': 'Mparticles/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 grid/FFT, f64 bin sums',
+        'data': 'synthetic: Zel\'dovich + RSD (f=0.5, axis 0) lattice from a seeded Gaussian field',
+        'config': {'workload': 'clustered field: %d^3 particles, TSC onto %d^3, Pk axis 0 '
+                               '(MA + delta + Pk per step)' % (dims, dims),
+                   'particles': n, 'grid': dims, 'box_mpc_h': box, 'mas': 'TSC', 'axis': axis},
+        'ma_ms': round(ma_ms, 4),
+        'pk_ms': round(pk_ms, 4),
+        'ma_mparticles_per_s': round(n / ma_ms / 1e3, 2) if ma_ms > 0 else None,
+        'kernels_ms': kernels,
+        'stages': stages,
+        'plain_gpu': plain,
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -102,6 +102,14 @@ int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */);
 int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
            float *grid, int dims, float BoxSize, int mas, int flags);
 
+/* Batched MA: nreal independent grids in one launch sequence.  Particles of grid r are
+ * pos[offsets[r] .. offsets[r+1]) (host array, nreal+1 entries, offsets[0] = 0), W is
+ * concatenated the same way (or NULL), grids holds nreal contiguous grids.  Same
+ * semantics as nreal calls of pyl_ma (each grid's result is bitwise the same as alone).
+ * Replaces a Python loop of MA calls (MAS_library.pyx:57-112) over realisations. */
+int pyl_ma_batch(pyl_ctx *ctx, const float *pos, const int64_t *offsets, int nreal, int axes,
+                 const float *W, float *grids, int dims, float BoxSize, int mas, int flags);
+
 /* Reference C kernels, same names and signatures as MAS_c.h:3-10 (host pointers,
  * threads ignored, += semantics, native 2D as in MAS_c.c:28-34).  They run on
  * a process-wide context on device 0.  On error they print pyl_last_error() to
@@ -153,6 +161,17 @@ typedef struct {
 int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas,
            int dims, double BoxSize, int axis, float scale, float shift, int flags,
            pyl_pk_out *out);
+
+/* Batched Pk/XPk: nreal realisations of nfields fields each, stored as nreal*nfields
+ * contiguous (dims,dims,dims) float32 cubes (realisation-major).  One batched rocFFT
+ * and one binning launch for the whole batch.  scales/shifts: nreal*nfields values
+ * for the prologue (NULL = none).  outs: nreal result structs (each sized as for
+ * pyl_pk with nfields fields).  Each realisation's result is bitwise the same as a
+ * single pyl_pk call on it.  Replaces a Python loop of Pk/XPk calls over
+ * realisations (Pk_library.pyx:263-420, :529-793). */
+int pyl_pk_batch(pyl_ctx *ctx, const float *deltas, int nreal, int nfields, const int *mas,
+                 int dims, double BoxSize, int axis, const float *scales, const float *shifts,
+                 int flags, pyl_pk_out *outs);
 
 /* Same binning pass on fields that are already in Fourier space:
  * delta_k[f] is (dims,dims,dims/2+1) complex64 (interleaved). */

@@ -52,7 +52,6 @@ def _check_f32(a, ndim, name):
 
 def _deposit(pos, number, BoxSize, mas, W, flags, require_contig=False, ctx=None):
     """Run pyl_ma on host or device arrays.  number is updated in place."""
-    ctx = ctx or N.default_context()
     axes = pos.shape[1]
     dims = number.shape[0]
     n = pos.shape[0]
@@ -61,6 +60,7 @@ def _deposit(pos, number, BoxSize, mas, W, flags, require_contig=False, ctx=None
         if not (N.is_device_array(pos) and N.is_device_array(number)
                 and (W is None or N.is_device_array(W))):
             raise ValueError('pos, number and W must all be on the GPU or all on the host')
+        ctx = ctx or N.default_context()
         with N.torch_stream(ctx, True):
             N.check(N.lib().pyl_ma(ctx.handle, N.device_ptr(pos), n, axes,
                                    None if W is None else N.device_ptr(W), N.device_ptr(number),
@@ -70,6 +70,7 @@ def _deposit(pos, number, BoxSize, mas, W, flags, require_contig=False, ctx=None
         for a, nm in ((pos, 'pos'), (number, 'number'), (W, 'W')):
             if a is not None and not a.flags['C_CONTIGUOUS']:
                 raise ValueError('ndarray is not C-contiguous')
+    ctx = ctx or N.default_context()
     p = np.ascontiguousarray(pos)
     w = None if W is None else np.ascontiguousarray(W)
     g = number if number.flags['C_CONTIGUOUS'] else np.ascontiguousarray(number)

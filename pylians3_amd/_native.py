@@ -69,6 +69,9 @@ EXPORTS = {
     'pyl_ma': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
                               ctypes.c_int, ctypes.c_int]),
+    'pyl_ma_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_float, ctypes.c_int, ctypes.c_int]),
     'NGP': (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
                    ctypes.c_int, ctypes.c_float, ctypes.c_int]),
     'CIC': (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
@@ -84,6 +87,10 @@ EXPORTS = {
                               ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                               ctypes.POINTER(PkOut)]),
+    'pyl_pk_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.POINTER(PkOut)]),
     'pyl_pk_from_fourier': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                            ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                            ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -96,6 +103,18 @@ EXPORTS = {
 }
 
 
+def _share_hip_runtime():
+    """PyTorch wheels bundle their own HIP runtime with the same soname
+    (libamdhip64.so.7) but load it under another name; if this library were
+    loaded first, a later ``import torch`` would bring a second HIP runtime
+    into the process and lose the GPU.  Importing torch first makes the
+    library bind to the runtime already in the process."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libpylians_hip.so (raises if it has not been built)."""
     global _LIB
@@ -105,6 +124,7 @@ def lib():
                 if not os.path.exists(LIB_PATH):
                     raise ImportError('libpylians_hip.so is not built (%s); run '
                                       '"python -m pylians3_amd.build"' % LIB_PATH)
+                _share_hip_runtime()
                 L = ctypes.CDLL(LIB_PATH)
                 for name, (res, args) in EXPORTS.items():
                     f = getattr(L, name)

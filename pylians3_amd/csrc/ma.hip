@@ -95,14 +95,29 @@ __device__ __forceinline__ int axis_foot(float x, float inv, int dims, float *C)
     }
 }
 
-// Geometry of the tiled deposit.  2D grids are 3D grids with one z plane.
+// Geometry of the tiled deposit.  2D grids are 3D grids with one z plane.  A
+// batch holds nreal grids of the same size; tile keys are r*ntiles + tile.
 struct TileGeom {
     int dims;          // nodes per axis (x, y and, for 3D, z)
     int dz;            // nodes along z: dims (3D) or 1 (2D)
     int T[3];          // tile extent per axis
     int nt[3];         // tiles per axis
-    long long ntiles;
+    long long ntiles;  // tiles per grid
+    long long ncell;   // nodes per grid
+    int nreal;         // grids in the batch
 };
+
+// Realisation owning particle i: offs[r] <= i < offs[r+1] (offs has nreal+1 entries).
+__device__ __forceinline__ int realisation_of(const long long *__restrict__ offs, int nreal,
+                                              long long i) {
+    int lo = 0, hi = nreal;  // invariant: offs[lo] <= i < offs[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (offs[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // Distinct tiles touched along one axis by the F consecutive nodes starting at
 // base (mod d).  Returns their count (1..3) in order.
@@ -148,6 +163,26 @@ __device__ __forceinline__ unsigned long long agg_inc(unsigned *counters, long l
     return slot;
 }
 
+// Count pass variant for weighted input: also raises wmax[key] to the largest
+// finite |W| bits of the lanes sharing the key (one atomicMax per group).
+__device__ __forceinline__ void agg_inc_max(unsigned *counters, unsigned *wmax, long long key,
+                                            bool active, unsigned wbits) {
+    const int lane = __lane_id();
+    unsigned long long pend = __ballot(active);
+    while (pend) {
+        int leader = __ffsll((unsigned long long)pend) - 1;
+        long long lk = __shfl(key, leader);
+        unsigned long long grp = __ballot(active && key == lk) & pend;
+        unsigned v = ((grp >> lane) & 1ull) ? wbits : 0u;
+        for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off));
+        if (lane == leader) {
+            atomicAdd(&counters[lk], (unsigned)__popcll(grp));
+            atomicMax(&wmax[lk], v);
+        }
+        pend &= ~grp;
+    }
+}
+
 template <int MAS>
 __device__ __forceinline__ void particle_tiles(const float *pos, long long i, int axes,
                                                float inv, const TileGeom &g, int tx[3],
@@ -168,21 +203,29 @@ __device__ __forceinline__ void particle_tiles(const float *pos, long long i, in
     }
 }
 
-// Pass 1: per-tile record counts and the global max |W| (as float bits).
+// Pass 1: per-tile record counts and, with weights, the per-tile max finite |W|
+// (as float bits; non-finite weights only poison the nodes they touch, as in the
+// reference).  Per-tile bounds make a tile's quantum depend on its own records
+// only, so a grid's result does not depend on what else is in the batch.
 template <int MAS>
 __global__ void __launch_bounds__(kBlock)
 tile_count_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
-                  int axes, float inv, TileGeom g, unsigned *__restrict__ counts,
-                  unsigned *__restrict__ wmax_bits) {
+                  int axes, float inv, TileGeom g, const long long *__restrict__ offs,
+                  unsigned *__restrict__ counts, unsigned *__restrict__ wmax_bits) {
     const long long stride = (long long)gridDim.x * blockDim.x;
-    unsigned wloc = 0;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
         long long i = base + threadIdx.x;
         bool valid = i < n;
         int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
+        long long rbase = 0;
+        unsigned wb = 0;
         if (valid) {
             particle_tiles<MAS>(pos, i, axes, inv, g, tx, ty, tz, nx, ny, nz);
-            if (W) wloc = max(wloc, __float_as_uint(fabsf(W[i])));
+            if (g.nreal > 1) rbase = (long long)realisation_of(offs, g.nreal, i) * g.ntiles;
+            if (W) {
+                wb = __float_as_uint(fabsf(W[i]));
+                if (wb >= 0x7f800000u) wb = 0;
+            }
         }
         int nc = valid ? nx * ny * nz : 0;
         for (int j = 0;; ++j) {
@@ -191,14 +234,11 @@ tile_count_kernel(const float *__restrict__ pos, const float *__restrict__ W, lo
             long long key = 0;
             if (act) {
                 int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
-                key = ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
+                key = rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
             }
-            agg_inc(counts, key, act, false);
+            if (W) agg_inc_max(counts, wmax_bits, key, act, wb);
+            else agg_inc(counts, key, act, false);
         }
-    }
-    if (W) {
-        for (int off = 32; off > 0; off >>= 1) wloc = max(wloc, (unsigned)__shfl_xor((int)wloc, off));
-        if (__lane_id() == 0) atomicMax(wmax_bits, wloc);
     }
 }
 
@@ -206,16 +246,19 @@ tile_count_kernel(const float *__restrict__ pos, const float *__restrict__ W, lo
 template <int MAS>
 __global__ void __launch_bounds__(kBlock)
 tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
-                    int axes, float inv, TileGeom g, const unsigned long long *__restrict__ offsets,
+                    int axes, float inv, TileGeom g, const long long *__restrict__ offs,
+                    const unsigned long long *__restrict__ offsets,
                     unsigned *__restrict__ cursors, float4 *__restrict__ records) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
         long long i = base + threadIdx.x;
         bool valid = i < n;
         int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
+        long long rbase = 0;
         float4 rec = make_float4(0.f, 0.f, 0.f, 1.f);
         if (valid) {
             particle_tiles<MAS>(pos, i, axes, inv, g, tx, ty, tz, nx, ny, nz);
+            if (g.nreal > 1) rbase = (long long)realisation_of(offs, g.nreal, i) * g.ntiles;
             rec.x = pos[(long long)axes * i + 0];
             rec.y = pos[(long long)axes * i + 1];
             rec.z = (axes == 3) ? pos[(long long)axes * i + 2] : 0.f;
@@ -228,7 +271,7 @@ tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, 
             long long key = 0;
             if (act) {
                 int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
-                key = ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
+                key = rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
             }
             unsigned long long slot = agg_inc(cursors, key, act, true);
             if (act) records[offsets[key] + slot] = rec;
@@ -325,7 +368,10 @@ tile_deposit_kernel(const float4 *__restrict__ records,
                     float *__restrict__ grid, int overwrite) {
     constexpr int F = MAS;
     extern __shared__ double acc[];
-    const long long tile = blockIdx.x;
+    const long long key = blockIdx.x;                 // r * ntiles + tile
+    const long long real = key / g.ntiles;
+    const long long tile = key - real * g.ntiles;
+    grid += real * g.ncell;
     const int tz = (int)(tile % g.nt[2]);
     const int ty = (int)((tile / g.nt[2]) % g.nt[1]);
     const int tx = (int)(tile / ((long long)g.nt[2] * g.nt[1]));
@@ -334,7 +380,7 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     const int ey = min(g.T[1], g.dims - oy);
     const int ez = (AXES == 3) ? min(g.T[2], g.dims - oz) : 1;
     const int ncell = ex * ey * ez;
-    const unsigned long long r0 = offsets[tile], r1 = offsets[tile + 1];
+    const unsigned long long r0 = offsets[key], r1 = offsets[key + 1];
 
     if (r0 == r1) {
         if (overwrite) {
@@ -352,7 +398,8 @@ tile_deposit_kernel(const float4 *__restrict__ records,
 
     // Quantum 2^q: every |contribution| <= weight_factor*max|W| (B-spline weights
     // are <= 1), so sum |c| <= count*wmax*factor < 2^e and 2^e/2^q = 2^52.
-    const double wmax = (double)__uint_as_float(*wmax_bits) * (double)weight_factor;
+    const double wmax =
+        (wmax_bits ? (double)__uint_as_float(wmax_bits[key]) : 1.0) * (double)weight_factor;
     const double bound = (double)(r1 - r0) * wmax;
     int e = 0;
     if (bound > 0.0 && bound <= 1.0e300) frexp(bound, &e);
@@ -467,7 +514,7 @@ int grid_blocks(long long n, int block) {
     return (int)b;
 }
 
-TileGeom make_geom(int dims, int axes) {
+TileGeom make_geom(int dims, int axes, int nreal) {
     TileGeom g{};
     g.dims = dims;
     g.dz = axes == 3 ? dims : 1;
@@ -481,38 +528,55 @@ TileGeom make_geom(int dims, int axes) {
     g.nt[1] = ceil_div(dims, g.T[1]);
     g.nt[2] = axes == 3 ? ceil_div(dims, g.T[2]) : 1;
     g.ntiles = (long long)g.nt[0] * g.nt[1] * g.nt[2];
+    g.ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
+    g.nreal = nreal;
     return g;
 }
 
+// Particle ranges of a batch on the device (offs[0..nreal]); NULL for one grid.
+int upload_offsets(Ctx *c, const long long *h_offs, int nreal, const long long **d_offs) {
+    *d_offs = nullptr;
+    if (nreal <= 1) return PYL_OK;
+    PYL_TRY(ensure(c, BUF_SCAN_TMP, sizeof(long long) * (size_t)(nreal + 1)));
+    long long *d = (long long *)c->bufs[BUF_SCAN_TMP].ptr;
+    PYL_HIP(hipMemcpyAsync(d, h_offs, sizeof(long long) * (size_t)(nreal + 1),
+                           hipMemcpyHostToDevice, c->stream));
+    PYL_HIP(hipStreamSynchronize(c->stream));   // h_offs may be a temporary
+    *d_offs = d;
+    return PYL_OK;
+}
+
 template <int MAS>
-int run_tiled(Ctx *c, const float *pos, long long n, int axes, const float *W, float *grid,
-              int dims, float inv, float weight_factor, int overwrite) {
-    TileGeom g = make_geom(dims, axes);
-    const long long nt = g.ntiles;
+int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal, int axes,
+              const float *W, float *grid, int dims, float inv, float weight_factor,
+              int overwrite) {
+    TileGeom g = make_geom(dims, axes, nreal);
+    const long long nt = g.ntiles * nreal;      // keys over the whole batch
     const int nscan_blocks = (int)((nt + kScanTile - 1) / kScanTile);
     PYL_CHECK(nscan_blocks <= kScanTile, PYL_EINVAL, "too many tiles (%lld)", nt);
-    // tile buffer: counts[nt] | cursors[nt] | wmax (1) | pad | offsets[nt+1] | block sums
-    size_t cnt_bytes = ((size_t)nt * 2 + 4) * sizeof(unsigned);
-    cnt_bytes = (cnt_bytes + 255) & ~(size_t)255;
+    PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
+    const long long *d_offs = nullptr;
+    PYL_TRY(upload_offsets(c, h_offs, nreal, &d_offs));
+    // tile buffer: counts[nt] | cursors[nt] | wmax[nt] (weighted only) | pad |
+    //              offsets[nt+1] | block sums
+    const size_t nwords = (size_t)nt * (W ? 3 : 2);
+    size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
     size_t off_bytes = ((size_t)nt + 1) * sizeof(unsigned long long);
     size_t bs_bytes = (size_t)nscan_blocks * sizeof(unsigned long long);
     PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes));
     char *tb = (char *)c->bufs[BUF_TILE_CNT].ptr;
     unsigned *counts = (unsigned *)tb;
     unsigned *cursors = counts + nt;
-    unsigned *wmax = cursors + nt;
+    unsigned *wmax = W ? cursors + nt : nullptr;   // NULL: unit weights
     unsigned long long *offsets = (unsigned long long *)(tb + cnt_bytes);
     unsigned long long *bsums = offsets + nt + 1;
 
-    PYL_HIP(hipMemsetAsync(counts, 0, ((size_t)nt * 2 + 4) * sizeof(unsigned), c->stream));
-    if (!W) {
-        unsigned one = 0x3f800000u;  // 1.0f
-        PYL_HIP(hipMemcpyAsync(wmax, &one, sizeof(one), hipMemcpyHostToDevice, c->stream));
-    }
+    PYL_HIP(hipMemsetAsync(counts, 0, nwords * sizeof(unsigned), c->stream));
     const int nb = grid_blocks(n, kBlock);
     if (n > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
-        tile_count_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, counts, wmax);
+        tile_count_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, d_offs,
+                                                             counts, wmax);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
     }
@@ -529,8 +593,8 @@ int run_tiled(Ctx *c, const float *pos, long long n, int axes, const float *W, f
     float4 *records = (float4 *)c->bufs[BUF_RECORDS].ptr;
     if (n > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
-        tile_scatter_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, offsets,
-                                                               cursors, records);
+        tile_scatter_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, d_offs,
+                                                               offsets, cursors, records);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
     }
@@ -548,20 +612,29 @@ int run_tiled(Ctx *c, const float *pos, long long n, int axes, const float *W, f
 }
 
 template <int MAS>
-int run_atomic(Ctx *c, const float *pos, long long n, int axes, const float *W, float *grid,
-               int dims, float inv, float weight_factor, int overwrite) {
-    long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
-    if (overwrite) PYL_HIP(hipMemsetAsync(grid, 0, (size_t)ncell * sizeof(float), c->stream));
+int run_atomic(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal,
+               int axes, const float *W, float *grid, int dims, float inv, float weight_factor,
+               int overwrite) {
+    const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
+    if (overwrite)
+        PYL_HIP(hipMemsetAsync(grid, 0, (size_t)ncell * nreal * sizeof(float), c->stream));
     if (n == 0) return PYL_OK;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_ATOMIC));
-    const int nb = grid_blocks(n, kBlock);
-    if (axes == 3)
-        atomic_deposit_kernel<MAS, 3><<<nb, kBlock, 0, c->stream>>>(pos, W, n, inv, dims,
-                                                                   weight_factor, grid);
-    else
-        atomic_deposit_kernel<MAS, 2><<<nb, kBlock, 0, c->stream>>>(pos, W, n, inv, dims,
-                                                                   weight_factor, grid);
-    PYL_HIP(hipGetLastError());
+    for (int r = 0; r < nreal; ++r) {
+        const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
+        if (e <= b) continue;
+        const float *p = pos + (long long)axes * b;
+        const float *w = W ? W + b : nullptr;
+        float *gr = grid + (long long)r * ncell;
+        const int nb = grid_blocks(e - b, kBlock);
+        if (axes == 3)
+            atomic_deposit_kernel<MAS, 3><<<nb, kBlock, 0, c->stream>>>(p, w, e - b, inv, dims,
+                                                                       weight_factor, gr);
+        else
+            atomic_deposit_kernel<MAS, 2><<<nb, kBlock, 0, c->stream>>>(p, w, e - b, inv, dims,
+                                                                       weight_factor, gr);
+        PYL_HIP(hipGetLastError());
+    }
     PYL_TRY(stage_end(c, PYL_STAGE_MA_ATOMIC));
     return PYL_OK;
 }
@@ -569,13 +642,16 @@ int run_atomic(Ctx *c, const float *pos, long long n, int axes, const float *W, 
 }  // namespace
 
 // Device-pointer MA used by the C ABI and by the batched driver.
-int ma_device(Ctx *c, const float *pos, long long n, int axes, const float *W, float *grid,
-              int dims, float BoxSize, int mas, int flags) {
+// h_offs: host particle ranges of nreal grids (nreal+1 entries, NULL when nreal == 1);
+// grid holds nreal contiguous grids.
+int ma_device(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal,
+              int axes, const float *W, float *grid, int dims, float BoxSize, int mas,
+              int flags) {
     // MAS_library.pyx:135 inv_cell_size = dims/BoxSize in float (MAS_c.c:19 computes it
     // in double and stores it to float, which rounds to the same value).
     const float inv = (float)dims / BoxSize;
     float weight_factor = 1.0f;
-    const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
+    const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1) * nreal;
     const int overwrite = (flags & PYL_MA_OVERWRITE) ? 1 : 0;
     if (axes == 2 && (flags & PYL_MA_CYTHON_2D) && mas != PYL_NGP) {
         // MA() runs the 3D kernel on a (d,d,1) view, so each deposit lands F times
@@ -593,12 +669,12 @@ int ma_device(Ctx *c, const float *pos, long long n, int axes, const float *W, f
     }
     const bool atomic = (flags & PYL_MA_ATOMIC) != 0;
     switch (mas) {
-#define PYL_MA_CASE(M)                                                                    \
-    case M:                                                                               \
-        return atomic ? run_atomic<M>(c, pos, n, axes, W, grid, dims, inv, weight_factor,  \
-                                      overwrite)                                          \
-                      : run_tiled<M>(c, pos, n, axes, W, grid, dims, inv, weight_factor,   \
-                                     overwrite);
+#define PYL_MA_CASE(M)                                                                      \
+    case M:                                                                                 \
+        return atomic ? run_atomic<M>(c, pos, n, h_offs, nreal, axes, W, grid, dims, inv,    \
+                                      weight_factor, overwrite)                             \
+                      : run_tiled<M>(c, pos, n, h_offs, nreal, axes, W, grid, dims, inv,     \
+                                     weight_factor, overwrite);
         PYL_MA_CASE(PYL_NGP)
         PYL_MA_CASE(PYL_CIC)
         PYL_MA_CASE(PYL_TSC)
@@ -613,8 +689,9 @@ int ma_device(Ctx *c, const float *pos, long long n, int axes, const float *W, f
 
 using namespace pyl;
 
-extern "C" int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
-                      float *grid, int dims, float BoxSize, int mas, int flags) {
+static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *offs, int nreal,
+                    int axes, const float *W, float *grid, int dims, float BoxSize, int mas,
+                    int flags) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
     Ctx *c = &ctx->c;
     PYL_CHECK(axes == 2 || axes == 3, PYL_EINVAL, "pos must have 2 or 3 columns (got %d)", axes);
@@ -623,12 +700,12 @@ extern "C" int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const
     PYL_CHECK(n >= 0, PYL_EINVAL, "negative particle count");
     PYL_CHECK(grid != nullptr && (pos != nullptr || n == 0), PYL_EINVAL, "NULL array");
     PYL_CHECK(BoxSize > 0.0f, PYL_EINVAL, "BoxSize must be positive");
-    const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
+    const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1) * nreal;
     PYL_CHECK(ncell < (1ll << 34), PYL_EINVAL, "grid too large");
     PYL_HIP(hipSetDevice(c->device));
     reset_stages(c);
     if (flags & PYL_PTR_DEVICE) {
-        PYL_TRY(ma_device(c, pos, n, axes, W, grid, dims, BoxSize, mas, flags));
+        PYL_TRY(ma_device(c, pos, n, offs, nreal, axes, W, grid, dims, BoxSize, mas, flags));
         return finish(c, flags);
     }
     // Host pointers: stage through device scratch.
@@ -642,10 +719,28 @@ extern "C" int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const
     if (W && n) PYL_HIP(hipMemcpyAsync(dW, W, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (!(flags & PYL_MA_OVERWRITE))
         PYL_HIP(hipMemcpyAsync(dgrid, grid, (size_t)ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    PYL_TRY(ma_device(c, dpos, n, axes, dW, dgrid, dims, BoxSize, mas, flags | PYL_PTR_DEVICE));
+    PYL_TRY(ma_device(c, dpos, n, offs, nreal, axes, dW, dgrid, dims, BoxSize, mas,
+                      flags | PYL_PTR_DEVICE));
     PYL_HIP(hipMemcpyAsync(grid, dgrid, (size_t)ncell * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     PYL_HIP(hipStreamSynchronize(c->stream));
     return PYL_OK;
+}
+
+extern "C" int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
+                      float *grid, int dims, float BoxSize, int mas, int flags) {
+    return ma_entry(ctx, pos, n, nullptr, 1, axes, W, grid, dims, BoxSize, mas, flags);
+}
+
+extern "C" int pyl_ma_batch(pyl_ctx *ctx, const float *pos, const int64_t *offsets, int nreal,
+                            int axes, const float *W, float *grids, int dims, float BoxSize,
+                            int mas, int flags) {
+    PYL_CHECK(offsets != nullptr && nreal >= 1, PYL_EINVAL, "need nreal >= 1 and offsets");
+    PYL_CHECK(offsets[0] == 0, PYL_EINVAL, "offsets[0] must be 0");
+    for (int r = 0; r < nreal; ++r)
+        PYL_CHECK(offsets[r + 1] >= offsets[r], PYL_EINVAL, "offsets must be non-decreasing");
+    static_assert(sizeof(int64_t) == sizeof(long long), "int64_t");
+    return ma_entry(ctx, pos, offsets[nreal], (const long long *)offsets, nreal, axes, W, grids,
+                    dims, BoxSize, mas, flags);
 }
 
 // ---- MAS_c.h:3-10 compatible entry points --------------------------------

@@ -176,16 +176,18 @@ int row_key(int kxx, int kyy, int dims, int axis) {
     return (int)s;
 }
 
-int get_bin_plan(Ctx *c, int dims, int axis, BinPlan **out) {
-    auto key = std::make_pair(dims, axis);
+int get_bin_plan(Ctx *c, int dims, int axis, int nreal, BinPlan **out) {
+    // rows per item: ~2048 items for one large grid; for a batch, ~16K items in total
+    // so that the partials stay small
+    const long long nrows_total = (long long)dims * dims;
+    const long long target = nreal > 1 ? std::max(1LL, 16384LL / nreal) : 2048LL;
+    const int R = (int)std::max(4LL, std::min(256LL, nrows_total / target));
+    auto key = std::make_tuple(dims, axis, R);
     auto it = c->bin_plans.find(key);
     if (it != c->bin_plans.end()) {
         *out = &it->second;
         return PYL_OK;
     }
-    // rows per item: aim for >= ~2048 items on large grids
-    const long long nrows_total = (long long)dims * dims;
-    const int R = (int)std::max(4LL, std::min(256LL, nrows_total / 2048));
     PkDims p = pk_dims(dims);
     const int nkeys = (axis == 2) ? p.kmax_per + 1 : p.m + 1;
     std::vector<std::vector<int>> by_key(nkeys);
@@ -245,7 +247,9 @@ struct BinArgs {
     const double *corr[2];   // MAS correction tables [dims] per field
     const int *rows;
     const int4 *items;
-    double *partials;        // [nitems][P]
+    double *partials;        // [nreal][nitems][P]
+    long long fstride;       // elements between realisations in dk[] (blockIdx.y = realisation)
+    int nitems;
     int dims, m, zlen, zpad, even, kmax, L2, L1;
     int P, off2, off1;       // partial layout: acc3 at 0, line2 at off2, line1 at off1
     int rmax;                // max rows per item (LDS row table size)
@@ -310,7 +314,7 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
             if (r >= nrows) break;
         }
         const int kx = rowi[3 * r + 0], ky = rowi[3 * r + 1];
-        const long long rowoff = (long long)rowi[3 * r + 2] * a.zlen;
+        const long long rowoff = (long long)blockIdx.y * a.fstride + (long long)rowi[3 * r + 2] * a.zlen;
         const int kxy2 = kx * kx + ky * ky;
         const bool kx_neg = kx < 0;
         const bool kx_edge = (kx == 0) || (a.even && kx == a.m);
@@ -433,27 +437,30 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
         }
     }
     __syncthreads();
-    double *out = a.partials + (long long)blockIdx.x * a.P;
+    double *out = a.partials + ((long long)blockIdx.y * a.nitems + blockIdx.x) * a.P;
     for (int i = tid; i < n3; i += blockDim.x) out[i] = acc3[i];
     for (int i = tid; i < a.L2 * S; i += blockDim.x) out[a.off2 + i] = line2[i];
     for (int i = tid; i < a.L1 * S; i += blockDim.x) out[a.off1 + i] = line1[i];
 }
 
-// Sum item partials in item order (deterministic).  Output layout:
-//   out3[(kmax+1)*NA3], out2[nb2*S] (reference 2D index), out1[(m+1)*S]
+// Sum item partials in item order (deterministic).  blockIdx.y = realisation.
+// Output layout per realisation: out3[(kmax+1)*NA3], out2[nb2*S] (reference 2D
+// index), out1[(m+1)*S].
 struct ReduceArgs {
     const double *partials;
     const int *key_items;
     double *out3, *out2, *out1;
     int nitems, nkeys, n3, L2, L1, S, P, off2, off1, axis, kmax_par;
+    long long nb2, n1;       // out2 / out1 elements per realisation (nb2*S, (m+1)*S)
 };
 
 __global__ void pk_reduce3_kernel(ReduceArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n3) return;
+    const double *part = a.partials + (long long)blockIdx.y * a.nitems * a.P;
     double s = 0.0;
-    for (int it = 0; it < a.nitems; ++it) s += a.partials[(long long)it * a.P + i];
-    a.out3[i] = s;
+    for (int it = 0; it < a.nitems; ++it) s += part[(long long)it * a.P + i];
+    a.out3[(long long)blockIdx.y * a.n3 + i] = s;
 }
 
 // One thread per (key, line index, spectrum): sums the key's items.
@@ -461,39 +468,51 @@ __global__ void pk_reduce2_kernel(ReduceArgs a) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long n = (long long)a.nkeys * a.L2 * a.S;
     if (i >= n) return;
+    const double *part = a.partials + (long long)blockIdx.y * a.nitems * a.P;
     int sp = (int)(i % a.S);
     int li = (int)((i / a.S) % a.L2);
     int key = (int)(i / ((long long)a.S * a.L2));
     double s = 0.0;
     for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
-        s += a.partials[(long long)it * a.P + a.off2 + li * a.S + sp];
+        s += part[(long long)it * a.P + a.off2 + li * a.S + sp];
     // reference 2D index: (kmax_par+1)*k_per + k_par
     long long idx2 = (a.axis == 2) ? (long long)(a.kmax_par + 1) * key + li
                                    : (long long)(a.kmax_par + 1) * li + key;
-    a.out2[idx2 * a.S + sp] = s;
+    a.out2[(long long)blockIdx.y * a.nb2 + idx2 * a.S + sp] = s;
 }
 
 __global__ void pk_reduce1_kernel(ReduceArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *part = a.partials + (long long)blockIdx.y * a.nitems * a.P;
+    double *out1 = a.out1 + (long long)blockIdx.y * a.n1;
     if (a.axis == 2) {
         if (i >= a.L1 * a.S) return;   // L1 = m+1 bins (k_par = kz), all items
         double s = 0.0;
-        for (int it = 0; it < a.nitems; ++it) s += a.partials[(long long)it * a.P + a.off1 + i];
-        a.out1[i] = s;
+        for (int it = 0; it < a.nitems; ++it) s += part[(long long)it * a.P + a.off1 + i];
+        out1[i] = s;
     } else {
         if (i >= a.nkeys * a.S) return;  // bin = key = k_par
         int key = i / a.S, sp = i % a.S;
         double s = 0.0;
         for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
-            s += a.partials[(long long)it * a.P + a.off1 + sp];
-        a.out1[key * a.S + sp] = s;
+            s += part[(long long)it * a.P + a.off1 + sp];
+        out1[key * a.S + sp] = s;
     }
 }
 
 // FFT prologue: out = (in / scale) - shift in float32 (the reference caller's
-// "delta /= mean; delta -= 1.0", Pk_snapshot.py:88), or a plain copy.
+// "delta /= mean; delta -= 1.0", Pk_snapshot.py:88), or a plain copy.  blockIdx.y
+// selects one of several contiguous cubes of n elements, each with its own
+// scale/shift when the device arrays are given.
 __global__ void prologue_kernel(const float *__restrict__ in, float *__restrict__ out,
-                                long long n, float scale, float shift, int apply) {
+                                long long n, const float *__restrict__ scales,
+                                const float *__restrict__ shifts, float scale, float shift,
+                                int apply) {
+    const long long cube = blockIdx.y;
+    in += cube * n;
+    out += cube * n;
+    if (scales) scale = scales[cube];
+    if (shifts) shift = shifts[cube];
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         float x = in[i];
@@ -555,7 +574,8 @@ struct HostAcc {
 
 template <int AXIS, int NFL, bool PHASE>
 int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
-               const double *const *corr, HostAcc &acc, const int *spec_out) {
+               long long fstride, int nreal, const double *const *corr,
+               std::vector<HostAcc> &accs, const int *spec_out) {
     constexpr int S = Spec<NFL>::S;
     constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);
     const int L2 = (AXIS == 2) ? p.zlen : p.kmax_per + 1;
@@ -567,15 +587,14 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     size_t lds = sizeof(double) * ((size_t)P + (size_t)p.zlen * NFL + (size_t)rmax * NFL) +
                  sizeof(int) * 3 * (size_t)rmax;
     PYL_CHECK(lds <= 160 * 1024, PYL_EINVAL, "grid too large for the binning kernel (%zu B LDS)", lds);
-    // scratch: partials + reduced outputs
-    const size_t part_bytes = sizeof(double) * (size_t)bp->nitems * P;
+    // scratch: partials + reduced outputs, per realisation
+    const size_t part_elems = (size_t)bp->nitems * P * nreal;
     const size_t out3 = (size_t)n3, out2 = (size_t)p.nb2 * S, out1 = (size_t)(p.m + 1) * S;
-    PYL_TRY(ensure(c, BUF_BINS, part_bytes + sizeof(double) * (out3 + out2 + out1)));
+    PYL_TRY(ensure(c, BUF_BINS, sizeof(double) * (part_elems + (out3 + out2 + out1) * nreal)));
     double *partials = (double *)c->bufs[BUF_BINS].ptr;
-    double *d3 = partials + (size_t)bp->nitems * P;
-    double *d2 = d3 + out3;
-    double *d1 = d2 + out2;
-    PYL_HIP(hipMemsetAsync(d2, 0, sizeof(double) * out2, c->stream));
+    double *d3 = partials + part_elems;
+    double *d2 = d3 + out3 * nreal;
+    double *d1 = d2 + out2 * nreal;
 
     BinArgs a{};
     for (int f = 0; f < NFL; ++f) {
@@ -585,55 +604,64 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     a.rows = bp->d_rows;
     a.items = bp->d_items;
     a.partials = partials;
+    a.fstride = fstride;
+    a.nitems = bp->nitems;
     a.dims = p.dims; a.m = p.m; a.zlen = p.zlen; a.zpad = p.zpad; a.even = p.even;
     a.kmax = p.kmax; a.L2 = L2; a.L1 = L1; a.P = P; a.off2 = off2; a.off1 = off1; a.rmax = rmax;
     auto kern = pk_bin_kernel<AXIS, NFL, PHASE>;
     PYL_TRY(set_lds(kern, lds));
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_BIN));
-    kern<<<bp->nitems, kBinBlock, lds, c->stream>>>(a);
+    kern<<<dim3(bp->nitems, nreal), kBinBlock, lds, c->stream>>>(a);
     PYL_HIP(hipGetLastError());
     ReduceArgs r{};
     r.partials = partials; r.key_items = bp->d_key_items; r.out3 = d3; r.out2 = d2; r.out1 = d1;
     r.nitems = bp->nitems; r.nkeys = bp->nkeys; r.n3 = n3; r.L2 = L2; r.L1 = L1; r.S = S;
     r.P = P; r.off2 = off2; r.off1 = off1; r.axis = AXIS; r.kmax_par = p.kmax_par;
-    pk_reduce3_kernel<<<ceil_div(n3, 256), 256, 0, c->stream>>>(r);
+    r.nb2 = (long long)out2; r.n1 = (long long)out1;
+    pk_reduce3_kernel<<<dim3(ceil_div(n3, 256), nreal), 256, 0, c->stream>>>(r);
     long long n2 = (long long)bp->nkeys * L2 * S;
-    pk_reduce2_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, c->stream>>>(r);
+    pk_reduce2_kernel<<<dim3((unsigned)((n2 + 255) / 256), nreal), 256, 0, c->stream>>>(r);
     int n1 = (AXIS == 2) ? L1 * S : bp->nkeys * S;
-    pk_reduce1_kernel<<<ceil_div(n1, 256), 256, 0, c->stream>>>(r);
+    pk_reduce1_kernel<<<dim3(ceil_div(n1, 256), nreal), 256, 0, c->stream>>>(r);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_PK_BIN));
-    std::vector<double> h3(out3), h2(out2), h1(out1);
-    PYL_HIP(hipMemcpyAsync(h3.data(), d3, sizeof(double) * out3, hipMemcpyDeviceToHost, c->stream));
-    PYL_HIP(hipMemcpyAsync(h2.data(), d2, sizeof(double) * out2, hipMemcpyDeviceToHost, c->stream));
-    PYL_HIP(hipMemcpyAsync(h1.data(), d1, sizeof(double) * out1, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> h3(out3 * nreal), h2(out2 * nreal), h1(out1 * nreal);
+    PYL_HIP(hipMemcpyAsync(h3.data(), d3, sizeof(double) * h3.size(), hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipMemcpyAsync(h2.data(), d2, sizeof(double) * h2.size(), hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipMemcpyAsync(h1.data(), d1, sizeof(double) * h1.size(), hipMemcpyDeviceToHost, c->stream));
     PYL_HIP(hipStreamSynchronize(c->stream));
     // scatter into the caller's spectrum columns
-    const int ST = acc.S_total;
-    for (int b = 0; b <= p.kmax; ++b) {
-        for (int sp = 0; sp < S; ++sp) {
-            int so = spec_out[sp];
-            if (so < 0) continue;
-            for (int l = 0; l < 3; ++l)
-                acc.P3[((size_t)b * 3 + l) * ST + so] = h3[(size_t)b * NA3 + 3 * sp + l];
+    for (int rr = 0; rr < nreal; ++rr) {
+        HostAcc &acc = accs[rr];
+        const double *H3 = h3.data() + out3 * rr, *H2 = h2.data() + out2 * rr,
+                     *H1 = h1.data() + out1 * rr;
+        const int ST = acc.S_total;
+        for (int b = 0; b <= p.kmax; ++b) {
+            for (int sp = 0; sp < S; ++sp) {
+                int so = spec_out[sp];
+                if (so < 0) continue;
+                for (int l = 0; l < 3; ++l)
+                    acc.P3[((size_t)b * 3 + l) * ST + so] = H3[(size_t)b * NA3 + 3 * sp + l];
+            }
+            if (PHASE) acc.phase[b] = H3[(size_t)b * NA3 + NA3 - 1];
         }
-        if (PHASE) acc.phase[b] = h3[(size_t)b * NA3 + NA3 - 1];
+        for (long long i = 0; i < p.nb2; ++i)
+            for (int sp = 0; sp < S; ++sp)
+                if (spec_out[sp] >= 0) acc.P2[(size_t)i * ST + spec_out[sp]] = H2[(size_t)i * S + sp];
+        for (int i = 0; i <= p.m; ++i)
+            for (int sp = 0; sp < S; ++sp)
+                if (spec_out[sp] >= 0) acc.P1[(size_t)i * ST + spec_out[sp]] = H1[(size_t)i * S + sp];
     }
-    for (long long i = 0; i < p.nb2; ++i)
-        for (int sp = 0; sp < S; ++sp)
-            if (spec_out[sp] >= 0) acc.P2[(size_t)i * ST + spec_out[sp]] = h2[(size_t)i * S + sp];
-    for (int i = 0; i <= p.m; ++i)
-        for (int sp = 0; sp < S; ++sp)
-            if (spec_out[sp] >= 0) acc.P1[(size_t)i * ST + spec_out[sp]] = h1[(size_t)i * S + sp];
     return PYL_OK;
 }
 
 template <int NFL, bool PHASE>
 int launch_bin_axis(Ctx *c, int axis, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
-                    const double *const *corr, HostAcc &acc, const int *spec_out) {
-    if (axis == 0) return launch_bin<0, NFL, PHASE>(c, bp, p, dk, corr, acc, spec_out);
-    if (axis == 1) return launch_bin<1, NFL, PHASE>(c, bp, p, dk, corr, acc, spec_out);
-    return launch_bin<2, NFL, PHASE>(c, bp, p, dk, corr, acc, spec_out);
+                    long long fstride, int nreal, const double *const *corr,
+                    std::vector<HostAcc> &accs, const int *spec_out) {
+    if (axis == 0) return launch_bin<0, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
+    if (axis == 1) return launch_bin<1, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
+    return launch_bin<2, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
 }
 
 // Normalisation, Pk_library.pyx:384-418 (and :735-791 for XPk).
@@ -707,13 +735,14 @@ int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch) {
     return PYL_OK;
 }
 
-// Binning of F device half-complex fields -> finalized output.
-int pk_bin_device(Ctx *c, const float2 *const *dk, int F, const int *mas, int dims,
-                  double BoxSize, int axis, int flags, pyl_pk_out *out) {
+// Binning of F device half-complex fields of nreal realisations -> finalized outputs.
+// Field f of realisation r is dk[f] + r*fstride.
+int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal, int F,
+                  const int *mas, int dims, double BoxSize, int axis, int flags, pyl_pk_out *outs) {
     const PkDims p = pk_dims(dims);
     const ModePlan *mp = get_mode_plan(c, dims, axis);
     BinPlan *bp = nullptr;
-    PYL_TRY(get_bin_plan(c, dims, axis, &bp));
+    PYL_TRY(get_bin_plan(c, dims, axis, nreal, &bp));
     // MAS tables (one per field, host libm, uploaded)
     std::vector<double> tabs((size_t)F * dims);
     for (int f = 0; f < F; ++f) mas_table(dims, mas[f], &tabs[(size_t)f * dims]);
@@ -722,19 +751,21 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, int F, const int *mas, int di
     PYL_HIP(hipMemcpyAsync(dtab, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice,
                            c->stream));
     const int S = F + F * (F - 1) / 2;
-    HostAcc acc;
-    acc.S_total = S;
-    acc.P3.assign((size_t)(p.kmax + 1) * 3 * S, 0.0);
-    acc.phase.assign(p.kmax + 1, 0.0);
-    acc.P2.assign((size_t)p.nb2 * S, 0.0);
-    acc.P1.assign((size_t)(p.m + 1) * S, 0.0);
+    std::vector<HostAcc> accs(nreal);
+    for (auto &acc : accs) {
+        acc.S_total = S;
+        acc.P3.assign((size_t)(p.kmax + 1) * 3 * S, 0.0);
+        acc.phase.assign(p.kmax + 1, 0.0);
+        acc.P2.assign((size_t)p.nb2 * S, 0.0);
+        acc.P1.assign((size_t)(p.m + 1) * S, 0.0);
+    }
     const bool phase = (F == 1) && !(flags & PYL_PK_NO_PHASE);
     if (F == 1) {
         const float2 *f0[1] = {dk[0]};
         const double *c0[1] = {dtab};
         int so[1] = {0};
-        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, c0, acc, so)));
-        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, c0, acc, so)));
+        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, nreal, c0, accs, so)));
+        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, nreal, c0, accs, so)));
     } else {
         // pairs i<j in lexicographic order; each launch yields auto i, auto j, cross ij
         std::vector<bool> have(F, false);
@@ -745,10 +776,11 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, int F, const int *mas, int di
                 const double *cp[2] = {dtab + (size_t)i * dims, dtab + (size_t)j * dims};
                 int so[3] = {have[i] ? -1 : i, have[j] ? -1 : j, F + xi};
                 have[i] = have[j] = true;
-                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, cp, acc, so)));
+                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, nreal, cp, accs, so)));
             }
     }
-    return finalize(mp, p, BoxSize, acc, phase, out);
+    for (int r = 0; r < nreal; ++r) PYL_TRY(finalize(mp, p, BoxSize, accs[r], phase, &outs[r]));
+    return PYL_OK;
 }
 
 }  // namespace pyl
@@ -838,7 +870,7 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
             float *x = (float *)deltas[f];
             if (apply) {
                 PYL_TRY(stage_begin(c, PYL_STAGE_PK_PREP));
-                prologue_kernel<<<nb, pb, 0, c->stream>>>(x, x, n, scale, shift, 1);
+                prologue_kernel<<<nb, pb, 0, c->stream>>>(x, x, n, nullptr, nullptr, scale, shift, 1);
                 PYL_HIP(hipGetLastError());
                 PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
             }
@@ -853,7 +885,8 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
             float *dst = in + (size_t)f * n;
             if (dev) {
                 if (apply) {
-                    prologue_kernel<<<nb, pb, 0, c->stream>>>(deltas[f], dst, n, scale, shift, 1);
+                    prologue_kernel<<<nb, pb, 0, c->stream>>>(deltas[f], dst, n, nullptr, nullptr, scale,
+                                                            shift, 1);
                     PYL_HIP(hipGetLastError());
                 } else {
                     PYL_HIP(hipMemcpyAsync(dst, deltas[f], n * sizeof(float), hipMemcpyDeviceToDevice,
@@ -863,7 +896,7 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
                 PYL_HIP(hipMemcpyAsync(dst, deltas[f], n * sizeof(float), hipMemcpyHostToDevice,
                                        c->stream));
                 if (apply) {
-                    prologue_kernel<<<nb, pb, 0, c->stream>>>(dst, dst, n, scale, shift, 1);
+                    prologue_kernel<<<nb, pb, 0, c->stream>>>(dst, dst, n, nullptr, nullptr, scale, shift, 1);
                     PYL_HIP(hipGetLastError());
                 }
             }
@@ -872,7 +905,64 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
         PYL_TRY(fft_device(c, in, dk, dims, nfields));
         for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nk;
     }
-    PYL_TRY(pk_bin_device(c, dks.data(), nfields, mas, dims, BoxSize, axis, flags, out));
+    PYL_TRY(pk_bin_device(c, dks.data(), 0, 1, nfields, mas, dims, BoxSize, axis, flags, out));
+    return finish(c, flags);
+}
+
+int pyl_pk_batch(pyl_ctx *ctx, const float *deltas, int nreal, int nfields, const int *mas,
+                 int dims, double BoxSize, int axis, const float *scales, const float *shifts,
+                 int flags, pyl_pk_out *outs) {
+    PYL_CHECK(ctx && deltas && mas && outs, PYL_EINVAL, "NULL argument");
+    PYL_CHECK(nreal >= 1 && nfields >= 1 && (long long)nreal * nfields <= 65535, PYL_EINVAL,
+              "need 1 <= nreal*nfields <= 65535");
+    PYL_CHECK(dims >= 2 && axis >= 0 && axis <= 2 && BoxSize > 0.0, PYL_EINVAL, "bad arguments");
+    for (int f = 0; f < nfields; ++f)
+        PYL_CHECK(mas[f] >= 0 && mas[f] <= 4, PYL_EINVAL, "bad MAS id %d", mas[f]);
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipSetDevice(c->device));
+    reset_stages(c);
+    const long long n = (long long)dims * dims * dims;
+    const long long nk = (long long)dims * dims * (dims / 2 + 1);
+    const int ncube = nreal * nfields;
+    const bool dev = (flags & PYL_PTR_DEVICE) != 0;
+    const bool destroy = dev && (flags & PYL_PK_DESTROY_INPUT);
+    const bool apply = scales != nullptr || shifts != nullptr;
+    PYL_TRY(ensure(c, BUF_FFT_OUT, (size_t)nk * ncube * sizeof(float2)));
+    float2 *dk = (float2 *)c->bufs[BUF_FFT_OUT].ptr;
+    float *in = (float *)deltas;
+    if (!destroy) {
+        PYL_TRY(ensure(c, BUF_FFT_IN, (size_t)n * ncube * sizeof(float)));
+        in = (float *)c->bufs[BUF_FFT_IN].ptr;
+    }
+    float *dsc = nullptr, *dsh = nullptr;
+    if (apply) {
+        std::vector<float> sc(ncube, 1.0f), sh(ncube, 0.0f);
+        for (int i = 0; i < ncube; ++i) {
+            if (scales) sc[i] = scales[i];
+            if (shifts) sh[i] = shifts[i];
+        }
+        PYL_TRY(ensure(c, BUF_SCAN_TMP, sizeof(float) * 2 * (size_t)ncube));
+        dsc = (float *)c->bufs[BUF_SCAN_TMP].ptr;
+        dsh = dsc + ncube;
+        PYL_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(float) * ncube, hipMemcpyHostToDevice, c->stream));
+        PYL_HIP(hipMemcpyAsync(dsh, sh.data(), sizeof(float) * ncube, hipMemcpyHostToDevice, c->stream));
+        PYL_HIP(hipStreamSynchronize(c->stream));   // sc/sh are host temporaries
+    }
+    PYL_TRY(stage_begin(c, PYL_STAGE_PK_PREP));
+    if (!destroy)
+        PYL_HIP(hipMemcpyAsync(in, deltas, (size_t)n * ncube * sizeof(float),
+                               dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (apply) {
+        const int nbx = (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048));
+        prologue_kernel<<<dim3(nbx, ncube), 256, 0, c->stream>>>(in, in, n, dsc, dsh, 1.0f, 0.0f, 1);
+        PYL_HIP(hipGetLastError());
+    }
+    PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
+    PYL_TRY(fft_device(c, in, dk, dims, ncube));
+    std::vector<const float2 *> dks(nfields);
+    for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nk;
+    PYL_TRY(pk_bin_device(c, dks.data(), (long long)nfields * nk, nreal, nfields, mas, dims,
+                          BoxSize, axis, flags, outs));
     return finish(c, flags);
 }
 
@@ -897,7 +987,7 @@ int pyl_pk_from_fourier(pyl_ctx *ctx, const float *const *delta_k, int nfields, 
             dks[f] = dk + (size_t)f * nk;
         }
     }
-    PYL_TRY(pk_bin_device(c, dks.data(), nfields, mas, dims, BoxSize, axis, flags, out));
+    PYL_TRY(pk_bin_device(c, dks.data(), 0, 1, nfields, mas, dims, BoxSize, axis, flags, out));
     return finish(c, flags);
 }
 

@@ -100,7 +100,7 @@ struct Ctx {
     DevBuf bufs[BUF_COUNT_];
     std::map<std::pair<int, int>, FFTPlan> fft_plans;      // (dims, batch)
     std::map<std::pair<int, int>, ModePlan> mode_plans;    // (dims, axis)
-    std::map<std::pair<int, int>, BinPlan> bin_plans;      // (dims, axis)
+    std::map<std::tuple<int, int, int>, BinPlan> bin_plans; // (dims, axis, rows per item)
     bool timing = false;
     hipEvent_t ev[PYL_STAGE_COUNT][2] = {};
     bool ev_used[PYL_STAGE_COUNT] = {};

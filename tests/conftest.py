@@ -19,12 +19,47 @@ CELL_RTOL = 1e-5   # per cell: |g - g_ref| <= 1e-5 * max(|g_ref|, mean |g_ref|)
 BIN_RTOL = 1e-5    # per bin:  |P - P_ref| <= 1e-5 * max(|P_ref|, floor)
 
 
-def assert_cells(g, g_ref, rtol=CELL_RTOL, what='grid'):
+U32 = 2.0 ** -24    # float32 unit roundoff
+
+
+def contrib_counts(pos, dims, box, mas, cython2d=True):
+    """Number of float32 additions the reference makes into each cell (its serial
+    loop adds every footprint weight, zero or not; the 2D expand_dims path adds
+    each (l, m) weight F times).  Bounds the reference's own summation error:
+    |fl(sum) - sum| <= n_adds * u * sum|x_i|."""
+    F = {'NGP': 1, 'CIC': 2, 'TSC': 3, 'PCS': 4}[mas]
+    x = pos.astype(np.float64) * (dims / box)
+    if mas == 'NGP':
+        base = np.floor(x + 0.5)
+    elif mas == 'CIC':
+        base = np.floor(x)
+    else:
+        base = np.floor(x - (F / 2.0)) + 1
+    base = base.astype(np.int64)
+    coord = pos.shape[1]
+    idx = [(base[:, ax, None] + np.arange(F)[None, :]) % dims for ax in range(coord)]  # (n, F)
+    if coord == 3:
+        cells = ((idx[0][:, :, None, None] * dims + idx[1][:, None, :, None]) * dims
+                 + idx[2][:, None, None, :])
+    else:
+        cells = idx[0][:, :, None] * dims + idx[1][:, None, :]
+    cnt = np.bincount(cells.ravel(), minlength=dims ** coord).astype(np.float64)
+    if coord == 2 and cython2d:
+        cnt *= F
+    return cnt.reshape((dims,) * coord)
+
+
+def assert_cells(g, g_ref, rtol=CELL_RTOL, what='grid', nadds=None):
+    """Per cell: |g - g_ref| <= 1e-5 max(|g_ref|, mean|g_ref|), widened where the
+    reference's serial float32 sum itself carries more error than that
+    ((nadds + 2) u |g_ref| for non-negative contributions)."""
     g = np.asarray(g, dtype=np.float64)
     g_ref = np.asarray(g_ref, dtype=np.float64)
     assert g.shape == g_ref.shape, (g.shape, g_ref.shape)
     floor = np.mean(np.abs(g_ref)) if g_ref.size else 0.0
     tol = rtol * np.maximum(np.abs(g_ref), floor)
+    if nadds is not None:
+        tol = np.maximum(tol, (np.asarray(nadds) + 2.0) * U32 * np.abs(g_ref))
     err = np.abs(g - g_ref)
     bad = err > tol
     if bad.any():
