@@ -42,6 +42,9 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain GPU (atomic) leg')
     ap.add_argument('--cpu-seconds', type=float, default=20.0, help='target CPU sample time')
+    ap.add_argument('--no-batched', action='store_true', help='skip the batched-realisations leg')
+    ap.add_argument('--no-serving', action='store_true', help='skip the serving leg')
+    ap.add_argument('--nreal', type=int, default=256, help='realisations in the batched leg')
     return ap.parse_args()
 
 
@@ -144,6 +147,165 @@ def cpu_baseline(pos_host, grid_host, dims, box, mas_name, axis, target_s):
     }
 
 
+def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
+    """Config 4: nreal independent 128^3 realisations (box 500): a Zel'dovich matter
+    field (128^3 particles) and Poisson(lambda) weighted tracers, lambda log-uniform
+    in [1e5, 2e6], CIC MA of both, then Pk of both and their cross spectrum (XPk).
+    Batched form (pyl_ma_batch x2 + pyl_pk_batch) against the plain per-call form
+    (pyl_ma x2 + pyl_pk per realisation), both device-resident."""
+    import numpy as np
+    import torch
+    from pylians3_amd import workloads
+    rng = np.random.default_rng(seed0)
+    lam = np.exp(rng.uniform(np.log(1e5), np.log(2e6), nreal))
+    counts = np.clip(rng.poisson(lam), 100000, 2000000).astype(np.int64)
+    matter, tr_pos, tr_w = [], [], []
+    for r in range(nreal):
+        psi = workloads.displacement_field(dims, box, seed0 + r)
+        q = (torch.arange(dims, device='cuda', dtype=torch.float32) + 0.5) * (box / dims)
+        lat = torch.stack(torch.meshgrid(q, q, q, indexing='ij'), -1).view(-1, 3)
+        matter.append(torch.remainder(lat + psi.view(3, -1).t(), box))
+        p, w = workloads.tracers(int(counts[r]), psi, box, seed0 + 5000 + r)
+        tr_pos.append(p)
+        tr_w.append(w)
+        del psi, lat
+    mpos = torch.cat(matter).contiguous()
+    mpos[mpos >= box] = 0.0
+    tpos = torch.cat(tr_pos).contiguous()
+    tw = torch.cat(tr_w).contiguous()
+    del matter, tr_pos, tr_w
+    moffs = np.arange(nreal + 1, dtype=np.int64) * dims ** 3
+    toffs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    ctx = N.Context(0)
+    grids = [torch.empty((nreal, dims, dims, dims), dtype=torch.float32, device='cuda')
+             for _ in range(2)]
+    allocs = [N.alloc_pk_out(dims, 2) for _ in range(nreal)]
+    outs = (N.PkOut * nreal)(*[a[0] for a in allocs])
+    mas = (ctypes.c_int * 2)(N.CIC, N.CIC)
+    tot_w = np.array([tw[toffs[r]:toffs[r + 1]].double().sum().item() for r in range(nreal)])
+    scales = np.stack([tot_w / dims ** 3, np.full(nreal, 1.0)], 1).astype(np.float32)
+    shifts = np.ones((nreal, 2), np.float32)
+    L = N.lib()
+    flags_ma = N.PTR_DEVICE | N.MA_OVERWRITE | N.NO_SYNC
+
+    def batched():
+        N.check(L.pyl_ma_batch(ctx.handle, ctypes.c_void_p(tpos.data_ptr()), toffs.ctypes.data,
+                               nreal, 3, ctypes.c_void_p(tw.data_ptr()),
+                               ctypes.c_void_p(grids[0].data_ptr()), dims, box, N.CIC, flags_ma))
+        N.check(L.pyl_ma_batch(ctx.handle, ctypes.c_void_p(mpos.data_ptr()), moffs.ctypes.data,
+                               nreal, 3, None, ctypes.c_void_p(grids[1].data_ptr()), dims, box,
+                               N.CIC, flags_ma))
+        ptrs = (ctypes.c_void_p * 2)(grids[0].data_ptr(), grids[1].data_ptr())
+        N.check(L.pyl_pk_batch(ctx.handle, ptrs, nreal, 2, mas, dims, box, 0,
+                               scales.ctypes.data, shifts.ctypes.data,
+                               N.PTR_DEVICE | N.PK_DESTROY_INPUT, outs))
+
+    one_out, _, _ = N.alloc_pk_out(dims, 2)
+
+    def per_call():
+        for r in range(nreal):
+            g0 = grids[0][r]
+            g1 = grids[1][r]
+            N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(tpos[toffs[r]:].data_ptr()),
+                             int(counts[r]), 3, ctypes.c_void_p(tw[toffs[r]:].data_ptr()),
+                             ctypes.c_void_p(g0.data_ptr()), dims, box, N.CIC,
+                             N.PTR_DEVICE | N.MA_OVERWRITE | N.MA_ATOMIC))
+            N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(mpos[moffs[r]:].data_ptr()),
+                             dims ** 3, 3, None, ctypes.c_void_p(g1.data_ptr()), dims, box, N.CIC,
+                             N.PTR_DEVICE | N.MA_OVERWRITE | N.MA_ATOMIC))
+            ptrs = (ctypes.c_void_p * 2)(g0.data_ptr(), g1.data_ptr())
+            N.check(L.pyl_pk(ctx.handle, ptrs, 2, mas, dims, box, 0, 1.0, 0.0, N.PTR_DEVICE,
+                             ctypes.byref(one_out)))
+
+    res = {}
+    for name, fn, reps in (('batched', batched, 3), ('per_call', per_call, 2)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / reps
+    npart = int(counts.sum()) + nreal * dims ** 3
+    ctx.close()
+    return {'workload': '%d x %d^3 realisations: ZA matter + Poisson lognormal-weighted tracers '
+                        '(lambda log-uniform 1e5-2e6), CIC MA of both, Pk + XPk, axis 0'
+                        % (nreal, dims),
+            'particles': npart,
+            'batched_s': round(res['batched'], 4),
+            'per_call_s': round(res['per_call'], 4),
+            'realisations_per_s': round(nreal / res['batched'], 2),
+            'per_call_realisations_per_s': round(nreal / res['per_call'], 2),
+            'speedup_vs_per_call': round(res['per_call'] / res['batched'], 2),
+            'per_call_form': 'plain atomic MA x2 + single-field-set Pk call per realisation'}
+
+
+def serving_leg(N, nreq=120, seed=5):
+    """Config 5: mixed MA+Pk / MA+XPk requests (grid 64^3-512^3, N log-uniform
+    1e5-1.3e8, MAS and axis drawn uniformly) against the continuous-batching server.
+    Plain form: one request at a time through the drop-in calls.  Served form:
+    Poisson arrivals at 0.9x the plain capacity, then a saturated burst."""
+    import numpy as np
+    import torch
+    from pylians3_amd import workloads
+    from pylians3_amd.serving import Request, Server
+    rng = np.random.default_rng(seed)
+    box = 1000.0
+    psi = workloads.displacement_field(256, box, seed)
+    nmax = 130_000_000
+    pool, poolw = workloads.tracers(nmax, psi, box, seed + 1)
+    del psi
+    reqs = []
+    for i in range(nreq):
+        d = int(rng.choice([64, 128, 256, 512]))
+        n = int(np.exp(rng.uniform(np.log(1e5), np.log(1.3e8))))
+        o = int(rng.integers(0, nmax - n + 1))
+        mas = str(rng.choice(['NGP', 'CIC', 'TSC', 'PCS']))
+        axis = int(rng.integers(0, 3))
+        if rng.random() < 0.5:
+            reqs.append(Request(pool[o:o + n], d, box, mas, axis))
+        else:
+            n2 = max(100000, n // 4)
+            o2 = int(rng.integers(0, nmax - n2 + 1))
+            reqs.append(Request(pool[o:o + n], d, box, mas, axis, pos2=pool[o2:o2 + n2],
+                                W2=poolw[o2:o2 + n2]))
+    # plain form: one request after the other, one slot, no fusion
+    with Server(slots=1, max_batch=1) as srv:
+        srv.submit(reqs[0]).result()
+        t0 = time.perf_counter()
+        plain_lat = np.array([srv.submit(r).result().latency_s for r in reqs])
+        t_plain = time.perf_counter() - t0
+    cap = nreq / t_plain
+    out = {'requests': nreq, 'plain_throughput_rps': round(cap, 2),
+           'plain_p50_ms': round(1e3 * np.percentile(plain_lat, 50), 3),
+           'plain_p99_ms': round(1e3 * np.percentile(plain_lat, 99), 3)}
+    with Server(slots=4, max_batch=32) as srv:
+        srv.submit(reqs[1]).result()
+        srv.completed.clear()
+        # Poisson arrivals at 0.9x the plain capacity
+        gaps = rng.exponential(1.0 / (0.9 * cap), nreq)
+        futs = []
+        t0 = time.perf_counter()
+        for r, gap in zip(reqs, gaps):
+            time.sleep(gap)
+            futs.append(srv.submit(r))
+        lat = np.array([f.result().latency_s for f in futs])
+        t_arr = time.perf_counter() - t0
+        srv.completed.clear()
+        t0 = time.perf_counter()
+        futs = [srv.submit(r) for r in reqs]
+        for f in futs:
+            f.result()
+        t_sat = time.perf_counter() - t0
+    out.update({'arrival_rate_rps': round(0.9 * cap, 2),
+                'p50_ms': round(1e3 * np.percentile(lat, 50), 3),
+                'p99_ms': round(1e3 * np.percentile(lat, 99), 3),
+                'served_window_s': round(t_arr, 3),
+                'saturated_throughput_rps': round(nreq / t_sat, 2),
+                'speedup_vs_plain': round((nreq / t_sat) / cap, 2)})
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get('WORLD_SIZE', '1'))
@@ -218,6 +380,22 @@ def main():
         log('CPU baseline leg')
         pos_host = pos.cpu().numpy()
         cpu = cpu_baseline(pos_host, grid_host, dims, box, 'TSC', axis, args.cpu_seconds)
+        del pos_host
+    del pipe
+    ctx.close()
+    del pos
+    torch.cuda.empty_cache()
+
+    batched = None
+    if not args.no_batched:
+        log('batched realisations leg (%d x 128^3)' % args.nreal)
+        batched = batched_leg(N, args.nreal)
+        torch.cuda.empty_cache()
+    serving = None
+    if not args.no_serving:
+        log('serving leg')
+        serving = serving_leg(N)
+        torch.cuda.empty_cache()
 
     value = n / (ms * 1e-3) / 1e6
     line = {
@@ -242,6 +420,8 @@ def main():
         'kernels_ms': kernels,
         'stages': stages,
         'plain_gpu': plain,
+        'batched': batched,
+        'serving': serving,
         'roofline': roofline,
         'cpu_baseline': cpu,
     }

@@ -123,6 +123,12 @@ void TSC(float *pos, float *number, float *W, long particles, int dims, int axes
 void PCS(float *pos, float *number, float *W, long particles, int dims, int axes,
          float BoxSize, int threads);
 
+/* CIC interpolation of a (dims,dims,dims) float32 grid at n positions (n,3):
+ * den[i] = sum of the 8 CIC-weighted neighbours (gather adjoint of CIC).
+ * Replaces CIC_interp, MAS_library.pyx:558-599. */
+int pyl_cic_interp(pyl_ctx *ctx, const float *density, int dims, float BoxSize,
+                   const float *pos, int64_t n, float *den, int flags);
+
 /* ---- FFT (Pk_library.pyx:117-130) -------------------------------------
  * Forward, unnormalised r2c of a (dims,dims,dims) float32 cube into a
  * (dims,dims,dims/2+1) complex64 array (interleaved re,im).  delta is not
@@ -162,16 +168,17 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
            int dims, double BoxSize, int axis, float scale, float shift, int flags,
            pyl_pk_out *out);
 
-/* Batched Pk/XPk: nreal realisations of nfields fields each, stored as nreal*nfields
- * contiguous (dims,dims,dims) float32 cubes (realisation-major).  One batched rocFFT
- * and one binning launch for the whole batch.  scales/shifts: nreal*nfields values
- * for the prologue (NULL = none).  outs: nreal result structs (each sized as for
- * pyl_pk with nfields fields).  Each realisation's result is bitwise the same as a
- * single pyl_pk call on it.  Replaces a Python loop of Pk/XPk calls over
- * realisations (Pk_library.pyx:263-420, :529-793). */
-int pyl_pk_batch(pyl_ctx *ctx, const float *deltas, int nreal, int nfields, const int *mas,
-                 int dims, double BoxSize, int axis, const float *scales, const float *shifts,
-                 int flags, pyl_pk_out *outs);
+/* Batched Pk/XPk: nreal realisations of nfields fields.  deltas[f] points to nreal
+ * contiguous (dims,dims,dims) float32 cubes of field f (field-major, so each field can
+ * live in its own array).  One batched rocFFT and one binning launch for the whole
+ * batch.  scales/shifts: nreal*nfields values, [r*nfields + f], for the prologue
+ * (NULL = none).  outs: nreal result structs (each sized as for pyl_pk with nfields
+ * fields).  Each realisation's result is bitwise the same as a single pyl_pk call on
+ * it.  Replaces a Python loop of Pk/XPk calls over realisations
+ * (Pk_library.pyx:263-420, :529-793). */
+int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfields,
+                 const int *mas, int dims, double BoxSize, int axis, const float *scales,
+                 const float *shifts, int flags, pyl_pk_out *outs);
 
 /* Same binning pass on fields that are already in Fourier space:
  * delta_k[f] is (dims,dims,dims/2+1) complex64 (interleaved). */

@@ -207,6 +207,39 @@ void ora_ma_f64(const float *pos, long n, int coord, const float *W,
     }
 }
 
+/*
+ * CIC interpolation of a 3D grid at particle positions, the gather adjoint of CIC:
+ * library/MAS_library/MAS_library.pyx:558-599.  Float arithmetic, terms summed
+ * left to right in the reference's order.
+ */
+void ora_cic_interp(const float *density, int dims, float BoxSize, const float *pos, long n,
+                    float *den)
+{
+    long i;
+    float inv_cell_size = (float)dims / BoxSize;
+    for (i = 0; i < n; i++) {
+        float u[3], d[3];
+        int id[3], iu[3], axis;
+        for (axis = 0; axis < 3; axis++) {
+            float dist = pos[3 * i + axis] * inv_cell_size;
+            u[axis] = dist - (float)(int)dist;
+            d[axis] = (float)(1.0 - (double)u[axis]);
+            id[axis] = wrap_index((long)(int)dist, dims);
+            iu[axis] = wrap_index((long)id[axis] + 1, dims);
+        }
+#define G(a, b, c) density[((long)(a) * dims + (b)) * dims + (c)]
+        den[i] = G(id[0], id[1], id[2]) * d[0] * d[1] * d[2] +
+                 G(id[0], id[1], iu[2]) * d[0] * d[1] * u[2] +
+                 G(id[0], iu[1], id[2]) * d[0] * u[1] * d[2] +
+                 G(id[0], iu[1], iu[2]) * d[0] * u[1] * u[2] +
+                 G(iu[0], id[1], id[2]) * u[0] * d[1] * d[2] +
+                 G(iu[0], id[1], iu[2]) * u[0] * d[1] * u[2] +
+                 G(iu[0], iu[1], id[2]) * u[0] * u[1] * d[2] +
+                 G(iu[0], iu[1], iu[2]) * u[0] * u[1] * u[2];
+#undef G
+    }
+}
+
 /* Single-axis footprint, exposed so tests can pin the closed forms. */
 int ora_axis_weights(float x, float inv_cell_size, int dims, int mas,
                      int *idx_out, float *C_out)

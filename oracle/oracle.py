@@ -63,6 +63,7 @@ def lib():
         L.ora_mas_correction.restype = ctypes.c_double
         L.ora_pos_redshift_space.argtypes = [fp, fp, ctypes.c_long, ctypes.c_float,
                                              ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.ora_cic_interp.argtypes = [fp, ctypes.c_int, ctypes.c_float, fp, ctypes.c_long, fp]
         _LIB = L
     return _LIB
 
@@ -131,6 +132,16 @@ def ma_reference(pos, number, BoxSize, MAS='CIC', W=None, flavour='strict', thre
                                 _p(Wc, ctypes.c_float), pos.shape[0], number.shape[0],
                                 pos.shape[1], float(BoxSize), int(threads))
     return number
+
+
+def cic_interp(density, BoxSize, pos):
+    """MAS_library.CIC_interp (MAS_library.pyx:558-599): float32 gather."""
+    density = np.ascontiguousarray(density, dtype=np.float32)
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    den = np.zeros(pos.shape[0], np.float32)
+    lib().ora_cic_interp(_p(density, ctypes.c_float), density.shape[0], float(BoxSize),
+                         _p(pos, ctypes.c_float), pos.shape[0], _p(den, ctypes.c_float))
+    return den
 
 
 def pos_redshift_space(pos, vel, BoxSize, Hubble, redshift, axis):

@@ -238,6 +238,29 @@ def PCSWc2D(pos, number, W, BoxSize, threads):
     _ckernel(N.PCS, pos, number, BoxSize, W, 2)
 
 
+def CIC_interp(density, BoxSize, pos, den):
+    """CIC-interpolate a 3D float32 grid at positions, writing den in place
+    (MAS_library.pyx:558-599)."""
+    _check_f32(density, 3, 'density')
+    _check_f32(pos, 2, 'pos')
+    _check_f32(den, 1, 'den')
+    if N.is_device_array(density):
+        ctx = N.default_context()
+        with N.torch_stream(ctx, True):
+            N.check(N.lib().pyl_cic_interp(ctx.handle, N.device_ptr(density), density.shape[0],
+                                           float(BoxSize), N.device_ptr(pos), pos.shape[0],
+                                           N.device_ptr(den), N.PTR_DEVICE))
+        return
+    ctx = N.default_context()
+    d = np.ascontiguousarray(density)
+    p = np.ascontiguousarray(pos)
+    out = den if den.flags['C_CONTIGUOUS'] else np.empty_like(den)
+    N.check(N.lib().pyl_cic_interp(ctx.handle, d.ctypes.data, d.shape[0], float(BoxSize),
+                                   p.ctypes.data, p.shape[0], out.ctypes.data, 0))
+    if out is not den:
+        den[...] = out
+
+
 def MA_atomic(pos, number, BoxSize, MAS='CIC', W=None):
     """The plain GPU form: per-particle global float atomics (not bitwise reproducible).
     Same semantics as MA for 3D grids; used as the comparison baseline."""

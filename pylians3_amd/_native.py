@@ -82,12 +82,14 @@ EXPORTS = {
                    ctypes.c_int, ctypes.c_float, ctypes.c_int]),
     'pyl_fft_r2c': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                    ctypes.c_int]),
+    'pyl_cic_interp': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
+                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int]),
     'pyl_pk_get_sizes': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(PkSizes)]),
     'pyl_pk': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                               ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                               ctypes.POINTER(PkOut)]),
-    'pyl_pk_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+    'pyl_pk_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.POINTER(PkOut)]),

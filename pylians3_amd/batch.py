@@ -4,7 +4,8 @@
         particles of realisation r are pos[offsets[r]:offsets[r+1]]; grids is
         (nreal, d, d, d) float32; same result per grid as MAS_library.MA on it.
     Pk_batch(deltas, BoxSize, axis=2, MAS=('CIC',), scales=None, shifts=None)
-        deltas is (nreal, nfields, d, d, d) float32; returns one result per
+        deltas is a list of nfields (nreal, d, d, d) float32 arrays (or one
+        (nreal, nfields, d, d, d) array); returns one result per
         realisation with the attributes of Pk_library.Pk (nfields == 1) or
         Pk_library.XPk (nfields >= 2).  The optional prologue applies
         delta = delta/scale - shift per field in float32 before the FFT.
@@ -56,12 +57,33 @@ def MA_batch(pos, offsets, grids, BoxSize, MAS='CIC', W=None, accumulate=True, a
     return grids
 
 
+def _fields(deltas):
+    """List of F field arrays, each (nreal, d, d, d) contiguous float32.  Accepts such
+    a list, or one (nreal, F, d, d, d) array (split into per-field copies)."""
+    if isinstance(deltas, (list, tuple)):
+        fields = list(deltas)
+    else:
+        fields = [deltas[:, f] for f in range(deltas.shape[1])]
+    out = []
+    for a in fields:
+        if N.is_device_array(a):
+            out.append(a.contiguous())
+        else:
+            out.append(np.ascontiguousarray(a, dtype=np.float32))
+    return out
+
+
 def Pk_batch(deltas, BoxSize, axis=2, MAS=('CIC',), scales=None, shifts=None, destroy=False,
              ctx=None):
     ctx = ctx or N.default_context()
-    nreal, F, dims = deltas.shape[0], deltas.shape[1], deltas.shape[2]
+    fields = _fields(deltas)
+    F = len(fields)
+    nreal, dims = fields[0].shape[0], fields[0].shape[1]
     if len(MAS) != F:
         raise ValueError('one MAS per field')
+    for a in fields:
+        if tuple(a.shape) != (nreal, dims, dims, dims):
+            raise ValueError('every field must be (nreal, dims, dims, dims)')
     mas = (ctypes.c_int * F)(*[MAS_function(m) for m in MAS])
     allocs = [N.alloc_pk_out(dims, F) for _ in range(nreal)]
     outs = (N.PkOut * nreal)(*[a[0] for a in allocs])
@@ -70,16 +92,13 @@ def Pk_batch(deltas, BoxSize, axis=2, MAS=('CIC',), scales=None, shifts=None, de
     for a in (sc, sh):
         if a is not None and a.size != nreal * F:
             raise ValueError('scales/shifts need nreal*nfields values')
-    args = lambda ptr, flags: (ctx.handle, ptr, nreal, F, mas, dims, float(BoxSize), int(axis),
-                               None if sc is None else sc.ctypes.data,
-                               None if sh is None else sh.ctypes.data, flags, outs)
-    if N.is_device_array(deltas):
-        flags = N.PTR_DEVICE | (N.PK_DESTROY_INPUT if destroy else 0)
-        with N.torch_stream(ctx, True):
-            N.check(N.lib().pyl_pk_batch(*args(N.device_ptr(deltas), flags)))
-    else:
-        d = np.ascontiguousarray(deltas, dtype=np.float32)
-        N.check(N.lib().pyl_pk_batch(*args(d.ctypes.data, 0)))
+    dev = N.is_device_array(fields[0])
+    ptrs = (ctypes.c_void_p * F)(*[(N.device_ptr(a) if dev else a.ctypes.data) for a in fields])
+    flags = (N.PTR_DEVICE | (N.PK_DESTROY_INPUT if destroy else 0)) if dev else 0
+    with N.torch_stream(ctx, dev):
+        N.check(N.lib().pyl_pk_batch(ctx.handle, ptrs, nreal, F, mas, dims, float(BoxSize),
+                                     int(axis), None if sc is None else sc.ctypes.data,
+                                     None if sh is None else sh.ctypes.data, flags, outs))
     X = F * (F - 1) // 2
     results = []
     for _, arrs, S in allocs:

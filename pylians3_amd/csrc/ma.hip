@@ -40,12 +40,21 @@ __device__ __forceinline__ int node_add(int base, int j, int d) {
 
 // Per-axis footprint.  Writes the first node (wrapped into [0,dims)) and the F
 // weights; node j of the footprint is (base + j) mod dims.
+//
+// The reference takes floor() of (double)dist - 1.5 (TSC) / - 2.0 (PCS) and
+// truncates (double)dist + 0.5 (NGP).  With k = floorf(dist) and the exact
+// fraction f = dist - k (exact for any float), these equal k - 2 + (f >= 0.5),
+// k - 2 and k + (f >= 0.5) (for dist >= -0.5) in integer arithmetic, for every
+// |dist| < 2^31.
 template <int MAS>
 __device__ __forceinline__ int axis_foot(float x, float inv, int dims, float *C) {
     if constexpr (MAS == PYL_NGP) {
         // MAS_library.pyx:288-289: <int>(pos*inv + 0.5), the 0.5 is a double
         C[0] = 1.0f;
-        return wrapi((int)((double)(x * inv) + 0.5), dims);
+        const float v = x * inv;
+        if (!(v >= -0.5f)) return wrapi((int)((double)v + 0.5), dims);  // truncation toward 0
+        const float k = floorf(v);
+        return wrapi((int)k + ((v - k) >= 0.5f ? 1 : 0), dims);
     } else if constexpr (MAS == PYL_CIC) {
         // MAS_library.pyx:151-156: u = dist - <int>dist, d = 1.0 - u
         float dist = x * inv;
@@ -57,7 +66,8 @@ __device__ __forceinline__ int axis_foot(float x, float inv, int dims, float *C)
     } else if constexpr (MAS == PYL_TSC) {
         // MAS_library.pyx:392-399
         float dist = x * inv;
-        int minimum = (int)floor((double)dist - 1.5);
+        const float kf = floorf(dist);
+        int minimum = (int)kf - ((dist - kf) >= 0.5f ? 1 : 2);   // floor(dist - 1.5)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             float diff = fabsf((float)(minimum + j + 1) - dist);
@@ -77,7 +87,7 @@ __device__ __forceinline__ int axis_foot(float x, float inv, int dims, float *C)
     } else {
         // PCS, MAS_library.pyx:485-492 (double polynomial, rounded to float)
         float dist = x * inv;
-        int minimum = (int)floor((double)dist - 2.0);
+        int minimum = (int)floorf(dist) - 2;                       // floor(dist - 2.0)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             float diff = fabsf((float)(minimum + j + 1) - dist);
@@ -139,48 +149,49 @@ __device__ __forceinline__ int axis_tiles(int base, int d, int T, int *t) {
     return n;
 }
 
-// Wave-aggregated atomic add of 1 per active lane on counters[key].  Returns
-// each lane's slot (old value + rank among same-key lanes).  Called by the full
-// wave (converged); inactive lanes pass active=false.
-__device__ __forceinline__ unsigned long long agg_inc(unsigned *counters, long long key,
-                                                      bool active, bool want_slot) {
-    const int lane = __lane_id();
-    unsigned long long pend = __ballot(active);
-    unsigned long long slot = 0;
-    while (pend) {
-        int leader = __ffsll((unsigned long long)pend) - 1;
-        long long lk = __shfl(key, leader);
-        unsigned long long grp = __ballot(active && key == lk) & pend;
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&counters[lk], (unsigned)__popcll(grp));
-        if (want_slot) {
-            base = __shfl(base, leader);
-            if ((grp >> lane) & 1ull)
-                slot = (unsigned long long)base + __popcll(grp & ((1ull << lane) - 1ull));
-        }
-        pend &= ~grp;
-    }
-    return slot;
+// ---- counting sort by tile -------------------------------------------------
+// Each workgroup owns a contiguous chunk of particles and counts its (particle,
+// tile) copies in an LDS hash table (open addressing, linear probing, no
+// deletion).  At the end one global atomic per distinct key reserves the chunk's
+// range inside the tile's segment; the (key, base) table is kept in global
+// memory so the scatter pass can hand out slots with LDS atomics only.  Keys
+// that do not fit in the table (more than kHashCap distinct tiles in a chunk)
+// take per-group global atomics on a separate overflow counter.  Lanes of a
+// wave that share a key are grouped first, so coherent input (a lattice, a
+// sorted catalogue) costs one table operation per wave and distinct tile.
+constexpr int kSortBlock = 512;
+constexpr int kHashBits = 13;
+constexpr int kHashCap = 1 << kHashBits;      // 8192 entries, 64 KB of LDS (key + value)
+constexpr int kProbe = 32;
+constexpr unsigned kEmpty = 0xffffffffu;
+
+__device__ __forceinline__ unsigned hslot(unsigned key) {
+    return (key * 0x9E3779B1u) >> (32 - kHashBits);
 }
 
-// Count pass variant for weighted input: also raises wmax[key] to the largest
-// finite |W| bits of the lanes sharing the key (one atomicMax per group).
-__device__ __forceinline__ void agg_inc_max(unsigned *counters, unsigned *wmax, long long key,
-                                            bool active, unsigned wbits) {
-    const int lane = __lane_id();
-    unsigned long long pend = __ballot(active);
-    while (pend) {
-        int leader = __ffsll((unsigned long long)pend) - 1;
-        long long lk = __shfl(key, leader);
-        unsigned long long grp = __ballot(active && key == lk) & pend;
-        unsigned v = ((grp >> lane) & 1ull) ? wbits : 0u;
-        for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off));
-        if (lane == leader) {
-            atomicAdd(&counters[lk], (unsigned)__popcll(grp));
-            atomicMax(&wmax[lk], v);
+__device__ __forceinline__ int tab_insert(unsigned *tkey, unsigned *tval, unsigned key,
+                                          unsigned n) {
+    const unsigned h = hslot(key);
+    for (int p = 0; p < kProbe; ++p) {
+        const unsigned s = (h + p) & (kHashCap - 1);
+        const unsigned prev = atomicCAS(&tkey[s], kEmpty, key);
+        if (prev == kEmpty || prev == key) {
+            atomicAdd(&tval[s], n);
+            return (int)s;
         }
-        pend &= ~grp;
     }
+    return -1;
+}
+
+__device__ __forceinline__ int tab_find(const unsigned *tkey, unsigned key) {
+    const unsigned h = hslot(key);
+    for (int p = 0; p < kProbe; ++p) {
+        const unsigned s = (h + p) & (kHashCap - 1);
+        const unsigned k = tkey[s];
+        if (k == key) return (int)s;
+        if (k == kEmpty) return -1;
+    }
+    return -1;
 }
 
 template <int MAS>
@@ -203,56 +214,100 @@ __device__ __forceinline__ void particle_tiles(const float *pos, long long i, in
     }
 }
 
-// Pass 1: per-tile record counts and, with weights, the per-tile max finite |W|
-// (as float bits; non-finite weights only poison the nodes they touch, as in the
-// reference).  Per-tile bounds make a tile's quantum depend on its own records
-// only, so a grid's result does not depend on what else is in the batch.
+__device__ __forceinline__ void chunk_range(long long n, long long chunk, long long &c0,
+                                            long long &c1) {
+    c0 = (long long)blockIdx.x * chunk;
+    c1 = min(n, c0 + chunk);
+}
+
+// Pass 1: per-tile counts.  counts[key] receives the table-counted copies (and
+// hands each chunk its base), ocounts[key] the overflowed ones.
 template <int MAS>
-__global__ void __launch_bounds__(kBlock)
-tile_count_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
-                  int axes, float inv, TileGeom g, const long long *__restrict__ offs,
-                  unsigned *__restrict__ counts, unsigned *__restrict__ wmax_bits) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
-        long long i = base + threadIdx.x;
-        bool valid = i < n;
+__global__ void __launch_bounds__(kSortBlock)
+tile_count_kernel(const float *__restrict__ pos, long long n, int axes, float inv, TileGeom g,
+                  const long long *__restrict__ offs, long long chunk,
+                  unsigned *__restrict__ counts, unsigned *__restrict__ ocounts,
+                  uint2 *__restrict__ dump) {
+    __shared__ unsigned tkey[kHashCap];
+    __shared__ unsigned tval[kHashCap];
+    for (int s = threadIdx.x; s < kHashCap; s += blockDim.x) {
+        tkey[s] = kEmpty;
+        tval[s] = 0;
+    }
+    __syncthreads();
+    const int lane = __lane_id();
+    long long c0, c1;
+    chunk_range(n, chunk, c0, c1);
+    for (long long base = c0; base < c1; base += blockDim.x) {
+        const long long i = base + threadIdx.x;
+        const bool valid = i < c1;
         int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
         long long rbase = 0;
-        unsigned wb = 0;
         if (valid) {
             particle_tiles<MAS>(pos, i, axes, inv, g, tx, ty, tz, nx, ny, nz);
             if (g.nreal > 1) rbase = (long long)realisation_of(offs, g.nreal, i) * g.ntiles;
-            if (W) {
-                wb = __float_as_uint(fabsf(W[i]));
-                if (wb >= 0x7f800000u) wb = 0;
-            }
         }
-        int nc = valid ? nx * ny * nz : 0;
+        const int nc = valid ? nx * ny * nz : 0;
         for (int j = 0;; ++j) {
-            bool act = j < nc;
+            const bool act = j < nc;
             if (!__any(act)) break;
-            long long key = 0;
+            unsigned key = kEmpty;
             if (act) {
                 int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
-                key = rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
+                key = (unsigned)(rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c]);
             }
-            if (W) agg_inc_max(counts, wmax_bits, key, act, wb);
-            else agg_inc(counts, key, act, false);
+            unsigned long long pend = __ballot(act);
+            while (pend) {
+                const int leader = __ffsll((unsigned long long)pend) - 1;
+                const unsigned lk = __shfl(key, leader);
+                const unsigned long long grp = __ballot(act && key == lk) & pend;
+                if (lane == leader) {
+                    const unsigned cnt = (unsigned)__popcll(grp);
+                    if (tab_insert(tkey, tval, lk, cnt) < 0) atomicAdd(&ocounts[lk], cnt);
+                }
+                pend &= ~grp;
+            }
         }
+    }
+    __syncthreads();
+    uint2 *dmp = dump + (long long)blockIdx.x * kHashCap;
+    for (int s = threadIdx.x; s < kHashCap; s += blockDim.x) {
+        const unsigned k = tkey[s];
+        uint2 d = make_uint2(kEmpty, 0u);
+        if (k != kEmpty) {
+            d.x = k;
+            d.y = atomicAdd(&counts[k], tval[s]);
+        }
+        dmp[s] = d;
     }
 }
 
-// Pass 3: write one record (x, y, z, w) per (particle, overlapped tile).
+// Pass 3: write one record (x, y, z, w) per (particle, overlapped tile).  Record
+// index = tile offset + chunk base + LDS cursor (table keys), or tile offset +
+// table count + global overflow cursor.
 template <int MAS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kSortBlock)
 tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
                     int axes, float inv, TileGeom g, const long long *__restrict__ offs,
+                    long long chunk, const uint2 *__restrict__ dump,
                     const unsigned long long *__restrict__ offsets,
-                    unsigned *__restrict__ cursors, float4 *__restrict__ records) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
-        long long i = base + threadIdx.x;
-        bool valid = i < n;
+                    const unsigned *__restrict__ counts, unsigned *__restrict__ ocursors,
+                    float4 *__restrict__ records) {
+    __shared__ unsigned tkey[kHashCap];
+    __shared__ unsigned tcur[kHashCap];
+    const uint2 *dmp = dump + (long long)blockIdx.x * kHashCap;
+    for (int s = threadIdx.x; s < kHashCap; s += blockDim.x) {
+        const uint2 d = dmp[s];
+        tkey[s] = d.x;
+        tcur[s] = d.y;
+    }
+    __syncthreads();
+    const int lane = __lane_id();
+    long long c0, c1;
+    chunk_range(n, chunk, c0, c1);
+    for (long long base = c0; base < c1; base += blockDim.x) {
+        const long long i = base + threadIdx.x;
+        const bool valid = i < c1;
         int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
         long long rbase = 0;
         float4 rec = make_float4(0.f, 0.f, 0.f, 1.f);
@@ -264,17 +319,33 @@ tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, 
             rec.z = (axes == 3) ? pos[(long long)axes * i + 2] : 0.f;
             rec.w = W ? W[i] : 1.0f;
         }
-        int nc = valid ? nx * ny * nz : 0;
+        const int nc = valid ? nx * ny * nz : 0;
         for (int j = 0;; ++j) {
-            bool act = j < nc;
+            const bool act = j < nc;
             if (!__any(act)) break;
-            long long key = 0;
+            unsigned key = kEmpty;
             if (act) {
                 int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
-                key = rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c];
+                key = (unsigned)(rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c]);
             }
-            unsigned long long slot = agg_inc(cursors, key, act, true);
-            if (act) records[offsets[key] + slot] = rec;
+            unsigned long long pend = __ballot(act);
+            unsigned long long slot = 0;
+            while (pend) {
+                const int leader = __ffsll((unsigned long long)pend) - 1;
+                const unsigned lk = __shfl(key, leader);
+                const unsigned long long grp = __ballot(act && key == lk) & pend;
+                unsigned long long gb = 0;
+                if (lane == leader) {
+                    const unsigned cnt = (unsigned)__popcll(grp);
+                    const int s = tab_find(tkey, lk);
+                    if (s >= 0) gb = offsets[lk] + atomicAdd(&tcur[s], cnt);
+                    else gb = offsets[lk] + counts[lk] + atomicAdd(&ocursors[lk], cnt);
+                }
+                gb = __shfl(gb, leader);
+                if ((grp >> lane) & 1ull) slot = gb + __popcll(grp & ((1ull << lane) - 1ull));
+                pend &= ~grp;
+            }
+            if (act) records[slot] = rec;
         }
     }
 }
@@ -303,14 +374,15 @@ __device__ __forceinline__ unsigned long long block_exclusive_scan(unsigned long
     return incl - v;
 }
 
+// The scanned value of element i is in[i] + in2[i].
 __global__ void __launch_bounds__(kScanBlock)
-scan_partial_kernel(const unsigned *__restrict__ in, long long n,
-                    unsigned long long *__restrict__ block_sums) {
+scan_partial_kernel(const unsigned *__restrict__ in, const unsigned *__restrict__ in2,
+                    long long n, unsigned long long *__restrict__ block_sums) {
     __shared__ unsigned long long sh[kScanBlock];
     long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
     unsigned long long s = 0;
     for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) s += in[base + k];
+        if (base + k < n) s += (unsigned long long)in[base + k] + in2[base + k];
     unsigned long long total;
     block_exclusive_scan(s, sh, total);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
@@ -338,7 +410,7 @@ scan_blocksums_kernel(unsigned long long *__restrict__ block_sums, int nblocks) 
 }
 
 __global__ void __launch_bounds__(kScanBlock)
-scan_final_kernel(const unsigned *__restrict__ in, long long n,
+scan_final_kernel(const unsigned *__restrict__ in, const unsigned *__restrict__ in2, long long n,
                   const unsigned long long *__restrict__ block_prefix,
                   unsigned long long *__restrict__ out) {
     __shared__ unsigned long long sh[kScanBlock];
@@ -346,7 +418,7 @@ scan_final_kernel(const unsigned *__restrict__ in, long long n,
     unsigned long long v[kScanItems];
     unsigned long long s = 0;
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < n) ? in[base + k] : 0ull;
+        v[k] = (base + k < n) ? (unsigned long long)in[base + k] + in2[base + k] : 0ull;
         s += v[k];
     }
     unsigned long long total;
@@ -364,10 +436,10 @@ template <int MAS, int AXES>
 __global__ void __launch_bounds__(kBlock)
 tile_deposit_kernel(const float4 *__restrict__ records,
                     const unsigned long long *__restrict__ offsets, float inv, TileGeom g,
-                    const unsigned *__restrict__ wmax_bits, float weight_factor,
-                    float *__restrict__ grid, int overwrite) {
+                    int weighted, float weight_factor, float *__restrict__ grid, int overwrite) {
     constexpr int F = MAS;
     extern __shared__ double acc[];
+    __shared__ unsigned wred[kBlock / 64];
     const long long key = blockIdx.x;                 // r * ntiles + tile
     const long long real = key / g.ntiles;
     const long long tile = key - real * g.ntiles;
@@ -396,56 +468,65 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     }
     for (int c = threadIdx.x; c < ncell; c += blockDim.x) acc[c] = 0.0;
 
-    // Quantum 2^q: every |contribution| <= weight_factor*max|W| (B-spline weights
-    // are <= 1), so sum |c| <= count*wmax*factor < 2^e and 2^e/2^q = 2^52.
-    const double wmax =
-        (wmax_bits ? (double)__uint_as_float(wmax_bits[key]) : 1.0) * (double)weight_factor;
-    const double bound = (double)(r1 - r0) * wmax;
-    int e = 0;
-    if (bound > 0.0 && bound <= 1.0e300) frexp(bound, &e);
-    else e = -1000;
+    // Per-tile bound on |contribution|: B-spline weights are <= 1, so every
+    // |c| <= weight_factor * max finite |W| of the tile's records.
+    float wmax = 1.0f;
+    if (weighted) {
+        unsigned m = 0;
+        for (unsigned long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+            unsigned b = __float_as_uint(fabsf(records[r].w));
+            if (b < 0x7f800000u) m = max(m, b);
+        }
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+        if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = m;
+        __syncthreads();
+        m = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = max(m, wred[i]);
+        wmax = __uint_as_float(m);
+    }
+    // Quantum 2^q: sum |c| <= count*wmax*factor < 2^e and 2^e/2^q = 2^52, so all
+    // partial sums of rint(c/2^q) are integers below 2^53: exact in float64.
+    const double bound = (double)(r1 - r0) * (double)wmax * (double)weight_factor;
+    int e = -1000;
+    if (bound > 0.0) frexp(bound, &e);
     int q = e - 52;
-    if (q < -1020) q = -1020;
-    const double to_q = ldexp(1.0, -q);
+    if (q < -127) q = -127;                 // keep 2^-q a normal float
+    const float to_q = ldexpf(1.0f, -q);
     const double from_q = ldexp(1.0, q);
     __syncthreads();
 
     for (unsigned long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        float4 rec = records[r];
+        const float4 rec = records[r];
         float Cx[4], Cy[4], Cz[4];
-        int bx = axis_foot<MAS>(rec.x, inv, g.dims, Cx);
-        int by = axis_foot<MAS>(rec.y, inv, g.dims, Cy);
+        const int bx = axis_foot<MAS>(rec.x, inv, g.dims, Cx);
+        const int by = axis_foot<MAS>(rec.y, inv, g.dims, Cy);
         int bz = 0;
         if (AXES == 3) bz = axis_foot<MAS>(rec.z, inv, g.dims, Cz);
-        const float w = rec.w * weight_factor;
-        const bool weighted = w != 1.0f;
+        // ((Cx*Cy)*Cz)*(W*2^-q) == 2^-q * (((Cx*Cy)*Cz)*W) exactly (power-of-two
+        // scaling commutes with float rounding), so the scaled product is the
+        // reference's float contribution in units of the quantum.
+        const float w = (rec.w * weight_factor) * to_q;
 #pragma unroll
         for (int l = 0; l < F; ++l) {
-            const int nx = node_add(bx, l, g.dims);
-            int lx = nx - ox;
+            const int lx = node_add(bx, l, g.dims) - ox;
             if (lx < 0 || lx >= ex) continue;
 #pragma unroll
             for (int m = 0; m < F; ++m) {
-                const int ny = node_add(by, m, g.dims);
-                int ly = ny - oy;
+                const int ly = node_add(by, m, g.dims) - oy;
                 if (ly < 0 || ly >= ey) continue;
-                float cxy = Cx[l] * Cy[m];
+                const float cxy = Cx[l] * Cy[m];
                 if (AXES == 3) {
+                    const int row = (lx * ey + ly) * ez;
 #pragma unroll
                     for (int k = 0; k < F; ++k) {
-                        const int nz = node_add(bz, k, g.dims);
-                        int lz = nz - oz;
+                        const int lz = node_add(bz, k, g.dims) - oz;
                         if (lz < 0 || lz >= ez) continue;
-                        float c = cxy * Cz[k];
-                        if (weighted) c = c * w;
-                        double qv = rint((double)c * to_q);
-                        atomicAdd(&acc[(lx * ey + ly) * ez + lz], qv);
+                        const float c = (cxy * Cz[k]) * w;
+                        atomicAdd(&acc[row + lz], (double)rintf(c));
                     }
                 } else {
-                    float c = cxy;          // the reference multiplies by C[2][n] = 1.0
-                    if (weighted) c = c * w;
-                    double qv = rint((double)c * to_q);
-                    atomicAdd(&acc[lx * ey + ly], qv);
+                    const float c = cxy * w;   // the reference multiplies by C[2][n] = 1.0
+                    atomicAdd(&acc[lx * ey + ly], (double)rintf(c));
                 }
             }
         }
@@ -498,6 +579,34 @@ atomic_deposit_kernel(const float *__restrict__ pos, const float *__restrict__ W
                 }
             }
         }
+    }
+}
+
+// CIC interpolation (gather), MAS_library.pyx:558-599: one thread per position,
+// the 8 products summed left to right in the reference's order.
+__global__ void __launch_bounds__(kBlock)
+cic_interp_kernel(const float *__restrict__ density, int dims, float inv,
+                  const float *__restrict__ pos, long long n, float *__restrict__ den) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float Cx[4], Cy[4], Cz[4];
+        const int bx = axis_foot<PYL_CIC>(pos[3 * i + 0], inv, dims, Cx);
+        const int by = axis_foot<PYL_CIC>(pos[3 * i + 1], inv, dims, Cy);
+        const int bz = axis_foot<PYL_CIC>(pos[3 * i + 2], inv, dims, Cz);
+        const long long x0 = bx, x1 = node_add(bx, 1, dims);
+        const long long y0 = by, y1 = node_add(by, 1, dims);
+        const long long z0 = bz, z1 = node_add(bz, 1, dims);
+        const long long D = dims;
+        auto G = [&](long long a, long long b, long long c) { return density[(a * D + b) * D + c]; };
+        float s = G(x0, y0, z0) * Cx[0] * Cy[0] * Cz[0];
+        s = s + G(x0, y0, z1) * Cx[0] * Cy[0] * Cz[1];
+        s = s + G(x0, y1, z0) * Cx[0] * Cy[1] * Cz[0];
+        s = s + G(x0, y1, z1) * Cx[0] * Cy[1] * Cz[1];
+        s = s + G(x1, y0, z0) * Cx[1] * Cy[0] * Cz[0];
+        s = s + G(x1, y0, z1) * Cx[1] * Cy[0] * Cz[1];
+        s = s + G(x1, y1, z0) * Cx[1] * Cy[1] * Cz[0];
+        s = s + G(x1, y1, z1) * Cx[1] * Cy[1] * Cz[1];
+        den[i] = s;
     }
 }
 
@@ -557,33 +666,38 @@ int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, in
     PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
     const long long *d_offs = nullptr;
     PYL_TRY(upload_offsets(c, h_offs, nreal, &d_offs));
-    // tile buffer: counts[nt] | cursors[nt] | wmax[nt] (weighted only) | pad |
-    //              offsets[nt+1] | block sums
-    const size_t nwords = (size_t)nt * (W ? 3 : 2);
-    size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
-    size_t off_bytes = ((size_t)nt + 1) * sizeof(unsigned long long);
-    size_t bs_bytes = (size_t)nscan_blocks * sizeof(unsigned long long);
-    PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes));
+    // one chunk of particles per sort workgroup
+    const int nchunk = (int)std::max(1LL, std::min(512LL, (n + 65535) / 65536));
+    const long long chunk = (n + nchunk - 1) / nchunk;
+    // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | pad | offsets[nt+1] |
+    //              block sums | per-chunk tables
+    const size_t nwords = (size_t)nt * 3;
+    const size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
+    const size_t off_bytes = (((size_t)nt + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255;
+    const size_t bs_bytes = ((size_t)nscan_blocks * sizeof(unsigned long long) + 255) & ~(size_t)255;
+    const size_t dump_bytes = (size_t)nchunk * kHashCap * sizeof(uint2);
+    PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + dump_bytes));
     char *tb = (char *)c->bufs[BUF_TILE_CNT].ptr;
     unsigned *counts = (unsigned *)tb;
-    unsigned *cursors = counts + nt;
-    unsigned *wmax = W ? cursors + nt : nullptr;   // NULL: unit weights
+    unsigned *ocounts = counts + nt;
+    unsigned *ocursors = ocounts + nt;
     unsigned long long *offsets = (unsigned long long *)(tb + cnt_bytes);
-    unsigned long long *bsums = offsets + nt + 1;
+    unsigned long long *bsums = (unsigned long long *)(tb + cnt_bytes + off_bytes);
+    uint2 *dump = (uint2 *)(tb + cnt_bytes + off_bytes + bs_bytes);
 
     PYL_HIP(hipMemsetAsync(counts, 0, nwords * sizeof(unsigned), c->stream));
-    const int nb = grid_blocks(n, kBlock);
     if (n > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
-        tile_count_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, d_offs,
-                                                             counts, wmax);
+        tile_count_kernel<MAS><<<nchunk, kSortBlock, 0, c->stream>>>(pos, n, axes, inv, g, d_offs,
+                                                                     chunk, counts, ocounts, dump);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
     }
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
-    scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, nt, bsums);
+    scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums);
     scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
-    scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, nt, bsums, offsets);
+    scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
+                                                                   offsets);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
     unsigned long long total = 0;
@@ -593,19 +707,20 @@ int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, in
     float4 *records = (float4 *)c->bufs[BUF_RECORDS].ptr;
     if (n > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
-        tile_scatter_kernel<MAS><<<nb, kBlock, 0, c->stream>>>(pos, W, n, axes, inv, g, d_offs,
-                                                               offsets, cursors, records);
+        tile_scatter_kernel<MAS><<<nchunk, kSortBlock, 0, c->stream>>>(
+            pos, W, n, axes, inv, g, d_offs, chunk, dump, offsets, counts, ocursors, records);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
     }
     const size_t lds = (size_t)g.T[0] * g.T[1] * g.T[2] * sizeof(double);
+    const int weighted = W != nullptr;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_DEPOSIT));
     if (axes == 3)
         tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kBlock, lds, c->stream>>>(
-            records, offsets, inv, g, wmax, weight_factor, grid, overwrite);
+            records, offsets, inv, g, weighted, weight_factor, grid, overwrite);
     else
         tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kBlock, lds, c->stream>>>(
-            records, offsets, inv, g, wmax, weight_factor, grid, overwrite);
+            records, offsets, inv, g, weighted, weight_factor, grid, overwrite);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));
     return PYL_OK;
@@ -741,6 +856,34 @@ extern "C" int pyl_ma_batch(pyl_ctx *ctx, const float *pos, const int64_t *offse
     static_assert(sizeof(int64_t) == sizeof(long long), "int64_t");
     return ma_entry(ctx, pos, offsets[nreal], (const long long *)offsets, nreal, axes, W, grids,
                     dims, BoxSize, mas, flags);
+}
+
+extern "C" int pyl_cic_interp(pyl_ctx *ctx, const float *density, int dims, float BoxSize,
+                              const float *pos, int64_t n, float *den, int flags) {
+    PYL_CHECK(ctx && density && den && (pos || n == 0), PYL_EINVAL, "NULL argument");
+    PYL_CHECK(dims >= 1 && n >= 0 && BoxSize > 0.0f, PYL_EINVAL, "bad arguments");
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipSetDevice(c->device));
+    if (n == 0) return PYL_OK;
+    const float inv = (float)dims / BoxSize;
+    const long long ncell = (long long)dims * dims * dims;
+    const float *dd = density, *dp = pos;
+    float *dden = den;
+    if (!(flags & PYL_PTR_DEVICE)) {
+        PYL_TRY(ensure(c, BUF_GRID, (size_t)ncell * sizeof(float)));
+        PYL_TRY(ensure(c, BUF_POS, (size_t)n * 3 * sizeof(float)));
+        PYL_TRY(ensure(c, BUF_W, (size_t)n * sizeof(float)));
+        dd = (const float *)c->bufs[BUF_GRID].ptr;
+        dp = (const float *)c->bufs[BUF_POS].ptr;
+        dden = (float *)c->bufs[BUF_W].ptr;
+        PYL_HIP(hipMemcpyAsync((void *)dd, density, (size_t)ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        PYL_HIP(hipMemcpyAsync((void *)dp, pos, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    cic_interp_kernel<<<grid_blocks(n, kBlock), kBlock, 0, c->stream>>>(dd, dims, inv, dp, n, dden);
+    PYL_HIP(hipGetLastError());
+    if (!(flags & PYL_PTR_DEVICE))
+        PYL_HIP(hipMemcpyAsync(den, dden, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return finish(c, flags);
 }
 
 // ---- MAS_c.h:3-10 compatible entry points --------------------------------

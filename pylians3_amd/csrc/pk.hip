@@ -454,13 +454,33 @@ struct ReduceArgs {
     long long nb2, n1;       // out2 / out1 elements per realisation (nb2*S, (m+1)*S)
 };
 
-__global__ void pk_reduce3_kernel(ReduceArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n3) return;
-    const double *part = a.partials + (long long)blockIdx.y * a.nitems * a.P;
+// Columns summed over every item (3D bins; the 1D line for axis 2) are reduced
+// in two fixed stages: chunks of kRedChunk items, then the chunk sums in order.
+constexpr int kRedChunk = 64;
+
+// tmp[r][ch][c] = sum over items of chunk ch of partials[r][item][col0 + c]
+__global__ void pk_reduce_chunks_kernel(const double *__restrict__ partials, int nitems, int P,
+                                        int col0, int ncol, double *__restrict__ tmp) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    const int ch = blockIdx.y, nch = gridDim.y;
+    const double *part = partials + (long long)blockIdx.z * nitems * P + col0 + c;
+    const int i0 = ch * kRedChunk, i1 = min(nitems, i0 + kRedChunk);
     double s = 0.0;
-    for (int it = 0; it < a.nitems; ++it) s += part[(long long)it * a.P + i];
-    a.out3[(long long)blockIdx.y * a.n3 + i] = s;
+#pragma unroll 8
+    for (int it = i0; it < i1; ++it) s += part[(long long)it * P];
+    tmp[((long long)blockIdx.z * nch + ch) * ncol + c] = s;
+}
+
+// out[r*ostride + c] = sum over chunks (in order) of tmp[r][ch][c]
+__global__ void pk_reduce_final_kernel(const double *__restrict__ tmp, int nch, int ncol,
+                                       double *__restrict__ out, long long ostride) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    const double *t = tmp + (long long)blockIdx.y * nch * ncol + c;
+    double s = 0.0;
+    for (int ch = 0; ch < nch; ++ch) s += t[(long long)ch * ncol];
+    out[(long long)blockIdx.y * ostride + c] = s;
 }
 
 // One thread per (key, line index, spectrum): sums the key's items.
@@ -481,23 +501,18 @@ __global__ void pk_reduce2_kernel(ReduceArgs a) {
     a.out2[(long long)blockIdx.y * a.nb2 + idx2 * a.S + sp] = s;
 }
 
+// Axes 0/1: 1D bin = key = k_par; sums the key's items.  (Axis 2 keeps one 1D line
+// per item and goes through the chunked reduction.)
 __global__ void pk_reduce1_kernel(ReduceArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nkeys * a.S) return;
     const double *part = a.partials + (long long)blockIdx.y * a.nitems * a.P;
     double *out1 = a.out1 + (long long)blockIdx.y * a.n1;
-    if (a.axis == 2) {
-        if (i >= a.L1 * a.S) return;   // L1 = m+1 bins (k_par = kz), all items
-        double s = 0.0;
-        for (int it = 0; it < a.nitems; ++it) s += part[(long long)it * a.P + a.off1 + i];
-        out1[i] = s;
-    } else {
-        if (i >= a.nkeys * a.S) return;  // bin = key = k_par
-        int key = i / a.S, sp = i % a.S;
-        double s = 0.0;
-        for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
-            s += part[(long long)it * a.P + a.off1 + sp];
-        out1[key * a.S + sp] = s;
-    }
+    int key = i / a.S, sp = i % a.S;
+    double s = 0.0;
+    for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
+        s += part[(long long)it * a.P + a.off1 + sp];
+    out1[key * a.S + sp] = s;
 }
 
 // FFT prologue: out = (in / scale) - shift in float32 (the reference caller's
@@ -590,7 +605,11 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     // scratch: partials + reduced outputs, per realisation
     const size_t part_elems = (size_t)bp->nitems * P * nreal;
     const size_t out3 = (size_t)n3, out2 = (size_t)p.nb2 * S, out1 = (size_t)(p.m + 1) * S;
-    PYL_TRY(ensure(c, BUF_BINS, sizeof(double) * (part_elems + (out3 + out2 + out1) * nreal)));
+    const int nch = ceil_div(bp->nitems, kRedChunk);
+    const int ncol_all = (AXIS == 2) ? std::max(n3, L1 * S) : n3;
+    const size_t tmp_elems = (size_t)nch * ncol_all * nreal;
+    PYL_TRY(ensure(c, BUF_BINS, sizeof(double) * (part_elems + (out3 + out2 + out1) * nreal +
+                                                 tmp_elems)));
     double *partials = (double *)c->bufs[BUF_BINS].ptr;
     double *d3 = partials + part_elems;
     double *d2 = d3 + out3 * nreal;
@@ -618,11 +637,22 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     r.nitems = bp->nitems; r.nkeys = bp->nkeys; r.n3 = n3; r.L2 = L2; r.L1 = L1; r.S = S;
     r.P = P; r.off2 = off2; r.off1 = off1; r.axis = AXIS; r.kmax_par = p.kmax_par;
     r.nb2 = (long long)out2; r.n1 = (long long)out1;
-    pk_reduce3_kernel<<<dim3(ceil_div(n3, 256), nreal), 256, 0, c->stream>>>(r);
+    double *tmp = d1 + out1 * nreal;
+    // 3D bins: every item contributes
+    pk_reduce_chunks_kernel<<<dim3(ceil_div(n3, 256), nch, nreal), 256, 0, c->stream>>>(
+        partials, bp->nitems, P, 0, n3, tmp);
+    pk_reduce_final_kernel<<<dim3(ceil_div(n3, 256), nreal), 256, 0, c->stream>>>(
+        tmp, nch, n3, d3, (long long)out3);
     long long n2 = (long long)bp->nkeys * L2 * S;
     pk_reduce2_kernel<<<dim3((unsigned)((n2 + 255) / 256), nreal), 256, 0, c->stream>>>(r);
-    int n1 = (AXIS == 2) ? L1 * S : bp->nkeys * S;
-    pk_reduce1_kernel<<<dim3(ceil_div(n1, 256), nreal), 256, 0, c->stream>>>(r);
+    if (AXIS == 2) {   // 1D bins are k_par = kz: every item contributes
+        pk_reduce_chunks_kernel<<<dim3(ceil_div(L1 * S, 256), nch, nreal), 256, 0, c->stream>>>(
+            partials, bp->nitems, P, off1, L1 * S, tmp);
+        pk_reduce_final_kernel<<<dim3(ceil_div(L1 * S, 256), nreal), 256, 0, c->stream>>>(
+            tmp, nch, L1 * S, d1, (long long)out1);
+    } else {
+        pk_reduce1_kernel<<<dim3(ceil_div(bp->nkeys * S, 256), nreal), 256, 0, c->stream>>>(r);
+    }
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_PK_BIN));
     std::vector<double> h3(out3 * nreal), h2(out2 * nreal), h1(out1 * nreal);
@@ -909,38 +939,42 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
     return finish(c, flags);
 }
 
-int pyl_pk_batch(pyl_ctx *ctx, const float *deltas, int nreal, int nfields, const int *mas,
-                 int dims, double BoxSize, int axis, const float *scales, const float *shifts,
-                 int flags, pyl_pk_out *outs) {
+int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfields,
+                 const int *mas, int dims, double BoxSize, int axis, const float *scales,
+                 const float *shifts, int flags, pyl_pk_out *outs) {
     PYL_CHECK(ctx && deltas && mas && outs, PYL_EINVAL, "NULL argument");
     PYL_CHECK(nreal >= 1 && nfields >= 1 && (long long)nreal * nfields <= 65535, PYL_EINVAL,
               "need 1 <= nreal*nfields <= 65535");
     PYL_CHECK(dims >= 2 && axis >= 0 && axis <= 2 && BoxSize > 0.0, PYL_EINVAL, "bad arguments");
-    for (int f = 0; f < nfields; ++f)
+    for (int f = 0; f < nfields; ++f) {
+        PYL_CHECK(deltas[f] != nullptr, PYL_EINVAL, "field %d is NULL", f);
         PYL_CHECK(mas[f] >= 0 && mas[f] <= 4, PYL_EINVAL, "bad MAS id %d", mas[f]);
+    }
     Ctx *c = &ctx->c;
     PYL_HIP(hipSetDevice(c->device));
     reset_stages(c);
     const long long n = (long long)dims * dims * dims;
     const long long nk = (long long)dims * dims * (dims / 2 + 1);
-    const int ncube = nreal * nfields;
+    const int ncube = nreal * nfields;                 // cube (f, r) at index f*nreal + r
     const bool dev = (flags & PYL_PTR_DEVICE) != 0;
     const bool destroy = dev && (flags & PYL_PK_DESTROY_INPUT);
     const bool apply = scales != nullptr || shifts != nullptr;
     PYL_TRY(ensure(c, BUF_FFT_OUT, (size_t)nk * ncube * sizeof(float2)));
     float2 *dk = (float2 *)c->bufs[BUF_FFT_OUT].ptr;
-    float *in = (float *)deltas;
+    float *in = nullptr;
     if (!destroy) {
         PYL_TRY(ensure(c, BUF_FFT_IN, (size_t)n * ncube * sizeof(float)));
         in = (float *)c->bufs[BUF_FFT_IN].ptr;
     }
     float *dsc = nullptr, *dsh = nullptr;
     if (apply) {
+        // caller order [r][f] -> cube order [f][r]
         std::vector<float> sc(ncube, 1.0f), sh(ncube, 0.0f);
-        for (int i = 0; i < ncube; ++i) {
-            if (scales) sc[i] = scales[i];
-            if (shifts) sh[i] = shifts[i];
-        }
+        for (int r = 0; r < nreal; ++r)
+            for (int f = 0; f < nfields; ++f) {
+                if (scales) sc[(size_t)f * nreal + r] = scales[(size_t)r * nfields + f];
+                if (shifts) sh[(size_t)f * nreal + r] = shifts[(size_t)r * nfields + f];
+            }
         PYL_TRY(ensure(c, BUF_SCAN_TMP, sizeof(float) * 2 * (size_t)ncube));
         dsc = (float *)c->bufs[BUF_SCAN_TMP].ptr;
         dsh = dsc + ncube;
@@ -948,21 +982,29 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *deltas, int nreal, int nfields, cons
         PYL_HIP(hipMemcpyAsync(dsh, sh.data(), sizeof(float) * ncube, hipMemcpyHostToDevice, c->stream));
         PYL_HIP(hipStreamSynchronize(c->stream));   // sc/sh are host temporaries
     }
+    const int nbx = (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048));
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_PREP));
-    if (!destroy)
-        PYL_HIP(hipMemcpyAsync(in, deltas, (size_t)n * ncube * sizeof(float),
-                               dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-    if (apply) {
-        const int nbx = (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048));
-        prologue_kernel<<<dim3(nbx, ncube), 256, 0, c->stream>>>(in, in, n, dsc, dsh, 1.0f, 0.0f, 1);
-        PYL_HIP(hipGetLastError());
+    for (int f = 0; f < nfields; ++f) {
+        float *dst = destroy ? (float *)deltas[f] : in + (size_t)f * nreal * n;
+        if (!destroy)
+            PYL_HIP(hipMemcpyAsync(dst, deltas[f], (size_t)n * nreal * sizeof(float),
+                                   dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+        if (apply) {
+            prologue_kernel<<<dim3(nbx, nreal), 256, 0, c->stream>>>(
+                dst, dst, n, dsc + (size_t)f * nreal, dsh + (size_t)f * nreal, 1.0f, 0.0f, 1);
+            PYL_HIP(hipGetLastError());
+        }
     }
     PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
-    PYL_TRY(fft_device(c, in, dk, dims, ncube));
+    if (destroy) {
+        for (int f = 0; f < nfields; ++f)
+            PYL_TRY(fft_device(c, (float *)deltas[f], dk + (size_t)f * nreal * nk, dims, nreal));
+    } else {
+        PYL_TRY(fft_device(c, in, dk, dims, ncube));
+    }
     std::vector<const float2 *> dks(nfields);
-    for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nk;
-    PYL_TRY(pk_bin_device(c, dks.data(), (long long)nfields * nk, nreal, nfields, mas, dims,
-                          BoxSize, axis, flags, outs));
+    for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nreal * nk;
+    PYL_TRY(pk_bin_device(c, dks.data(), nk, nreal, nfields, mas, dims, BoxSize, axis, flags, outs));
     return finish(c, flags);
 }
 

@@ -187,7 +187,7 @@ class Server:
         return N.device_ptr(a) if N.is_device_array(a) else a.ctypes.data
 
     def _grid_fields(self, ctx, reqs, grids):
-        """MA of every field of every request into grids (nreal, F, d, d, d), overwrite."""
+        """MA of field f of every request into grids[f] (nreal, d, d, d), overwrite."""
         import torch
         d = reqs[0].dims
         F = reqs[0].nfields
@@ -197,20 +197,18 @@ class Server:
             Ws = [r.W if f == 0 else r.W2 for r in reqs]
             counts = [p.shape[0] for p in pos]
             offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-            if len(reqs) == 1:
-                P, Wc = pos[0], Ws[0]
-            else:
-                P = torch.cat([torch.as_tensor(p, device='cuda') for p in pos])
-                Wc = None if Ws[0] is None else torch.cat([torch.as_tensor(w, device='cuda') for w in Ws])
-            P = torch.as_tensor(P, device='cuda').contiguous()
-            Wc = None if Wc is None else torch.as_tensor(Wc, device='cuda').contiguous()
-            target = grids[:, f].contiguous() if F > 1 else grids[:, 0]
+            P = torch.cat([torch.as_tensor(p, device='cuda') for p in pos]) if len(reqs) > 1 \
+                else torch.as_tensor(pos[0], device='cuda')
+            P = P.contiguous()
+            Wc = None
+            if Ws[0] is not None:
+                Wc = torch.cat([torch.as_tensor(w, device='cuda') for w in Ws]) if len(reqs) > 1 \
+                    else torch.as_tensor(Ws[0], device='cuda')
+                Wc = Wc.contiguous()
             N.check(N.lib().pyl_ma_batch(ctx.handle, N.device_ptr(P), offs.ctypes.data, len(reqs), 3,
                                          None if Wc is None else N.device_ptr(Wc),
-                                         N.device_ptr(target), d, reqs[0].BoxSize, mas,
+                                         N.device_ptr(grids[f]), d, reqs[0].BoxSize, mas,
                                          N.PTR_DEVICE | N.MA_OVERWRITE))
-            if F > 1:
-                grids[:, f].copy_(target)
         # mean density per field as the reference caller computes it (Pk_snapshot.py:77-83):
         # len(pos)/dims**3, or sum(M)/dims**3 with weights (float64 sum per request)
         means = np.zeros((len(reqs), F), np.float64)
@@ -227,7 +225,8 @@ class Server:
         d = reqs[0].dims
         F = reqs[0].nfields
         with torch.cuda.stream(stream):
-            grids = torch.empty((len(reqs), F, d, d, d), dtype=torch.float32, device='cuda')
+            grids = [torch.empty((len(reqs), d, d, d), dtype=torch.float32, device='cuda')
+                     for _ in range(F)]
             means = self._grid_fields(ctx, reqs, grids)
             means[means == 0] = 1.0
             mas = (ctypes.c_int * F)(*([MAS_function(reqs[0].MAS)] * F))
@@ -235,7 +234,8 @@ class Server:
             outs = (N.PkOut * len(reqs))(*[a[0] for a in allocs])
             sc = np.ascontiguousarray(means, dtype=np.float32)
             sh = np.ones_like(sc)
-            N.check(N.lib().pyl_pk_batch(ctx.handle, N.device_ptr(grids), len(reqs), F, mas, d,
+            ptrs = (ctypes.c_void_p * F)(*[N.device_ptr(g) for g in grids])
+            N.check(N.lib().pyl_pk_batch(ctx.handle, ptrs, len(reqs), F, mas, d,
                                          reqs[0].BoxSize, reqs[0].axis, sc.ctypes.data,
                                          sh.ctypes.data, N.PTR_DEVICE | N.PK_DESTROY_INPUT, outs))
             stream.synchronize()

@@ -96,6 +96,50 @@ def zeldovich(d, box, seed, f_rsd=0.0, rsd_axis=0, device='cuda', chunk_planes=N
     return pos
 
 
+def displacement_field(d, box, seed, device='cuda'):
+    """Zel'dovich displacement psi on a d^3 grid: (3, d, d, d) float32 (Mpc/h)."""
+    import torch
+    g = torch.Generator(device=device).manual_seed(int(seed))
+    w = torch.randn((d, d, d), generator=g, device=device, dtype=torch.float32)
+    wk = torch.fft.rfftn(w)
+    del w
+    k1, kz = _kgrid(d, box, device)
+    A = _sigma8_amplitude()
+    kx, ky, kzz = k1.view(d, 1, 1), k1.view(1, d, 1), kz.view(1, 1, d // 2 + 1)
+    k2 = kx * kx + ky * ky + kzz * kzz
+    k = torch.sqrt(k2)
+    k[0, 0, 0] = 1.0
+    amp = torch.sqrt(A * k ** -1.5 * torch.exp(-(k / 0.5) ** 2) * (d ** 3) / box ** 3)
+    amp[0, 0, 0] = 0.0
+    dk = wk * amp
+    k2[0, 0, 0] = 1.0
+    psi = torch.empty((3, d, d, d), device=device, dtype=torch.float32)
+    for ax, kv in enumerate((kx, ky, kzz)):
+        pk = dk * (1j * kv / k2)
+        pk[0, 0, 0] = 0.0
+        psi[ax] = torch.fft.irfftn(pk, s=(d, d, d))
+    return psi
+
+
+def tracers(n, psi, box, seed, weights=True, device='cuda'):
+    """n tracers at q ~ U[0, box)^3 moved by psi CIC-interpolated at q (the batched
+    and serving recipe), with lognormal(0, 1) weights.  Returns (pos, W)."""
+    import torch
+    from . import MAS_library as MASL
+    g = torch.Generator(device=device).manual_seed(int(seed))
+    q = torch.rand((n, 3), generator=g, device=device, dtype=torch.float32) * float(box)
+    disp = torch.empty((n,), device=device, dtype=torch.float32)
+    pos = torch.empty_like(q)
+    for ax in range(3):
+        MASL.CIC_interp(psi[ax].contiguous(), box, q, disp)
+        pos[:, ax] = torch.remainder(q[:, ax] + disp, box)
+    pos[pos >= box] = 0.0
+    W = None
+    if weights:
+        W = torch.exp(torch.randn((n,), generator=g, device=device, dtype=torch.float32))
+    return pos.contiguous(), W
+
+
 def config_c2(device='cuda'):
     """Clustered field: 512^3 ZA particles, box 1000 Mpc/h, RSD along axis 0 (f=0.5), seed 2."""
     return zeldovich(512, 1000.0, 2, f_rsd=0.5, rsd_axis=0, device=device)

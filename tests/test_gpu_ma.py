@@ -77,8 +77,12 @@ def test_ma_grid_sizes_and_tile_edges(oracle, dims, mas):
 def test_ma_edge_positions(oracle, mas):
     print('case edges', mas, flush=True)
     box, dims = 64.0, 32
+    # inv = dims/box = 0.5: x = 2k+1 puts dist exactly on k + 0.5 (the TSC/NGP
+    # branch points), its float neighbours just either side
+    half = np.float32(7.0)
     vals = np.array([0.0, box, np.nextafter(np.float32(box), np.float32(0)), 1.0, 2.0, 2.5,
-                     31.999, 32.0, 63.0], np.float32)
+                     31.999, 32.0, 63.0, half, np.nextafter(half, np.float32(0)),
+                     np.nextafter(half, np.float32(100)), np.float32(1e-8)], np.float32)
     g = np.stack(np.meshgrid(vals, vals, vals, indexing='ij'), -1).reshape(-1, 3)
     ref = np.zeros((dims,) * 3, np.float32)
     oracle.ma_f32(g, ref, box, mas)
@@ -113,8 +117,10 @@ def test_ma_clustered_one_cell(oracle, mas):
     assert_cells(g, ref, what='clustered %s' % mas, nadds=contrib_counts(pos, dims, box, mas))
     err64 = np.abs(g - g64) / np.maximum(np.abs(g64), 1e-30)
     err_ref = np.abs(ref - g64) / np.maximum(np.abs(g64), 1e-30)
+    # vs float64: what remains is the float32 rounding of dist = x*inv in the
+    # reference arithmetic (~1e-6 of a weight), never worse than the reference's
     dense = g64 > 0.1 * g64.max()
-    assert err64[dense].max() < 1e-6, err64[dense].max()
+    assert err64[dense].max() < 5e-6, err64[dense].max()
     assert err64[dense].max() <= err_ref[dense].max()
 
 
@@ -178,3 +184,20 @@ def test_ma_large_mass_conservation(mas):
     grid2 = torch.zeros_like(grid)
     MASL.MA(pos, grid2, 1000.0, mas)
     assert torch.equal(grid, grid2)
+
+
+def test_cic_interp_vs_oracle(oracle):
+    import torch
+    from pylians3_amd import MAS_library as MASL
+    rng = np.random.default_rng(40)
+    for dims, box in ((32, 100.0), (17, 5.0)):
+        g = rng.standard_normal((dims,) * 3).astype(np.float32)
+        pos = (rng.random((40000, 3)) * box).astype(np.float32)
+        pos[:3] = np.float32(box)
+        den = np.zeros(40000, np.float32)
+        MASL.CIC_interp(g, box, pos, den)
+        ref = oracle.cic_interp(g, box, pos)
+        np.testing.assert_allclose(den, ref, rtol=2e-6, atol=2e-6 * np.abs(g).max())
+        tden = torch.zeros(40000, device='cuda')
+        MASL.CIC_interp(torch.from_numpy(g).cuda(), box, torch.from_numpy(pos).cuda(), tden)
+        np.testing.assert_array_equal(tden.cpu().numpy(), den)

@@ -202,3 +202,20 @@ def test_redshift_space_oracle(oracle):
     out2 = oracle.pos_redshift_space(np.array([[999.5, 0, 0]], np.float32),
                                      np.array([[100.0, 0, 0]], np.float32), 1000.0, 100.0, 0.0, 0)
     assert out2[0, 0] == np.float32(np.fmod(np.float32(1000.5), np.float32(1000.0)))
+
+
+def test_cic_interp_oracle_adjoint_and_closed_form(oracle):
+    # CIC_interp is the adjoint of CIC: <MA_CIC(p, w), g> = sum_i w_i interp(g, p_i)
+    rng = np.random.default_rng(31)
+    dims, box = 12, 30.0
+    pos = (rng.random((500, 3)) * box).astype(np.float32)
+    w = rng.random(500)
+    g = rng.standard_normal((dims,) * 3).astype(np.float32)
+    grid64 = oracle.ma_f64(pos, dims, box, 'CIC', w.astype(np.float32))
+    lhs = np.sum(grid64 * g.astype(np.float64))
+    rhs = np.sum(w.astype(np.float32).astype(np.float64) * oracle.cic_interp(g, box, pos))
+    assert abs(lhs - rhs) < 1e-5 * np.sum(np.abs(grid64 * g))
+    # a linear field is reproduced exactly between nodes (x = 2.5 cells -> 2.5 along x)
+    lin = np.tile(np.arange(dims, dtype=np.float32)[:, None, None], (1, dims, dims))
+    v = oracle.cic_interp(lin, box, np.array([[2.5 * box / dims, 1.0, 1.0]], np.float32))
+    assert abs(v[0] - 2.5) < 1e-5
