@@ -57,7 +57,7 @@ class Pipeline:
         self.pos, self.dims, self.box, self.mas, self.axis = pos, dims, box, mas, axis
         self.np = pos.shape[0]
         self.grid = torch.empty((dims,) * 3, dtype=torch.float32, device=pos.device)
-        self.mean = float(self.np) / dims ** 3
+        self.mean = float(ctypes.c_float(float(self.np) / dims ** 3).value)
         self.out, self.arrs, _ = N.alloc_pk_out(dims, 1)
         self.flags_ma = N.PTR_DEVICE | N.MA_OVERWRITE | N.NO_SYNC | (N.MA_ATOMIC if atomic else 0)
         self.gp = ctypes.c_void_p(self.grid.data_ptr())
@@ -67,12 +67,14 @@ class Pipeline:
         self.nsteps = 0
 
     def step(self):
+        # MA fused with delta = rho/mean - 1 (Pk_snapshot.py:80-88), then Pk on delta
         N, L = self.N, self.N.lib()
-        N.check(L.pyl_ma(self.ctx.handle, ctypes.c_void_p(self.pos.data_ptr()), self.np, 3, None,
-                         self.gp, self.dims, self.box, self.mas, self.flags_ma))
+        N.check(L.pyl_ma_delta(self.ctx.handle, ctypes.c_void_p(self.pos.data_ptr()), self.np, 3,
+                               None, self.gp, self.dims, self.box, self.mas, self.mean, 1.0,
+                               self.flags_ma))
         t_ma = self.ctx.stage_times()
         N.check(L.pyl_pk(self.ctx.handle, self.ptrs, 1, self.masid, self.dims, self.box, self.axis,
-                         self.mean, 1.0, N.PTR_DEVICE | N.PK_DESTROY_INPUT, ctypes.byref(self.out)))
+                         1.0, 0.0, N.PTR_DEVICE | N.PK_DESTROY_INPUT, ctypes.byref(self.out)))
         t_pk = self.ctx.stage_times()
         for k, v in list(t_ma.items()) + list(t_pk.items()):
             if v > 0:
@@ -200,7 +202,7 @@ def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
                                scales.ctypes.data, shifts.ctypes.data,
                                N.PTR_DEVICE | N.PK_DESTROY_INPUT, outs))
 
-    one_out, _, _ = N.alloc_pk_out(dims, 2)
+    one_out, one_arrs, _ = N.alloc_pk_out(dims, 2)   # one_arrs keeps the result arrays alive
 
     def per_call():
         for r in range(nreal):

@@ -12,6 +12,7 @@
  *
  * Reference interfaces replaced (library/ paths are in the reference tree):
  *   pyl_ma            MAS_library.MA                library/MAS_library/MAS_library.pyx:57-112
+ *   pyl_ma_delta      MA + "delta /= mean; delta -= 1.0"  library/Pk_library/Pk_snapshot.py:80-88
  *                     and its kernels NGP..PCSW      MAS_library.pyx:123-545
  *   NGP/CIC/TSC/PCS   the C kernels, same signature  library/MAS_library/MAS_c.h:3-10 (MAS_c.c:8-266)
  *   pyl_fft_r2c       FFT3Dr_f                       library/Pk_library/Pk_library.pyx:117-130
@@ -22,6 +23,8 @@
  *   pyl_mas_correction  MAS_correction               Pk_library.pyx:83-84
  *   pyl_pos_redshift_space  pos_redshift_space       library/redshift_space_library/
  *                                                    redshift_space_library.pyx:29-46
+ *   pyl_pk_plane      Pk_plane (F=1), XPk_plane (F=2) Pk_library.pyx:435-511, :1093-1220
+ *   pyl_correct_mas   correct_MAS                    Pk_library.pyx:1882-1939
  */
 #ifndef PYLIANS_HIP_H
 #define PYLIANS_HIP_H
@@ -95,12 +98,22 @@ int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */);
  * W       (n,) float32 or NULL
  * grid    (dims,dims[,dims]) float32, C order; grid += deposit (or = with OVERWRITE)
  * mas     PYL_NGP..PYL_PCS
- * Default algorithm: particles binned by tile with a device counting sort, each
- * tile summed in LDS with exact (order-independent) accumulation, then written
- * once -- bitwise reproducible.  PYL_MA_ATOMIC selects per-particle global atomics.
+ * Default algorithm: particles binned by home tile with a device counting sort,
+ * each tile plus its halo summed in LDS with exact (order-independent)
+ * accumulation, the halos added in a fixed order -- bitwise reproducible.
+ * PYL_MA_ATOMIC selects per-particle global atomics.
  */
 int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
            float *grid, int dims, float BoxSize, int mas, int flags);
+
+/* MA fused with the density contrast: delta = deposit/scale - shift in float32,
+ * written once (overwrite).  The caller's two steps after MA in the reference
+ * pipeline (library/Pk_library/Pk_snapshot.py:80-88, "delta /= mean; delta -= 1.0")
+ * with scale = the mean and shift = 1; scale = 1, shift = 0 gives pyl_ma with
+ * PYL_MA_OVERWRITE. */
+int pyl_ma_delta(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
+                 float *delta, int dims, float BoxSize, int mas, float scale, float shift,
+                 int flags);
 
 /* Batched MA: nreal independent grids in one launch sequence.  Particles of grid r are
  * pos[offsets[r] .. offsets[r+1]) (host array, nreal+1 entries, offsets[0] = 0), W is
@@ -199,6 +212,27 @@ double pyl_mas_correction(double x, int p);
  * pos/vel (n,3) float32; pos[:,axis] += vel[:,axis]*(1+z)/H, then periodic wrap. */
 int pyl_pos_redshift_space(pyl_ctx *ctx, float *pos, const float *vel, int64_t n,
                            float BoxSize, float Hubble, float redshift, int axis, int flags);
+
+/* ---- 2D spectra of (grid,grid) maps ------------------------------------
+ * Pk_plane (nfields = 1) and XPk_plane (nfields = 2): 2D r2c FFT, MAS
+ * deconvolution (float factor, complex64 scaling), binning by int(|k|), then the
+ * reference normalisation with the DC bin dropped.  Outputs have nbins =
+ * pyl_pk_plane_nbins(grid) entries: k[nbins], Nmodes[nbins] and
+ * Pk[nbins*S] with S = 1, or S = 3 for two fields (auto 1, auto 2, cross),
+ * [i*S + s].  Deterministic: every bin is summed in a fixed order. */
+int pyl_pk_plane_nbins(int grid);
+int pyl_pk_plane(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas,
+                 int grid, double BoxSize, int flags, double *k, double *Nmodes, double *Pk);
+
+/* ---- MAS deconvolution of a field (correct_MAS) ------------------------
+ * out = unnormalised inverse r2c transform (FFTW_BACKWARD convention, so the
+ * result is dims^3 times the deconvolved field) of the forward transform of delta
+ * with its independent modes divided by the MAS window.  The redundant half of
+ * the kz = 0 and kz = dims/2 planes is left unscaled, as the reference does, and
+ * enters the real inverse transform through its Hermitian part.  delta and out
+ * are (dims,dims,dims) float32 and may alias. */
+int pyl_correct_mas(pyl_ctx *ctx, const float *delta, float *out, int dims, double BoxSize,
+                    int mas, int flags);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,10 @@ def lib():
         L.ora_pos_redshift_space.argtypes = [fp, fp, ctypes.c_long, ctypes.c_float,
                                              ctypes.c_float, ctypes.c_float, ctypes.c_int]
         L.ora_cic_interp.argtypes = [fp, ctypes.c_int, ctypes.c_float, fp, ctypes.c_long, fp]
+        L.ora_pk_plane_loop.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ip, dp, dp, dp]
+        L.ora_correct_mas_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -339,3 +343,136 @@ def direct_dft_r2c(x):
     E = np.exp(-2j * np.pi * np.outer(n, n) / d)
     X = np.einsum('ai,bj,ck,ijk->abc', E, E, E, x.astype(np.complex128))
     return X[:, :, :d // 2 + 1]
+
+
+# ----------------------------------------------------------------------------- 2D spectra
+def frequencies_2D(BoxSize, grid):
+    """Pk_library.pyx:64-69."""
+    kF = 2.0 * np.pi / BoxSize
+    middle = grid // 2
+    kN = middle * kF
+    return kF, kN, middle, middle, int(np.sqrt(middle ** 2 + middle ** 2))
+
+
+def expected_modes_2D(grid):
+    """check_number_modes_2D closed form (Pk_library.pyx:102-114)."""
+    own = 1 if grid % 2 == 1 else 4
+    return (grid ** 2 - own) // 2 + own
+
+
+def fft2_f32(delta):
+    import scipy.fft
+    return scipy.fft.rfftn(np.asarray(delta, dtype=np.float32))
+
+
+def fft2_f64(delta):
+    import scipy.fft
+    return scipy.fft.rfftn(np.asarray(delta, dtype=np.float64))
+
+
+def raw_plane_bins(dk_list, grid, mas_list, is_f64=False):
+    """Mode loop of Pk_plane (one field) or XPk_plane (two fields): raw sums."""
+    F = len(dk_list)
+    S = 1 if F == 1 else 3
+    kmax = int(np.sqrt(2 * (grid // 2) ** 2))
+    ct = np.complex128 if is_f64 else np.complex64
+    dk = np.ascontiguousarray(np.stack([np.asarray(d, dtype=ct) for d in dk_list]))
+    assert dk.shape[1:] == (grid, grid // 2 + 1)
+    out = dict(k=np.zeros(kmax + 1), Pk=np.zeros((kmax + 1, S)), N=np.zeros(kmax + 1))
+    mas = (ctypes.c_int * F)(*[MAS_INDEX.get(m, 0) if isinstance(m, str) or m is None else int(m)
+                               for m in mas_list])
+    lib().ora_pk_plane_loop(dk.ctypes.data, 1 if is_f64 else 0, F, grid, mas,
+                            _p(out['k'], ctypes.c_double), _p(out['Pk'], ctypes.c_double),
+                            _p(out['N'], ctypes.c_double))
+    return out
+
+
+def pk_plane(delta, BoxSize, MAS='CIC', f64=False):
+    """Pk_library.Pk_plane restated: attributes k, Pk, Nmodes (Pk_library.pyx:499-507)."""
+    grid = delta.shape[0]
+    kF = 2.0 * np.pi / BoxSize
+    dk = fft2_f64(delta) if f64 else fft2_f32(delta)
+    raw = raw_plane_bins([dk], grid, [MAS], f64)
+    assert int(np.sum(raw['N'])) == expected_modes_2D(grid)
+    r = _Result()
+    N = raw['N'][1:].copy()
+    r.k = (raw['k'][1:] / N) * kF
+    r.Pk = (raw['Pk'][1:, 0] / N) * (BoxSize / grid ** 2) ** 2
+    r.Nmodes = N
+    r.raw = raw
+    return r
+
+
+def xpk_plane(delta1, delta2, BoxSize, MAS1=None, MAS2=None, f64=False):
+    """Pk_library.XPk_plane restated: k, Pk (nb,2), XPk, Nmodes, r (Pk_library.pyx:1205-1218)."""
+    grid = delta1.shape[0]
+    kF = 2.0 * np.pi / BoxSize
+    dks = [fft2_f64(d) if f64 else fft2_f32(d) for d in (delta1, delta2)]
+    raw = raw_plane_bins(dks, grid, [MAS1, MAS2], f64)
+    fact = (BoxSize / grid ** 2) ** 2
+    N = raw['N'][1:].copy()
+    r = _Result()
+    r.k = (raw['k'][1:] / N) * kF
+    r.Pk = (raw['Pk'][1:, 0:2] / N[:, None]) * fact
+    r.XPk = (raw['Pk'][1:, 2] / N) * fact
+    r.Nmodes = N
+    r.r = r.XPk / np.sqrt(r.Pk[:, 0] * r.Pk[:, 1])
+    r.raw = raw
+    return r
+
+
+def xpk_2d(delta1, delta2, BoxSize, axis=2, MAS1='CIC', MAS2='CIC', f64=False):
+    """Pk_library.XPk_2D restated: [kpar, kper, Pk1, Pk2, PkX, Nmodes]
+    (Pk_library.pyx:1754-1865; the 2D bins of the 3D mode loop, kept with the DC bin)."""
+    dims = delta1.shape[0]
+    dks = [fft_f64(d) if f64 else fft_f32(d) for d in (delta1, delta2)]
+    raw = raw_bins(dks, dims, axis, [MAS1, MAS2], is_f64=f64)
+    kF, kN, kmax_par, kmax_per, kmax = frequencies(1.0, dims)
+    kF = 2.0 * np.pi / BoxSize
+    nb2 = (kmax_par + 1) * (kmax_per + 1)
+    kpar = np.zeros(nb2)
+    kper = np.zeros(nb2)
+    for ipar in range(kmax_par + 1):
+        for iper in range(kmax_per + 1):
+            index = (kmax_par + 1) * iper + ipar
+            kpar[index] = 0.5 * (ipar + ipar + 1) * kF
+            kper[index] = 0.5 * (iper + iper + 1) * kF
+    N = raw['N2D'].copy()
+    fact = (BoxSize / dims ** 2) ** 3
+    with np.errstate(divide='ignore', invalid='ignore'):
+        Pk1 = raw['Pk2D'][:, 0] * fact / N
+        Pk2 = raw['Pk2D'][:, 1] * fact / N
+        PkX = raw['Pk2D'][:, 2] * fact / N
+    return [kpar, kper, Pk1, Pk2, PkX, N]
+
+
+def correct_mas(delta, BoxSize, MAS='CIC', f64=False):
+    """Pk_library.correct_MAS restated (Pk_library.pyx:1882-1936): r2c FFT, MAS
+    deconvolution of the independent modes, unnormalised c2r (FFTW_BACKWARD).
+
+    The inverse real transform reads the redundant half of the kz = 0 and
+    kz = middle planes through its Hermitian part, so each of those modes ends
+    up with (1 + c)/2 of the correction c; scipy's irfftn and FFTW both behave
+    this way (a complex transform over x, y followed by a c2r along z that
+    uses only the real part of the kz = 0 and kz = middle terms)."""
+    import scipy.fft
+    dims = delta.shape[0]
+    dk = np.ascontiguousarray(fft_f64(delta) if f64 else fft_f32(delta))
+    lib().ora_correct_mas_modes(dk.ctypes.data, 1 if f64 else 0, dims,
+                                MAS_INDEX.get(MAS, 0) if isinstance(MAS, str) else int(MAS))
+    out = scipy.fft.irfftn(dk, s=delta.shape, norm='forward')
+    return out.astype(np.float64 if f64 else np.float32)
+
+
+def brute_mode_counts_2D(grid):
+    """Enumerate all grid^2 modes, keep one of each Hermitian pair, histogram by int(|k|)."""
+    middle = grid // 2
+    kmax = int(np.sqrt(2 * middle ** 2))
+    k = np.arange(grid)
+    k = np.where(k > middle, k - grid, k)
+    KX, KY = np.meshgrid(k, k, indexing='ij')
+    fi = (KX % grid) * grid + (KY % grid)
+    fj = ((-KX) % grid) * grid + ((-KY) % grid)
+    keep = fi <= fj
+    k2 = KX[keep] ** 2 + KY[keep] ** 2
+    return np.bincount(np.floor(np.sqrt(k2)).astype(np.int64), minlength=kmax + 1).astype(np.float64)

@@ -7,6 +7,9 @@
  *   library/Pk_library/Pk_library.pyx:83-84    MAS_correction
  *   library/Pk_library/Pk_library.pyx:311-378  Pk  mode loop (serial)
  *   library/Pk_library/Pk_library.pyx:623-732  XPk mode loop (serial)
+ *   library/Pk_library/Pk_library.pyx:466-497  Pk_plane mode loop
+ *   library/Pk_library/Pk_library.pyx:1155-1203 XPk_plane mode loop
+ *   library/Pk_library/Pk_library.pyx:1910-1929 correct_MAS mode scaling
  * The loops take the half-complex FFT output (dims, dims, dims/2+1) and fill
  * the raw per-bin sums; normalisation (Pk_library.pyx:384-418, 735-791) is in
  * oracle/oracle.py.  The float32 variant keeps the reference's types: float
@@ -142,3 +145,102 @@ void ora_pk_loop(const void *dk, int is_f64, int F, int dims, int axis,
 
 /* MAS_correction exported for the closed-form tests (Pk_library.pyx:83-84). */
 double ora_mas_correction(double x, int p) { return mas_correction(x, p); }
+
+/*
+ * 2D mode loops of Pk_plane (F == 1, Pk_library.pyx:466-497) and XPk_plane
+ * (F == 2, Pk_library.pyx:1155-1203).  dk: F half-complex planes
+ * (grid, grid/2+1), interleaved (re,im).  Outputs per |k| bin (kmax+1 bins,
+ * kmax = (int)sqrt(2*middle^2)): k2D, N, Pk[(kmax+1)*S] with S = 1 (F == 1) or
+ * 3 (two autos, then the cross).  float32 variant: float MAS factor (the
+ * double product of the two axis corrections, rounded), complex64 scaling,
+ * double squares and sums.
+ */
+void ora_pk_plane_loop(const void *dk, int is_f64, int F, int grid, const int *mas_index,
+                       double *k2D, double *Pk, double *N)
+{
+    int middle = grid / 2;
+    int S = (F == 1) ? 1 : 3;
+    long zlen = middle + 1;
+    double prefact = M_PI / grid;
+    int kxx, kyy, f;
+    for (kxx = 0; kxx < grid; kxx++) {
+        int kx = (kxx > middle) ? kxx - grid : kxx;
+        double cx[2], cy[2];
+        for (f = 0; f < F; f++) cx[f] = mas_correction(prefact * kx, mas_index[f]);
+        for (kyy = 0; kyy < middle + 1; kyy++) {
+            int ky = kyy;
+            double k, re[2], im[2];
+            int k_index;
+            for (f = 0; f < F; f++) cy[f] = mas_correction(prefact * ky, mas_index[f]);
+            if (ky == 0 || (ky == middle && grid % 2 == 0))
+                if (kx < 0) continue;
+            k = sqrt((double)(kx * kx + ky * ky));
+            k_index = (int)k;
+            for (f = 0; f < F; f++) {
+                long e = (long)f * grid * zlen + (long)kxx * zlen + kyy;
+                if (is_f64) {
+                    const double *d = (const double *)dk + 2 * e;
+                    double mf = cx[f] * cy[f];
+                    re[f] = d[0] * mf; im[f] = d[1] * mf;
+                } else {
+                    const float *d = (const float *)dk + 2 * e;
+                    float mf = (float)(cx[f] * cy[f]);
+                    float r = d[0] * mf, m = d[1] * mf;
+                    re[f] = (double)r; im[f] = (double)m;
+                }
+            }
+            k2D[k_index] += k;
+            N[k_index] += 1.0;
+            if (F == 1) {
+                Pk[k_index] += re[0] * re[0] + im[0] * im[0];
+            } else {
+                Pk[(long)k_index * 3 + 0] += re[0] * re[0] + im[0] * im[0];
+                Pk[(long)k_index * 3 + 1] += re[1] * re[1] + im[1] * im[1];
+                Pk[(long)k_index * 3 + 2] += re[0] * re[1] + im[0] * im[1];
+            }
+        }
+    }
+}
+
+/*
+ * MAS deconvolution of correct_MAS (Pk_library.pyx:1910-1929) applied in place
+ * to a half-complex cube (dims, dims, dims/2+1): independent modes are scaled
+ * by the float MAS factor in complex64 (or double for is_f64); the redundant
+ * half of the kz = 0 and kz = middle planes is left as it is, as the reference
+ * does.  The caller runs the unnormalised inverse transform.
+ */
+void ora_correct_mas_modes(void *dk, int is_f64, int dims, int mas_index)
+{
+    int middle = dims / 2;
+    long zlen = middle + 1;
+    double prefact = M_PI / dims;
+    int kxx, kyy, kzz;
+    for (kxx = 0; kxx < dims; kxx++) {
+        int kx = (kxx > middle) ? kxx - dims : kxx;
+        double c0 = mas_correction(prefact * kx, mas_index);
+        for (kyy = 0; kyy < dims; kyy++) {
+            int ky = (kyy > middle) ? kyy - dims : kyy;
+            double c1 = mas_correction(prefact * ky, mas_index);
+            for (kzz = 0; kzz < middle + 1; kzz++) {
+                int kz = kzz;
+                double c2 = mas_correction(prefact * kz, mas_index);
+                long e = ((long)kxx * dims + kyy) * zlen + kzz;
+                if (kz == 0 || (kz == middle && dims % 2 == 0)) {
+                    if (kx < 0) continue;
+                    else if (kx == 0 || (kx == middle && dims % 2 == 0)) {
+                        if (ky < 0) continue;
+                    }
+                }
+                if (is_f64) {
+                    double *d = (double *)dk + 2 * e;
+                    double mf = c0 * c1 * c2;
+                    d[0] *= mf; d[1] *= mf;
+                } else {
+                    float *d = (float *)dk + 2 * e;
+                    float mf = (float)(c0 * c1 * c2);
+                    d[0] = d[0] * mf; d[1] = d[1] * mf;
+                }
+            }
+        }
+    }
+}

@@ -69,6 +69,9 @@ EXPORTS = {
     'pyl_ma': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
                               ctypes.c_int, ctypes.c_int]),
+    'pyl_ma_delta': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int]),
     'pyl_ma_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_float, ctypes.c_int, ctypes.c_int]),
@@ -102,6 +105,12 @@ EXPORTS = {
     'pyl_pos_redshift_space': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_int64, ctypes.c_float, ctypes.c_float,
                                               ctypes.c_float, ctypes.c_int, ctypes.c_int]),
+    'pyl_pk_plane_nbins': (ctypes.c_int, [ctypes.c_int]),
+    'pyl_pk_plane': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, _dp, _dp, _dp]),
+    'pyl_correct_mas': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int]),
 }
 
 
@@ -218,6 +227,7 @@ def alloc_pk_out(dims, nfields):
                 Pk2D=np.zeros((s.n2d, S)), k3D=np.zeros(s.n3d), Nmodes3D=np.zeros(s.n3d),
                 Pk3D=np.zeros((s.n3d, 3, S)), Pkphase=np.zeros(s.n3d))
     out = PkOut(**{k: v.ctypes.data_as(_dp) for k, v in arrs.items()})
+    out._keep = arrs          # the struct holds raw pointers into these arrays
     return out, arrs, S
 
 

@@ -85,8 +85,8 @@ static std::once_flag g_rocfft_once;
 
 // Resident rocFFT plan: forward, single precision, out-of-place r2c of `batch`
 // contiguous dims^3 cubes.  Created once per (dims, batch) and kept.
-int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out) {
-    auto key = std::make_pair(dims, batch);
+int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out, int kind) {
+    auto key = std::make_tuple(dims, batch, kind);
     auto it = c->fft_plans.find(key);
     if (it != c->fft_plans.end()) {
         *out = &it->second;
@@ -97,12 +97,14 @@ int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out) {
     p.dims = dims;
     p.batch = batch;
     size_t lengths[3] = {(size_t)dims, (size_t)dims, (size_t)dims};  // fastest first
-    rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace,
-                                          rocfft_transform_type_real_forward,
-                                          rocfft_precision_single, 3, lengths, (size_t)batch,
+    const size_t rank = (kind == FFT_R2C_2D) ? 2 : 3;
+    const rocfft_transform_type type = (kind == FFT_C2R_3D) ? rocfft_transform_type_real_inverse
+                                                            : rocfft_transform_type_real_forward;
+    rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type,
+                                          rocfft_precision_single, rank, lengths, (size_t)batch,
                                           nullptr);
-    PYL_CHECK(st == rocfft_status_success, PYL_EFFT, "rocfft_plan_create(%d^3) failed (%d)",
-              dims, (int)st);
+    PYL_CHECK(st == rocfft_status_success, PYL_EFFT, "rocfft_plan_create(%d^%d, kind %d) failed (%d)",
+              dims, (int)rank, kind, (int)st);
     st = rocfft_plan_get_work_buffer_size(p.plan, &p.work_bytes);
     if (st != rocfft_status_success) {
         destroy_plan(p);

@@ -8,15 +8,23 @@
 // Two algorithms:
 //  * atomic: one thread per particle, one global float atomic per footprint
 //    node (the plain GPU form; summation order is not reproducible).
-//  * tiled (default): the grid is cut into tiles; a counting sort places a
-//    copy of each particle in every tile its footprint overlaps; one workgroup
-//    per tile sums the footprints of its records into an LDS accumulator and
-//    writes each node of its tile exactly once.  Contributions are rounded to
-//    an integer multiple of a per-tile power-of-two quantum 2^q chosen so
-//    that every partial sum is an integer below 2^53: the float64 LDS adds are
-//    then exact, so the result does not depend on the order of the atomics and
-//    is bitwise reproducible.  The quantum is ~2^-52 of the tile's weight bound
-//    (count x max|W|), far below float32 resolution.
+//  * tiled (default): the grid is cut into tiles (16^3 in 3D, 64^2 in 2D).
+//    A particle belongs to the tile of its first footprint node (its home
+//    tile).  A counting sort groups the particles by home tile; one workgroup
+//    per tile then adds the full footprints of its particles into an LDS box
+//    that is the tile plus an upper halo of F-1 nodes per axis, so no node
+//    test is needed.  Box cells that no other tile touches (the tile minus its
+//    lower band of F-1 nodes) are written to the grid directly; the band and
+//    the halo go to a partial-sum buffer, and a finalize pass adds, for every
+//    band cell, the tile's own value and the halo values of its lower
+//    neighbours in a fixed order.
+//    Inside a box, contributions are rounded to integer multiples of a
+//    per-tile power-of-two quantum 2^q chosen so that every partial sum is an
+//    integer below 2^53: the float64 LDS adds are exact, so the box does not
+//    depend on the order of the atomics.  With the fixed-order finalize the
+//    grid is bitwise reproducible, and identical whether a realisation is
+//    deposited alone or in a batch.  The quantum is ~2^-52 of the tile's
+//    weight bound (count x max|W|), far below float32 resolution.
 #include "pyl_internal.h"
 
 namespace pyl {
@@ -24,328 +32,289 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ int wrapi(int i, int d) {
+__host__ __device__ __forceinline__ int wrapi(int i, int d) {
     int r = i % d;
     return r < 0 ? r + d : r;
 }
 
+// raw mod d: two selects for the values a footprint of a position in [-L, 2L)
+// produces, a division only for anything further out.
+__host__ __device__ __forceinline__ int wrap_node(int raw, int d) {
+    int r = raw + (raw < 0 ? d : 0);
+    r -= (r >= d ? d : 0);
+    if ((unsigned)r >= (unsigned)d) r = wrapi(raw, d);
+    return r;
+}
+
 // (base + j) mod d for base in [0,d) and 0 <= j < 4; the second test only
 // fires on grids smaller than the footprint (d < 4).
-__device__ __forceinline__ int node_add(int base, int j, int d) {
+__host__ __device__ __forceinline__ int node_add(int base, int j, int d) {
     int r = base + j;
     if (r >= d) r -= d;
     if (r >= d) r %= d;
     return r;
 }
 
-// Per-axis footprint.  Writes the first node (wrapped into [0,dims)) and the F
-// weights; node j of the footprint is (base + j) mod dims.
+// First footprint node along one axis, before the periodic wrap, from
+// dist = pos * inv_cell_size (a float product, as in the reference).
 //
 // The reference takes floor() of (double)dist - 1.5 (TSC) / - 2.0 (PCS) and
 // truncates (double)dist + 0.5 (NGP).  With k = floorf(dist) and the exact
 // fraction f = dist - k (exact for any float), these equal k - 2 + (f >= 0.5),
 // k - 2 and k + (f >= 0.5) (for dist >= -0.5) in integer arithmetic, for every
-// |dist| < 2^31.
+// |dist| < 2^31.  The footprint nodes are raw, raw+1, ..., raw+F-1.
 template <int MAS>
-__device__ __forceinline__ int axis_foot(float x, float inv, int dims, float *C) {
+__host__ __device__ __forceinline__ int first_node(float dist) {
     if constexpr (MAS == PYL_NGP) {
         // MAS_library.pyx:288-289: <int>(pos*inv + 0.5), the 0.5 is a double
+        if (!(dist >= -0.5f)) return (int)((double)dist + 0.5);   // truncation toward 0
+        const float k = floorf(dist);
+        return (int)k + ((dist - k) >= 0.5f ? 1 : 0);
+    } else if constexpr (MAS == PYL_CIC) {
+        return (int)dist;                                   // MAS_library.pyx:151-153
+    } else if constexpr (MAS == PYL_TSC) {
+        const float k = floorf(dist);                       // floor(dist - 1.5) + 1
+        return (int)k - ((dist - k) >= 0.5f ? 0 : 1);
+    } else {
+        return (int)floorf(dist) - 1;                       // floor(dist - 2.0) + 1
+    }
+}
+
+// Footprint weights of the nodes raw .. raw+F-1.
+template <int MAS>
+__host__ __device__ __forceinline__ void foot_weights(float dist, int raw, float *C) {
+    if constexpr (MAS == PYL_NGP) {
         C[0] = 1.0f;
-        const float v = x * inv;
-        if (!(v >= -0.5f)) return wrapi((int)((double)v + 0.5), dims);  // truncation toward 0
-        const float k = floorf(v);
-        return wrapi((int)k + ((v - k) >= 0.5f ? 1 : 0), dims);
     } else if constexpr (MAS == PYL_CIC) {
         // MAS_library.pyx:151-156: u = dist - <int>dist, d = 1.0 - u
-        float dist = x * inv;
-        int i = (int)dist;
-        float u = dist - (float)i;
+        const float u = dist - (float)raw;
         C[1] = u;
         C[0] = 1.0f - u;
-        return wrapi(i, dims);
     } else if constexpr (MAS == PYL_TSC) {
-        // MAS_library.pyx:392-399
-        float dist = x * inv;
-        const float kf = floorf(dist);
-        int minimum = (int)kf - ((dist - kf) >= 0.5f ? 1 : 2);   // floor(dist - 1.5)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            float diff = fabsf((float)(minimum + j + 1) - dist);
-            float c;
-            if (diff < 0.5f) {
-                float d2 = diff * diff;          // float product, then 0.75 - it
-                c = 0.75f - d2;
-            } else if (diff < 1.5f) {
-                float a = 1.5f - diff;           // exact
-                c = 0.5f * (a * a);              // == (float)(0.5*(1.5-d)*(1.5-d))
-            } else {
-                c = 0.0f;
-            }
-            C[j] = c;
-        }
-        return wrapi(minimum + 1, dims);
+        // MAS_library.pyx:392-399.  With raw = floor(dist - 0.5) the distances
+        // to the three nodes are d0, d2 in [0.5, 1.5] and d1 in [0, 0.5], also
+        // after float rounding, so node 1 always takes the inner branch and
+        // nodes 0, 2 the outer one (at a branch boundary both formulas give the
+        // same float).  diff*diff is a float product, then 0.75 - it;
+        // 0.5*(1.5-d)^2 rounds once like the reference's double expression
+        // (1.5 - d is exact, its square fits in a double).
+        const float d0 = fabsf((float)raw - dist);
+        const float d1 = fabsf((float)(raw + 1) - dist);
+        const float d2 = fabsf((float)(raw + 2) - dist);
+        const float a0 = 1.5f - d0, a2 = 1.5f - d2;
+        C[0] = 0.5f * (a0 * a0);
+        C[1] = 0.75f - d1 * d1;
+        C[2] = 0.5f * (a2 * a2);
     } else {
-        // PCS, MAS_library.pyx:485-492 (double polynomial, rounded to float)
-        float dist = x * inv;
-        int minimum = (int)floorf(dist) - 2;                       // floor(dist - 2.0)
+        // PCS, MAS_library.pyx:485-492 (double polynomials, rounded to float).
+        // With raw = floor(dist) - 1, d0 and d3 lie in [1, 2] and d1, d2 in
+        // [0, 1] (after float rounding too); at d = 1 and d = 2 both branches
+        // of the reference give the same double.
+        float d[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = fabsf((float)(raw + j) - dist);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            float diff = fabsf((float)(minimum + j + 1) - dist);
-            double dd = (double)diff;
-            float c;
-            if (diff < 1.0f)
-                c = (float)((4.0 - 6.0 * dd * dd + 3.0 * dd * dd * dd) / 6.0);
-            else if (diff < 2.0f)
-                c = (float)((2.0 - dd) * (2.0 - dd) * (2.0 - dd) / 6.0);
+            const double dd = (double)d[j];
+            if (j == 1 || j == 2)
+                C[j] = (float)((4.0 - 6.0 * dd * dd + 3.0 * dd * dd * dd) / 6.0);
             else
-                c = 0.0f;
-            C[j] = c;
+                C[j] = (float)((2.0 - dd) * (2.0 - dd) * (2.0 - dd) / 6.0);
         }
-        return wrapi(minimum + 1, dims);
     }
 }
 
-// Geometry of the tiled deposit.  2D grids are 3D grids with one z plane.  A
-// batch holds nreal grids of the same size; tile keys are r*ntiles + tile.
+// Per-axis footprint: writes the F weights, returns the first node wrapped into
+// [0, dims); node j of the footprint is (first + j) mod dims.
+template <int MAS>
+__device__ __forceinline__ int axis_foot(float x, float inv, int dims, float *C) {
+    const float dist = x * inv;
+    const int raw = first_node<MAS>(dist);
+    foot_weights<MAS>(dist, raw, C);
+    return wrap_node(raw, dims);
+}
+
+// ---- tile geometry -----------------------------------------------------------
+// Tiles split each axis into nt pieces; tile i covers the nodes with
+// floor(node*nt/dims) == i, i.e. [origin(i), origin(i+1)) with
+// origin(i) = ceil(i*dims/nt): all extents are dims/nt rounded up or down, so
+// none is shorter than the footprint's reach when nt >= 2.  2D grids have one
+// node along z.  A batch holds nreal grids; tile keys are r*ntiles + tile.
+constexpr int kT3 = 16;        // nominal 3D tile extent
+constexpr int kT2 = 64;        // nominal 2D tile extent
+
 struct TileGeom {
-    int dims;          // nodes per axis (x, y and, for 3D, z)
-    int dz;            // nodes along z: dims (3D) or 1 (2D)
-    int T[3];          // tile extent per axis
-    int nt[3];         // tiles per axis
+    int dims;          // nodes per axis
+    int nt;            // tiles along x and y (and z in 3D)
+    int ntz;           // tiles along z: nt (3D) or 1 (2D)
+    int shift;         // log2(tile extent) when dims % T == 0, else -1
     long long ntiles;  // tiles per grid
     long long ncell;   // nodes per grid
-    int nreal;         // grids in the batch
+    int pslot;         // partial values stored per tile
+    const int *orig;   // device table of tile_origin(i), i = 0..nt (same on every axis)
 };
 
-// Realisation owning particle i: offs[r] <= i < offs[r+1] (offs has nreal+1 entries).
-__device__ __forceinline__ int realisation_of(const long long *__restrict__ offs, int nreal,
-                                              long long i) {
-    int lo = 0, hi = nreal;  // invariant: offs[lo] <= i < offs[hi]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (offs[mid] <= i) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+__host__ __device__ __forceinline__ int tile_origin(int i, int nt, int dims) {
+    return (int)(((long long)i * dims + nt - 1) / nt);
 }
 
-// Distinct tiles touched along one axis by the F consecutive nodes starting at
-// base (mod d).  Returns their count (1..3) in order.
-template <int F>
-__device__ __forceinline__ int axis_tiles(int base, int d, int T, int *t) {
-    int n = 0;
-    int last = -1;
-#pragma unroll
-    for (int j = 0; j < F; ++j) {
-        int node = node_add(base, j, d);
-        int ti = node / T;
-        if (ti != last) {
-            bool seen = false;
-            for (int k = 0; k < n; ++k) seen |= (t[k] == ti);
-            if (!seen) t[n++] = ti;
-            last = ti;
-        }
-    }
-    return n;
+__host__ __device__ __forceinline__ int tile_of(int node, const TileGeom &g) {
+    return g.shift >= 0 ? (node >> g.shift) : (int)(((long long)node * g.nt) / g.dims);
 }
 
-// ---- counting sort by tile -------------------------------------------------
-// Each workgroup owns a contiguous chunk of particles and counts its (particle,
-// tile) copies in an LDS hash table (open addressing, linear probing, no
-// deletion).  At the end one global atomic per distinct key reserves the chunk's
-// range inside the tile's segment; the (key, base) table is kept in global
-// memory so the scatter pass can hand out slots with LDS atomics only.  Keys
-// that do not fit in the table (more than kHashCap distinct tiles in a chunk)
-// take per-group global atomics on a separate overflow counter.  Lanes of a
-// wave that share a key are grouped first, so coherent input (a lattice, a
-// sorted catalogue) costs one table operation per wave and distinct tile.
-constexpr int kSortBlock = 512;
-constexpr int kHashBits = 13;
-constexpr int kHashCap = 1 << kHashBits;      // 8192 entries, 64 KB of LDS (key + value)
-constexpr int kProbe = 32;
-constexpr unsigned kEmpty = 0xffffffffu;
-
-__device__ __forceinline__ unsigned hslot(unsigned key) {
-    return (key * 0x9E3779B1u) >> (32 - kHashBits);
+// Number of values of a box (extents e + F - 1 per axis, z extent 1 in 2D)
+// outside its interior [F-1, e) per axis, and the compact index of such a cell.
+// The layout runs over three slabs: x outside the interior; x inside and y
+// outside; x and y inside and z outside (3D only).
+template <int F, int AXES>
+__host__ __device__ __forceinline__ int halo_count(int ex, int ey, int ez) {
+    const int bx = ex + F - 1, by = ey + F - 1;
+    const int ix = ex - F + 1 > 0 ? ex - F + 1 : 0;
+    const int iy = ey - F + 1 > 0 ? ey - F + 1 : 0;
+    if (AXES == 2) return bx * by - ix * iy;
+    const int bz = ez + F - 1;
+    const int iz = ez - F + 1 > 0 ? ez - F + 1 : 0;
+    return bx * by * bz - ix * iy * iz;
 }
 
-__device__ __forceinline__ int tab_insert(unsigned *tkey, unsigned *tval, unsigned key,
-                                          unsigned n) {
-    const unsigned h = hslot(key);
-    for (int p = 0; p < kProbe; ++p) {
-        const unsigned s = (h + p) & (kHashCap - 1);
-        const unsigned prev = atomicCAS(&tkey[s], kEmpty, key);
-        if (prev == kEmpty || prev == key) {
-            atomicAdd(&tval[s], n);
-            return (int)s;
-        }
+template <int F, int AXES>
+__host__ __device__ __forceinline__ int halo_index(int x, int y, int z, int ex, int ey, int ez) {
+    const int by = ey + F - 1;
+    const int bz = (AXES == 3) ? ez + F - 1 : 1;
+    const int ix = ex - F + 1 > 0 ? ex - F + 1 : 0;
+    const int iy = ey - F + 1 > 0 ? ey - F + 1 : 0;
+    const bool xin = x >= F - 1 && x < ex;
+    const bool yin = y >= F - 1 && y < ey;
+    if (!xin) {
+        const int xi = x < F - 1 ? x : x - ix;
+        return (xi * by + y) * bz + z;
     }
-    return -1;
+    const int sa = (ex + F - 1 - ix) * by * bz;
+    if (!yin) {
+        const int yi = y < F - 1 ? y : y - iy;
+        return sa + ((x - (F - 1)) * (by - iy) + yi) * bz + z;
+    }
+    const int sb = ix * (by - iy) * bz;
+    const int iz = ez - F + 1 > 0 ? ez - F + 1 : 0;
+    const int zi = z < F - 1 ? z : z - iz;
+    return sa + sb + ((x - (F - 1)) * iy + (y - (F - 1))) * (bz - iz) + zi;
 }
 
-__device__ __forceinline__ int tab_find(const unsigned *tkey, unsigned key) {
-    const unsigned h = hslot(key);
-    for (int p = 0; p < kProbe; ++p) {
-        const unsigned s = (h + p) & (kHashCap - 1);
-        const unsigned k = tkey[s];
-        if (k == key) return (int)s;
-        if (k == kEmpty) return -1;
-    }
-    return -1;
+// ---- counting sort by home tile -----------------------------------------------
+// Each sort workgroup owns a chunk of consecutive particles of one realisation.
+// It counts its particles per home tile in an LDS window of kWin consecutive
+// keys centred on its first particle's key (coherent input -- a lattice, a
+// snapshot in ID order, a sorted catalogue -- keeps almost every key in the
+// window); keys outside the window take a global atomic on a separate
+// overflow counter.  The window counts are flushed with one global atomic per
+// non-zero key, which also hands the chunk its base inside the key's
+// segment; the scatter pass reloads those bases and assigns slots with LDS
+// atomics only.
+constexpr int kSortBlock = 1024;
+constexpr int kWin = 16384;                    // 64 KB of LDS
+
+struct Chunk {
+    long long c0, c1;   // particle range
+    int r;              // realisation
+    int wbase;          // first key of the LDS window (written by the count pass)
+};
+
+template <int MAS>
+__host__ __device__ __forceinline__ int home_tile(float dist, const TileGeom &g) {
+    return tile_of(wrap_node(first_node<MAS>(dist), g.dims), g);
 }
 
 template <int MAS>
-__device__ __forceinline__ void particle_tiles(const float *pos, long long i, int axes,
-                                               float inv, const TileGeom &g, int tx[3],
-                                               int ty[3], int tz[3], int &nx, int &ny,
-                                               int &nz) {
-    constexpr int F = MAS;
-    float C[4];
-    int bx = axis_foot<MAS>(pos[(long long)axes * i + 0], inv, g.dims, C);
-    int by = axis_foot<MAS>(pos[(long long)axes * i + 1], inv, g.dims, C);
-    nx = axis_tiles<F>(bx, g.dims, g.T[0], tx);
-    ny = axis_tiles<F>(by, g.dims, g.T[1], ty);
-    if (axes == 3) {
-        int bz = axis_foot<MAS>(pos[(long long)axes * i + 2], inv, g.dims, C);
-        nz = axis_tiles<F>(bz, g.dims, g.T[2], tz);
-    } else {
-        nz = 1;
-        tz[0] = 0;
-    }
+__device__ __forceinline__ unsigned particle_key(const float *__restrict__ pos, long long i,
+                                                 int axes, float inv, const TileGeom &g,
+                                                 long long rbase, float3 &dist) {
+    dist.x = pos[(long long)axes * i + 0] * inv;
+    dist.y = pos[(long long)axes * i + 1] * inv;
+    dist.z = (axes == 3) ? pos[(long long)axes * i + 2] * inv : 0.0f;
+    const int tx = home_tile<MAS>(dist.x, g);
+    const int ty = home_tile<MAS>(dist.y, g);
+    const int tz = (axes == 3) ? home_tile<MAS>(dist.z, g) : 0;
+    return (unsigned)(rbase + ((long long)tx * g.nt + ty) * g.ntz + tz);
 }
 
-__device__ __forceinline__ void chunk_range(long long n, long long chunk, long long &c0,
-                                            long long &c1) {
-    c0 = (long long)blockIdx.x * chunk;
-    c1 = min(n, c0 + chunk);
-}
-
-// Pass 1: per-tile counts.  counts[key] receives the table-counted copies (and
-// hands each chunk its base), ocounts[key] the overflowed ones.
 template <int MAS>
 __global__ void __launch_bounds__(kSortBlock)
-tile_count_kernel(const float *__restrict__ pos, long long n, int axes, float inv, TileGeom g,
-                  const long long *__restrict__ offs, long long chunk,
-                  unsigned *__restrict__ counts, unsigned *__restrict__ ocounts,
-                  uint2 *__restrict__ dump) {
-    __shared__ unsigned tkey[kHashCap];
-    __shared__ unsigned tval[kHashCap];
-    for (int s = threadIdx.x; s < kHashCap; s += blockDim.x) {
-        tkey[s] = kEmpty;
-        tval[s] = 0;
+tile_count_kernel(const float *__restrict__ pos, int axes, float inv, TileGeom g,
+                  Chunk *__restrict__ chunks, unsigned *__restrict__ counts,
+                  unsigned *__restrict__ ocounts, unsigned *__restrict__ dump) {
+    __shared__ unsigned hist[kWin];
+    const Chunk ch = chunks[blockIdx.x];
+    const long long rbase = (long long)ch.r * g.ntiles;
+    // every thread derives the window from the chunk's first particle
+    long long k0 = rbase;
+    if (ch.c1 > ch.c0) {
+        float3 d;
+        k0 = particle_key<MAS>(pos, ch.c0, axes, inv, g, rbase, d);
+    }
+    long long wbl = k0 - kWin / 2;
+    const long long hi = rbase + g.ntiles - kWin;       // keep the window in this grid's keys
+    if (wbl > hi) wbl = hi;
+    if (wbl < rbase) wbl = rbase;
+    if (threadIdx.x == 0) chunks[blockIdx.x].wbase = (int)wbl;
+    for (int s = threadIdx.x; s < kWin; s += blockDim.x) hist[s] = 0u;
+    __syncthreads();
+    const unsigned wb = (unsigned)wbl;
+    for (long long i = ch.c0 + threadIdx.x; i < ch.c1; i += blockDim.x) {
+        float3 d;
+        const unsigned key = particle_key<MAS>(pos, i, axes, inv, g, rbase, d);
+        const unsigned s = key - wb;
+        if (s < (unsigned)kWin) atomicAdd(&hist[s], 1u);
+        else atomicAdd(&ocounts[key], 1u);
     }
     __syncthreads();
-    const int lane = __lane_id();
-    long long c0, c1;
-    chunk_range(n, chunk, c0, c1);
-    for (long long base = c0; base < c1; base += blockDim.x) {
-        const long long i = base + threadIdx.x;
-        const bool valid = i < c1;
-        int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
-        long long rbase = 0;
-        if (valid) {
-            particle_tiles<MAS>(pos, i, axes, inv, g, tx, ty, tz, nx, ny, nz);
-            if (g.nreal > 1) rbase = (long long)realisation_of(offs, g.nreal, i) * g.ntiles;
-        }
-        const int nc = valid ? nx * ny * nz : 0;
-        for (int j = 0;; ++j) {
-            const bool act = j < nc;
-            if (!__any(act)) break;
-            unsigned key = kEmpty;
-            if (act) {
-                int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
-                key = (unsigned)(rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c]);
-            }
-            unsigned long long pend = __ballot(act);
-            while (pend) {
-                const int leader = __ffsll((unsigned long long)pend) - 1;
-                const unsigned lk = __shfl(key, leader);
-                const unsigned long long grp = __ballot(act && key == lk) & pend;
-                if (lane == leader) {
-                    const unsigned cnt = (unsigned)__popcll(grp);
-                    if (tab_insert(tkey, tval, lk, cnt) < 0) atomicAdd(&ocounts[lk], cnt);
-                }
-                pend &= ~grp;
-            }
-        }
-    }
-    __syncthreads();
-    uint2 *dmp = dump + (long long)blockIdx.x * kHashCap;
-    for (int s = threadIdx.x; s < kHashCap; s += blockDim.x) {
-        const unsigned k = tkey[s];
-        uint2 d = make_uint2(kEmpty, 0u);
-        if (k != kEmpty) {
-            d.x = k;
-            d.y = atomicAdd(&counts[k], tval[s]);
-        }
-        dmp[s] = d;
+    unsigned *dmp = dump + (long long)blockIdx.x * kWin;
+    for (int s = threadIdx.x; s < kWin; s += blockDim.x) {
+        const unsigned v = hist[s];
+        if (v) dmp[s] = atomicAdd(&counts[wb + s], v);
     }
 }
 
-// Pass 3: write one record (x, y, z, w) per (particle, overlapped tile).  Record
-// index = tile offset + chunk base + LDS cursor (table keys), or tile offset +
-// table count + global overflow cursor.
+// One record (dist_x, dist_y, dist_z, w) per particle at offsets[key] + slot:
+// window keys take slot = chunk base + LDS cursor, overflow keys slot =
+// counts[key] + global cursor.  Weighted input also records max|W| per
+// realisation (float bits of the largest finite |W|).
 template <int MAS>
 __global__ void __launch_bounds__(kSortBlock)
-tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
-                    int axes, float inv, TileGeom g, const long long *__restrict__ offs,
-                    long long chunk, const uint2 *__restrict__ dump,
-                    const unsigned long long *__restrict__ offsets,
+tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, int axes,
+                    float inv, TileGeom g, const Chunk *__restrict__ chunks,
+                    const unsigned *__restrict__ dump, const unsigned long long *__restrict__ offsets,
                     const unsigned *__restrict__ counts, unsigned *__restrict__ ocursors,
-                    float4 *__restrict__ records) {
-    __shared__ unsigned tkey[kHashCap];
-    __shared__ unsigned tcur[kHashCap];
-    const uint2 *dmp = dump + (long long)blockIdx.x * kHashCap;
-    for (int s = threadIdx.x; s < kHashCap; s += blockDim.x) {
-        const uint2 d = dmp[s];
-        tkey[s] = d.x;
-        tcur[s] = d.y;
-    }
+                    unsigned *__restrict__ wmax_bits, float4 *__restrict__ records) {
+    __shared__ unsigned cur[kWin];
+    const Chunk ch = chunks[blockIdx.x];
+    const long long rbase = (long long)ch.r * g.ntiles;
+    const unsigned wb = (unsigned)ch.wbase;
+    const unsigned *dmp = dump + (long long)blockIdx.x * kWin;
+    for (int s = threadIdx.x; s < kWin; s += blockDim.x) cur[s] = dmp[s];
     __syncthreads();
-    const int lane = __lane_id();
-    long long c0, c1;
-    chunk_range(n, chunk, c0, c1);
-    for (long long base = c0; base < c1; base += blockDim.x) {
-        const long long i = base + threadIdx.x;
-        const bool valid = i < c1;
-        int tx[3], ty[3], tz[3], nx = 0, ny = 0, nz = 0;
-        long long rbase = 0;
-        float4 rec = make_float4(0.f, 0.f, 0.f, 1.f);
-        if (valid) {
-            particle_tiles<MAS>(pos, i, axes, inv, g, tx, ty, tz, nx, ny, nz);
-            if (g.nreal > 1) rbase = (long long)realisation_of(offs, g.nreal, i) * g.ntiles;
-            rec.x = pos[(long long)axes * i + 0];
-            rec.y = pos[(long long)axes * i + 1];
-            rec.z = (axes == 3) ? pos[(long long)axes * i + 2] : 0.f;
-            rec.w = W ? W[i] : 1.0f;
-        }
-        const int nc = valid ? nx * ny * nz : 0;
-        for (int j = 0;; ++j) {
-            const bool act = j < nc;
-            if (!__any(act)) break;
-            unsigned key = kEmpty;
-            if (act) {
-                int a = j % nx, b = (j / nx) % ny, c = j / (nx * ny);
-                key = (unsigned)(rbase + ((long long)tx[a] * g.nt[1] + ty[b]) * g.nt[2] + tz[c]);
-            }
-            unsigned long long pend = __ballot(act);
-            unsigned long long slot = 0;
-            while (pend) {
-                const int leader = __ffsll((unsigned long long)pend) - 1;
-                const unsigned lk = __shfl(key, leader);
-                const unsigned long long grp = __ballot(act && key == lk) & pend;
-                unsigned long long gb = 0;
-                if (lane == leader) {
-                    const unsigned cnt = (unsigned)__popcll(grp);
-                    const int s = tab_find(tkey, lk);
-                    if (s >= 0) gb = offsets[lk] + atomicAdd(&tcur[s], cnt);
-                    else gb = offsets[lk] + counts[lk] + atomicAdd(&ocursors[lk], cnt);
-                }
-                gb = __shfl(gb, leader);
-                if ((grp >> lane) & 1ull) slot = gb + __popcll(grp & ((1ull << lane) - 1ull));
-                pend &= ~grp;
-            }
-            if (act) records[slot] = rec;
+    unsigned m = 0;
+    for (long long i = ch.c0 + threadIdx.x; i < ch.c1; i += blockDim.x) {
+        float3 d;
+        const unsigned key = particle_key<MAS>(pos, i, axes, inv, g, rbase, d);
+        const float w = W ? W[i] : 1.0f;
+        const unsigned s = key - wb;
+        unsigned long long slot = offsets[key];
+        if (s < (unsigned)kWin) slot += atomicAdd(&cur[s], 1u);
+        else slot += (unsigned long long)counts[key] + atomicAdd(&ocursors[key], 1u);
+        records[slot] = make_float4(d.x, d.y, d.z, w);
+        const unsigned b = __float_as_uint(fabsf(w));
+        if (b < 0x7f800000u) m = max(m, b);
+    }
+    if (W) {
+        // block max through the (no longer needed) cursor table
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) cur[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned mm = 0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mm = max(mm, cur[k]);
+            atomicMax(&wmax_bits[ch.r], mm);
         }
     }
 }
@@ -430,67 +399,96 @@ scan_final_kernel(const unsigned *__restrict__ in, const unsigned *__restrict__ 
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) out[n] = ex;
 }
 
-// Pass 4: one workgroup per tile.  LDS accumulator of T0*T1*T2 doubles holding
-// integer multiples of 2^q (exact sums), then each node written once.
+// ---- deposit -----------------------------------------------------------------
+// Output transform of a final node value v: overwrite ? v/scale - shift : grid + v.
+struct OutMode {
+    int overwrite;
+    float scale, shift;
+};
+
+__device__ __forceinline__ void store_node(float *__restrict__ grid, long long node, float v,
+                                           const OutMode &om) {
+    grid[node] = om.overwrite ? v / om.scale - om.shift : grid[node] + v;
+}
+
+struct TileBox {
+    int tx, ty, tz;        // tile coordinates
+    int ox, oy, oz;        // origin
+    int ex, ey, ez;        // extent
+};
+
+template <int AXES>
+__host__ __device__ __forceinline__ TileBox tile_box(int tile, const TileGeom &g) {
+    TileBox b;
+    b.tz = tile % g.ntz;
+    b.ty = (tile / g.ntz) % g.nt;
+    b.tx = tile / (g.ntz * g.nt);
+    b.ox = g.orig[b.tx];
+    b.oy = g.orig[b.ty];
+    b.ex = g.orig[b.tx + 1] - b.ox;
+    b.ey = g.orig[b.ty + 1] - b.oy;
+    if (AXES == 3) {
+        b.oz = g.orig[b.tz];
+        b.ez = g.orig[b.tz + 1] - b.oz;
+    } else {
+        b.oz = 0;
+        b.ez = 1;
+    }
+    return b;
+}
+
+// One workgroup per tile.  LDS box of BX*BY*BZ doubles (the tile plus the
+// upper halo) holding integer multiples of 2^q; every record adds its whole
+// footprint at fixed offsets from its first node.
 template <int MAS, int AXES>
 __global__ void __launch_bounds__(kBlock)
 tile_deposit_kernel(const float4 *__restrict__ records,
-                    const unsigned long long *__restrict__ offsets, float inv, TileGeom g,
-                    int weighted, float weight_factor, float *__restrict__ grid, int overwrite) {
+                    const unsigned long long *__restrict__ offsets, TileGeom g, int weighted,
+                    float weight_factor, const unsigned *__restrict__ wmax_bits,
+                    float *__restrict__ grid, float *__restrict__ partials, OutMode om) {
     constexpr int F = MAS;
-    extern __shared__ double acc[];
-    __shared__ unsigned wred[kBlock / 64];
-    const long long key = blockIdx.x;                 // r * ntiles + tile
-    const long long real = key / g.ntiles;
-    const long long tile = key - real * g.ntiles;
+    constexpr int T = (AXES == 3) ? kT3 : kT2;
+    constexpr int BY = T + F - 1;
+    constexpr int BZ = (AXES == 3) ? T + F - 1 : 1;
+    constexpr int BX = T + F - 1;
+    constexpr int NB = BX * BY * BZ;
+    __shared__ double acc[NB];
+    const int key = blockIdx.x;                       // r * ntiles + tile
+    const int real = key / (int)g.ntiles;
+    const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
     grid += real * g.ncell;
-    const int tz = (int)(tile % g.nt[2]);
-    const int ty = (int)((tile / g.nt[2]) % g.nt[1]);
-    const int tx = (int)(tile / ((long long)g.nt[2] * g.nt[1]));
-    const int ox = tx * g.T[0], oy = ty * g.T[1], oz = tz * g.T[2];
-    const int ex = min(g.T[0], g.dims - ox);
-    const int ey = min(g.T[1], g.dims - oy);
-    const int ez = (AXES == 3) ? min(g.T[2], g.dims - oz) : 1;
-    const int ncell = ex * ey * ez;
     const unsigned long long r0 = offsets[key], r1 = offsets[key + 1];
+    const int ex = tb.ex, ey = tb.ey, ez = tb.ez;
 
     if (r0 == r1) {
-        if (overwrite) {
-            for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-                int lz = c % ez, ly = (c / ez) % ey, lx = c / (ez * ey);
-                long long node = AXES == 3
-                    ? ((long long)(ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz)
-                    : (long long)(ox + lx) * g.dims + (oy + ly);
-                grid[node] = 0.0f;
+        // Empty tile: its interior is zero; the finalize reads no halo from it.
+        if (om.overwrite) {
+            const int n = ex * ey * ez;
+            for (int c = threadIdx.x; c < n; c += blockDim.x) {
+                const int lz = c % ez, ly = (c / ez) % ey, lx = c / (ez * ey);
+                if (lx < F - 1 || ly < F - 1 || (AXES == 3 && lz < F - 1)) continue;
+                const long long node = AXES == 3
+                    ? ((long long)(tb.ox + lx) * g.dims + (tb.oy + ly)) * g.dims + (tb.oz + lz)
+                    : (long long)(tb.ox + lx) * g.dims + (tb.oy + ly);
+                store_node(grid, node, 0.0f, om);
             }
         }
         return;
     }
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) acc[c] = 0.0;
+    for (int c = threadIdx.x; c < NB; c += blockDim.x) acc[c] = 0.0;
 
-    // Per-tile bound on |contribution|: B-spline weights are <= 1, so every
-    // |c| <= weight_factor * max finite |W| of the tile's records.
-    float wmax = 1.0f;
-    if (weighted) {
-        unsigned m = 0;
-        for (unsigned long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-            unsigned b = __float_as_uint(fabsf(records[r].w));
-            if (b < 0x7f800000u) m = max(m, b);
-        }
-        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
-        if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = m;
-        __syncthreads();
-        m = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = max(m, wred[i]);
-        wmax = __uint_as_float(m);
-    }
-    // Quantum 2^q: sum |c| <= count*wmax*factor < 2^e and 2^e/2^q = 2^52, so all
-    // partial sums of rint(c/2^q) are integers below 2^53: exact in float64.
+    // Quantum 2^q: the footprint weights of a particle sum to 1, so the sum of
+    // the tile's |contributions| is at most count * weight_factor * max finite
+    // |W| of the realisation (to float rounding), which is < 2^e.  With
+    // 2^e / 2^q = 2^52 every partial sum of rint(c / 2^q) is an integer below
+    // 2^53: exact in float64.
+    const float wmax = weighted ? __uint_as_float(wmax_bits[real]) : 1.0f;
     const double bound = (double)(r1 - r0) * (double)wmax * (double)weight_factor;
     int e = -1000;
     if (bound > 0.0) frexp(bound, &e);
     int q = e - 52;
     if (q < -127) q = -127;                 // keep 2^-q a normal float
+    if (q > 127) q = 127;
     const float to_q = ldexpf(1.0f, -q);
     const double from_q = ldexp(1.0, q);
     __syncthreads();
@@ -498,47 +496,141 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     for (unsigned long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const float4 rec = records[r];
         float Cx[4], Cy[4], Cz[4];
-        const int bx = axis_foot<MAS>(rec.x, inv, g.dims, Cx);
-        const int by = axis_foot<MAS>(rec.y, inv, g.dims, Cy);
-        int bz = 0;
-        if (AXES == 3) bz = axis_foot<MAS>(rec.z, inv, g.dims, Cz);
+        const int rx = first_node<MAS>(rec.x);
+        const int ry = first_node<MAS>(rec.y);
+        foot_weights<MAS>(rec.x, rx, Cx);
+        foot_weights<MAS>(rec.y, ry, Cy);
+        int lx = wrap_node(rx, g.dims) - tb.ox;
+        int ly = wrap_node(ry, g.dims) - tb.oy;
+        lx = min(max(lx, 0), ex - 1);       // records of this tile satisfy 0 <= l < e
+        ly = min(max(ly, 0), ey - 1);
+        int lz = 0;
+        if (AXES == 3) {
+            const int rz = first_node<MAS>(rec.z);
+            foot_weights<MAS>(rec.z, rz, Cz);
+            lz = wrap_node(rz, g.dims) - tb.oz;
+            lz = min(max(lz, 0), ez - 1);
+        }
         // ((Cx*Cy)*Cz)*(W*2^-q) == 2^-q * (((Cx*Cy)*Cz)*W) exactly (power-of-two
         // scaling commutes with float rounding), so the scaled product is the
         // reference's float contribution in units of the quantum.
         const float w = (rec.w * weight_factor) * to_q;
+        double *a = acc + (lx * BY + ly) * BZ + lz;
 #pragma unroll
         for (int l = 0; l < F; ++l) {
-            const int lx = node_add(bx, l, g.dims) - ox;
-            if (lx < 0 || lx >= ex) continue;
 #pragma unroll
             for (int m = 0; m < F; ++m) {
-                const int ly = node_add(by, m, g.dims) - oy;
-                if (ly < 0 || ly >= ey) continue;
                 const float cxy = Cx[l] * Cy[m];
                 if (AXES == 3) {
-                    const int row = (lx * ey + ly) * ez;
 #pragma unroll
                     for (int k = 0; k < F; ++k) {
-                        const int lz = node_add(bz, k, g.dims) - oz;
-                        if (lz < 0 || lz >= ez) continue;
                         const float c = (cxy * Cz[k]) * w;
-                        atomicAdd(&acc[row + lz], (double)rintf(c));
+                        atomicAdd(a + (l * BY + m) * BZ + k, (double)rintf(c));
                     }
                 } else {
                     const float c = cxy * w;   // the reference multiplies by C[2][n] = 1.0
-                    atomicAdd(&acc[lx * ey + ly], (double)rintf(c));
+                    atomicAdd(a + l * BY + m, (double)rintf(c));
                 }
             }
         }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        int lz = c % ez, ly = (c / ez) % ey, lx = c / (ez * ey);
-        long long node = AXES == 3
-            ? ((long long)(ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz)
-            : (long long)(ox + lx) * g.dims + (oy + ly);
-        float v = (float)(acc[c] * from_q);
-        grid[node] = overwrite ? v : grid[node] + v;
+    float *part = partials + (long long)key * g.pslot;
+    for (int c = threadIdx.x; c < NB; c += blockDim.x) {
+        const int bz = c % BZ, by = (c / BZ) % BY, bx = c / (BZ * BY);
+        if (bx >= ex + F - 1 || by >= ey + F - 1 || (AXES == 3 && bz >= ez + F - 1)) continue;
+        const float v = (float)(acc[c] * from_q);
+        const bool inner = bx >= F - 1 && bx < ex && by >= F - 1 && by < ey &&
+                           (AXES == 2 || (bz >= F - 1 && bz < ez));
+        if (inner) {
+            const long long node = AXES == 3
+                ? ((long long)(tb.ox + bx) * g.dims + (tb.oy + by)) * g.dims + (tb.oz + bz)
+                : (long long)(tb.ox + bx) * g.dims + (tb.oy + by);
+            store_node(grid, node, v, om);
+        } else {
+            part[halo_index<F, AXES>(bx, by, bz, ex, ey, ez)] = v;
+        }
+    }
+}
+
+// Images of tile-local coordinate l along one axis: (tile, box coordinate)
+// pairs whose box cell lands on l.  Own tile at l; with nt >= 2 the lower
+// neighbour's halo at l + e(t-1) when l < F-1; with one tile along the axis
+// the tile's own halo wraps onto it, l + k*dims for k >= 1.
+template <int F>
+__host__ __device__ __forceinline__ int axis_images(int l, int t, int nt, int dims, const int *orig,
+                                           int *it, int *ib) {
+    it[0] = t;
+    ib[0] = l;
+    int n = 1;
+    if (l >= F - 1) return n;
+    if (nt >= 2) {
+        const int tp = (t == 0) ? nt - 1 : t - 1;
+        it[1] = tp;
+        ib[1] = l + (orig[tp + 1] - orig[tp]);
+        return 2;
+    }
+    for (int b = l + dims; b < dims + F - 1; b += dims) {
+        it[n] = 0;
+        ib[n] = b;
+        ++n;
+    }
+    return n;
+}
+
+// Band cells (l < F-1 along some axis) of every tile: own band value plus the
+// halo values of the lower neighbours, summed in double in a fixed order, then
+// written once.  Empty tiles contribute nothing.
+template <int MAS, int AXES>
+__global__ void __launch_bounds__(kBlock)
+tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
+                     const float *__restrict__ partials, float *__restrict__ grid, OutMode om) {
+    constexpr int F = MAS;
+    const int key = blockIdx.x;
+    const int real = key / (int)g.ntiles;
+    const long long kbase = (long long)real * g.ntiles;
+    const TileBox tb = tile_box<AXES>(key - (int)kbase, g);
+    grid += real * g.ncell;
+    const int ex = tb.ex, ey = tb.ey, ez = tb.ez;
+    const int nbx = min(F - 1, ex), nby = min(F - 1, ey), nbz = (AXES == 3) ? min(F - 1, ez) : 0;
+    const int sa = nbx * ey * ez;
+    const int sb = (ex - nbx) * nby * ez;
+    const int sc = (AXES == 3) ? (ex - nbx) * (ey - nby) * nbz : 0;
+    const int nband = sa + sb + sc;
+    for (int t = threadIdx.x; t < nband; t += blockDim.x) {
+        int lx, ly, lz;
+        if (t < sa) {
+            lz = t % ez; ly = (t / ez) % ey; lx = t / (ez * ey);
+        } else if (t < sa + sb) {
+            const int u = t - sa;
+            lz = u % ez; ly = (u / ez) % nby; lx = nbx + u / (ez * nby);
+        } else {
+            const int u = t - sa - sb;
+            lz = u % nbz; ly = nby + (u / nbz) % (ey - nby); lx = nbx + u / (nbz * (ey - nby));
+        }
+        int itx[4], ibx[4], ity[4], iby[4], itz[4], ibz[4];
+        const int nx = axis_images<F>(lx, tb.tx, g.nt, g.dims, g.orig, itx, ibx);
+        const int ny = axis_images<F>(ly, tb.ty, g.nt, g.dims, g.orig, ity, iby);
+        int nz = 1;
+        itz[0] = 0;
+        ibz[0] = 0;
+        if (AXES == 3) nz = axis_images<F>(lz, tb.tz, g.ntz, g.dims, g.orig, itz, ibz);
+        double v = 0.0;
+        for (int a = 0; a < nx; ++a)
+            for (int b = 0; b < ny; ++b)
+                for (int c = 0; c < nz; ++c) {
+                    const long long vkey = kbase + ((long long)itx[a] * g.nt + ity[b]) * g.ntz + itz[c];
+                    if (offsets[vkey] == offsets[vkey + 1]) continue;
+                    const int vex = g.orig[itx[a] + 1] - g.orig[itx[a]];
+                    const int vey = g.orig[ity[b] + 1] - g.orig[ity[b]];
+                    const int vez = (AXES == 3) ? g.orig[itz[c] + 1] - g.orig[itz[c]] : 1;
+                    v += (double)partials[vkey * g.pslot +
+                                          halo_index<F, AXES>(ibx[a], iby[b], ibz[c], vex, vey, vez)];
+                }
+        const long long node = AXES == 3
+            ? ((long long)(tb.ox + lx) * g.dims + (tb.oy + ly)) * g.dims + (tb.oz + lz)
+            : (long long)(tb.ox + lx) * g.dims + (tb.oy + ly);
+        store_node(grid, node, (float)v, om);
     }
 }
 
@@ -616,6 +708,13 @@ __global__ void scale_grid_kernel(float *__restrict__ grid, long long n, float d
         grid[i] = grid[i] / divisor;
 }
 
+// grid = grid / scale - shift (the output transform of the atomic form).
+__global__ void delta_grid_kernel(float *__restrict__ grid, long long n, float scale, float shift) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        grid[i] = grid[i] / scale - shift;
+}
+
 int grid_blocks(long long n, int block) {
     long long b = (n + block - 1) / block;
     if (b < 1) b = 1;
@@ -623,73 +722,84 @@ int grid_blocks(long long n, int block) {
     return (int)b;
 }
 
-TileGeom make_geom(int dims, int axes, int nreal) {
+TileGeom make_geom(int dims, int axes, int mas) {
     TileGeom g{};
+    const int T = axes == 3 ? kT3 : kT2;
     g.dims = dims;
-    g.dz = axes == 3 ? dims : 1;
-    if (axes == 3) {
-        g.T[0] = g.T[1] = g.T[2] = dims < 16 ? dims : 16;
-    } else {
-        g.T[0] = g.T[1] = dims < 64 ? dims : 64;
-        g.T[2] = 1;
-    }
-    g.nt[0] = ceil_div(dims, g.T[0]);
-    g.nt[1] = ceil_div(dims, g.T[1]);
-    g.nt[2] = axes == 3 ? ceil_div(dims, g.T[2]) : 1;
-    g.ntiles = (long long)g.nt[0] * g.nt[1] * g.nt[2];
+    g.nt = dims <= T ? 1 : (dims + T - 1) / T;
+    g.ntz = axes == 3 ? g.nt : 1;
+    g.shift = -1;
+    if (g.nt == 1) g.shift = 30;                       // every node in tile 0
+    else if (dims % T == 0) g.shift = (axes == 3) ? 4 : 6;
+    g.ntiles = (long long)g.nt * g.nt * g.ntz;
     g.ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
-    g.nreal = nreal;
+    const int emax = (dims + g.nt - 1) / g.nt;
+    const int ez = axes == 3 ? emax : 1;
+    int ps = 0;
+    switch (mas) {
+        case PYL_NGP: ps = axes == 3 ? halo_count<1, 3>(emax, emax, ez) : halo_count<1, 2>(emax, emax, ez); break;
+        case PYL_CIC: ps = axes == 3 ? halo_count<2, 3>(emax, emax, ez) : halo_count<2, 2>(emax, emax, ez); break;
+        case PYL_TSC: ps = axes == 3 ? halo_count<3, 3>(emax, emax, ez) : halo_count<3, 2>(emax, emax, ez); break;
+        default: ps = axes == 3 ? halo_count<4, 3>(emax, emax, ez) : halo_count<4, 2>(emax, emax, ez); break;
+    }
+    g.pslot = ps > 0 ? ps : 1;
     return g;
-}
-
-// Particle ranges of a batch on the device (offs[0..nreal]); NULL for one grid.
-int upload_offsets(Ctx *c, const long long *h_offs, int nreal, const long long **d_offs) {
-    *d_offs = nullptr;
-    if (nreal <= 1) return PYL_OK;
-    PYL_TRY(ensure(c, BUF_SCAN_TMP, sizeof(long long) * (size_t)(nreal + 1)));
-    long long *d = (long long *)c->bufs[BUF_SCAN_TMP].ptr;
-    PYL_HIP(hipMemcpyAsync(d, h_offs, sizeof(long long) * (size_t)(nreal + 1),
-                           hipMemcpyHostToDevice, c->stream));
-    PYL_HIP(hipStreamSynchronize(c->stream));   // h_offs may be a temporary
-    *d_offs = d;
-    return PYL_OK;
 }
 
 template <int MAS>
 int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal, int axes,
               const float *W, float *grid, int dims, float inv, float weight_factor,
-              int overwrite) {
-    TileGeom g = make_geom(dims, axes, nreal);
+              const OutMode &om) {
+    TileGeom g = make_geom(dims, axes, MAS);
     const long long nt = g.ntiles * nreal;      // keys over the whole batch
     const int nscan_blocks = (int)((nt + kScanTile - 1) / kScanTile);
     PYL_CHECK(nscan_blocks <= kScanTile, PYL_EINVAL, "too many tiles (%lld)", nt);
     PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
-    const long long *d_offs = nullptr;
-    PYL_TRY(upload_offsets(c, h_offs, nreal, &d_offs));
-    // one chunk of particles per sort workgroup
-    const int nchunk = (int)std::max(1LL, std::min(512LL, (n + 65535) / 65536));
-    const long long chunk = (n + nchunk - 1) / nchunk;
-    // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | pad | offsets[nt+1] |
-    //              block sums | per-chunk tables
-    const size_t nwords = (size_t)nt * 3;
+    // Chunks of consecutive particles, never straddling two realisations.
+    const long long target = std::max(16384LL, std::min(1LL << 20, (n + 1023) / 1024));
+    std::vector<Chunk> chunks;
+    for (int r = 0; r < nreal; ++r) {
+        const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
+        for (long long s = b; s < e; s += target) chunks.push_back({s, std::min(e, s + target), r, 0});
+    }
+    const int nchunk = (int)chunks.size();
+    PYL_CHECK(nchunk < (1 << 30), PYL_EINVAL, "too many particle chunks");
+    // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | wmax[nreal] | pad |
+    //              offsets[nt+1] | block sums | chunk table | per-chunk window bases
+    const size_t nwords = (size_t)nt * 3 + (size_t)nreal;
     const size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t off_bytes = (((size_t)nt + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255;
     const size_t bs_bytes = ((size_t)nscan_blocks * sizeof(unsigned long long) + 255) & ~(size_t)255;
-    const size_t dump_bytes = (size_t)nchunk * kHashCap * sizeof(uint2);
-    PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + dump_bytes));
+    const size_t ch_bytes = ((size_t)std::max(nchunk, 1) * sizeof(Chunk) + 255) & ~(size_t)255;
+    const size_t org_bytes = ((size_t)(g.nt + 1) * sizeof(int) + 255) & ~(size_t)255;
+    const size_t dump_bytes = (size_t)std::max(nchunk, 1) * kWin * sizeof(unsigned);
+    PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
+                                        dump_bytes));
     char *tb = (char *)c->bufs[BUF_TILE_CNT].ptr;
     unsigned *counts = (unsigned *)tb;
     unsigned *ocounts = counts + nt;
     unsigned *ocursors = ocounts + nt;
+    unsigned *wmax = ocursors + nt;
     unsigned long long *offsets = (unsigned long long *)(tb + cnt_bytes);
     unsigned long long *bsums = (unsigned long long *)(tb + cnt_bytes + off_bytes);
-    uint2 *dump = (uint2 *)(tb + cnt_bytes + off_bytes + bs_bytes);
+    Chunk *dchunks = (Chunk *)(tb + cnt_bytes + off_bytes + bs_bytes);
+    int *dorig = (int *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes);
+    unsigned *dump = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes);
+    std::vector<int> orig(g.nt + 1);
+    for (int i = 0; i <= g.nt; ++i) orig[i] = tile_origin(i, g.nt, dims);
+    g.orig = dorig;
+    PYL_TRY(ensure(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float)));
+    float *partials = (float *)c->bufs[BUF_PARTIAL].ptr;
 
     PYL_HIP(hipMemsetAsync(counts, 0, nwords * sizeof(unsigned), c->stream));
-    if (n > 0) {
+    PYL_HIP(hipMemcpyAsync(dorig, orig.data(), orig.size() * sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+    if (nchunk > 0) {
+        PYL_HIP(hipMemcpyAsync(dchunks, chunks.data(), (size_t)nchunk * sizeof(Chunk),
+                               hipMemcpyHostToDevice, c->stream));
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
-        tile_count_kernel<MAS><<<nchunk, kSortBlock, 0, c->stream>>>(pos, n, axes, inv, g, d_offs,
-                                                                     chunk, counts, ocounts, dump);
+        tile_count_kernel<MAS><<<nchunk, kSortBlock, 0, c->stream>>>(pos, axes, inv, g, dchunks,
+                                                                     counts, ocounts, dump);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
     }
@@ -702,25 +812,33 @@ int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, in
     PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
     unsigned long long total = 0;
     PYL_HIP(hipMemcpyAsync(&total, offsets + nt, sizeof(total), hipMemcpyDeviceToHost, c->stream));
-    PYL_HIP(hipStreamSynchronize(c->stream));
+    PYL_HIP(hipStreamSynchronize(c->stream));   // also keeps `chunks`, `orig` alive until uploaded
+    PYL_CHECK(total == (unsigned long long)n, PYL_EHIP, "tile sort lost particles (%llu of %lld)",
+              total, n);
     PYL_TRY(ensure(c, BUF_RECORDS, (size_t)(total ? total : 1) * sizeof(float4)));
     float4 *records = (float4 *)c->bufs[BUF_RECORDS].ptr;
-    if (n > 0) {
+    if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
         tile_scatter_kernel<MAS><<<nchunk, kSortBlock, 0, c->stream>>>(
-            pos, W, n, axes, inv, g, d_offs, chunk, dump, offsets, counts, ocursors, records);
+            pos, W, axes, inv, g, dchunks, dump, offsets, counts, ocursors, wmax, records);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
     }
-    const size_t lds = (size_t)g.T[0] * g.T[1] * g.T[2] * sizeof(double);
     const int weighted = W != nullptr;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_DEPOSIT));
-    if (axes == 3)
-        tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kBlock, lds, c->stream>>>(
-            records, offsets, inv, g, weighted, weight_factor, grid, overwrite);
-    else
-        tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kBlock, lds, c->stream>>>(
-            records, offsets, inv, g, weighted, weight_factor, grid, overwrite);
+    if (axes == 3) {
+        tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(
+            records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
+        if (MAS > 1)
+            tile_finalize_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
+                                                                                grid, om);
+    } else {
+        tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kBlock, 0, c->stream>>>(
+            records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
+        if (MAS > 1)
+            tile_finalize_kernel<MAS, 2><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
+                                                                                grid, om);
+    }
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));
     return PYL_OK;
@@ -729,11 +847,10 @@ int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, in
 template <int MAS>
 int run_atomic(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal,
                int axes, const float *W, float *grid, int dims, float inv, float weight_factor,
-               int overwrite) {
+               const OutMode &om) {
     const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
-    if (overwrite)
+    if (om.overwrite)
         PYL_HIP(hipMemsetAsync(grid, 0, (size_t)ncell * nreal * sizeof(float), c->stream));
-    if (n == 0) return PYL_OK;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_ATOMIC));
     for (int r = 0; r < nreal; ++r) {
         const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
@@ -750,6 +867,11 @@ int run_atomic(Ctx *c, const float *pos, long long n, const long long *h_offs, i
                                                                        weight_factor, gr);
         PYL_HIP(hipGetLastError());
     }
+    if (om.overwrite && (om.scale != 1.0f || om.shift != 0.0f)) {
+        delta_grid_kernel<<<grid_blocks(ncell * nreal, kBlock), kBlock, 0, c->stream>>>(
+            grid, ncell * nreal, om.scale, om.shift);
+        PYL_HIP(hipGetLastError());
+    }
     PYL_TRY(stage_end(c, PYL_STAGE_MA_ATOMIC));
     return PYL_OK;
 }
@@ -758,22 +880,23 @@ int run_atomic(Ctx *c, const float *pos, long long n, const long long *h_offs, i
 
 // Device-pointer MA used by the C ABI and by the batched driver.
 // h_offs: host particle ranges of nreal grids (nreal+1 entries, NULL when nreal == 1);
-// grid holds nreal contiguous grids.
+// grid holds nreal contiguous grids.  With PYL_MA_OVERWRITE the grid receives
+// deposit/scale - shift (scale = 1, shift = 0: the plain deposit).
 int ma_device(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal,
               int axes, const float *W, float *grid, int dims, float BoxSize, int mas,
-              int flags) {
+              int flags, float scale, float shift) {
     // MAS_library.pyx:135 inv_cell_size = dims/BoxSize in float (MAS_c.c:19 computes it
     // in double and stores it to float, which rounds to the same value).
     const float inv = (float)dims / BoxSize;
     float weight_factor = 1.0f;
     const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1) * nreal;
-    const int overwrite = (flags & PYL_MA_OVERWRITE) ? 1 : 0;
+    OutMode om{(flags & PYL_MA_OVERWRITE) ? 1 : 0, scale, shift};
     if (axes == 2 && (flags & PYL_MA_CYTHON_2D) && mas != PYL_NGP) {
         // MA() runs the 3D kernel on a (d,d,1) view, so each deposit lands F times
         // in the same cell; with renormalize_2D the whole grid is then divided by F
         // (MAS_library.pyx:84-110).  Equivalent: grid/F + deposit, or grid + F*deposit.
         if (flags & PYL_MA_RENORM_2D) {
-            if (!overwrite) {
+            if (!om.overwrite) {
                 scale_grid_kernel<<<grid_blocks(ncell, kBlock), kBlock, 0, c->stream>>>(
                     grid, ncell, (float)mas);
                 PYL_HIP(hipGetLastError());
@@ -787,9 +910,9 @@ int ma_device(Ctx *c, const float *pos, long long n, const long long *h_offs, in
 #define PYL_MA_CASE(M)                                                                      \
     case M:                                                                                 \
         return atomic ? run_atomic<M>(c, pos, n, h_offs, nreal, axes, W, grid, dims, inv,    \
-                                      weight_factor, overwrite)                             \
+                                      weight_factor, om)                                    \
                       : run_tiled<M>(c, pos, n, h_offs, nreal, axes, W, grid, dims, inv,     \
-                                     weight_factor, overwrite);
+                                     weight_factor, om);
         PYL_MA_CASE(PYL_NGP)
         PYL_MA_CASE(PYL_CIC)
         PYL_MA_CASE(PYL_TSC)
@@ -806,7 +929,7 @@ using namespace pyl;
 
 static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *offs, int nreal,
                     int axes, const float *W, float *grid, int dims, float BoxSize, int mas,
-                    int flags) {
+                    int flags, float scale = 1.0f, float shift = 0.0f) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
     Ctx *c = &ctx->c;
     PYL_CHECK(axes == 2 || axes == 3, PYL_EINVAL, "pos must have 2 or 3 columns (got %d)", axes);
@@ -815,12 +938,14 @@ static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *
     PYL_CHECK(n >= 0, PYL_EINVAL, "negative particle count");
     PYL_CHECK(grid != nullptr && (pos != nullptr || n == 0), PYL_EINVAL, "NULL array");
     PYL_CHECK(BoxSize > 0.0f, PYL_EINVAL, "BoxSize must be positive");
+    PYL_CHECK(scale != 0.0f, PYL_EINVAL, "scale must be non-zero");
     const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1) * nreal;
     PYL_CHECK(ncell < (1ll << 34), PYL_EINVAL, "grid too large");
     PYL_HIP(hipSetDevice(c->device));
     reset_stages(c);
     if (flags & PYL_PTR_DEVICE) {
-        PYL_TRY(ma_device(c, pos, n, offs, nreal, axes, W, grid, dims, BoxSize, mas, flags));
+        PYL_TRY(ma_device(c, pos, n, offs, nreal, axes, W, grid, dims, BoxSize, mas, flags, scale,
+                          shift));
         return finish(c, flags);
     }
     // Host pointers: stage through device scratch.
@@ -835,7 +960,7 @@ static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *
     if (!(flags & PYL_MA_OVERWRITE))
         PYL_HIP(hipMemcpyAsync(dgrid, grid, (size_t)ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
     PYL_TRY(ma_device(c, dpos, n, offs, nreal, axes, dW, dgrid, dims, BoxSize, mas,
-                      flags | PYL_PTR_DEVICE));
+                      flags | PYL_PTR_DEVICE, scale, shift));
     PYL_HIP(hipMemcpyAsync(grid, dgrid, (size_t)ncell * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     PYL_HIP(hipStreamSynchronize(c->stream));
     return PYL_OK;
@@ -844,6 +969,13 @@ static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *
 extern "C" int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
                       float *grid, int dims, float BoxSize, int mas, int flags) {
     return ma_entry(ctx, pos, n, nullptr, 1, axes, W, grid, dims, BoxSize, mas, flags);
+}
+
+extern "C" int pyl_ma_delta(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
+                            float *delta, int dims, float BoxSize, int mas, float scale,
+                            float shift, int flags) {
+    return ma_entry(ctx, pos, n, nullptr, 1, axes, W, delta, dims, BoxSize, mas,
+                    flags | PYL_MA_OVERWRITE, scale, shift);
 }
 
 extern "C" int pyl_ma_batch(pyl_ctx *ctx, const float *pos, const int64_t *offsets, int nreal,

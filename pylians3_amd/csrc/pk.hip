@@ -565,18 +565,6 @@ int set_lds(K kernel, size_t bytes) {
     return PYL_OK;
 }
 
-// Host-side MAS tables: corr[kk] = MAS_correction(pi/dims * k(kk), p) with the
-// reference's libm calls (Pk_library.pyx:83-84, prefact :310).
-void mas_table(int dims, int p, double *out) {
-    const int m = dims / 2;
-    const double prefact = M_PI / dims;
-    for (int kk = 0; kk < dims; ++kk) {
-        int k = kk > m ? kk - dims : kk;
-        double x = prefact * k;
-        out[kk] = (x == 0.0) ? 1.0 : std::pow(x / std::sin(x), (double)p);
-    }
-}
-
 // Run the binning of NFL fields (device half-complex arrays) -> reduced sums
 // appended into host arrays of spectra [s_off, s_off+S).
 struct HostAcc {

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cmath>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -89,6 +90,7 @@ enum BufId {
     BUF_BINS,        // per-bin accumulators
     BUF_MAS_TAB,     // MAS correction tables
     BUF_VEL,
+    BUF_PARTIAL,     // per-tile halo partial sums of the tiled deposit
     BUF_COUNT_
 };
 
@@ -98,7 +100,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     size_t budget = 0;
     DevBuf bufs[BUF_COUNT_];
-    std::map<std::pair<int, int>, FFTPlan> fft_plans;      // (dims, batch)
+    std::map<std::tuple<int, int, int>, FFTPlan> fft_plans; // (dims, batch, kind)
     std::map<std::pair<int, int>, ModePlan> mode_plans;    // (dims, axis)
     std::map<std::tuple<int, int, int>, BinPlan> bin_plans; // (dims, axis, rows per item)
     bool timing = false;
@@ -115,8 +117,23 @@ int stage_end(Ctx *c, int stage);
 int finish(Ctx *c, int flags);   // synchronise unless PYL_NO_SYNC on device pointers
 
 // FFT plan (resident, one per (dims, batch))
-int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out);
+// Transform kinds of the resident rocFFT plans (all single precision, out of place).
+enum FFTKind { FFT_R2C_3D = 0, FFT_C2R_3D = 1, FFT_R2C_2D = 2 };
+int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out, int kind = FFT_R2C_3D);
 int run_fft(Ctx *c, FFTPlan *p, void *in, void *out);
+
+// Host-side MAS tables: out[kk] = MAS_correction(pi/dims * k(kk), p) for the
+// signed frequency k(kk) of FFT index kk, with the reference's libm calls
+// (Pk_library.pyx:83-84, prefact :310).
+inline void mas_table(int dims, int p, double *out) {
+    const int m = dims / 2;
+    const double prefact = M_PI / dims;
+    for (int kk = 0; kk < dims; ++kk) {
+        int k = kk > m ? kk - dims : kk;
+        double x = prefact * k;
+        out[kk] = (x == 0.0) ? 1.0 : std::pow(x / std::sin(x), (double)p);
+    }
+}
 
 // Mode-count plan (host computed, cached)
 const ModePlan *get_mode_plan(Ctx *c, int dims, int axis);

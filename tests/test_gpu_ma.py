@@ -201,3 +201,75 @@ def test_cic_interp_vs_oracle(oracle):
         tden = torch.zeros(40000, device='cuda')
         MASL.CIC_interp(torch.from_numpy(g).cuda(), box, torch.from_numpy(pos).cuda(), tden)
         np.testing.assert_array_equal(tden.cpu().numpy(), den)
+
+
+def _lattice(d, box, seed, disp=3.0):
+    """d^3 particles in lattice (C) order, displaced by disp cells rms: the coherent
+    input order of a simulation snapshot."""
+    rng = np.random.default_rng(seed)
+    q = (np.arange(d) + 0.5) * (box / d)
+    p = np.stack(np.meshgrid(q, q, q, indexing='ij'), -1).reshape(-1, 3)
+    p = p + rng.normal(0.0, disp * box / d, p.shape)
+    return (p % box).astype(np.float32) % np.float32(box)
+
+
+@pytest.mark.parametrize('mas', SCHEMES)
+def test_ma_lattice_order_vs_oracle(oracle, mas):
+    # many sort chunks, coherent keys: the LDS-window counting path
+    print('case lattice', mas, flush=True)
+    box, d = 300.0, 80
+    pos = _lattice(d, box, 21)
+    ref = np.zeros((d,) * 3, np.float32)
+    oracle.ma_f32(pos, ref, box, mas)
+    g = _gpu_ma(pos, d, box, mas)
+    assert_cells(g, ref, what='lattice %s' % mas, nadds=contrib_counts(pos, d, box, mas))
+
+
+def test_ma_many_tiles_random_order(oracle):
+    # more tiles than the LDS window holds, random order: most keys take the
+    # global overflow counters
+    import torch
+    from pylians3_amd import MAS_library as MASL
+    print('case overflow keys', flush=True)
+    box, d, n = 500.0, 432, 150000
+    pos = _pos(n, 3, box, 22)
+    W = np.random.default_rng(23).random(n).astype(np.float32)
+    tg = torch.zeros((d,) * 3, dtype=torch.float32, device='cuda')
+    MASL.MA(torch.from_numpy(pos).cuda(), tg, box, 'TSC', torch.from_numpy(W).cuda())
+    g = tg.cpu().numpy()
+    del tg
+    ref = np.zeros((d,) * 3, np.float32)
+    oracle.ma_f32(pos, ref, box, 'TSC', W)
+    assert_cells(g, ref, what='overflow keys')
+
+
+@pytest.mark.parametrize('mas', ['CIC', 'TSC', 'PCS'])
+def test_ma_delta_equals_ma_then_normalise(oracle, mas):
+    # pyl_ma_delta == MA (overwrite) followed by "delta /= mean; delta -= 1" in float32
+    import ctypes
+    import torch
+    from pylians3_amd import _native as N
+    box, d = 200.0, 48
+    pos = torch.from_numpy(_lattice(d, box, 24)).cuda()
+    mean = np.float32(pos.shape[0] / d ** 3)
+    ctx = N.default_context()
+    a = torch.empty((d,) * 3, dtype=torch.float32, device='cuda')
+    b = torch.empty_like(a)
+    mid = {'CIC': N.CIC, 'TSC': N.TSC, 'PCS': N.PCS}[mas]
+    with N.torch_stream(ctx, True):
+        N.check(N.lib().pyl_ma(ctx.handle, N.device_ptr(pos), pos.shape[0], 3, None, N.device_ptr(a),
+                               d, box, mid, N.PTR_DEVICE | N.MA_OVERWRITE))
+        N.check(N.lib().pyl_ma_delta(ctx.handle, N.device_ptr(pos), pos.shape[0], 3, None,
+                                     N.device_ptr(b), d, box, mid, float(mean), 1.0, N.PTR_DEVICE))
+    torch.cuda.synchronize()
+    ah = a.cpu().numpy()
+    expect = ah / mean - np.float32(1.0)
+    np.testing.assert_array_equal(b.cpu().numpy(), expect)
+    # the atomic form applies the same transform
+    c = torch.empty_like(a)
+    with N.torch_stream(ctx, True):
+        N.check(N.lib().pyl_ma_delta(ctx.handle, N.device_ptr(pos), pos.shape[0], 3, None,
+                                     N.device_ptr(c), d, box, mid, float(mean), 1.0,
+                                     N.PTR_DEVICE | N.MA_ATOMIC))
+    torch.cuda.synchronize()
+    assert_cells(c.cpu().numpy() + 1.0, expect + 1.0, what='atomic delta')
