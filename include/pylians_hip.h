@@ -13,6 +13,7 @@
  * Reference interfaces replaced (library/ paths are in the reference tree):
  *   pyl_ma            MAS_library.MA                library/MAS_library/MAS_library.pyx:57-112
  *   pyl_ma_delta      MA + "delta /= mean; delta -= 1.0"  library/Pk_library/Pk_snapshot.py:80-88
+ *   pyl_ma_rsd        pos_redshift_space + MA       redshift_space_library.pyx:29-46 + MAS_library.pyx:57-112
  *                     and its kernels NGP..PCSW      MAS_library.pyx:123-545
  *   NGP/CIC/TSC/PCS   the C kernels, same signature  library/MAS_library/MAS_c.h:3-10 (MAS_c.c:8-266)
  *   pyl_fft_r2c       FFT3Dr_f                       library/Pk_library/Pk_library.pyx:117-130
@@ -114,6 +115,16 @@ int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
 int pyl_ma_delta(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
                  float *delta, int dims, float BoxSize, int mas, float scale, float shift,
                  int flags);
+
+/* MA of redshift-space positions, the shift of pos_redshift_space
+ * (library/redshift_space_library/redshift_space_library.pyx:29-46) fused into the
+ * position load: each particle is deposited at pos[:,axis] + vel[:,axis]*(1+z)/H,
+ * wrapped as the reference does; pos itself is not modified.  pos and vel are
+ * (n,3) float32.  With PYL_MA_OVERWRITE the grid receives deposit/scale - shift
+ * (as pyl_ma_delta); without it, grid += deposit and scale/shift must be 1/0. */
+int pyl_ma_rsd(pyl_ctx *ctx, const float *pos, const float *vel, int64_t n, const float *W,
+               float *grid, int dims, float BoxSize, int mas, float Hubble, float redshift,
+               int axis, float scale, float shift, int flags);
 
 /* Batched MA: nreal independent grids in one launch sequence.  Particles of grid r are
  * pos[offsets[r] .. offsets[r+1]) (host array, nreal+1 entries, offsets[0] = 0), W is

@@ -268,3 +268,46 @@ def MA_atomic(pos, number, BoxSize, MAS='CIC', W=None):
     _check_f32(number, pos.shape[1], 'number')
     flags = N.MA_ATOMIC | (N.MA_CYTHON_2D | N.MA_RENORM_2D if pos.shape[1] == 2 else 0)
     _deposit(pos, number, BoxSize, _MAS_IDS[MAS], W, flags)
+
+
+def MA_rsd(pos, vel, number, BoxSize, Hubble, redshift, axis, MAS='CIC', W=None, overwrite=False,
+           scale=1.0, shift=0.0):
+    """MA of the redshift-space positions pos[:,axis] + vel[:,axis]*(1+z)/H (wrapped as
+    redshift_space_library.pyx:40-46) without modifying pos: the pos_redshift_space
+    + MA sequence of the reference pipelines in one pass over the particles.
+
+    3D only.  number += deposit, or with overwrite=True number = deposit/scale - shift
+    (scale = mean density and shift = 1 give the density contrast directly)."""
+    _check_f32(pos, 2, 'pos')
+    _check_f32(vel, 2, 'vel')
+    _check_f32(number, 3, 'number')
+    if pos.shape[1] != 3 or vel.shape != pos.shape:
+        raise ValueError('pos and vel must both be (N, 3)')
+    if W is not None:
+        _check_f32(W, 1, 'W')
+    if MAS not in _MAS_IDS:
+        print('option not valid!!!')
+        sys.exit()
+    flags = N.MA_OVERWRITE if overwrite else 0
+    dims, n = number.shape[0], pos.shape[0]
+    ctx = N.default_context()
+    arrs = [pos, vel, number] + ([] if W is None else [W])
+    dev = [N.is_device_array(a) for a in arrs]
+    if any(dev) and not all(dev):
+        raise ValueError('pos, vel, number and W must all be on the GPU or all on the host')
+    if all(dev):
+        with N.torch_stream(ctx, True):
+            N.check(N.lib().pyl_ma_rsd(ctx.handle, N.device_ptr(pos), N.device_ptr(vel), n,
+                                       None if W is None else N.device_ptr(W), N.device_ptr(number),
+                                       dims, float(BoxSize), _MAS_IDS[MAS], float(Hubble),
+                                       float(redshift), int(axis), float(scale), float(shift),
+                                       flags | N.PTR_DEVICE))
+        return
+    p, v = np.ascontiguousarray(pos), np.ascontiguousarray(vel)
+    w = None if W is None else np.ascontiguousarray(W)
+    if not number.flags['C_CONTIGUOUS']:
+        raise ValueError('number must be C-contiguous')
+    N.check(N.lib().pyl_ma_rsd(ctx.handle, p.ctypes.data, v.ctypes.data, n,
+                               None if w is None else w.ctypes.data, number.ctypes.data, dims,
+                               float(BoxSize), _MAS_IDS[MAS], float(Hubble), float(redshift),
+                               int(axis), float(scale), float(shift), flags))

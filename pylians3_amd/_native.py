@@ -72,6 +72,10 @@ EXPORTS = {
     'pyl_ma_delta': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int]),
+    'pyl_ma_rsd': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
+                                  ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_int]),
     'pyl_ma_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_float, ctypes.c_int, ctypes.c_int]),

@@ -19,18 +19,19 @@
 //    band cell, the tile's own value and the halo values of its lower
 //    neighbours in a fixed order.
 //    Inside a box, contributions are rounded to integer multiples of a
-//    per-tile power-of-two quantum 2^q chosen so that every partial sum is an
-//    integer below 2^53: the float64 LDS adds are exact, so the box does not
+//    power-of-two quantum 2^q (2^-30 of the realisation's largest |W|) and
+//    summed with 64-bit integer LDS atomics: exact, so the box does not
 //    depend on the order of the atomics.  With the fixed-order finalize the
 //    grid is bitwise reproducible, and identical whether a realisation is
-//    deposited alone or in a batch.  The quantum is ~2^-52 of the tile's
-//    weight bound (count x max|W|), far below float32 resolution.
+//    deposited alone or in a batch.  The rounding is below 2^-31 of the
+//    weight scale per contribution, well under float32 resolution of a cell.
 #include "pyl_internal.h"
 
 namespace pyl {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kDepBlock = 512;     // deposit workgroup (one tile)
 
 __host__ __device__ __forceinline__ int wrapi(int i, int d) {
     int r = i % d;
@@ -158,9 +159,6 @@ __host__ __device__ __forceinline__ int tile_origin(int i, int nt, int dims) {
     return (int)(((long long)i * dims + nt - 1) / nt);
 }
 
-__host__ __device__ __forceinline__ int tile_of(int node, const TileGeom &g) {
-    return g.shift >= 0 ? (node >> g.shift) : (int)(((long long)node * g.nt) / g.dims);
-}
 
 // Number of values of a box (extents e + F - 1 per axis, z extent 1 in 2D)
 // outside its interior [F-1, e) per axis, and the compact index of such a cell.
@@ -203,7 +201,8 @@ __host__ __device__ __forceinline__ int halo_index(int x, int y, int z, int ex, 
 // ---- counting sort by home tile -----------------------------------------------
 // Each sort workgroup owns a chunk of consecutive particles of one realisation.
 // It counts its particles per home tile in an LDS window of kWin consecutive
-// keys centred on its first particle's key (coherent input -- a lattice, a
+// keys (cyclic within the realisation's key range, so the periodic wrap in x
+// stays close) centred on its first particle's key (coherent input -- a lattice, a
 // snapshot in ID order, a sorted catalogue -- keeps almost every key in the
 // window); keys outside the window take a global atomic on a separate
 // overflow counter.  The window counts are flushed with one global atomic per
@@ -216,30 +215,78 @@ constexpr int kWin = 16384;                    // 64 KB of LDS
 struct Chunk {
     long long c0, c1;   // particle range
     int r;              // realisation
-    int wbase;          // first key of the LDS window (written by the count pass)
+    int wbase;          // window start, relative to the realisation's first key (count pass)
 };
 
-template <int MAS>
+// Home tile along one axis; POW2: every tile has extent 2^shift (or nt == 1).
+template <int MAS, bool POW2>
 __host__ __device__ __forceinline__ int home_tile(float dist, const TileGeom &g) {
-    return tile_of(wrap_node(first_node<MAS>(dist), g.dims), g);
+    const int node = wrap_node(first_node<MAS>(dist), g.dims);
+    if constexpr (POW2) return node >> g.shift;
+    else return (int)(((long long)node * g.nt) / g.dims);
 }
 
-template <int MAS>
-__device__ __forceinline__ unsigned particle_key(const float *__restrict__ pos, long long i,
-                                                 int axes, float inv, const TileGeom &g,
-                                                 long long rbase, float3 &dist) {
-    dist.x = pos[(long long)axes * i + 0] * inv;
-    dist.y = pos[(long long)axes * i + 1] * inv;
-    dist.z = (axes == 3) ? pos[(long long)axes * i + 2] * inv : 0.0f;
-    const int tx = home_tile<MAS>(dist.x, g);
-    const int ty = home_tile<MAS>(dist.y, g);
-    const int tz = (axes == 3) ? home_tile<MAS>(dist.z, g) : 0;
+// Optional redshift-space shift applied to one coordinate as the position is
+// read (redshift_space_library.pyx:29-46 fused into the MA load; pos itself is
+// not modified).  When off, vel points at pos with vstride = axes and the shift
+// is skipped, so the extra load stays in bounds and hits the same cache lines.
+struct RSD {
+    const float *vel;
+    float factor;      // (1 + z) / H, rounded to float as in the reference
+    float box;
+    int axis;
+    int vstride;
+    int on;
+};
+
+template <int AXES>
+__device__ __forceinline__ float3 load_dist(const float *__restrict__ pos, const RSD &rsd,
+                                            long long i, float inv) {
+    const float *p = pos + (long long)AXES * i;
+    float x = p[0], y = p[1], z = (AXES == 3) ? p[2] : 0.0f;
+    const float v = rsd.vel[(long long)rsd.vstride * i + rsd.axis];
+    if (rsd.on) {
+        if (rsd.axis == 0) x = rsd_shift(x, v, rsd.factor, rsd.box);
+        else if (rsd.axis == 1) y = rsd_shift(y, v, rsd.factor, rsd.box);
+        else z = rsd_shift(z, v, rsd.factor, rsd.box);
+    }
+    float3 d;
+    d.x = x * inv;
+    d.y = y * inv;
+    d.z = z * inv;
+    return d;
+}
+
+template <int MAS, int AXES, bool POW2>
+__device__ __forceinline__ unsigned dist_key(const float3 &d, const TileGeom &g, long long rbase) {
+    const int tx = home_tile<MAS, POW2>(d.x, g);
+    const int ty = home_tile<MAS, POW2>(d.y, g);
+    const int tz = (AXES == 3) ? home_tile<MAS, POW2>(d.z, g) : 0;
     return (unsigned)(rbase + ((long long)tx * g.nt + ty) * g.ntz + tz);
 }
 
-template <int MAS>
+// Window slot of a key: (key - rbase - w0) mod ntiles; slots >= kWin are outside.
+__device__ __forceinline__ unsigned win_slot(unsigned key, long long rbase, int w0,
+                                             long long ntiles) {
+    long long s = (long long)key - rbase - w0;
+    if (s < 0) s += ntiles;
+    return (unsigned)s;
+}
+
+__device__ __forceinline__ long long win_key(int s, long long rbase, int w0, long long ntiles) {
+    long long k = (long long)w0 + s;
+    if (k >= ntiles) k -= ntiles;
+    return rbase + k;
+}
+
+// Particles per thread and loop trip: the loads of kUnroll particles are issued
+// (unconditionally, at clamped indices) before any of them is used, so their
+// latencies overlap.
+constexpr int kUnroll = 4;
+
+template <int MAS, int AXES, bool POW2>
 __global__ void __launch_bounds__(kSortBlock)
-tile_count_kernel(const float *__restrict__ pos, int axes, float inv, TileGeom g,
+tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
                   Chunk *__restrict__ chunks, unsigned *__restrict__ counts,
                   unsigned *__restrict__ ocounts, unsigned *__restrict__ dump) {
     __shared__ unsigned hist[kWin];
@@ -247,30 +294,39 @@ tile_count_kernel(const float *__restrict__ pos, int axes, float inv, TileGeom g
     const long long rbase = (long long)ch.r * g.ntiles;
     // every thread derives the window from the chunk's first particle
     long long k0 = rbase;
-    if (ch.c1 > ch.c0) {
-        float3 d;
-        k0 = particle_key<MAS>(pos, ch.c0, axes, inv, g, rbase, d);
+    if (ch.c1 > ch.c0) k0 = dist_key<MAS, AXES, POW2>(load_dist<AXES>(pos, rsd, ch.c0, inv), g, rbase);
+    // window start relative to the realisation's first key, taken cyclically so
+    // that particles wrapped across the periodic x boundary stay inside it
+    int w0 = 0;
+    if (g.ntiles > kWin) {
+        long long w = (k0 - rbase - kWin / 2) % g.ntiles;
+        w0 = (int)(w < 0 ? w + g.ntiles : w);
     }
-    long long wbl = k0 - kWin / 2;
-    const long long hi = rbase + g.ntiles - kWin;       // keep the window in this grid's keys
-    if (wbl > hi) wbl = hi;
-    if (wbl < rbase) wbl = rbase;
-    if (threadIdx.x == 0) chunks[blockIdx.x].wbase = (int)wbl;
+    if (threadIdx.x == 0) chunks[blockIdx.x].wbase = w0;
     for (int s = threadIdx.x; s < kWin; s += blockDim.x) hist[s] = 0u;
     __syncthreads();
-    const unsigned wb = (unsigned)wbl;
-    for (long long i = ch.c0 + threadIdx.x; i < ch.c1; i += blockDim.x) {
-        float3 d;
-        const unsigned key = particle_key<MAS>(pos, i, axes, inv, g, rbase, d);
-        const unsigned s = key - wb;
-        if (s < (unsigned)kWin) atomicAdd(&hist[s], 1u);
-        else atomicAdd(&ocounts[key], 1u);
+    const long long step = (long long)blockDim.x * kUnroll;
+    const long long last = ch.c1 - 1;
+    for (long long base = ch.c0 + threadIdx.x; base < ch.c1; base += step) {
+        float3 d[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+            d[u] = load_dist<AXES>(pos, rsd, min(base + (long long)u * blockDim.x, last), inv);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (base + (long long)u * blockDim.x < ch.c1) {
+                const unsigned key = dist_key<MAS, AXES, POW2>(d[u], g, rbase);
+                const unsigned s = win_slot(key, rbase, w0, g.ntiles);
+                if (s < (unsigned)kWin) atomicAdd(&hist[s], 1u);
+                else atomicAdd(&ocounts[key], 1u);
+            }
+        }
     }
     __syncthreads();
     unsigned *dmp = dump + (long long)blockIdx.x * kWin;
     for (int s = threadIdx.x; s < kWin; s += blockDim.x) {
         const unsigned v = hist[s];
-        if (v) dmp[s] = atomicAdd(&counts[wb + s], v);
+        if (v) dmp[s] = atomicAdd(&counts[win_key(s, rbase, w0, g.ntiles)], v);
     }
 }
 
@@ -278,34 +334,56 @@ tile_count_kernel(const float *__restrict__ pos, int axes, float inv, TileGeom g
 // window keys take slot = chunk base + LDS cursor, overflow keys slot =
 // counts[key] + global cursor.  Weighted input also records max|W| per
 // realisation (float bits of the largest finite |W|).
-template <int MAS>
+template <int MAS, int AXES, bool POW2, bool WT>
 __global__ void __launch_bounds__(kSortBlock)
-tile_scatter_kernel(const float *__restrict__ pos, const float *__restrict__ W, int axes,
-                    float inv, TileGeom g, const Chunk *__restrict__ chunks,
+tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restrict__ W, float inv,
+                    TileGeom g, const Chunk *__restrict__ chunks,
                     const unsigned *__restrict__ dump, const unsigned long long *__restrict__ offsets,
                     const unsigned *__restrict__ counts, unsigned *__restrict__ ocursors,
                     unsigned *__restrict__ wmax_bits, float4 *__restrict__ records) {
     __shared__ unsigned cur[kWin];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
-    const unsigned wb = (unsigned)ch.wbase;
+    const int w0 = ch.wbase;
     const unsigned *dmp = dump + (long long)blockIdx.x * kWin;
     for (int s = threadIdx.x; s < kWin; s += blockDim.x) cur[s] = dmp[s];
     __syncthreads();
     unsigned m = 0;
-    for (long long i = ch.c0 + threadIdx.x; i < ch.c1; i += blockDim.x) {
-        float3 d;
-        const unsigned key = particle_key<MAS>(pos, i, axes, inv, g, rbase, d);
-        const float w = W ? W[i] : 1.0f;
-        const unsigned s = key - wb;
-        unsigned long long slot = offsets[key];
-        if (s < (unsigned)kWin) slot += atomicAdd(&cur[s], 1u);
-        else slot += (unsigned long long)counts[key] + atomicAdd(&ocursors[key], 1u);
-        records[slot] = make_float4(d.x, d.y, d.z, w);
-        const unsigned b = __float_as_uint(fabsf(w));
-        if (b < 0x7f800000u) m = max(m, b);
+    const long long step = (long long)blockDim.x * kUnroll;
+    const long long last = ch.c1 - 1;
+    for (long long base = ch.c0 + threadIdx.x; base < ch.c1; base += step) {
+        float3 d[kUnroll];
+        float w[kUnroll];
+        unsigned key[kUnroll];
+        unsigned long long slot[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const long long i = min(base + (long long)u * blockDim.x, last);
+            d[u] = load_dist<AXES>(pos, rsd, i, inv);
+            w[u] = WT ? W[i] : 1.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            key[u] = dist_key<MAS, AXES, POW2>(d[u], g, rbase);
+            slot[u] = 0;
+            if (base + (long long)u * blockDim.x < ch.c1) {
+                const unsigned s = win_slot(key[u], rbase, w0, g.ntiles);
+                if (s < (unsigned)kWin) slot[u] = atomicAdd(&cur[s], 1u);
+                else slot[u] = (unsigned long long)counts[key[u]] + atomicAdd(&ocursors[key[u]], 1u);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (base + (long long)u * blockDim.x < ch.c1) {
+                records[offsets[key[u]] + slot[u]] = make_float4(d[u].x, d[u].y, d[u].z, w[u]);
+                if (WT) {
+                    const unsigned b = __float_as_uint(fabsf(w[u]));
+                    if (b < 0x7f800000u) m = max(m, b);
+                }
+            }
+        }
     }
-    if (W) {
+    if (WT) {
         // block max through the (no longer needed) cursor table
         for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
         __syncthreads();
@@ -441,7 +519,7 @@ __host__ __device__ __forceinline__ TileBox tile_box(int tile, const TileGeom &g
 // upper halo) holding integer multiples of 2^q; every record adds its whole
 // footprint at fixed offsets from its first node.
 template <int MAS, int AXES>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kDepBlock)
 tile_deposit_kernel(const float4 *__restrict__ records,
                     const unsigned long long *__restrict__ offsets, TileGeom g, int weighted,
                     float weight_factor, const unsigned *__restrict__ wmax_bits,
@@ -452,7 +530,7 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     constexpr int BZ = (AXES == 3) ? T + F - 1 : 1;
     constexpr int BX = T + F - 1;
     constexpr int NB = BX * BY * BZ;
-    __shared__ double acc[NB];
+    __shared__ unsigned long long acc[NB];
     const int key = blockIdx.x;                       // r * ntiles + tile
     const int real = key / (int)g.ntiles;
     const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
@@ -475,26 +553,30 @@ tile_deposit_kernel(const float4 *__restrict__ records,
         }
         return;
     }
-    for (int c = threadIdx.x; c < NB; c += blockDim.x) acc[c] = 0.0;
+    for (int c = threadIdx.x; c < NB; c += blockDim.x) acc[c] = 0ull;
 
-    // Quantum 2^q: the footprint weights of a particle sum to 1, so the sum of
-    // the tile's |contributions| is at most count * weight_factor * max finite
-    // |W| of the realisation (to float rounding), which is < 2^e.  With
-    // 2^e / 2^q = 2^52 every partial sum of rint(c / 2^q) is an integer below
-    // 2^53: exact in float64.
+    // Quantum 2^q: footprint weights are <= 1, so |contribution| <=
+    // weight_factor * max finite |W| of the realisation < 2^e.  With q = e - 30
+    // every rounded contribution is an int32 and every box sum an int64 for
+    // any count below 2^33: exact integer atomics, independent of their order.
     const float wmax = weighted ? __uint_as_float(wmax_bits[real]) : 1.0f;
-    const double bound = (double)(r1 - r0) * (double)wmax * (double)weight_factor;
+    const double bound = (double)wmax * (double)weight_factor;
     int e = -1000;
     if (bound > 0.0) frexp(bound, &e);
-    int q = e - 52;
+    int q = e - 30;
     if (q < -127) q = -127;                 // keep 2^-q a normal float
     if (q > 127) q = 127;
     const float to_q = ldexpf(1.0f, -q);
     const double from_q = ldexp(1.0, q);
     __syncthreads();
 
-    for (unsigned long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const float4 rec = records[r];
+    // the next record is loaded while the current one is deposited
+    unsigned long long r = r0 + threadIdx.x;
+    float4 nxt = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (r < r1) nxt = records[r];
+    for (; r < r1; r += blockDim.x) {
+        const float4 rec = nxt;
+        if (r + blockDim.x < r1) nxt = records[r + blockDim.x];
         float Cx[4], Cy[4], Cz[4];
         const int rx = first_node<MAS>(rec.x);
         const int ry = first_node<MAS>(rec.y);
@@ -515,7 +597,7 @@ tile_deposit_kernel(const float4 *__restrict__ records,
         // scaling commutes with float rounding), so the scaled product is the
         // reference's float contribution in units of the quantum.
         const float w = (rec.w * weight_factor) * to_q;
-        double *a = acc + (lx * BY + ly) * BZ + lz;
+        unsigned long long *a = acc + (lx * BY + ly) * BZ + lz;
 #pragma unroll
         for (int l = 0; l < F; ++l) {
 #pragma unroll
@@ -525,11 +607,12 @@ tile_deposit_kernel(const float4 *__restrict__ records,
 #pragma unroll
                     for (int k = 0; k < F; ++k) {
                         const float c = (cxy * Cz[k]) * w;
-                        atomicAdd(a + (l * BY + m) * BZ + k, (double)rintf(c));
+                        atomicAdd(a + (l * BY + m) * BZ + k,
+                                  (unsigned long long)(long long)__float2int_rn(c));
                     }
                 } else {
                     const float c = cxy * w;   // the reference multiplies by C[2][n] = 1.0
-                    atomicAdd(a + l * BY + m, (double)rintf(c));
+                    atomicAdd(a + l * BY + m, (unsigned long long)(long long)__float2int_rn(c));
                 }
             }
         }
@@ -539,7 +622,7 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     for (int c = threadIdx.x; c < NB; c += blockDim.x) {
         const int bz = c % BZ, by = (c / BZ) % BY, bx = c / (BZ * BY);
         if (bx >= ex + F - 1 || by >= ey + F - 1 || (AXES == 3 && bz >= ez + F - 1)) continue;
-        const float v = (float)(acc[c] * from_q);
+        const float v = (float)((double)(long long)acc[c] * from_q);
         const bool inner = bx >= F - 1 && bx < ex && by >= F - 1 && by < ey &&
                            (AXES == 2 || (bz >= F - 1 && bz < ez));
         if (inner) {
@@ -553,44 +636,63 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     }
 }
 
-// Images of tile-local coordinate l along one axis: (tile, box coordinate)
-// pairs whose box cell lands on l.  Own tile at l; with nt >= 2 the lower
-// neighbour's halo at l + e(t-1) when l < F-1; with one tile along the axis
-// the tile's own halo wraps onto it, l + k*dims for k >= 1.
+// Images along one axis of a tile t: the tiles whose boxes cover its nodes and
+// the offset from a tile-local coordinate l to their box coordinate.  Image 0
+// is t itself (offset 0); with nt >= 2, image 1 is the lower neighbour, whose
+// halo starts at its extent e(t-1); with one tile along the axis the tile's own
+// halo wraps onto it at offsets dims, 2*dims, ...  An image applies to l when
+// l + offset lies inside its box, i.e. < extent + F - 1.
 template <int F>
-__host__ __device__ __forceinline__ int axis_images(int l, int t, int nt, int dims, const int *orig,
-                                           int *it, int *ib) {
+__host__ __device__ __forceinline__ int axis_images(int t, int nt, int dims, const int *orig,
+                                                    int *it, int *io, int *ie) {
     it[0] = t;
-    ib[0] = l;
-    int n = 1;
-    if (l >= F - 1) return n;
+    io[0] = 0;
+    ie[0] = orig[t + 1] - orig[t];
+    if (F == 1) return 1;
     if (nt >= 2) {
         const int tp = (t == 0) ? nt - 1 : t - 1;
         it[1] = tp;
-        ib[1] = l + (orig[tp + 1] - orig[tp]);
+        io[1] = orig[tp + 1] - orig[tp];
+        ie[1] = io[1];
         return 2;
     }
-    for (int b = l + dims; b < dims + F - 1; b += dims) {
+    int n = 1;
+    for (int off = dims; off < dims + F - 1 && n < 4; off += dims) {
         it[n] = 0;
-        ib[n] = b;
+        io[n] = off;
+        ie[n] = dims;
         ++n;
     }
     return n;
 }
 
 // Band cells (l < F-1 along some axis) of every tile: own band value plus the
-// halo values of the lower neighbours, summed in double in a fixed order, then
-// written once.  Empty tiles contribute nothing.
+// halo values of the lower neighbours, summed in double in a fixed order (the
+// image order), then written once.  Empty tiles contribute nothing.
 template <int MAS, int AXES>
 __global__ void __launch_bounds__(kBlock)
 tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
                      const float *__restrict__ partials, float *__restrict__ grid, OutMode om) {
     constexpr int F = MAS;
+    __shared__ long long s_pbase[64];   // partial base of image combination, -1 if empty
     const int key = blockIdx.x;
     const int real = key / (int)g.ntiles;
     const long long kbase = (long long)real * g.ntiles;
     const TileBox tb = tile_box<AXES>(key - (int)kbase, g);
     grid += real * g.ncell;
+    int itx[4], iox[4], iex[4], ity[4], ioy[4], iey[4], itz[4], ioz[4], iez[4];
+    const int nx = axis_images<F>(tb.tx, g.nt, g.dims, g.orig, itx, iox, iex);
+    const int ny = axis_images<F>(tb.ty, g.nt, g.dims, g.orig, ity, ioy, iey);
+    int nz = 1;
+    itz[0] = 0; ioz[0] = 0; iez[0] = 1;
+    if (AXES == 3) nz = axis_images<F>(tb.tz, g.ntz, g.dims, g.orig, itz, ioz, iez);
+    const int ncomb = nx * ny * nz;
+    if ((int)threadIdx.x < ncomb) {
+        const int c = threadIdx.x % nz, b = (threadIdx.x / nz) % ny, a = threadIdx.x / (nz * ny);
+        const long long vkey = kbase + ((long long)itx[a] * g.nt + ity[b]) * g.ntz + itz[c];
+        s_pbase[threadIdx.x] = (offsets[vkey] == offsets[vkey + 1]) ? -1 : vkey * g.pslot;
+    }
+    __syncthreads();
     const int ex = tb.ex, ey = tb.ey, ez = tb.ez;
     const int nbx = min(F - 1, ex), nby = min(F - 1, ey), nbz = (AXES == 3) ? min(F - 1, ez) : 0;
     const int sa = nbx * ey * ez;
@@ -608,25 +710,22 @@ tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
             const int u = t - sa - sb;
             lz = u % nbz; ly = nby + (u / nbz) % (ey - nby); lx = nbx + u / (nbz * (ey - nby));
         }
-        int itx[4], ibx[4], ity[4], iby[4], itz[4], ibz[4];
-        const int nx = axis_images<F>(lx, tb.tx, g.nt, g.dims, g.orig, itx, ibx);
-        const int ny = axis_images<F>(ly, tb.ty, g.nt, g.dims, g.orig, ity, iby);
-        int nz = 1;
-        itz[0] = 0;
-        ibz[0] = 0;
-        if (AXES == 3) nz = axis_images<F>(lz, tb.tz, g.ntz, g.dims, g.orig, itz, ibz);
         double v = 0.0;
-        for (int a = 0; a < nx; ++a)
-            for (int b = 0; b < ny; ++b)
+        for (int a = 0; a < nx; ++a) {
+            const int bx = lx + iox[a];
+            if (bx >= iex[a] + F - 1) continue;
+            for (int b = 0; b < ny; ++b) {
+                const int by = ly + ioy[b];
+                if (by >= iey[b] + F - 1) continue;
                 for (int c = 0; c < nz; ++c) {
-                    const long long vkey = kbase + ((long long)itx[a] * g.nt + ity[b]) * g.ntz + itz[c];
-                    if (offsets[vkey] == offsets[vkey + 1]) continue;
-                    const int vex = g.orig[itx[a] + 1] - g.orig[itx[a]];
-                    const int vey = g.orig[ity[b] + 1] - g.orig[ity[b]];
-                    const int vez = (AXES == 3) ? g.orig[itz[c] + 1] - g.orig[itz[c]] : 1;
-                    v += (double)partials[vkey * g.pslot +
-                                          halo_index<F, AXES>(ibx[a], iby[b], ibz[c], vex, vey, vez)];
+                    const int bz = lz + ioz[c];
+                    if (AXES == 3 && bz >= iez[c] + F - 1) continue;
+                    const long long pb = s_pbase[(a * ny + b) * nz + c];
+                    if (pb < 0) continue;
+                    v += (double)partials[pb + halo_index<F, AXES>(bx, by, bz, iex[a], iey[b], iez[c])];
                 }
+            }
+        }
         const long long node = AXES == 3
             ? ((long long)(tb.ox + lx) * g.dims + (tb.oy + ly)) * g.dims + (tb.oz + lz)
             : (long long)(tb.ox + lx) * g.dims + (tb.oy + ly);
@@ -637,16 +736,24 @@ tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
 // Plain form: one thread per particle, one global float atomic per node.
 template <int MAS, int AXES>
 __global__ void __launch_bounds__(kBlock)
-atomic_deposit_kernel(const float *__restrict__ pos, const float *__restrict__ W, long long n,
-                      float inv, int dims, float weight_factor, float *__restrict__ grid) {
+atomic_deposit_kernel(const float *__restrict__ pos, RSD rsd, const float *__restrict__ W,
+                      long long n, float inv, int dims, float weight_factor,
+                      float *__restrict__ grid) {
     constexpr int F = MAS;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         float Cx[4], Cy[4], Cz[4];
-        int bx = axis_foot<MAS>(pos[(long long)AXES * i + 0], inv, dims, Cx);
-        int by = axis_foot<MAS>(pos[(long long)AXES * i + 1], inv, dims, Cy);
+        const float3 d = load_dist<AXES>(pos, rsd, i, inv);
+        const int rx = first_node<MAS>(d.x), ry = first_node<MAS>(d.y);
+        foot_weights<MAS>(d.x, rx, Cx);
+        foot_weights<MAS>(d.y, ry, Cy);
+        const int bx = wrap_node(rx, dims), by = wrap_node(ry, dims);
         int bz = 0;
-        if (AXES == 3) bz = axis_foot<MAS>(pos[(long long)AXES * i + 2], inv, dims, Cz);
+        if (AXES == 3) {
+            const int rz = first_node<MAS>(d.z);
+            foot_weights<MAS>(d.z, rz, Cz);
+            bz = wrap_node(rz, dims);
+        }
         float w = (W ? W[i] : 1.0f) * weight_factor;
         const bool weighted = w != 1.0f;
 #pragma unroll
@@ -747,16 +854,17 @@ TileGeom make_geom(int dims, int axes, int mas) {
 }
 
 template <int MAS>
-int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal, int axes,
-              const float *W, float *grid, int dims, float inv, float weight_factor,
-              const OutMode &om) {
+int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long long *h_offs,
+              int nreal, int axes, const float *W, float *grid, int dims, float inv,
+              float weight_factor, const OutMode &om) {
     TileGeom g = make_geom(dims, axes, MAS);
+    const bool pow2 = g.shift >= 0;
     const long long nt = g.ntiles * nreal;      // keys over the whole batch
     const int nscan_blocks = (int)((nt + kScanTile - 1) / kScanTile);
     PYL_CHECK(nscan_blocks <= kScanTile, PYL_EINVAL, "too many tiles (%lld)", nt);
     PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
     // Chunks of consecutive particles, never straddling two realisations.
-    const long long target = std::max(16384LL, std::min(1LL << 20, (n + 1023) / 1024));
+    const long long target = std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
     std::vector<Chunk> chunks;
     for (int r = 0; r < nreal; ++r) {
         const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
@@ -798,8 +906,9 @@ int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, in
         PYL_HIP(hipMemcpyAsync(dchunks, chunks.data(), (size_t)nchunk * sizeof(Chunk),
                                hipMemcpyHostToDevice, c->stream));
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
-        tile_count_kernel<MAS><<<nchunk, kSortBlock, 0, c->stream>>>(pos, axes, inv, g, dchunks,
-                                                                     counts, ocounts, dump);
+        auto kc = (axes == 3) ? (pow2 ? tile_count_kernel<MAS, 3, true> : tile_count_kernel<MAS, 3, false>)
+                              : (pow2 ? tile_count_kernel<MAS, 2, true> : tile_count_kernel<MAS, 2, false>);
+        kc<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, inv, g, dchunks, counts, ocounts, dump);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
     }
@@ -819,21 +928,29 @@ int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, in
     float4 *records = (float4 *)c->bufs[BUF_RECORDS].ptr;
     if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
-        tile_scatter_kernel<MAS><<<nchunk, kSortBlock, 0, c->stream>>>(
-            pos, W, axes, inv, g, dchunks, dump, offsets, counts, ocursors, wmax, records);
+        auto ks = W ? ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true>
+                                           : tile_scatter_kernel<MAS, 3, false, true>)
+                                   : (pow2 ? tile_scatter_kernel<MAS, 2, true, true>
+                                           : tile_scatter_kernel<MAS, 2, false, true>))
+                    : ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, false>
+                                           : tile_scatter_kernel<MAS, 3, false, false>)
+                                   : (pow2 ? tile_scatter_kernel<MAS, 2, true, false>
+                                           : tile_scatter_kernel<MAS, 2, false, false>));
+        ks<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, W, inv, g, dchunks, dump, offsets, counts,
+                                                 ocursors, wmax, records);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
     }
     const int weighted = W != nullptr;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_DEPOSIT));
     if (axes == 3) {
-        tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(
+        tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kDepBlock, 0, c->stream>>>(
             records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
         if (MAS > 1)
             tile_finalize_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
                                                                                 grid, om);
     } else {
-        tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kBlock, 0, c->stream>>>(
+        tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kDepBlock, 0, c->stream>>>(
             records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
         if (MAS > 1)
             tile_finalize_kernel<MAS, 2><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
@@ -845,9 +962,9 @@ int run_tiled(Ctx *c, const float *pos, long long n, const long long *h_offs, in
 }
 
 template <int MAS>
-int run_atomic(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal,
-               int axes, const float *W, float *grid, int dims, float inv, float weight_factor,
-               const OutMode &om) {
+int run_atomic(Ctx *c, const float *pos, const RSD &rsd, long long n, const long long *h_offs,
+               int nreal, int axes, const float *W, float *grid, int dims, float inv,
+               float weight_factor, const OutMode &om) {
     const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
     if (om.overwrite)
         PYL_HIP(hipMemsetAsync(grid, 0, (size_t)ncell * nreal * sizeof(float), c->stream));
@@ -857,13 +974,15 @@ int run_atomic(Ctx *c, const float *pos, long long n, const long long *h_offs, i
         if (e <= b) continue;
         const float *p = pos + (long long)axes * b;
         const float *w = W ? W + b : nullptr;
+        RSD rr = rsd;
+        rr.vel = rsd.vel + (long long)rsd.vstride * b;
         float *gr = grid + (long long)r * ncell;
         const int nb = grid_blocks(e - b, kBlock);
         if (axes == 3)
-            atomic_deposit_kernel<MAS, 3><<<nb, kBlock, 0, c->stream>>>(p, w, e - b, inv, dims,
+            atomic_deposit_kernel<MAS, 3><<<nb, kBlock, 0, c->stream>>>(p, rr, w, e - b, inv, dims,
                                                                        weight_factor, gr);
         else
-            atomic_deposit_kernel<MAS, 2><<<nb, kBlock, 0, c->stream>>>(p, w, e - b, inv, dims,
+            atomic_deposit_kernel<MAS, 2><<<nb, kBlock, 0, c->stream>>>(p, rr, w, e - b, inv, dims,
                                                                        weight_factor, gr);
         PYL_HIP(hipGetLastError());
     }
@@ -884,7 +1003,10 @@ int run_atomic(Ctx *c, const float *pos, long long n, const long long *h_offs, i
 // deposit/scale - shift (scale = 1, shift = 0: the plain deposit).
 int ma_device(Ctx *c, const float *pos, long long n, const long long *h_offs, int nreal,
               int axes, const float *W, float *grid, int dims, float BoxSize, int mas,
-              int flags, float scale, float shift) {
+              int flags, float scale, float shift, const float *vel, float rsd_factor,
+              int rsd_axis) {
+    RSD rsd{pos, 0.0f, BoxSize, 0, axes, 0};
+    if (vel) rsd = RSD{vel, rsd_factor, BoxSize, rsd_axis, 3, 1};
     // MAS_library.pyx:135 inv_cell_size = dims/BoxSize in float (MAS_c.c:19 computes it
     // in double and stores it to float, which rounds to the same value).
     const float inv = (float)dims / BoxSize;
@@ -909,9 +1031,9 @@ int ma_device(Ctx *c, const float *pos, long long n, const long long *h_offs, in
     switch (mas) {
 #define PYL_MA_CASE(M)                                                                      \
     case M:                                                                                 \
-        return atomic ? run_atomic<M>(c, pos, n, h_offs, nreal, axes, W, grid, dims, inv,    \
+        return atomic ? run_atomic<M>(c, pos, rsd, n, h_offs, nreal, axes, W, grid, dims, inv, \
                                       weight_factor, om)                                    \
-                      : run_tiled<M>(c, pos, n, h_offs, nreal, axes, W, grid, dims, inv,     \
+                      : run_tiled<M>(c, pos, rsd, n, h_offs, nreal, axes, W, grid, dims, inv,  \
                                      weight_factor, om);
         PYL_MA_CASE(PYL_NGP)
         PYL_MA_CASE(PYL_CIC)
@@ -929,7 +1051,8 @@ using namespace pyl;
 
 static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *offs, int nreal,
                     int axes, const float *W, float *grid, int dims, float BoxSize, int mas,
-                    int flags, float scale = 1.0f, float shift = 0.0f) {
+                    int flags, float scale = 1.0f, float shift = 0.0f,
+                    const float *vel = nullptr, float rsd_factor = 0.0f, int rsd_axis = 0) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
     Ctx *c = &ctx->c;
     PYL_CHECK(axes == 2 || axes == 3, PYL_EINVAL, "pos must have 2 or 3 columns (got %d)", axes);
@@ -945,7 +1068,7 @@ static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *
     reset_stages(c);
     if (flags & PYL_PTR_DEVICE) {
         PYL_TRY(ma_device(c, pos, n, offs, nreal, axes, W, grid, dims, BoxSize, mas, flags, scale,
-                          shift));
+                          shift, vel, rsd_factor, rsd_axis));
         return finish(c, flags);
     }
     // Host pointers: stage through device scratch.
@@ -955,12 +1078,19 @@ static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *
     float *dpos = (float *)c->bufs[BUF_POS].ptr;
     float *dgrid = (float *)c->bufs[BUF_GRID].ptr;
     float *dW = W ? (float *)c->bufs[BUF_W].ptr : nullptr;
+    const float *dvel = nullptr;
+    if (vel) {
+        PYL_TRY(ensure(c, BUF_VEL, (size_t)(n ? n : 1) * 3 * sizeof(float)));
+        dvel = (const float *)c->bufs[BUF_VEL].ptr;
+        if (n) PYL_HIP(hipMemcpyAsync((void *)dvel, vel, (size_t)n * 3 * sizeof(float),
+                                      hipMemcpyHostToDevice, c->stream));
+    }
     if (n) PYL_HIP(hipMemcpyAsync(dpos, pos, (size_t)n * axes * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (W && n) PYL_HIP(hipMemcpyAsync(dW, W, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (!(flags & PYL_MA_OVERWRITE))
         PYL_HIP(hipMemcpyAsync(dgrid, grid, (size_t)ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
     PYL_TRY(ma_device(c, dpos, n, offs, nreal, axes, dW, dgrid, dims, BoxSize, mas,
-                      flags | PYL_PTR_DEVICE, scale, shift));
+                      flags | PYL_PTR_DEVICE, scale, shift, dvel, rsd_factor, rsd_axis));
     PYL_HIP(hipMemcpyAsync(grid, dgrid, (size_t)ncell * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     PYL_HIP(hipStreamSynchronize(c->stream));
     return PYL_OK;
@@ -976,6 +1106,22 @@ extern "C" int pyl_ma_delta(pyl_ctx *ctx, const float *pos, int64_t n, int axes,
                             float shift, int flags) {
     return ma_entry(ctx, pos, n, nullptr, 1, axes, W, delta, dims, BoxSize, mas,
                     flags | PYL_MA_OVERWRITE, scale, shift);
+}
+
+extern "C" int pyl_ma_rsd(pyl_ctx *ctx, const float *pos, const float *vel, int64_t n,
+                          const float *W, float *grid, int dims, float BoxSize, int mas,
+                          float Hubble, float redshift, int axis, float scale, float shift,
+                          int flags) {
+    PYL_CHECK(vel != nullptr || n == 0, PYL_EINVAL, "NULL velocities");
+    PYL_CHECK(axis >= 0 && axis <= 2, PYL_EINVAL, "axis must be 0, 1 or 2");
+    PYL_CHECK(Hubble != 0.0f, PYL_EINVAL, "Hubble must be non-zero");
+    PYL_CHECK((flags & PYL_MA_OVERWRITE) || (scale == 1.0f && shift == 0.0f), PYL_EINVAL,
+              "scale/shift need PYL_MA_OVERWRITE");
+    // redshift_space_library.pyx:36: factor = (1.0 + redshift)/Hubble in double,
+    // stored to a float
+    const float factor = (float)((1.0 + (double)redshift) / (double)Hubble);
+    return ma_entry(ctx, pos, n, nullptr, 1, 3, W, grid, dims, BoxSize, mas, flags, scale, shift,
+                    vel ? vel : pos, factor, axis);
 }
 
 extern "C" int pyl_ma_batch(pyl_ctx *ctx, const float *pos, const int64_t *offsets, int nreal,

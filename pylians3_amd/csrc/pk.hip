@@ -543,17 +543,7 @@ __global__ void rsd_kernel(float *__restrict__ pos, const float *__restrict__ ve
                            float BoxSize, float factor, int axis) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float p = pos[3 * i + axis];
-        float v = vel[3 * i + axis];
-        float dv = v * factor;
-        p = p + dv;
-        // redshift_space_library.pyx:40-46.  The "while <0" loop is bounded here:
-        // after 64 box lengths the remainder is folded with fmodf.
-        int guard = 0;
-        while (p < 0.0f && guard < 64) { p += BoxSize; ++guard; }
-        if (p < 0.0f) p = fmodf(p, BoxSize) + BoxSize;
-        if (p > BoxSize) p = fmodf(p, BoxSize);
-        pos[3 * i + axis] = p;
+        pos[3 * i + axis] = rsd_shift(pos[3 * i + axis], vel[3 * i + axis], factor, BoxSize);
     }
 }
 

@@ -135,6 +135,22 @@ inline void mas_table(int dims, int p, double *out) {
     }
 }
 
+// Redshift-space shift of one coordinate, redshift_space_library.pyx:40-46:
+// p + v*factor (float), "while p < 0: p += BoxSize", "if p > BoxSize: p = p % BoxSize"
+// (C fmod under cdivision).  The while loop is bounded: after 64 box lengths the
+// remainder is folded with fmodf.
+__device__ __forceinline__ float rsd_shift(float p, float v, float factor, float box) {
+    p = p + v * factor;
+    int guard = 0;
+    while (p < 0.0f && guard < 64) {
+        p += box;
+        ++guard;
+    }
+    if (p < 0.0f) p = fmodf(p, box) + box;
+    if (p > box) p = fmodf(p, box);
+    return p;
+}
+
 // Mode-count plan (host computed, cached)
 const ModePlan *get_mode_plan(Ctx *c, int dims, int axis);
 void compute_mode_plan(int dims, int axis, ModePlan *mp);
