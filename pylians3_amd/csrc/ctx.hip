@@ -97,7 +97,7 @@ int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out, int kind) {
     p.dims = dims;
     p.batch = batch;
     size_t lengths[3] = {(size_t)dims, (size_t)dims, (size_t)dims};  // fastest first
-    const size_t rank = (kind == FFT_R2C_2D) ? 2 : 3;
+    const size_t rank = (kind == FFT_R2C_2D) ? 2 : (kind == FFT_R2C_1D ? 1 : 3);
     const rocfft_transform_type type = (kind == FFT_C2R_3D) ? rocfft_transform_type_real_inverse
                                                             : rocfft_transform_type_real_forward;
     rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type,
@@ -203,6 +203,8 @@ int pyl_ctx_trim(pyl_ctx *ctx) {
         (void)hipFree(kv.second.d_key_items);
     }
     c->bin_plans.clear();
+    for (auto &kv : c->twiddles) (void)hipFree(kv.second);
+    c->twiddles.clear();
     return PYL_OK;
 }
 

@@ -1,0 +1,209 @@
+// Forward 3D real-to-complex FFT of power-of-two cubes (16..1024 per side).
+//
+// Same transform as FFT3Dr_f (library/Pk_library/Pk_library.pyx:117-130, FFTW
+// forward, unnormalised, e^{-i}): out[x][y][kz], kz in [0, N/2].  Three passes
+// over HBM:
+//   z: batched 1D r2c along the contiguous axis (rocFFT 1D plan, N^2 rows);
+//   y: complex FFTs along y, in place;
+//   x: complex FFTs along x, in place.
+// The y and x passes use the same column kernel: a workgroup loads C
+// consecutive kz columns (C*8 bytes contiguous per point) of one (x or y) line
+// set into LDS, runs C Stockham FFTs of length N with radix-8 stages (radix 4
+// or 2 for the remainder) in registers, exchanging through LDS between stages,
+// and writes the block back.  Twiddles exp(-2 pi i k/N) are computed on the
+// host in double and rounded to float.  Other sizes use a rocFFT 3D plan.
+#include "pyl_internal.h"
+
+namespace pyl {
+namespace {
+
+constexpr int kFftBlock = 256;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (-i)
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+template <int R>
+__device__ __forceinline__ void dft(float2 *v) {
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const float2 s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    } else {
+        // radix 8 = two radix-4 halves (even, odd) and the twiddles w8^k
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4>(e);
+        dft<4>(o);
+        const float h = 0.70710678118654752f;
+        // o[k] *= exp(-2 pi i k/8)
+        o[1] = make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+        o[2] = mul_mi(o[2]);
+        o[3] = make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+}
+
+// One Stockham stage of radix R over C interleaved columns held in LDS
+// (element (n, c) at buf[n*C + c]).  Span Ns: sub-transforms of length Ns are
+// complete; this stage makes them Ns*R long.
+template <int N, int C, int R>
+__device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw, int Ns) {
+    constexpr int NB = (N / R) * C;                          // butterflies
+    constexpr int PER = (NB + kFftBlock - 1) / kFftBlock;    // per thread
+    float2 v[PER][R];
+    int dst[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = threadIdx.x + p * kFftBlock;
+        dst[p] = -1;
+        if (b < NB) {
+            const int c = b % C, j = b / C;
+            const int k = j % Ns;
+            const int tstep = (N / (Ns * R)) * k;
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                float2 x = buf[(j + m * (N / R)) * C + c];
+                if (m > 0) x = cmul(x, tw[(m * tstep) & (N - 1)]);
+                v[p][m] = x;
+            }
+            dft<R>(v[p]);
+            dst[p] = ((j / Ns) * Ns * R + k) * C + c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        if (dst[p] >= 0) {
+#pragma unroll
+            for (int m = 0; m < R; ++m) buf[dst[p] + m * Ns * C] = v[p][m];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int C>
+__device__ __forceinline__ void fft_columns(float2 *buf, const float2 *__restrict__ tw) {
+    int Ns = 1;
+    int rem = N;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (rem >= 8) {
+            stage<N, C, 8>(buf, tw, Ns);
+            Ns *= 8;
+            rem /= 8;
+        }
+    }
+    if (rem == 4) stage<N, C, 4>(buf, tw, Ns);
+    else if (rem == 2) stage<N, C, 2>(buf, tw, Ns);
+}
+
+// In-place FFT of length N along one axis of a batch of complex arrays:
+// element n of column (o, z) of batch b at data[b*bstride + o*ostride + n*nstride + z],
+// z in [0, zlen).  Grid: (ceil(zlen/C), nother, nbatch).
+template <int N, int C>
+__global__ void __launch_bounds__(kFftBlock)
+fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, long long bstride,
+               int zlen, const float2 *__restrict__ tw) {
+    __shared__ float2 buf[N * C];
+    const int z0 = blockIdx.x * C;
+    const int cw = min(C, zlen - z0);
+    float2 *base = data + (long long)blockIdx.z * bstride + (long long)blockIdx.y * ostride + z0;
+    for (int t = threadIdx.x; t < N * C; t += kFftBlock) {
+        const int c = t % C, n = t / C;
+        buf[t] = (c < cw) ? base[(long long)n * nstride + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_columns<N, C>(buf, tw);
+    for (int t = threadIdx.x; t < N * C; t += kFftBlock) {
+        const int c = t % C, n = t / C;
+        if (c < cw) base[(long long)n * nstride + c] = buf[t];
+    }
+}
+
+template <int N>
+int launch_cols(Ctx *c, float2 *data, long long nstride, long long ostride, long long bstride,
+                int zlen, int nother, int nbatch, const float2 *tw) {
+    constexpr int C = (N >= 1024) ? 4 : (N >= 512 ? 8 : 16);
+    dim3 grid((unsigned)ceil_div(zlen, C), (unsigned)nother, (unsigned)nbatch);
+    fft_col_kernel<N, C><<<grid, kFftBlock, 0, c->stream>>>(data, nstride, ostride, bstride, zlen, tw);
+    PYL_HIP(hipGetLastError());
+    return PYL_OK;
+}
+
+int cols(Ctx *c, int n, float2 *data, long long nstride, long long ostride, long long bstride,
+         int zlen, int nother, int nbatch, const float2 *tw) {
+    switch (n) {
+        case 16: return launch_cols<16>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+        case 32: return launch_cols<32>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+        case 64: return launch_cols<64>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+        case 128: return launch_cols<128>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+        case 256: return launch_cols<256>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+        case 512: return launch_cols<512>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+        case 1024: return launch_cols<1024>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+    }
+    set_error("no column FFT of length %d", n);
+    return PYL_EINVAL;
+}
+
+}  // namespace
+
+bool fft_own_size(int dims) {
+    return dims >= 16 && dims <= 1024 && (dims & (dims - 1)) == 0;
+}
+
+// Twiddle table exp(-2 pi i k/N), k < N, resident per N.
+int get_twiddles(Ctx *c, int n, const float2 **out) {
+    auto it = c->twiddles.find(n);
+    if (it == c->twiddles.end()) {
+        std::vector<float2> h(n);
+        for (int k = 0; k < n; ++k) {
+            const double a = -2.0 * M_PI * (double)k / (double)n;
+            h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        void *d = nullptr;
+        PYL_HIP(hipMalloc(&d, sizeof(float2) * n));
+        PYL_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+        it = c->twiddles.emplace(n, d).first;
+    }
+    *out = (const float2 *)it->second;
+    return PYL_OK;
+}
+
+// out = r2c FFT of `batch` contiguous (N,N,N) float cubes, (N,N,N/2+1) each.
+// `in` may be overwritten.
+int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch) {
+    if (!fft_own_size(dims)) {
+        FFTPlan *plan = nullptr;
+        PYL_TRY(get_fft_plan(c, dims, batch, &plan, FFT_R2C_3D));
+        return run_fft(c, plan, in, out);
+    }
+    const long long N = dims, Z = dims / 2 + 1;
+    FFTPlan *zplan = nullptr;
+    PYL_TRY(get_fft_plan(c, dims, (int)(N * N * batch), &zplan, FFT_R2C_1D));
+    PYL_TRY(run_fft(c, zplan, in, out));
+    const float2 *tw = nullptr;
+    PYL_TRY(get_twiddles(c, dims, &tw));
+    // y: n = y (stride Z), other = x (stride N*Z)
+    PYL_TRY(cols(c, dims, out, Z, N * Z, N * N * Z, (int)Z, dims, batch, tw));
+    // x: n = x (stride N*Z), other = y (stride Z)
+    PYL_TRY(cols(c, dims, out, N * Z, Z, N * N * Z, (int)Z, dims, batch, tw));
+    return PYL_OK;
+}
+
+}  // namespace pyl

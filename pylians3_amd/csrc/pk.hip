@@ -735,10 +735,8 @@ int finalize(const ModePlan *mp, const PkDims &p, double BoxSize, const HostAcc 
 }  // namespace
 
 int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch) {
-    FFTPlan *plan = nullptr;
-    PYL_TRY(get_fft_plan(c, dims, batch, &plan));
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_FFT));
-    PYL_TRY(run_fft(c, plan, in, out));
+    PYL_TRY(fft_r2c_3d(c, in, out, dims, batch));
     PYL_TRY(stage_end(c, PYL_STAGE_PK_FFT));
     return PYL_OK;
 }

@@ -266,11 +266,10 @@ int pyl_correct_mas(pyl_ctx *ctx, const float *delta, float *out, int dims, doub
                            c->stream));
     PYL_HIP(hipMemcpyAsync(in, delta, (size_t)n * sizeof(float),
                            dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-    FFTPlan *fwd = nullptr, *inv = nullptr;
-    PYL_TRY(get_fft_plan(c, dims, 1, &fwd, FFT_R2C_3D));
+    FFTPlan *inv = nullptr;
     PYL_TRY(get_fft_plan(c, dims, 1, &inv, FFT_C2R_3D));
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_FFT));
-    PYL_TRY(run_fft(c, fwd, in, dk));
+    PYL_TRY(fft_r2c_3d(c, in, dk, dims, 1));
     PYL_TRY(stage_end(c, PYL_STAGE_PK_FFT));
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_BIN));
     mas_scale_kernel<<<plane_blocks(nk), 256, 0, c->stream>>>(dk, dims, dtab);

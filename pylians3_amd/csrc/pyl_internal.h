@@ -103,6 +103,7 @@ struct Ctx {
     std::map<std::tuple<int, int, int>, FFTPlan> fft_plans; // (dims, batch, kind)
     std::map<std::pair<int, int>, ModePlan> mode_plans;    // (dims, axis)
     std::map<std::tuple<int, int, int>, BinPlan> bin_plans; // (dims, axis, rows per item)
+    std::map<int, void *> twiddles;                          // exp(-2 pi i k/N), per N
     bool timing = false;
     hipEvent_t ev[PYL_STAGE_COUNT][2] = {};
     bool ev_used[PYL_STAGE_COUNT] = {};
@@ -118,8 +119,11 @@ int finish(Ctx *c, int flags);   // synchronise unless PYL_NO_SYNC on device poi
 
 // FFT plan (resident, one per (dims, batch))
 // Transform kinds of the resident rocFFT plans (all single precision, out of place).
-enum FFTKind { FFT_R2C_3D = 0, FFT_C2R_3D = 1, FFT_R2C_2D = 2 };
+enum FFTKind { FFT_R2C_3D = 0, FFT_C2R_3D = 1, FFT_R2C_2D = 2, FFT_R2C_1D = 3 };
 int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out, int kind = FFT_R2C_3D);
+// Forward r2c of `batch` contiguous cubes (own passes for power-of-two sizes).
+int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch);
+bool fft_own_size(int dims);
 int run_fft(Ctx *c, FFTPlan *p, void *in, void *out);
 
 // Host-side MAS tables: out[kk] = MAS_correction(pi/dims * k(kk), p) for the

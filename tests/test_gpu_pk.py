@@ -169,3 +169,29 @@ def test_pk_deterministic(oracle):
     b = PKL.Pk(delta, 500.0, axis=0, MAS='CIC', verbose=False)
     np.testing.assert_array_equal(a.Pk, b.Pk)
     np.testing.assert_array_equal(a.Pk2D, b.Pk2D)
+
+
+@pytest.mark.parametrize('dims', [16, 64, 128, 256, 512])
+def test_fft_power_of_two_vs_scipy(oracle, dims):
+    # the own z/y/x passes (power-of-two sides) against scipy's pocketfft in
+    # float64: max error within a few float32 ulps of the largest mode, and the
+    # float32 oracle FFT is no closer
+    import torch
+    from pylians3_amd import Pk_library as PKL
+    print('case fft', dims, flush=True)
+    rng = np.random.default_rng(dims)
+    x = rng.standard_normal((dims,) * 3).astype(np.float32)
+    ref = oracle.fft_f64(x)
+    got = PKL.FFT3Dr_f(torch.from_numpy(x).cuda(), 1).cpu().numpy()
+    scale = np.sqrt(np.mean(np.abs(ref) ** 2))
+    err = np.abs(got - ref).max() / scale
+    err32 = np.abs(oracle.fft_f32(x) - ref).max() / scale
+    assert err < max(3 * err32, 1e-6), (err, err32)
+    # a plane wave lands in exactly one mode with amplitude d^3/2
+    i = np.arange(dims)
+    X, Y, Z = np.meshgrid(i, i, i, indexing='ij')
+    pw = np.cos(2 * np.pi * (3 * X + 1 * Y + 5 * Z) / dims).astype(np.float32)
+    g = PKL.FFT3Dr_f(pw, 1)
+    assert abs(g[3, 1, 5] - dims ** 3 / 2) < 1e-4 * dims ** 3
+    g[3, 1, 5] = 0
+    assert np.abs(g).max() < 1e-4 * dims ** 3
