@@ -88,6 +88,36 @@ class Pipeline:
         return {k: v / max(self.nsteps, 1) for k, v in self.stage_sum.items()}
 
 
+def algorithmic_bytes(n, dims):
+    """Algorithmic HBM bytes per launch of each timed kernel of the MA + Pk step
+    (n particles, dims^3 grid, TSC, unweighted): what the kernel must move at
+    minimum, not what it moves."""
+    nk = dims * dims * (dims // 2 + 1)
+    return {
+        'ma_count': 12.0 * n,                    # positions read once
+        'ma_scatter': 12.0 * n + 16.0 * n,       # positions read, one 16 B record written
+        'ma_deposit': 16.0 * n + 4.0 * dims ** 3,  # records read, grid written once
+        'pk_fft': 4.0 * dims ** 3 + 8.0 * nk,    # real grid read, half-complex written once
+        'pk_bin': 8.0 * nk,                      # half-complex read once
+    }
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC FETCH_SIZE and
+    WRITE_SIZE passes on this workload, from the committed summary
+    profiles/<kernel>_512_tsc_traffic.txt (line 'traffic_bytes: <int>'), else None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        '%s_512_tsc_traffic.txt' % kernel)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith('traffic_bytes:'):
+                    return int(line.split(':', 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def time_steps(pipe, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -351,31 +381,27 @@ def main():
                  'ma_ms': round(sp.get('ma_atomic', 0.0), 4)}
         del pp
 
-    ma_ms = sum(st.get(k, 0.0) for k in ('ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit'))
+    ma_ms = sum(st.get(k, 0.0) for k in ('ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit',
+                                          'ma_final'))
     pk_ms = sum(st.get(k, 0.0) for k in ('pk_prep', 'pk_fft', 'pk_bin'))
     nk = dims * dims * (dims // 2 + 1)
     B_ma = 12.0 * n + 4.0 * dims ** 3            # positions once + grid written once
     B_pk = 4.0 * dims ** 3 + 16.0 * nk           # real grid, half-complex written + read once
-    B_bin = 8.0 * nk                             # binning pass: half-complex read once
-    stage_bytes = {'ma': (B_ma, ma_ms), 'pk': (B_pk, pk_ms), 'pk_bin': (B_bin, st.get('pk_bin', 0.0))}
+    stage_bytes = {'ma': (B_ma, ma_ms), 'pk': (B_pk, pk_ms)}
     stages = {k: {'ms': round(t, 4), 'algorithmic_GB': round(b / 1e9, 4),
                   'GBps': round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None}
               for k, (b, t) in stage_bytes.items()}
     kernels = {k: round(v, 4) for k, v in st.items()}
-    # dominant kernel: the longest single stage
-    dom = max(st.items(), key=lambda kv: kv[1])[0]
-    dom_bytes = {'ma_deposit': None, 'pk_bin': B_bin}.get(dom)
-    roofline = None
-    if dom_bytes is None:
-        # account the whole MA stage (its kernels share the MA algorithmic bytes)
-        ach = B_ma / (ma_ms * 1e-3) / 1e9 if dom.startswith('ma') else B_pk / (pk_ms * 1e-3) / 1e9
-        roofline = {'bound': 'hbm', 'kernel': dom, 'scope': 'ma stage' if dom.startswith('ma')
-                    else 'pk stage', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
-                    'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None}
-    else:
-        ach = dom_bytes / (st[dom] * 1e-3) / 1e9
-        roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
-                    'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None}
+    kernel_bytes = algorithmic_bytes(n, dims)
+    # dominant kernel: the longest timed stage (HIP events on the library's stream)
+    dom = max((k for k in st if k in kernel_bytes), key=lambda k: st[k])
+    ach = kernel_bytes[dom] / (st[dom] * 1e-3) / 1e9
+    roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
+                'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
+                'algorithmic_bytes': int(kernel_bytes[dom]),
+                'traffic': committed_traffic(dom)}
+    per_kernel_gbps = {k: round(kernel_bytes[k] / (t * 1e-3) / 1e9, 1)
+                       for k, t in st.items() if k in kernel_bytes and t > 0}
 
     cpu = None
     if not args.no_cpu:
@@ -418,6 +444,7 @@ def main():
                    'particles': n, 'grid': dims, 'box_mpc_h': box, 'mas': 'TSC', 'axis': axis},
         'ma_ms': round(ma_ms, 4),
         'pk_ms': round(pk_ms, 4),
+        'kernel_GBps': per_kernel_gbps,
         'ma_mparticles_per_s': round(n / ma_ms / 1e3, 2) if ma_ms > 0 else None,
         'kernels_ms': kernels,
         'stages': stages,

@@ -90,7 +90,8 @@ int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out);
 #define PYL_STAGE_PK_PREP    5
 #define PYL_STAGE_PK_FFT     6
 #define PYL_STAGE_PK_BIN     7
-#define PYL_STAGE_COUNT      8
+#define PYL_STAGE_MA_FINAL   8   /* halo finalize of the tiled MA */
+#define PYL_STAGE_COUNT      9
 int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable);
 int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */);
 

@@ -25,7 +25,7 @@ PK_DESTROY_INPUT = 0x0040
 PK_NO_PHASE = 0x0080
 
 STAGES = ['ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit', 'ma_atomic', 'pk_prep', 'pk_fft',
-          'pk_bin']
+          'pk_bin', 'ma_final']
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)

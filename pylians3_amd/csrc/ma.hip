@@ -943,21 +943,25 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     }
     const int weighted = W != nullptr;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_DEPOSIT));
-    if (axes == 3) {
+    if (axes == 3)
         tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kDepBlock, 0, c->stream>>>(
             records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
-        if (MAS > 1)
-            tile_finalize_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
-                                                                                grid, om);
-    } else {
+    else
         tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kDepBlock, 0, c->stream>>>(
             records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
-        if (MAS > 1)
-            tile_finalize_kernel<MAS, 2><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
-                                                                                grid, om);
-    }
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));
+    if (MAS > 1) {
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_FINAL));
+        if (axes == 3)
+            tile_finalize_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
+                                                                                grid, om);
+        else
+            tile_finalize_kernel<MAS, 2><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
+                                                                                grid, om);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_FINAL));
+    }
     return PYL_OK;
 }
 
