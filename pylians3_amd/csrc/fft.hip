@@ -3,7 +3,8 @@
 // Same transform as FFT3Dr_f (library/Pk_library/Pk_library.pyx:117-130, FFTW
 // forward, unnormalised, e^{-i}): out[x][y][kz], kz in [0, N/2].  Three passes
 // over HBM:
-//   z: batched 1D r2c along the contiguous axis (rocFFT 1D plan, N^2 rows);
+//   z: r2c of the contiguous rows (N/2-point complex FFTs of the even/odd
+//      pairs, then the split into the N/2+1 output modes);
 //   y: complex FFTs along y, in place;
 //   x: complex FFTs along x, in place.
 // The y and x passes use the same column kernel: a workgroup loads C
@@ -60,17 +61,18 @@ __device__ __forceinline__ void dft(float2 *v) {
 }
 
 // One Stockham stage of radix R over C interleaved columns held in LDS
-// (element (n, c) at buf[n*C + c]).  Span Ns: sub-transforms of length Ns are
-// complete; this stage makes them Ns*R long.
-template <int N, int C, int R>
+// (element (n, c) at buf[n*CP + c], CP >= C a padded stride).  Span Ns:
+// sub-transforms of length Ns are complete; this stage makes them Ns*R long.
+// BLK threads; every thread reads its inputs before any thread writes.
+template <int N, int C, int CP, int R, int BLK>
 __device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw, int Ns) {
     constexpr int NB = (N / R) * C;                          // butterflies
-    constexpr int PER = (NB + kFftBlock - 1) / kFftBlock;    // per thread
+    constexpr int PER = (NB + BLK - 1) / BLK;                // per thread
     float2 v[PER][R];
     int dst[PER];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        const int b = threadIdx.x + p * kFftBlock;
+        const int b = threadIdx.x + p * BLK;
         dst[p] = -1;
         if (b < NB) {
             const int c = b % C, j = b / C;
@@ -78,12 +80,12 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw
             const int tstep = (N / (Ns * R)) * k;
 #pragma unroll
             for (int m = 0; m < R; ++m) {
-                float2 x = buf[(j + m * (N / R)) * C + c];
+                float2 x = buf[(j + m * (N / R)) * CP + c];
                 if (m > 0) x = cmul(x, tw[(m * tstep) & (N - 1)]);
                 v[p][m] = x;
             }
             dft<R>(v[p]);
-            dst[p] = ((j / Ns) * Ns * R + k) * C + c;
+            dst[p] = ((j / Ns) * Ns * R + k) * CP + c;
         }
     }
     __syncthreads();
@@ -91,57 +93,119 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw
     for (int p = 0; p < PER; ++p) {
         if (dst[p] >= 0) {
 #pragma unroll
-            for (int m = 0; m < R; ++m) buf[dst[p] + m * Ns * C] = v[p][m];
+            for (int m = 0; m < R; ++m) buf[dst[p] + m * Ns * CP] = v[p][m];
         }
     }
     __syncthreads();
 }
 
-template <int N, int C>
+template <int N, int C, int CP, int BLK>
 __device__ __forceinline__ void fft_columns(float2 *buf, const float2 *__restrict__ tw) {
     int Ns = 1;
     int rem = N;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         if (rem >= 8) {
-            stage<N, C, 8>(buf, tw, Ns);
+            stage<N, C, CP, 8, BLK>(buf, tw, Ns);
             Ns *= 8;
             rem /= 8;
         }
     }
-    if (rem == 4) stage<N, C, 4>(buf, tw, Ns);
-    else if (rem == 2) stage<N, C, 2>(buf, tw, Ns);
+    if (rem == 4) stage<N, C, CP, 4, BLK>(buf, tw, Ns);
+    else if (rem == 2) stage<N, C, CP, 2, BLK>(buf, tw, Ns);
 }
 
 // In-place FFT of length N along one axis of a batch of complex arrays:
 // element n of column (o, z) of batch b at data[b*bstride + o*ostride + n*nstride + z],
 // z in [0, zlen).  Grid: (ceil(zlen/C), nother, nbatch).
-template <int N, int C>
-__global__ void __launch_bounds__(kFftBlock)
+template <int N, int C, int BLK>
+__global__ void __launch_bounds__(BLK)
 fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, long long bstride,
                int zlen, const float2 *__restrict__ tw) {
+    constexpr int PER = N * C / BLK;
+    static_assert(PER * BLK == N * C, "block shape");
     __shared__ float2 buf[N * C];
     const int z0 = blockIdx.x * C;
     const int cw = min(C, zlen - z0);
     float2 *base = data + (long long)blockIdx.z * bstride + (long long)blockIdx.y * ostride + z0;
-    for (int t = threadIdx.x; t < N * C; t += kFftBlock) {
+    // all loads issued before any is used (clamped column index: always in range)
+    float2 v[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int t = threadIdx.x + p * BLK;
         const int c = t % C, n = t / C;
-        buf[t] = (c < cw) ? base[(long long)n * nstride + c] : make_float2(0.f, 0.f);
+        v[p] = base[(long long)n * nstride + min(c, cw - 1)];
+    }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int t = threadIdx.x + p * BLK;
+        buf[t] = (t % C < cw) ? v[p] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_columns<N, C>(buf, tw);
-    for (int t = threadIdx.x; t < N * C; t += kFftBlock) {
+    fft_columns<N, C, C, BLK>(buf, tw);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int t = threadIdx.x + p * BLK;
         const int c = t % C, n = t / C;
         if (c < cw) base[(long long)n * nstride + c] = buf[t];
+    }
+}
+
+// Real-to-complex FFT of rows of N floats (contiguous), R rows per workgroup:
+// z[j] = x[2j] + i x[2j+1] is transformed with N/2-point Stockham FFTs, then
+// X[k] = E[k] + e^{-2 pi i k/N} O[k] with E[k] = (Z[k] + conj(Z[N/2-k]))/2 and
+// O[k] = -i (Z[k] - conj(Z[N/2-k]))/2, k = 0..N/2 (indices mod N/2).  Output
+// rows of N/2+1 complex.
+template <int N, int R>
+__global__ void __launch_bounds__(kFftBlock)
+fft_r2c_rows_kernel(const float *__restrict__ in, float2 *__restrict__ out, long long nrows,
+                    const float2 *__restrict__ tw_half, const float2 *__restrict__ tw_full) {
+    constexpr int H = N / 2;
+    constexpr int RP = R + 1;                      // padded LDS stride: rows land in distinct banks
+    constexpr int PER = H * R / kFftBlock;
+    static_assert(PER * kFftBlock == H * R, "block shape");
+    __shared__ float2 buf[H * RP];
+    const long long row0 = (long long)blockIdx.x * R;
+    const int rw = (int)min((long long)R, nrows - row0);
+    // R consecutive rows are N*R contiguous floats: read them as float2 pairs
+    const float2 *src = (const float2 *)(in + row0 * N);
+    float2 v[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int t = threadIdx.x + p * kFftBlock;        // pair index within the block
+        const int r = min(t / H, rw - 1);
+        v[p] = src[(long long)r * H + t % H];
+    }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int t = threadIdx.x + p * kFftBlock;
+        const int r = t / H, j = t % H;
+        buf[j * RP + r] = v[p];                           // column r = row r
+    }
+    __syncthreads();
+    fft_columns<H, R, RP, kFftBlock>(buf, tw_half);
+    float2 *dst = out + row0 * (H + 1);
+    for (int t = threadIdx.x; t < (H + 1) * R; t += kFftBlock) {
+        const int r = t / (H + 1), k = t % (H + 1);
+        if (r >= rw) continue;
+        const float2 a = buf[(k % H) * RP + r];
+        const float2 b = buf[((H - k) % H) * RP + r];     // Z[N/2 - k]
+        const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+        // O = -i/2 (a - conj(b)) = ((a.y + b.y)/2, -(a.x - b.x)/2)
+        const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+        const float2 w = tw_full[k];                      // k <= N/2 < N
+        dst[(long long)r * (H + 1) + k] = cadd(e, cmul(w, o));
     }
 }
 
 template <int N>
 int launch_cols(Ctx *c, float2 *data, long long nstride, long long ostride, long long bstride,
                 int zlen, int nother, int nbatch, const float2 *tw) {
-    constexpr int C = (N >= 1024) ? 4 : (N >= 512 ? 8 : 16);
+    // 16 complex (128 B) contiguous per point; 64 KB of LDS at most
+    constexpr int C = (N >= 1024) ? 8 : 16;
+    constexpr int BLK = (N >= 512) ? 512 : 256;
     dim3 grid((unsigned)ceil_div(zlen, C), (unsigned)nother, (unsigned)nbatch);
-    fft_col_kernel<N, C><<<grid, kFftBlock, 0, c->stream>>>(data, nstride, ostride, bstride, zlen, tw);
+    fft_col_kernel<N, C, BLK><<<grid, BLK, 0, c->stream>>>(data, nstride, ostride, bstride, zlen, tw);
     PYL_HIP(hipGetLastError());
     return PYL_OK;
 }
@@ -158,6 +222,31 @@ int cols(Ctx *c, int n, float2 *data, long long nstride, long long ostride, long
         case 1024: return launch_cols<1024>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
     }
     set_error("no column FFT of length %d", n);
+    return PYL_EINVAL;
+}
+
+template <int N>
+int launch_rows(Ctx *c, const float *in, float2 *out, long long nrows, const float2 *twh,
+                const float2 *twf) {
+    constexpr int R = (N >= 1024) ? 8 : (N >= 512 ? 16 : 32);
+    const long long nb = (nrows + R - 1) / R;
+    fft_r2c_rows_kernel<N, R><<<(unsigned)nb, kFftBlock, 0, c->stream>>>(in, out, nrows, twh, twf);
+    PYL_HIP(hipGetLastError());
+    return PYL_OK;
+}
+
+int rows(Ctx *c, int n, const float *in, float2 *out, long long nrows, const float2 *twh,
+         const float2 *twf) {
+    switch (n) {
+        case 16: return launch_rows<16>(c, in, out, nrows, twh, twf);
+        case 32: return launch_rows<32>(c, in, out, nrows, twh, twf);
+        case 64: return launch_rows<64>(c, in, out, nrows, twh, twf);
+        case 128: return launch_rows<128>(c, in, out, nrows, twh, twf);
+        case 256: return launch_rows<256>(c, in, out, nrows, twh, twf);
+        case 512: return launch_rows<512>(c, in, out, nrows, twh, twf);
+        case 1024: return launch_rows<1024>(c, in, out, nrows, twh, twf);
+    }
+    set_error("no row FFT of length %d", n);
     return PYL_EINVAL;
 }
 
@@ -194,11 +283,11 @@ int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch) {
         return run_fft(c, plan, in, out);
     }
     const long long N = dims, Z = dims / 2 + 1;
-    FFTPlan *zplan = nullptr;
-    PYL_TRY(get_fft_plan(c, dims, (int)(N * N * batch), &zplan, FFT_R2C_1D));
-    PYL_TRY(run_fft(c, zplan, in, out));
-    const float2 *tw = nullptr;
+    const float2 *tw = nullptr, *twh = nullptr;
     PYL_TRY(get_twiddles(c, dims, &tw));
+    PYL_TRY(get_twiddles(c, dims / 2, &twh));
+    // z: real rows -> half-complex rows
+    PYL_TRY(rows(c, dims, in, out, N * N * batch, twh, tw));
     // y: n = y (stride Z), other = x (stride N*Z)
     PYL_TRY(cols(c, dims, out, Z, N * Z, N * N * Z, (int)Z, dims, batch, tw));
     // x: n = x (stride N*Z), other = y (stride Z)
