@@ -9,7 +9,7 @@ shift along axis 0 (f = 0.5); 134,217,728 float32 positions resident in HBM.
 One step = the reference caller's chain (Pk_snapshot.py:77-89):
     grid = MA(pos, TSC)            (grid written, not accumulated: it starts at zero)
     delta = grid / mean - 1        (fused into the FFT input pass)
-    Pk(delta, axis=0, MAS='TSC')   (rocFFT r2c + fused binning + finalisation)
+    Pk(delta, axis=0, MAS='TSC')   (own r2c FFT passes + fused binning + finalisation)
 value = particles / step time (Mparticles/s).  The plain GPU form (per-particle
 global atomics) and a CPU baseline are timed in the same run.
 
@@ -45,6 +45,9 @@ def parse():
     ap.add_argument('--no-batched', action='store_true', help='skip the batched-realisations leg')
     ap.add_argument('--no-serving', action='store_true', help='skip the serving leg')
     ap.add_argument('--nreal', type=int, default=256, help='realisations in the batched leg')
+    ap.add_argument('--calib-copy', action='store_true',
+                    help='one untimed 512 MiB device-to-device copy before the steps: the '
+                         'known-size kernel that scales PMC FETCH_SIZE/WRITE_SIZE to bytes')
     return ap.parse_args()
 
 
@@ -63,8 +66,7 @@ class Pipeline:
         self.gp = ctypes.c_void_p(self.grid.data_ptr())
         self.ptrs = (ctypes.c_void_p * 1)(self.grid.data_ptr())
         self.masid = (ctypes.c_int * 1)(mas)
-        self.stage_sum = {}
-        self.nsteps = 0
+        self.totals = {}
 
     def step(self):
         # MA fused with delta = rho/mean - 1 (Pk_snapshot.py:80-88), then Pk on delta
@@ -72,20 +74,21 @@ class Pipeline:
         N.check(L.pyl_ma_delta(self.ctx.handle, ctypes.c_void_p(self.pos.data_ptr()), self.np, 3,
                                None, self.gp, self.dims, self.box, self.mas, self.mean, 1.0,
                                self.flags_ma))
-        t_ma = self.ctx.stage_times()
         N.check(L.pyl_pk(self.ctx.handle, self.ptrs, 1, self.masid, self.dims, self.box, self.axis,
                          1.0, 0.0, N.PTR_DEVICE | N.PK_DESTROY_INPUT, ctypes.byref(self.out)))
-        t_pk = self.ctx.stage_times()
-        for k, v in list(t_ma.items()) + list(t_pk.items()):
-            if v > 0:
-                self.stage_sum[k] = self.stage_sum.get(k, 0.0) + v
-        self.nsteps += 1
 
     def reset(self):
-        self.stage_sum, self.nsteps = {}, 0
+        # clears the stage totals; the stage events of the timed steps are read
+        # back only after the last step (HIP events on the library's stream)
+        self.ctx.enable_timing(True)
+        self.totals = {}
+
+    def collect(self):
+        self.totals = self.ctx.stage_totals()
 
     def stage_avg(self):
-        return {k: v / max(self.nsteps, 1) for k, v in self.stage_sum.items()}
+        """Average ms per launch of every stage that ran in the timed steps."""
+        return {k: ms / n for k, (ms, n) in self.totals.items() if n > 0 and ms > 0}
 
 
 def algorithmic_bytes(n, dims):
@@ -95,8 +98,8 @@ def algorithmic_bytes(n, dims):
     nk = dims * dims * (dims // 2 + 1)
     return {
         'ma_count': 12.0 * n,                    # positions read once
-        'ma_scatter': 12.0 * n + 16.0 * n,       # positions read, one 16 B record written
-        'ma_deposit': 16.0 * n + 4.0 * dims ** 3,  # records read, grid written once
+        'ma_scatter': 12.0 * n + 12.0 * n,       # positions read, one 12 B record written
+        'ma_deposit': 12.0 * n + 4.0 * dims ** 3,  # records read, grid written once
         'pk_fft': 4.0 * dims ** 3 + 8.0 * nk,    # real grid read, half-complex written once
         'pk_bin': 8.0 * nk,                      # half-complex read once
     }
@@ -124,11 +127,15 @@ def time_steps(pipe, steps, warmup):
         pipe.step()
     torch.cuda.synchronize()
     pipe.reset()
+    pipe.ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         pipe.step()
+    pipe.ctx.synchronize()
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps * 1e3
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    pipe.collect()
+    return ms
 
 
 def cpu_baseline(pos_host, grid_host, dims, box, mas_name, axis, target_s):
@@ -357,6 +364,13 @@ def main():
     n = pos.shape[0]
     ctx = N.Context(0)
     ctx.enable_timing(True)
+    if args.calib_copy:
+        src = torch.ones(1 << 27, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log('calibration copy: %d bytes read, %d bytes written' % (src.numel() * 4, dst.numel() * 4))
+        del src, dst
 
     log('tiled MA + Pk: warmup %d, steps %d' % (args.warmup, args.steps))
     pipe = Pipeline(N, ctx, pos, dims, box, mas, axis)

@@ -92,8 +92,13 @@ int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out);
 #define PYL_STAGE_PK_BIN     7
 #define PYL_STAGE_MA_FINAL   8   /* halo finalize of the tiled MA */
 #define PYL_STAGE_COUNT      9
-int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable);
+int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable);   /* also clears the totals */
+/* Stage times of the latest call (ms; 0 for stages it did not run). */
 int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */);
+/* Summed stage times and call counts since timing was enabled; recording them
+ * needs no synchronisation between calls (read back here, after the stream). */
+int pyl_ctx_stage_totals(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */,
+                         long long *count_out /* [PYL_STAGE_COUNT] */);
 
 /* ---- mass assignment (MAS_library.pyx:57-545) --------------------------
  * pos     (n, axes) float32, C order (x,y[,z] per particle)

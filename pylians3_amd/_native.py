@@ -66,6 +66,7 @@ EXPORTS = {
     'pyl_ctx_memory': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
     'pyl_ctx_enable_timing': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     'pyl_ctx_stage_times': (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    'pyl_ctx_stage_totals': (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     'pyl_ma': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
                               ctypes.c_int, ctypes.c_int]),
@@ -199,9 +200,17 @@ class Context:
         check(lib().pyl_ctx_enable_timing(self.handle, 1 if on else 0))
 
     def stage_times(self):
+        """Stage times (ms) of the latest call."""
         buf = (ctypes.c_double * len(STAGES))()
         check(lib().pyl_ctx_stage_times(self.handle, buf))
         return dict(zip(STAGES, list(buf)))
+
+    def stage_totals(self):
+        """{stage: (summed ms, calls)} since timing was enabled (no per-call sync)."""
+        ms = (ctypes.c_double * len(STAGES))()
+        cnt = (ctypes.c_longlong * len(STAGES))()
+        check(lib().pyl_ctx_stage_totals(self.handle, ms, cnt))
+        return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
 
 
 _tls = threading.local()

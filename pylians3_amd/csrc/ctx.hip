@@ -55,16 +55,36 @@ void reset_stages(Ctx *c) {
     for (int s = 0; s < PYL_STAGE_COUNT; ++s) c->ev_used[s] = false;
 }
 
+// Adds a recorded pair to the stage totals (waits for its end event).
+// A pair whose call failed before its end event was recorded is dropped.
+static int harvest(Ctx *c, int stage, int slot) {
+    if (!c->ev_pending[stage][slot]) return PYL_OK;
+    c->ev_pending[stage][slot] = false;
+    PYL_HIP(hipEventSynchronize(c->ev[stage][slot][1]));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[stage][slot][0], c->ev[stage][slot][1]) != hipSuccess) {
+        (void)hipGetLastError();
+        return PYL_OK;
+    }
+    c->ev_total_ms[stage] += ms;
+    c->ev_count[stage] += 1;
+    return PYL_OK;
+}
+
 int stage_begin(Ctx *c, int stage) {
     if (!c->timing || c->ev_used[stage]) return PYL_OK;
-    PYL_HIP(hipEventRecord(c->ev[stage][0], c->stream));
+    const int slot = (c->ev_slot[stage] + 1) % Ctx::kEvRing;
+    PYL_TRY(harvest(c, stage, slot));             // recorded kEvRing calls ago
+    PYL_HIP(hipEventRecord(c->ev[stage][slot][0], c->stream));
+    c->ev_slot[stage] = slot;
     c->ev_used[stage] = true;
+    c->ev_pending[stage][slot] = true;
     return PYL_OK;
 }
 
 int stage_end(Ctx *c, int stage) {
-    if (!c->timing) return PYL_OK;
-    PYL_HIP(hipEventRecord(c->ev[stage][1], c->stream));
+    if (!c->timing || !c->ev_used[stage]) return PYL_OK;
+    PYL_HIP(hipEventRecord(c->ev[stage][c->ev_slot[stage]][1], c->stream));
     return PYL_OK;
 }
 
@@ -176,10 +196,11 @@ int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
         hbm_budget = (size_t)(0.9 * (double)fr);
     }
     c->budget = hbm_budget;
-    for (int s = 0; s < PYL_STAGE_COUNT; ++s) {
-        PYL_HIP(hipEventCreate(&c->ev[s][0]));
-        PYL_HIP(hipEventCreate(&c->ev[s][1]));
-    }
+    for (int s = 0; s < PYL_STAGE_COUNT; ++s)
+        for (int k = 0; k < Ctx::kEvRing; ++k) {
+            PYL_HIP(hipEventCreate(&c->ev[s][k][0]));
+            PYL_HIP(hipEventCreate(&c->ev[s][k][1]));
+        }
     c->ev_ready = true;
     *out = h.release();
     return PYL_OK;
@@ -213,10 +234,11 @@ int pyl_ctx_destroy(pyl_ctx *ctx) {
     int s = pyl_ctx_trim(ctx);
     Ctx *c = &ctx->c;
     if (c->ev_ready)
-        for (int i = 0; i < PYL_STAGE_COUNT; ++i) {
-            (void)hipEventDestroy(c->ev[i][0]);
-            (void)hipEventDestroy(c->ev[i][1]);
-        }
+        for (int i = 0; i < PYL_STAGE_COUNT; ++i)
+            for (int k = 0; k < Ctx::kEvRing; ++k) {
+                if (c->ev[i][k][0]) (void)hipEventDestroy(c->ev[i][k][0]);
+                if (c->ev[i][k][1]) (void)hipEventDestroy(c->ev[i][k][1]);
+            }
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete ctx;
     return s;
@@ -243,8 +265,15 @@ int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out) {
 
 int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
-    ctx->c.timing = enable != 0;
-    for (int s = 0; s < PYL_STAGE_COUNT; ++s) ctx->c.ev_used[s] = false;
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipStreamSynchronize(c->stream));   // no recorded pair is still in flight
+    c->timing = enable != 0;
+    for (int s = 0; s < PYL_STAGE_COUNT; ++s) {
+        c->ev_used[s] = false;
+        c->ev_total_ms[s] = 0.0;
+        c->ev_count[s] = 0;
+        for (int k = 0; k < Ctx::kEvRing; ++k) c->ev_pending[s][k] = false;
+    }
     return PYL_OK;
 }
 
@@ -254,10 +283,23 @@ int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out) {
     PYL_HIP(hipStreamSynchronize(c->stream));
     for (int s = 0; s < PYL_STAGE_COUNT; ++s) {
         ms_out[s] = 0.0;
-        if (!c->ev_used[s]) continue;
+        if (!c->timing || !c->ev_used[s]) continue;
+        const int k = c->ev_slot[s];
         float ms = 0.f;
-        PYL_HIP(hipEventElapsedTime(&ms, c->ev[s][0], c->ev[s][1]));
+        PYL_HIP(hipEventElapsedTime(&ms, c->ev[s][k][0], c->ev[s][k][1]));
         ms_out[s] = ms;
+    }
+    return PYL_OK;
+}
+
+int pyl_ctx_stage_totals(pyl_ctx *ctx, double *ms_out, long long *count_out) {
+    PYL_CHECK(ctx && ms_out && count_out, PYL_EINVAL, "NULL argument");
+    Ctx *c = &ctx->c;
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < PYL_STAGE_COUNT; ++s) {
+        for (int k = 0; k < Ctx::kEvRing; ++k) PYL_TRY(harvest(c, s, k));
+        ms_out[s] = c->ev_total_ms[s];
+        count_out[s] = c->ev_count[s];
     }
     return PYL_OK;
 }

@@ -121,13 +121,25 @@ __device__ __forceinline__ void fft_columns(float2 *buf, const float2 *__restric
 template <int N, int C, int BLK>
 __global__ void __launch_bounds__(BLK)
 fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, long long bstride,
-               int zlen, const float2 *__restrict__ tw) {
+               int zlen, int nother, const float2 *__restrict__ tw) {
     constexpr int PER = N * C / BLK;
     static_assert(PER * BLK == N * C, "block shape");
     __shared__ float2 buf[N * C];
-    const int z0 = blockIdx.x * C;
+    // Workgroups are dealt to the 8 XCDs round-robin by launch index, and each
+    // XCD has its own L2.  Rows are 8*zlen bytes long, so a group's 128-byte
+    // column segment straddles two cache lines shared with the neighbouring
+    // column groups: logical index w puts all column groups of one line set on
+    // the same XCD, next to each other in time, and the shared lines are
+    // fetched from HBM once.
+    const unsigned ngz = (unsigned)((zlen + C - 1) / C);
+    const unsigned total = gridDim.x, per = total / 8u;
+    const unsigned L = blockIdx.x;
+    const unsigned w = (L < per * 8u) ? (L % 8u) * per + L / 8u : L;
+    const unsigned zg = w % ngz, t = w / ngz;
+    const unsigned line = t % (unsigned)nother, b = t / (unsigned)nother;
+    const int z0 = (int)zg * C;
     const int cw = min(C, zlen - z0);
-    float2 *base = data + (long long)blockIdx.z * bstride + (long long)blockIdx.y * ostride + z0;
+    float2 *base = data + (long long)b * bstride + (long long)line * ostride + z0;
     // all loads issued before any is used (clamped column index: always in range)
     float2 v[PER];
 #pragma unroll
@@ -204,8 +216,10 @@ int launch_cols(Ctx *c, float2 *data, long long nstride, long long ostride, long
     // 16 complex (128 B) contiguous per point; 64 KB of LDS at most
     constexpr int C = (N >= 1024) ? 8 : 16;
     constexpr int BLK = (N >= 512) ? 512 : 256;
-    dim3 grid((unsigned)ceil_div(zlen, C), (unsigned)nother, (unsigned)nbatch);
-    fft_col_kernel<N, C, BLK><<<grid, BLK, 0, c->stream>>>(data, nstride, ostride, bstride, zlen, tw);
+    const long long nwg = (long long)ceil_div(zlen, C) * nother * nbatch;
+    PYL_CHECK(nwg < (1ll << 31), PYL_EINVAL, "FFT column launch too large (%lld workgroups)", nwg);
+    fft_col_kernel<N, C, BLK><<<(unsigned)nwg, BLK, 0, c->stream>>>(data, nstride, ostride, bstride,
+                                                                     zlen, nother, tw);
     PYL_HIP(hipGetLastError());
     return PYL_OK;
 }

@@ -216,6 +216,8 @@ struct Chunk {
     long long c0, c1;   // particle range
     int r;              // realisation
     int wbase;          // window start, relative to the realisation's first key (count pass)
+    int smin, smax;     // first and last non-empty window slot (count pass; smin > smax: none)
+    int pad0, pad1;
 };
 
 // Home tile along one axis; POW2: every tile has extent 2^shift (or nt == 1).
@@ -324,9 +326,35 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
     }
     __syncthreads();
     unsigned *dmp = dump + (long long)blockIdx.x * kWin;
+    int smin = kWin, smax = -1;
     for (int s = threadIdx.x; s < kWin; s += blockDim.x) {
         const unsigned v = hist[s];
-        if (v) dmp[s] = atomicAdd(&counts[win_key(s, rbase, w0, g.ntiles)], v);
+        if (v) {
+            dmp[s] = atomicAdd(&counts[win_key(s, rbase, w0, g.ntiles)], v);
+            smin = min(smin, s);
+            smax = max(smax, s);
+        }
+    }
+    // the scatter pass reloads only the slots in [smin, smax]
+    for (int off = 32; off > 0; off >>= 1) {
+        smin = min(smin, __shfl_xor(smin, off));
+        smax = max(smax, __shfl_xor(smax, off));
+    }
+    __syncthreads();                           // hist is reused for the wave results
+    int *red = (int *)hist;
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = smin;
+        red[2 * (threadIdx.x >> 6) + 1] = smax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = kWin, b = -1;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            a = min(a, red[2 * k]);
+            b = max(b, red[2 * k + 1]);
+        }
+        chunks[blockIdx.x].smin = a;
+        chunks[blockIdx.x].smax = b;
     }
 }
 
@@ -340,13 +368,14 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
                     TileGeom g, const Chunk *__restrict__ chunks,
                     const unsigned *__restrict__ dump, const unsigned long long *__restrict__ offsets,
                     const unsigned *__restrict__ counts, unsigned *__restrict__ ocursors,
-                    unsigned *__restrict__ wmax_bits, float4 *__restrict__ records) {
+                    unsigned *__restrict__ wmax_bits, float *__restrict__ records) {
     __shared__ unsigned cur[kWin];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
     const int w0 = ch.wbase;
     const unsigned *dmp = dump + (long long)blockIdx.x * kWin;
-    for (int s = threadIdx.x; s < kWin; s += blockDim.x) cur[s] = dmp[s];
+    // only slots in [smin, smax] were counted; no other slot is looked up
+    for (int s = ch.smin + threadIdx.x; s <= ch.smax; s += blockDim.x) cur[s] = dmp[s];
     __syncthreads();
     unsigned m = 0;
     const long long step = (long long)blockDim.x * kUnroll;
@@ -375,7 +404,15 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             if (base + (long long)u * blockDim.x < ch.c1) {
-                records[offsets[key[u]] + slot[u]] = make_float4(d[u].x, d[u].y, d[u].z, w[u]);
+                const unsigned long long at = offsets[key[u]] + slot[u];
+                if (WT) {
+                    reinterpret_cast<float4 *>(records)[at] = make_float4(d[u].x, d[u].y, d[u].z, w[u]);
+                } else {
+                    float *o = records + 3 * at;
+                    o[0] = d[u].x;
+                    o[1] = d[u].y;
+                    o[2] = d[u].z;
+                }
                 if (WT) {
                     const unsigned b = __float_as_uint(fabsf(w[u]));
                     if (b < 0x7f800000u) m = max(m, b);
@@ -515,13 +552,118 @@ __host__ __device__ __forceinline__ TileBox tile_box(int tile, const TileGeom &g
     return b;
 }
 
-// One workgroup per tile.  LDS box of BX*BY*BZ doubles (the tile plus the
-// upper halo) holding integer multiples of 2^q; every record adds its whole
-// footprint at fixed offsets from its first node.
+// Sorted particle records: (dist_x, dist_y, dist_z, W) as float4 when the
+// particles carry weights, (dist_x, dist_y, dist_z) packed in 12 bytes
+// otherwise (w reads as 1).
+template <bool RW>
+__device__ __forceinline__ float4 load_record(const float *__restrict__ records,
+                                              unsigned long long i) {
+    if (RW) return reinterpret_cast<const float4 *>(records)[i];
+    const float *p = records + 3 * i;
+    return make_float4(p[0], p[1], p[2], 1.0f);
+}
+
+// Box offset of a record's first footprint node (records of this tile satisfy
+// 0 <= l < e on every axis; the clamp only guards the LDS box).
 template <int MAS, int AXES>
+__device__ __forceinline__ int box_base(const float4 &rec, const TileGeom &g, const TileBox &tb) {
+    constexpr int F = MAS;
+    constexpr int T = (AXES == 3) ? kT3 : kT2;
+    constexpr int BY = T + F - 1;
+    constexpr int BZ = (AXES == 3) ? T + F - 1 : 1;
+    const int lx = min(max(wrap_node(first_node<MAS>(rec.x), g.dims) - tb.ox, 0), tb.ex - 1);
+    const int ly = min(max(wrap_node(first_node<MAS>(rec.y), g.dims) - tb.oy, 0), tb.ey - 1);
+    int lz = 0;
+    if (AXES == 3) lz = min(max(wrap_node(first_node<MAS>(rec.z), g.dims) - tb.oz, 0), tb.ez - 1);
+    return (lx * BY + ly) * BZ + lz;
+}
+
+// Adds one record's footprint to the LDS box at compile-time offsets from `a`.
+// ((Cx*Cy)*Cz)*(W*2^-q) == 2^-q * (((Cx*Cy)*Cz)*W) exactly (power-of-two
+// scaling commutes with float rounding), so the scaled product is the
+// reference's float contribution in units of the quantum.  Without weights
+// (Cx*Cy)*(Cz*2^-q) is the same number: it can differ only where (Cx*Cy)*Cz is
+// subnormal, far below half a quantum, and both round to 0.
+template <int MAS, int AXES, bool WT>
+__device__ __forceinline__ void deposit_record(unsigned long long *a, const float4 &rec,
+                                               float to_q, float weight_factor) {
+    constexpr int F = MAS;
+    constexpr int T = (AXES == 3) ? kT3 : kT2;
+    constexpr int BY = T + F - 1;
+    constexpr int BZ = (AXES == 3) ? T + F - 1 : 1;
+    float Cx[4], Cy[4], Cz[4];
+    foot_weights<MAS>(rec.x, first_node<MAS>(rec.x), Cx);
+    foot_weights<MAS>(rec.y, first_node<MAS>(rec.y), Cy);
+    if (AXES == 3) foot_weights<MAS>(rec.z, first_node<MAS>(rec.z), Cz);
+    if (WT) {
+        const float w = (rec.w * weight_factor) * to_q;
+#pragma unroll
+        for (int l = 0; l < F; ++l) {
+#pragma unroll
+            for (int m = 0; m < F; ++m) {
+                const float cxy = Cx[l] * Cy[m];
+                if (AXES == 3) {
+#pragma unroll
+                    for (int k = 0; k < F; ++k) {
+                        const float c = (cxy * Cz[k]) * w;
+                        atomicAdd(a + (l * BY + m) * BZ + k,
+                                  (unsigned long long)(long long)__float2int_rn(c));
+                    }
+                } else {
+                    const float c = cxy * w;   // the reference multiplies by C[2][n] = 1.0
+                    atomicAdd(a + l * BY + m, (unsigned long long)(long long)__float2int_rn(c));
+                }
+            }
+        }
+    } else {
+        float Cs[4];                         // last axis' weights in quanta (>= 0)
+#pragma unroll
+        for (int k = 0; k < F; ++k) Cs[k] = (AXES == 3 ? Cz[k] : Cy[k]) * to_q;
+#pragma unroll
+        for (int l = 0; l < F; ++l) {
+            if (AXES == 3) {
+#pragma unroll
+                for (int m = 0; m < F; ++m) {
+                    const float cxy = Cx[l] * Cy[m];
+#pragma unroll
+                    for (int k = 0; k < F; ++k)
+                        atomicAdd(a + (l * BY + m) * BZ + k,
+                                  (unsigned long long)(unsigned)__float2int_rn(cxy * Cs[k]));
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < F; ++m)
+                    atomicAdd(a + l * BY + m,
+                              (unsigned long long)(unsigned)__float2int_rn(Cx[l] * Cs[m]));
+            }
+        }
+    }
+}
+
+// Records each wave stages per round set in the bank-balanced deposit: KR per
+// lane (fewer for the PCS 3D box, so that two workgroups still fit one CU).
+template <int MAS, int AXES>
+constexpr int dep_kr() { return (MAS == 4 && AXES == 3) ? 2 : 4; }
+
+// One workgroup per tile.  LDS box of BX*BY*BZ 64-bit integers (the tile plus
+// the upper halo) holding multiples of 2^q; every record adds its whole
+// footprint at fixed offsets from its first node.  WT = false: unit weights
+// and weight_factor 1, so every contribution is >= 0 and the power-of-two
+// scale can be folded into the last axis' weights.
+//
+// Bank balancing (F >= 2): a footprint's F^AXES atomics all share the lanes'
+// box offsets up to a constant, so their LDS bank conflicts are fixed by the
+// offsets' residues mod 32 (32 banks of 8-byte words).  Each wave takes KR*64
+// consecutive records, ranks them by residue (LDS counters), and deals them
+// out in residue order so that every half-wave of every round takes every
+// (2*KR)-th record of that order: about one record per residue.  On random
+// offsets this halves the cost of a ds_add_u64 (12.6 -> 6.2 clk per wave in
+// a microbenchmark of the two extremes).  The sums are integers, so the
+// order of the atomics does not change any bit of the result.
+template <int MAS, int AXES, bool WT, bool RW>
 __global__ void __launch_bounds__(kDepBlock)
-tile_deposit_kernel(const float4 *__restrict__ records,
-                    const unsigned long long *__restrict__ offsets, TileGeom g, int weighted,
+tile_deposit_kernel(const float *__restrict__ records,
+                    const unsigned long long *__restrict__ offsets, TileGeom g,
                     float weight_factor, const unsigned *__restrict__ wmax_bits,
                     float *__restrict__ grid, float *__restrict__ partials, OutMode om) {
     constexpr int F = MAS;
@@ -530,7 +672,13 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     constexpr int BZ = (AXES == 3) ? T + F - 1 : 1;
     constexpr int BX = T + F - 1;
     constexpr int NB = BX * BY * BZ;
+    constexpr bool BAL = F >= 2;
+    constexpr int KR = BAL ? dep_kr<MAS, AXES>() : 1;
+    constexpr int NW = kDepBlock / 64;                 // waves
+    constexpr int WB = KR * 64;                        // records per wave round set
     __shared__ unsigned long long acc[NB];
+    __shared__ float4 stage[BAL ? NW * WB : 1];
+    __shared__ unsigned whist[BAL ? NW * 32 : 1];
     const int key = blockIdx.x;                       // r * ntiles + tile
     const int real = key / (int)g.ntiles;
     const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
@@ -559,7 +707,7 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     // weight_factor * max finite |W| of the realisation < 2^e.  With q = e - 30
     // every rounded contribution is an int32 and every box sum an int64 for
     // any count below 2^33: exact integer atomics, independent of their order.
-    const float wmax = weighted ? __uint_as_float(wmax_bits[real]) : 1.0f;
+    const float wmax = (WT && wmax_bits) ? __uint_as_float(wmax_bits[real]) : 1.0f;
     const double bound = (double)wmax * (double)weight_factor;
     int e = -1000;
     if (bound > 0.0) frexp(bound, &e);
@@ -570,51 +718,82 @@ tile_deposit_kernel(const float4 *__restrict__ records,
     const double from_q = ldexp(1.0, q);
     __syncthreads();
 
-    // the next record is loaded while the current one is deposited
-    unsigned long long r = r0 + threadIdx.x;
-    float4 nxt = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (r < r1) nxt = records[r];
-    for (; r < r1; r += blockDim.x) {
-        const float4 rec = nxt;
-        if (r + blockDim.x < r1) nxt = records[r + blockDim.x];
-        float Cx[4], Cy[4], Cz[4];
-        const int rx = first_node<MAS>(rec.x);
-        const int ry = first_node<MAS>(rec.y);
-        foot_weights<MAS>(rec.x, rx, Cx);
-        foot_weights<MAS>(rec.y, ry, Cy);
-        int lx = wrap_node(rx, g.dims) - tb.ox;
-        int ly = wrap_node(ry, g.dims) - tb.oy;
-        lx = min(max(lx, 0), ex - 1);       // records of this tile satisfy 0 <= l < e
-        ly = min(max(ly, 0), ey - 1);
-        int lz = 0;
-        if (AXES == 3) {
-            const int rz = first_node<MAS>(rec.z);
-            foot_weights<MAS>(rec.z, rz, Cz);
-            lz = wrap_node(rz, g.dims) - tb.oz;
-            lz = min(max(lz, 0), ez - 1);
+    if constexpr (!BAL) {
+        // the next record is loaded while the current one is deposited
+        unsigned long long r = r0 + threadIdx.x;
+        float4 nxt = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < r1) nxt = load_record<RW>(records, r);
+        for (; r < r1; r += blockDim.x) {
+            const float4 rec = nxt;
+            if (r + blockDim.x < r1) nxt = load_record<RW>(records, r + blockDim.x);
+            deposit_record<MAS, AXES, WT>(acc + box_base<MAS, AXES>(rec, g, tb), rec, to_q,
+                                          weight_factor);
         }
-        // ((Cx*Cy)*Cz)*(W*2^-q) == 2^-q * (((Cx*Cy)*Cz)*W) exactly (power-of-two
-        // scaling commutes with float rounding), so the scaled product is the
-        // reference's float contribution in units of the quantum.
-        const float w = (rec.w * weight_factor) * to_q;
-        unsigned long long *a = acc + (lx * BY + ly) * BZ + lz;
+    } else {
+        const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+        float4 *st = stage + wv * WB;
+        unsigned *wh = whist + wv * 32;
+        // wave wv takes the round sets starting at r0 + (wv + NW*j)*WB; the next
+        // set's records are loaded (clamped into the tile) while this one runs
+        unsigned long long b0 = r0 + (unsigned long long)wv * WB;
+        float4 nxt[KR];
 #pragma unroll
-        for (int l = 0; l < F; ++l) {
+        for (int k = 0; k < KR; ++k) nxt[k] = load_record<RW>(records, min(b0 + ln + 64 * k, r1 - 1));
+        for (; b0 < r1; b0 += (unsigned long long)NW * WB) {
+            float4 rec[KR];
 #pragma unroll
-            for (int m = 0; m < F; ++m) {
-                const float cxy = Cx[l] * Cy[m];
-                if (AXES == 3) {
+            for (int k = 0; k < KR; ++k) rec[k] = nxt[k];
+            const int nb = (int)min((unsigned long long)WB, r1 - b0);   // records in this set
+            const unsigned long long b1 = b0 + (unsigned long long)NW * WB;
+            if (b1 < r1) {
 #pragma unroll
-                    for (int k = 0; k < F; ++k) {
-                        const float c = (cxy * Cz[k]) * w;
-                        atomicAdd(a + (l * BY + m) * BZ + k,
-                                  (unsigned long long)(long long)__float2int_rn(c));
-                    }
-                } else {
-                    const float c = cxy * w;   // the reference multiplies by C[2][n] = 1.0
-                    atomicAdd(a + l * BY + m, (unsigned long long)(long long)__float2int_rn(c));
+                for (int k = 0; k < KR; ++k)
+                    nxt[k] = load_record<RW>(records, min(b1 + ln + 64 * k, r1 - 1));
+            }
+            // rank within residue class (LDS ops of one wave run in issue order)
+            if (ln < 32) wh[ln] = 0u;
+            __builtin_amdgcn_wave_barrier();
+            int base[KR];
+            unsigned cls[KR], rank[KR];
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                base[k] = box_base<MAS, AXES>(rec[k], g, tb);
+                cls[k] = (unsigned)base[k] & 31u;
+                rank[k] = (ln + 64 * k < nb) ? atomicAdd(&wh[cls[k]], 1u) : 0u;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // exclusive prefix of the 32 class counts
+            const unsigned h = (ln < 32) ? wh[ln] : 0u;
+            unsigned incl = h;
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                const unsigned t = __shfl_up(incl, o);
+                if (ln >= o) incl += t;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (ln < 32) wh[ln] = incl - h;
+            __builtin_amdgcn_wave_barrier();
+            // position p in residue order -> round p % KR, lane ((p/KR)&1)*32 + p/(2KR)
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                if (ln + 64 * k < nb) {
+                    const unsigned p = wh[cls[k]] + rank[k];
+                    const unsigned slot = (p % KR) * 64 + ((p / KR) & 1u) * 32 + p / (2 * KR);
+                    st[slot] = make_float4(rec[k].x, rec[k].y, rec[k].z,
+                                           WT ? rec[k].w : __int_as_float(base[k]));
                 }
             }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < KR; ++r) {
+                const int p = (ln & 31) * (2 * KR) + (ln >> 5) * KR + r;
+                if (p < nb) {
+                    const float4 rr = st[r * 64 + ln];
+                    const int bb = WT ? box_base<MAS, AXES>(rr, g, tb) : __float_as_int(rr.w);
+                    deposit_record<MAS, AXES, WT>(acc + bb, rr, to_q, weight_factor);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
     __syncthreads();
@@ -666,6 +845,16 @@ __host__ __device__ __forceinline__ int axis_images(int t, int nt, int dims, con
     return n;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by launch index, and each XCD
+// has its own L2.  The finalize of a tile reads halo partials that its
+// neighbours along z (adjacent keys) and y (nt keys away) also read, a few
+// bytes apart; this remap gives each XCD a contiguous range of keys, in key
+// order, so that those shared cache lines come from HBM once.
+__device__ __forceinline__ int xcd_contiguous_block() {
+    const unsigned total = gridDim.x, per = total / 8u, L = blockIdx.x;
+    return (int)((L < per * 8u) ? (L % 8u) * per + L / 8u : L);
+}
+
 // Band cells (l < F-1 along some axis) of every tile: own band value plus the
 // halo values of the lower neighbours, summed in double in a fixed order (the
 // image order), then written once.  Empty tiles contribute nothing.
@@ -675,7 +864,7 @@ tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
                      const float *__restrict__ partials, float *__restrict__ grid, OutMode om) {
     constexpr int F = MAS;
     __shared__ long long s_pbase[64];   // partial base of image combination, -1 if empty
-    const int key = blockIdx.x;
+    const int key = xcd_contiguous_block();
     const int real = key / (int)g.ntiles;
     const long long kbase = (long long)real * g.ntiles;
     const TileBox tb = tile_box<AXES>(key - (int)kbase, g);
@@ -730,6 +919,72 @@ tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
             ? ((long long)(tb.ox + lx) * g.dims + (tb.oy + ly)) * g.dims + (tb.oz + lz)
             : (long long)(tb.ox + lx) * g.dims + (tb.oy + ly);
         store_node(grid, node, (float)v, om);
+    }
+}
+
+// The same finalize for 3D grids cut into equal tiles of kT3 nodes with at
+// least two tiles per axis (dims % kT3 == 0, dims > kT3).  Every lower
+// neighbour then has extent kT3, so each band cell (lx, ly, lz) takes image 1
+// along an axis exactly when its coordinate there is below F-1, and the
+// partial indices fold to constants.  The up to 8 loads of a cell are issued
+// before any is summed; the sum runs in the image order of the general kernel
+// (a, b, c nested), so both kernels give the same bits.
+template <int MAS>
+__global__ void __launch_bounds__(kBlock)
+tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
+                             const float *__restrict__ partials, float *__restrict__ grid,
+                             OutMode om) {
+    constexpr int F = MAS, T = kT3, L = F - 1;
+    constexpr int NA = L * T * T;                       // lx < L
+    constexpr int NB = (T - L) * L * T;                 // lx >= L, ly < L
+    constexpr int NC = (T - L) * (T - L) * L;           // lx, ly >= L, lz < L
+    constexpr int NBAND = NA + NB + NC;
+    __shared__ long long s_pbase[8];                    // image (a, b, c) at (a*2 + b)*2 + c
+    const int key = xcd_contiguous_block();
+    const int real = key / (int)g.ntiles;
+    const long long kbase = (long long)real * g.ntiles;
+    const int tile = key - (int)kbase;
+    const int tz = tile % g.nt, ty = (tile / g.nt) % g.nt, tx = tile / (g.nt * g.nt);
+    if (threadIdx.x < 8) {
+        const int a = threadIdx.x >> 2, b = (threadIdx.x >> 1) & 1, c = threadIdx.x & 1;
+        const int ix = a ? (tx == 0 ? g.nt - 1 : tx - 1) : tx;
+        const int iy = b ? (ty == 0 ? g.nt - 1 : ty - 1) : ty;
+        const int iz = c ? (tz == 0 ? g.nt - 1 : tz - 1) : tz;
+        const long long vkey = kbase + ((long long)ix * g.nt + iy) * g.nt + iz;
+        s_pbase[threadIdx.x] = (offsets[vkey] == offsets[vkey + 1]) ? -1 : vkey * g.pslot;
+    }
+    __syncthreads();
+    grid += real * g.ncell;
+    const long long ox = (long long)tx * T, oy = (long long)ty * T, oz = (long long)tz * T;
+    for (int t = threadIdx.x; t < NBAND; t += kBlock) {
+        int lx, ly, lz;
+        if (t < NA) {
+            lz = t % T; ly = (t / T) % T; lx = t / (T * T);
+        } else if (t < NA + NB) {
+            const int u = t - NA;
+            lz = u % T; ly = (u / T) % L; lx = L + u / (T * L);
+        } else {
+            const int u = t - NA - NB;
+            lz = u % L; ly = L + (u / L) % (T - L); lx = L + u / (L * (T - L));
+        }
+        float x[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+            const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
+            const long long pb = s_pbase[i];
+            x[i] = 0.0f;
+            if (on && pb >= 0)
+                x[i] = partials[pb + halo_index<F, 3>(lx + a * T, ly + b * T, lz + c * T, T, T, T)];
+        }
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+            const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
+            if (on && s_pbase[i] >= 0) v += (double)x[i];
+        }
+        store_node(grid, ((ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz), (float)v, om);
     }
 }
 
@@ -924,8 +1179,9 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     PYL_HIP(hipStreamSynchronize(c->stream));   // also keeps `chunks`, `orig` alive until uploaded
     PYL_CHECK(total == (unsigned long long)n, PYL_EHIP, "tile sort lost particles (%llu of %lld)",
               total, n);
-    PYL_TRY(ensure(c, BUF_RECORDS, (size_t)(total ? total : 1) * sizeof(float4)));
-    float4 *records = (float4 *)c->bufs[BUF_RECORDS].ptr;
+    const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
+    PYL_TRY(ensure(c, BUF_RECORDS, (size_t)(total ? total : 1) * rec_bytes));
+    float *records = (float *)c->bufs[BUF_RECORDS].ptr;
     if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
         auto ks = W ? ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true>
@@ -941,19 +1197,23 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
     }
-    const int weighted = W != nullptr;
+    const bool unit = W == nullptr && weight_factor == 1.0f;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_DEPOSIT));
-    if (axes == 3)
-        tile_deposit_kernel<MAS, 3><<<(unsigned)nt, kDepBlock, 0, c->stream>>>(
-            records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
-    else
-        tile_deposit_kernel<MAS, 2><<<(unsigned)nt, kDepBlock, 0, c->stream>>>(
-            records, offsets, g, weighted, weight_factor, wmax, grid, partials, om);
+    auto kd = (axes == 3)
+        ? (unit ? tile_deposit_kernel<MAS, 3, false, false>
+                : (W ? tile_deposit_kernel<MAS, 3, true, true> : tile_deposit_kernel<MAS, 3, true, false>))
+        : (unit ? tile_deposit_kernel<MAS, 2, false, false>
+                : (W ? tile_deposit_kernel<MAS, 2, true, true> : tile_deposit_kernel<MAS, 2, true, false>));
+    kd<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(records, offsets, g, weight_factor,
+                                                 W ? wmax : nullptr, grid, partials, om);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));
     if (MAS > 1) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_FINAL));
-        if (axes == 3)
+        if (axes == 3 && g.nt >= 2 && dims % kT3 == 0)
+            tile_finalize_uniform_kernel<MAS><<<(unsigned)nt, kBlock, 0, c->stream>>>(
+                offsets, g, partials, grid, om);
+        else if (axes == 3)
             tile_finalize_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
                                                                                 grid, om);
         else

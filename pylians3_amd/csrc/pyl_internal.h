@@ -104,9 +104,17 @@ struct Ctx {
     std::map<std::pair<int, int>, ModePlan> mode_plans;    // (dims, axis)
     std::map<std::tuple<int, int, int>, BinPlan> bin_plans; // (dims, axis, rows per item)
     std::map<int, void *> twiddles;                          // exp(-2 pi i k/N), per N
+    // Stage timing: a ring of event pairs per stage, so that timing many calls
+    // needs no host synchronisation between them.  A pair is read back when its
+    // ring slot comes round again (kEvRing calls later) or on request.
+    static constexpr int kEvRing = 64;
     bool timing = false;
-    hipEvent_t ev[PYL_STAGE_COUNT][2] = {};
-    bool ev_used[PYL_STAGE_COUNT] = {};
+    hipEvent_t ev[PYL_STAGE_COUNT][kEvRing][2] = {};
+    int ev_slot[PYL_STAGE_COUNT] = {};                 // slot of the latest call
+    bool ev_used[PYL_STAGE_COUNT] = {};                // stage ran in the latest call
+    bool ev_pending[PYL_STAGE_COUNT][kEvRing] = {};    // recorded, not yet added to the totals
+    double ev_total_ms[PYL_STAGE_COUNT] = {};
+    long long ev_count[PYL_STAGE_COUNT] = {};
     bool ev_ready = false;
 };
 

@@ -1,0 +1,259 @@
+// CPU emulation of the tiled mass-assignment index logic of csrc/ma.hip: home
+// tiles, halo boxes, halo_index, band enumeration, axis_images, and the
+// uniform-tile finalize.  Uses the same device helpers (compiled for the host)
+// and checks, for every scheme, 2D and 3D, many grid sizes and edge positions:
+//   - every LDS box index, partial slot and grid node is in bounds;
+//   - every partial slot is written once and read only after it is written;
+//   - every grid node is written exactly once;
+//   - the result equals a direct float64 deposit;
+//   - the uniform-tile finalize gives the same bits as the general one.
+// Built and run by tests/test_ma_index_emul.py (CPU only, no GPU calls).
+#include "../../pylians3_amd/csrc/ma.hip"
+#include <cassert>
+#include <random>
+
+using namespace pyl;
+static long long uni_checked = 0;
+
+template <int MAS, int AXES>
+static int run_case(int dims, int n, int pattern, bool weighted, unsigned seed) {
+    constexpr int F = MAS;
+    constexpr int T = (AXES == 3) ? kT3 : kT2;
+    constexpr int BY = T + F - 1;
+    constexpr int BZ = (AXES == 3) ? T + F - 1 : 1;
+    constexpr int BX = T + F - 1;
+    constexpr int NB = BX * BY * BZ;
+    const float box = 100.0f;
+    const float inv = (float)dims / box;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> U(0.0f, box);
+    std::normal_distribution<float> Nd(0.0f, 1.0f);
+    std::vector<float> pos((size_t)n * AXES), W(n, 1.0f);
+    for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < AXES; ++a) {
+            float x;
+            if (pattern == 0) x = U(rng);
+            else if (pattern == 1) {   // lattice-ish + displacement
+                int side = (int)std::ceil(std::pow((double)n, 1.0 / AXES));
+                int c = (a == 0) ? i / (side * (AXES == 3 ? side : 1)) : (a == 1 ? (i / (AXES == 3 ? side : 1)) % side : i % side);
+                x = (c + 0.5f) * box / side + 3.0f * box / dims * Nd(rng);
+                x = std::fmod(x, box); if (x < 0) x += box; if (x >= box) x = 0.0f;
+            } else {                     // edges: nodes, half nodes, box ends
+                int k = rng() % (dims + 1);
+                float fr[6] = {0.0f, 0.5f, 0.25f, 0.999999f, 0.5000001f, 0.4999999f};
+                x = (k + fr[rng() % 6]) * box / dims;
+                if (rng() % 7 == 0) x = std::nextafter(box, 0.0f);
+                if (x >= box) x = 0.0f;
+            }
+            pos[(size_t)i * AXES + a] = x;
+        }
+        if (weighted) W[i] = std::exp(Nd(rng));
+    }
+    TileGeom g = make_geom(dims, AXES, MAS);
+    std::vector<int> orig(g.nt + 1);
+    for (int i = 0; i <= g.nt; ++i) orig[i] = tile_origin(i, g.nt, dims);
+    g.orig = orig.data();
+    // direct float64 deposit
+    const long long ncell = g.ncell;
+    std::vector<double> ref(ncell, 0.0);
+    for (int i = 0; i < n; ++i) {
+        float C[3][4];
+        int b[3] = {0, 0, 0};
+        for (int a = 0; a < AXES; ++a) {
+            float dist = pos[(size_t)i * AXES + a] * inv;
+            int raw = first_node<MAS>(dist);
+            foot_weights<MAS>(dist, raw, C[a]);
+            b[a] = wrap_node(raw, dims);
+        }
+        for (int l = 0; l < F; ++l)
+            for (int m = 0; m < F; ++m)
+                for (int k = 0; k < (AXES == 3 ? F : 1); ++k) {
+                    long long node = AXES == 3
+                        ? ((long long)node_add(b[0], l, dims) * dims + node_add(b[1], m, dims)) * dims + node_add(b[2], k, dims)
+                        : (long long)node_add(b[0], l, dims) * dims + node_add(b[1], m, dims);
+                    double c = (double)C[0][l] * C[1][m] * (AXES == 3 ? C[2][k] : 1.0f) * W[i];
+                    ref[node] += c;
+                }
+    }
+    // keys and records (the sort order inside a tile does not matter)
+    std::vector<std::vector<int>> bucket(g.ntiles);
+    for (int i = 0; i < n; ++i) {
+        int t[3] = {0, 0, 0};
+        for (int a = 0; a < AXES; ++a) t[a] = g.shift >= 0 ? home_tile<MAS, true>(pos[(size_t)i * AXES + a] * inv, g) : home_tile<MAS, false>(pos[(size_t)i * AXES + a] * inv, g);
+        long long key = ((long long)t[0] * g.nt + t[1]) * g.ntz + t[2];
+        assert(key >= 0 && key < g.ntiles);
+        bucket[key].push_back(i);
+    }
+    std::vector<double> grid(ncell, 0.0);
+    std::vector<int> writes(ncell, 0);
+    std::vector<double> part((size_t)g.ntiles * g.pslot, -1e300);
+    for (long long key = 0; key < g.ntiles; ++key) {
+        TileBox tb = tile_box<AXES>((int)key, g);
+        assert(tb.ex >= 1 && tb.ex <= T && tb.ey >= 1 && tb.ey <= T && tb.ez >= 1);
+        assert((halo_count<F, AXES>(tb.ex, tb.ey, tb.ez)) <= g.pslot);
+        std::vector<double> acc(NB, 0.0);
+        for (int i : bucket[key]) {
+            float C[3][4];
+            int l[3] = {0, 0, 0};
+            int o[3] = {tb.ox, tb.oy, tb.oz}, e[3] = {tb.ex, tb.ey, tb.ez};
+            for (int a = 0; a < AXES; ++a) {
+                float dist = pos[(size_t)i * AXES + a] * inv;
+                int raw = first_node<MAS>(dist);
+                foot_weights<MAS>(dist, raw, C[a]);
+                l[a] = wrap_node(raw, dims) - o[a];
+                assert(l[a] >= 0 && l[a] < e[a]);
+            }
+            for (int x = 0; x < F; ++x)
+                for (int y = 0; y < F; ++y)
+                    for (int z = 0; z < (AXES == 3 ? F : 1); ++z) {
+                        int idx = ((l[0] + x) * BY + (l[1] + y)) * BZ + l[2] + z;
+                        assert(idx >= 0 && idx < NB);
+                        acc[idx] += (double)C[0][x] * C[1][y] * (AXES == 3 ? C[2][z] : 1.0f) * W[i];
+                    }
+        }
+        const bool empty = bucket[key].empty();
+        for (int c = 0; c < NB; ++c) {
+            const int bz = c % BZ, by = (c / BZ) % BY, bx = c / (BZ * BY);
+            if (bx >= tb.ex + F - 1 || by >= tb.ey + F - 1 || (AXES == 3 && bz >= tb.ez + F - 1)) {
+                assert(acc[c] == 0.0);
+                continue;
+            }
+            const bool inner = bx >= F - 1 && bx < tb.ex && by >= F - 1 && by < tb.ey &&
+                               (AXES == 2 || (bz >= F - 1 && bz < tb.ez));
+            if (inner) {
+                long long node = AXES == 3
+                    ? ((long long)(tb.ox + bx) * dims + (tb.oy + by)) * dims + (tb.oz + bz)
+                    : (long long)(tb.ox + bx) * dims + (tb.oy + by);
+                assert(node >= 0 && node < ncell);
+                grid[node] = acc[c];
+                writes[node]++;
+            } else if (!empty) {
+                int h = halo_index<F, AXES>(bx, by, bz, tb.ex, tb.ey, tb.ez);
+                assert(h >= 0 && h < (halo_count<F, AXES>(tb.ex, tb.ey, tb.ez)));
+                assert(part[key * g.pslot + h] == -1e300);   // each slot written once
+                part[key * g.pslot + h] = acc[c];
+            }
+        }
+    }
+    if (F > 1) {
+        for (long long key = 0; key < g.ntiles; ++key) {
+            TileBox tb = tile_box<AXES>((int)key, g);
+            const int ex = tb.ex, ey = tb.ey, ez = tb.ez;
+            const int nbx = std::min(F - 1, ex), nby = std::min(F - 1, ey), nbz = (AXES == 3) ? std::min(F - 1, ez) : 0;
+            const int sa = nbx * ey * ez;
+            const int sb = (ex - nbx) * nby * ez;
+            const int sc = (AXES == 3) ? (ex - nbx) * (ey - nby) * nbz : 0;
+            for (int t = 0; t < sa + sb + sc; ++t) {
+                int lx, ly, lz;
+                if (t < sa) { lz = t % ez; ly = (t / ez) % ey; lx = t / (ez * ey); }
+                else if (t < sa + sb) { int u = t - sa; lz = u % ez; ly = (u / ez) % nby; lx = nbx + u / (ez * nby); }
+                else { int u = t - sa - sb; lz = u % nbz; ly = nby + (u / nbz) % (ey - nby); lx = nbx + u / (nbz * (ey - nby)); }
+                assert(lx < ex && ly < ey && lz < ez);
+                int itx[4], iox[4], iex[4], ity[4], ioy[4], iey[4], itz[4], ioz[4], iez[4];
+                int nx = axis_images<F>(tb.tx, g.nt, dims, g.orig, itx, iox, iex);
+                int ny = axis_images<F>(tb.ty, g.nt, dims, g.orig, ity, ioy, iey);
+                int nz = 1; itz[0] = 0; ioz[0] = 0; iez[0] = 1;
+                if (AXES == 3) nz = axis_images<F>(tb.tz, g.ntz, dims, g.orig, itz, ioz, iez);
+                assert(nx * ny * nz <= 64);
+                double v = 0.0;
+                for (int a = 0; a < nx; ++a) {
+                    int bx = lx + iox[a];
+                    if (bx >= iex[a] + F - 1) continue;
+                    for (int b = 0; b < ny; ++b) {
+                        int by = ly + ioy[b];
+                        if (by >= iey[b] + F - 1) continue;
+                        for (int c = 0; c < nz; ++c) {
+                            int bz = lz + ioz[c];
+                            if (AXES == 3 && bz >= iez[c] + F - 1) continue;
+                            long long vkey = ((long long)itx[a] * g.nt + ity[b]) * g.ntz + itz[c];
+                            if (bucket[vkey].empty()) continue;
+                            int h = halo_index<F, AXES>(bx, by, bz, iex[a], iey[b], iez[c]);
+                            assert(h >= 0 && h < (halo_count<F, AXES>(iex[a], iey[b], iez[c])));
+                            double pv = part[vkey * g.pslot + h];
+                            assert(pv != -1e300);
+                            v += pv;
+                        }
+                    }
+                }
+                long long node = AXES == 3
+                    ? ((long long)(tb.ox + lx) * dims + (tb.oy + ly)) * dims + (tb.oz + lz)
+                    : (long long)(tb.ox + lx) * dims + (tb.oy + ly);
+                grid[node] = v;
+                writes[node]++;
+            }
+            // uniform-tile finalize (tile_finalize_uniform_kernel): same cells, same sums
+            if (AXES == 3 && g.nt >= 2 && dims % kT3 == 0) {
+                constexpr int L = F - 1;
+                const int NA = L * T * T, NBb = (T - L) * L * T, NC = (T - L) * (T - L) * L;
+                assert(NA + NBb + NC == sa + sb + sc);
+                const int tz = (int)key % g.nt, ty = ((int)key / g.nt) % g.nt, tx = (int)key / (g.nt * g.nt);
+                assert(tx == tb.tx && ty == tb.ty && tz == tb.tz && tb.ox == tx * T && tb.oz == tz * T);
+                long long pbase[8];
+                for (int i = 0; i < 8; ++i) {
+                    const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+                    const int ix = a ? (tx == 0 ? g.nt - 1 : tx - 1) : tx;
+                    const int iy = b ? (ty == 0 ? g.nt - 1 : ty - 1) : ty;
+                    const int iz = c ? (tz == 0 ? g.nt - 1 : tz - 1) : tz;
+                    const long long vkey = ((long long)ix * g.nt + iy) * g.nt + iz;
+                    pbase[i] = bucket[vkey].empty() ? -1 : vkey * g.pslot;
+                }
+                for (int t = 0; t < NA + NBb + NC; ++t) {
+                    int lx, ly, lz;
+                    if (t < NA) { lz = t % T; ly = (t / T) % T; lx = t / (T * T); }
+                    else if (t < NA + NBb) { int u = t - NA; lz = u % T; ly = (u / T) % L; lx = L + u / (T * L); }
+                    else { int u = t - NA - NBb; lz = u % L; ly = L + (u / L) % (T - L); lx = L + u / (L * (T - L)); }
+                    double v = 0.0;
+                    for (int i = 0; i < 8; ++i) {
+                        const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+                        const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
+                        if (!(on && pbase[i] >= 0)) continue;
+                        const int h = halo_index<F, 3>(lx + a * T, ly + b * T, lz + c * T, T, T, T);
+                        assert(h >= 0 && h < g.pslot);
+                        const double pv = part[pbase[i] + h];
+                        assert(pv != -1e300);
+                        v += pv;
+                    }
+                    const long long node = ((long long)(tb.ox + lx) * dims + (tb.oy + ly)) * dims + (tb.oz + lz);
+                    assert(node >= 0 && node < ncell);
+                    if (v != grid[node]) { printf("  uniform finalize differs at node %lld\n", node); assert(0); }
+                    uni_checked++;
+                }
+            }
+        }
+    }
+    int bad = 0;
+    double tot = 0;
+    for (long long c = 0; c < ncell; ++c) tot += std::fabs(ref[c]);
+    for (long long c = 0; c < ncell; ++c) {
+        if (writes[c] != 1) { if (bad < 3) printf("  node %lld written %d times\n", c, writes[c]); ++bad; }
+        if (std::fabs(grid[c] - ref[c]) > 1e-9 * (std::fabs(ref[c]) + tot / ncell)) {
+            if (bad < 3) printf("  node %lld got %.17g ref %.17g\n", c, grid[c], ref[c]);
+            ++bad;
+        }
+    }
+    printf("MAS %d axes %d dims %3d n %6d pattern %d w %d: %s (nt %d pslot %d)\n", MAS, AXES, dims, n,
+           pattern, (int)weighted, bad ? "FAIL" : "ok", g.nt, g.pslot);
+    return bad;
+}
+
+template <int MAS>
+static int all_dims() {
+    int bad = 0;
+    const int d3[] = {1, 2, 3, 4, 5, 7, 8, 15, 16, 17, 20, 24, 31, 32, 33, 40, 48, 50, 64};
+    const int d2[] = {1, 2, 3, 5, 16, 63, 64, 65, 100, 128, 129, 200};
+    unsigned seed = 1;
+    for (int d : d3)
+        for (int pat = 0; pat < 3; ++pat)
+            bad += run_case<MAS, 3>(d, 3000, pat, pat == 2, seed++);
+    for (int d : d2)
+        for (int pat = 0; pat < 3; ++pat)
+            bad += run_case<MAS, 2>(d, 3000, pat, pat == 1, seed++);
+    return bad;
+}
+
+int main() {
+    int bad = all_dims<PYL_NGP>() + all_dims<PYL_CIC>() + all_dims<PYL_TSC>() + all_dims<PYL_PCS>();
+    printf("uniform finalize cells checked: %lld\n", uni_checked);
+    printf(bad ? "FAILURES: %d\n" : "all ok\n", bad);
+    return bad != 0;
+}
