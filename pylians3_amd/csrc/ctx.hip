@@ -91,6 +91,44 @@ int stage_end(Ctx *c, int stage) {
 int finish(Ctx *c, int flags) {
     if ((flags & PYL_PTR_DEVICE) && (flags & PYL_NO_SYNC)) return PYL_OK;
     PYL_HIP(hipStreamSynchronize(c->stream));
+    return check_total(c);
+}
+
+int pinned(Ctx *c, PinnedBuf &b, size_t bytes, void **out) {
+    if (b.pending) {
+        PYL_HIP(hipEventSynchronize(b.done));
+        b.pending = false;
+    }
+    if (bytes > b.cap) {
+        if (b.ptr) PYL_HIP(hipHostFree(b.ptr));
+        b.ptr = nullptr;
+        b.cap = 0;
+        const size_t need = bytes + (bytes >> 3);
+        hipError_t e = hipHostMalloc(&b.ptr, need, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            b.ptr = nullptr;
+            set_error("hipHostMalloc(%zu) failed: %s", need, hipGetErrorString(e));
+            return PYL_ENOMEM;
+        }
+        b.cap = need;
+    }
+    *out = b.ptr;
+    return PYL_OK;
+}
+
+int pinned_done(Ctx *c, PinnedBuf &b) {
+    PYL_HIP(hipEventRecord(b.done, c->stream));
+    b.pending = true;
+    return PYL_OK;
+}
+
+int check_total(Ctx *c) {
+    if (c->total_expect < 0) return PYL_OK;
+    const long long expect = c->total_expect;
+    c->total_expect = -1;
+    PYL_HIP(hipEventSynchronize(c->ev_total));
+    PYL_CHECK(*c->h_total == (unsigned long long)expect, PYL_EHIP,
+              "tile sort lost particles (%llu of %lld)", *c->h_total, expect);
     return PYL_OK;
 }
 
@@ -202,6 +240,10 @@ int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
             PYL_HIP(hipEventCreate(&c->ev[s][k][1]));
         }
     c->ev_ready = true;
+    PYL_HIP(hipEventCreateWithFlags(&c->pin_up.done, hipEventDisableTiming));
+    PYL_HIP(hipEventCreateWithFlags(&c->pin_down.done, hipEventDisableTiming));
+    PYL_HIP(hipEventCreateWithFlags(&c->ev_total, hipEventDisableTiming));
+    PYL_HIP(hipHostMalloc((void **)&c->h_total, sizeof(unsigned long long), hipHostMallocDefault));
     *out = h.release();
     return PYL_OK;
 }
@@ -226,6 +268,14 @@ int pyl_ctx_trim(pyl_ctx *ctx) {
     c->bin_plans.clear();
     for (auto &kv : c->twiddles) (void)hipFree(kv.second);
     c->twiddles.clear();
+    for (auto &kv : c->mas_tabs) (void)hipFree(kv.second);
+    c->mas_tabs.clear();
+    for (PinnedBuf *b : {&c->pin_up, &c->pin_down}) {
+        if (b->ptr) (void)hipHostFree(b->ptr);
+        b->ptr = nullptr;
+        b->cap = 0;
+        b->pending = false;
+    }
     return PYL_OK;
 }
 
@@ -239,6 +289,9 @@ int pyl_ctx_destroy(pyl_ctx *ctx) {
                 if (c->ev[i][k][0]) (void)hipEventDestroy(c->ev[i][k][0]);
                 if (c->ev[i][k][1]) (void)hipEventDestroy(c->ev[i][k][1]);
             }
+    for (hipEvent_t e : {c->pin_up.done, c->pin_down.done, c->ev_total})
+        if (e) (void)hipEventDestroy(e);
+    if (c->h_total) (void)hipHostFree(c->h_total);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete ctx;
     return s;
@@ -254,7 +307,7 @@ int pyl_ctx_synchronize(pyl_ctx *ctx) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
     PYL_HIP(hipSetDevice(ctx->c.device));
     PYL_HIP(hipStreamSynchronize(ctx->c.stream));
-    return PYL_OK;
+    return check_total(&ctx->c);
 }
 
 int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out) {

@@ -18,12 +18,12 @@
 //    the halo go to a partial-sum buffer, and a finalize pass adds, for every
 //    band cell, the tile's own value and the halo values of its lower
 //    neighbours in a fixed order.
-//    Inside a box, contributions are rounded to integer multiples of a
+//    Inside a box, contributions are truncated to integer multiples of a
 //    power-of-two quantum 2^q (2^-30 of the realisation's largest |W|) and
 //    summed with 64-bit integer LDS atomics: exact, so the box does not
 //    depend on the order of the atomics.  With the fixed-order finalize the
 //    grid is bitwise reproducible, and identical whether a realisation is
-//    deposited alone or in a batch.  The rounding is below 2^-31 of the
+//    deposited alone or in a batch.  The truncation is below 2^-30 of the
 //    weight scale per contribution, well under float32 resolution of a cell.
 #include "pyl_internal.h"
 
@@ -368,7 +368,8 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
                     TileGeom g, const Chunk *__restrict__ chunks,
                     const unsigned *__restrict__ dump, const unsigned long long *__restrict__ offsets,
                     const unsigned *__restrict__ counts, unsigned *__restrict__ ocursors,
-                    unsigned *__restrict__ wmax_bits, float *__restrict__ records) {
+                    unsigned *__restrict__ wmax_bits, float *__restrict__ records,
+                    unsigned long long rcap) {
     __shared__ unsigned cur[kWin];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
@@ -405,6 +406,7 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
         for (int u = 0; u < kUnroll; ++u) {
             if (base + (long long)u * blockDim.x < ch.c1) {
                 const unsigned long long at = offsets[key[u]] + slot[u];
+                if (at >= rcap) continue;               // only after a miscount (reported)
                 if (WT) {
                     reinterpret_cast<float4 *>(records)[at] = make_float4(d[u].x, d[u].y, d[u].z, w[u]);
                 } else {
@@ -581,9 +583,11 @@ __device__ __forceinline__ int box_base(const float4 &rec, const TileGeom &g, co
 // Adds one record's footprint to the LDS box at compile-time offsets from `a`.
 // ((Cx*Cy)*Cz)*(W*2^-q) == 2^-q * (((Cx*Cy)*Cz)*W) exactly (power-of-two
 // scaling commutes with float rounding), so the scaled product is the
-// reference's float contribution in units of the quantum.  Without weights
-// (Cx*Cy)*(Cz*2^-q) is the same number: it can differ only where (Cx*Cy)*Cz is
-// subnormal, far below half a quantum, and both round to 0.
+// reference's float contribution in units of the quantum; it is truncated
+// toward zero to an integer (|error| < 1 quantum = 2^-30 of the weight bound,
+// one conversion instruction).  Without weights (Cx*Cy)*(Cz*2^-q) is the same
+// number: it can differ only where (Cx*Cy)*Cz is subnormal, far below one
+// quantum, and both truncate to 0.
 template <int MAS, int AXES, bool WT>
 __device__ __forceinline__ void deposit_record(unsigned long long *a, const float4 &rec,
                                                float to_q, float weight_factor) {
@@ -606,12 +610,11 @@ __device__ __forceinline__ void deposit_record(unsigned long long *a, const floa
 #pragma unroll
                     for (int k = 0; k < F; ++k) {
                         const float c = (cxy * Cz[k]) * w;
-                        atomicAdd(a + (l * BY + m) * BZ + k,
-                                  (unsigned long long)(long long)__float2int_rn(c));
+                        atomicAdd(a + (l * BY + m) * BZ + k, (unsigned long long)(long long)(int)c);
                     }
                 } else {
                     const float c = cxy * w;   // the reference multiplies by C[2][n] = 1.0
-                    atomicAdd(a + l * BY + m, (unsigned long long)(long long)__float2int_rn(c));
+                    atomicAdd(a + l * BY + m, (unsigned long long)(long long)(int)c);
                 }
             }
         }
@@ -628,13 +631,12 @@ __device__ __forceinline__ void deposit_record(unsigned long long *a, const floa
 #pragma unroll
                     for (int k = 0; k < F; ++k)
                         atomicAdd(a + (l * BY + m) * BZ + k,
-                                  (unsigned long long)(unsigned)__float2int_rn(cxy * Cs[k]));
+                                  (unsigned long long)(unsigned)(cxy * Cs[k]));
                 }
             } else {
 #pragma unroll
                 for (int m = 0; m < F; ++m)
-                    atomicAdd(a + l * BY + m,
-                              (unsigned long long)(unsigned)__float2int_rn(Cx[l] * Cs[m]));
+                    atomicAdd(a + l * BY + m, (unsigned long long)(unsigned)(Cx[l] * Cs[m]));
             }
         }
     }
@@ -662,7 +664,7 @@ constexpr int dep_kr() { return (MAS == 4 && AXES == 3) ? 2 : 4; }
 // order of the atomics does not change any bit of the result.
 template <int MAS, int AXES, bool WT, bool RW>
 __global__ void __launch_bounds__(kDepBlock)
-tile_deposit_kernel(const float *__restrict__ records,
+tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                     const unsigned long long *__restrict__ offsets, TileGeom g,
                     float weight_factor, const unsigned *__restrict__ wmax_bits,
                     float *__restrict__ grid, float *__restrict__ partials, OutMode om) {
@@ -683,7 +685,8 @@ tile_deposit_kernel(const float *__restrict__ records,
     const int real = key / (int)g.ntiles;
     const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
     grid += real * g.ncell;
-    const unsigned long long r0 = offsets[key], r1 = offsets[key + 1];
+    // record range, clamped to the buffer (a miscount is reported by the host)
+    const unsigned long long r1 = min(offsets[key + 1], rcap), r0 = min(offsets[key], r1);
     const int ex = tb.ex, ey = tb.ey, ez = tb.ez;
 
     if (r0 == r1) {
@@ -1155,11 +1158,17 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     float *partials = (float *)c->bufs[BUF_PARTIAL].ptr;
 
     PYL_HIP(hipMemsetAsync(counts, 0, nwords * sizeof(unsigned), c->stream));
-    PYL_HIP(hipMemcpyAsync(dorig, orig.data(), orig.size() * sizeof(int), hipMemcpyHostToDevice,
-                           c->stream));
+    // chunk table and tile origins go up through pinned staging (asynchronous DMA)
+    void *stage = nullptr;
+    const size_t ch_up = (size_t)nchunk * sizeof(Chunk), org_up = orig.size() * sizeof(int);
+    PYL_TRY(pinned(c, c->pin_up, ch_up + org_up, &stage));
+    if (ch_up) std::memcpy(stage, chunks.data(), ch_up);
+    std::memcpy((char *)stage + ch_up, orig.data(), org_up);
+    PYL_HIP(hipMemcpyAsync(dorig, (char *)stage + ch_up, org_up, hipMemcpyHostToDevice, c->stream));
+    if (nchunk > 0)
+        PYL_HIP(hipMemcpyAsync(dchunks, stage, ch_up, hipMemcpyHostToDevice, c->stream));
+    PYL_TRY(pinned_done(c, c->pin_up));
     if (nchunk > 0) {
-        PYL_HIP(hipMemcpyAsync(dchunks, chunks.data(), (size_t)nchunk * sizeof(Chunk),
-                               hipMemcpyHostToDevice, c->stream));
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
         auto kc = (axes == 3) ? (pow2 ? tile_count_kernel<MAS, 3, true> : tile_count_kernel<MAS, 3, false>)
                               : (pow2 ? tile_count_kernel<MAS, 2, true> : tile_count_kernel<MAS, 2, false>);
@@ -1174,14 +1183,19 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                                                                    offsets);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
-    unsigned long long total = 0;
-    PYL_HIP(hipMemcpyAsync(&total, offsets + nt, sizeof(total), hipMemcpyDeviceToHost, c->stream));
-    PYL_HIP(hipStreamSynchronize(c->stream));   // also keeps `chunks`, `orig` alive until uploaded
-    PYL_CHECK(total == (unsigned long long)n, PYL_EHIP, "tile sort lost particles (%llu of %lld)",
-              total, n);
+    // offsets[nt] must equal n; it is read back asynchronously and checked at the
+    // next synchronising call on this context.  Until then the scatter and the
+    // deposit keep every record index below n, so a bad count cannot write or
+    // read outside the record buffer.
+    PYL_TRY(check_total(c));                    // the previous call's check, if still pending
+    PYL_HIP(hipMemcpyAsync(c->h_total, offsets + nt, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
+    PYL_HIP(hipEventRecord(c->ev_total, c->stream));
+    c->total_expect = n;
     const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
-    PYL_TRY(ensure(c, BUF_RECORDS, (size_t)(total ? total : 1) * rec_bytes));
+    PYL_TRY(ensure(c, BUF_RECORDS, (size_t)(n ? n : 1) * rec_bytes));
     float *records = (float *)c->bufs[BUF_RECORDS].ptr;
+    const unsigned long long rcap = (unsigned long long)n;
     if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
         auto ks = W ? ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true>
@@ -1193,7 +1207,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                                    : (pow2 ? tile_scatter_kernel<MAS, 2, true, false>
                                            : tile_scatter_kernel<MAS, 2, false, false>));
         ks<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, W, inv, g, dchunks, dump, offsets, counts,
-                                                 ocursors, wmax, records);
+                                                 ocursors, wmax, records, rcap);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
     }
@@ -1204,7 +1218,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                 : (W ? tile_deposit_kernel<MAS, 3, true, true> : tile_deposit_kernel<MAS, 3, true, false>))
         : (unit ? tile_deposit_kernel<MAS, 2, false, false>
                 : (W ? tile_deposit_kernel<MAS, 2, true, true> : tile_deposit_kernel<MAS, 2, true, false>));
-    kd<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(records, offsets, g, weight_factor,
+    kd<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(records, rcap, offsets, g, weight_factor,
                                                  W ? wmax : nullptr, grid, partials, om);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));

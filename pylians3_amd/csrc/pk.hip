@@ -654,16 +654,17 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     }
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_PK_BIN));
-    std::vector<double> h3(out3 * nreal), h2(out2 * nreal), h1(out1 * nreal);
-    PYL_HIP(hipMemcpyAsync(h3.data(), d3, sizeof(double) * h3.size(), hipMemcpyDeviceToHost, c->stream));
-    PYL_HIP(hipMemcpyAsync(h2.data(), d2, sizeof(double) * h2.size(), hipMemcpyDeviceToHost, c->stream));
-    PYL_HIP(hipMemcpyAsync(h1.data(), d1, sizeof(double) * h1.size(), hipMemcpyDeviceToHost, c->stream));
+    // d3 | d2 | d1 are contiguous: one DMA into pinned memory
+    const size_t nout = (out3 + out2 + out1) * nreal;
+    void *hp = nullptr;
+    PYL_TRY(pinned(c, c->pin_down, sizeof(double) * nout, &hp));
+    PYL_HIP(hipMemcpyAsync(hp, d3, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream));
     PYL_HIP(hipStreamSynchronize(c->stream));
+    const double *h3 = (const double *)hp, *h2 = h3 + out3 * nreal, *h1 = h2 + out2 * nreal;
     // scatter into the caller's spectrum columns
     for (int rr = 0; rr < nreal; ++rr) {
         HostAcc &acc = accs[rr];
-        const double *H3 = h3.data() + out3 * rr, *H2 = h2.data() + out2 * rr,
-                     *H1 = h1.data() + out1 * rr;
+        const double *H3 = h3 + out3 * rr, *H2 = h2 + out2 * rr, *H1 = h1 + out1 * rr;
         const int ST = acc.S_total;
         for (int b = 0; b <= p.kmax; ++b) {
             for (int sp = 0; sp < S; ++sp) {
@@ -764,19 +765,35 @@ int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch) {
 
 // Binning of F device half-complex fields of nreal realisations -> finalized outputs.
 // Field f of realisation r is dk[f] + r*fstride.
+// Device-resident MAS correction table for (dims, p), built once with the host's libm.
+int get_mas_tab(Ctx *c, int dims, int p, const double **out) {
+    auto key = std::make_pair(dims, p);
+    auto it = c->mas_tabs.find(key);
+    if (it == c->mas_tabs.end()) {
+        std::vector<double> tab(dims);
+        mas_table(dims, p, tab.data());
+        void *d = nullptr;
+        PYL_HIP(hipMalloc(&d, sizeof(double) * dims));
+        hipError_t e = hipMemcpy(d, tab.data(), sizeof(double) * dims, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            set_error("MAS table upload failed: %s", hipGetErrorString(e));
+            return PYL_EHIP;
+        }
+        it = c->mas_tabs.emplace(key, d).first;
+    }
+    *out = (const double *)it->second;
+    return PYL_OK;
+}
+
 int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal, int F,
                   const int *mas, int dims, double BoxSize, int axis, int flags, pyl_pk_out *outs) {
     const PkDims p = pk_dims(dims);
     const ModePlan *mp = get_mode_plan(c, dims, axis);
     BinPlan *bp = nullptr;
     PYL_TRY(get_bin_plan(c, dims, axis, nreal, &bp));
-    // MAS tables (one per field, host libm, uploaded)
-    std::vector<double> tabs((size_t)F * dims);
-    for (int f = 0; f < F; ++f) mas_table(dims, mas[f], &tabs[(size_t)f * dims]);
-    PYL_TRY(ensure(c, BUF_MAS_TAB, tabs.size() * sizeof(double)));
-    double *dtab = (double *)c->bufs[BUF_MAS_TAB].ptr;
-    PYL_HIP(hipMemcpyAsync(dtab, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice,
-                           c->stream));
+    std::vector<const double *> tab(F);
+    for (int f = 0; f < F; ++f) PYL_TRY(get_mas_tab(c, dims, mas[f], &tab[f]));
     const int S = F + F * (F - 1) / 2;
     std::vector<HostAcc> accs(nreal);
     for (auto &acc : accs) {
@@ -789,7 +806,7 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
     const bool phase = (F == 1) && !(flags & PYL_PK_NO_PHASE);
     if (F == 1) {
         const float2 *f0[1] = {dk[0]};
-        const double *c0[1] = {dtab};
+        const double *c0[1] = {tab[0]};
         int so[1] = {0};
         if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, nreal, c0, accs, so)));
         else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, nreal, c0, accs, so)));
@@ -800,7 +817,7 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
         for (int i = 0; i < F; ++i)
             for (int j = i + 1; j < F; ++j, ++xi) {
                 const float2 *fp[2] = {dk[i], dk[j]};
-                const double *cp[2] = {dtab + (size_t)i * dims, dtab + (size_t)j * dims};
+                const double *cp[2] = {tab[i], tab[j]};
                 int so[3] = {have[i] ? -1 : i, have[j] ? -1 : j, F + xi};
                 have[i] = have[j] = true;
                 PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, nreal, cp, accs, so)));

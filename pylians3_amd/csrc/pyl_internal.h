@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <cmath>
 #include <map>
 #include <memory>
@@ -50,6 +51,17 @@ void set_error(const char *fmt, ...);
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
+};
+
+// Pinned host staging (hipHostMalloc): small per-call uploads and result
+// downloads go through it, so the copies are plain DMA that do not block the
+// host.  `done` is recorded after the copies that read or fill it; the next
+// user waits for it before touching the memory.
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
 };
 
 // ---- resident plans --------------------------------------------------------
@@ -104,6 +116,14 @@ struct Ctx {
     std::map<std::pair<int, int>, ModePlan> mode_plans;    // (dims, axis)
     std::map<std::tuple<int, int, int>, BinPlan> bin_plans; // (dims, axis, rows per item)
     std::map<int, void *> twiddles;                          // exp(-2 pi i k/N), per N
+    std::map<std::pair<int, int>, void *> mas_tabs;          // MAS correction table per (dims, p)
+    PinnedBuf pin_up;                                        // MA chunk table + tile origins
+    PinnedBuf pin_down;                                      // Pk bin sums
+    // Tile-sort consistency check of the latest tiled MA, read back without a
+    // synchronisation of its own: checked at the next synchronising call.
+    unsigned long long *h_total = nullptr;                   // pinned
+    long long total_expect = -1;
+    hipEvent_t ev_total = nullptr;
     // Stage timing: a ring of event pairs per stage, so that timing many calls
     // needs no host synchronisation between them.  A pair is read back when its
     // ring slot comes round again (kEvRing calls later) or on request.
@@ -124,6 +144,10 @@ void reset_stages(Ctx *c);
 int stage_begin(Ctx *c, int stage);
 int stage_end(Ctx *c, int stage);
 int finish(Ctx *c, int flags);   // synchronise unless PYL_NO_SYNC on device pointers
+// Waits for the previous user of `b`, grows it to `bytes` and returns it.
+int pinned(Ctx *c, PinnedBuf &b, size_t bytes, void **out);
+int pinned_done(Ctx *c, PinnedBuf &b);   // records b.done after the copies just enqueued
+int check_total(Ctx *c);                 // reports a tile sort that lost particles
 
 // FFT plan (resident, one per (dims, batch))
 // Transform kinds of the resident rocFFT plans (all single precision, out of place).
