@@ -92,7 +92,10 @@ int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out);
 #define PYL_STAGE_PK_BIN     7
 #define PYL_STAGE_MA_FINAL   8   /* halo finalize of the tiled MA */
 #define PYL_STAGE_COUNT      9
-int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable);   /* also clears the totals */
+int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable);   /* all stages; also clears the totals */
+/* Times only the stages whose bit (1u << PYL_STAGE_*) is set: each timed stage adds
+ * two HIP events to the stream, so a caller timing one kernel records only its own. */
+int pyl_ctx_enable_stage_timing(pyl_ctx *ctx, unsigned stage_mask);
 /* Stage times of the latest call (ms; 0 for stages it did not run). */
 int pyl_ctx_stage_times(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */);
 /* Summed stage times and call counts since timing was enabled; recording them

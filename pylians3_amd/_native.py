@@ -65,6 +65,7 @@ EXPORTS = {
     'pyl_ctx_trim': (ctypes.c_int, [ctypes.c_void_p]),
     'pyl_ctx_memory': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
     'pyl_ctx_enable_timing': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    'pyl_ctx_enable_stage_timing': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     'pyl_ctx_stage_times': (ctypes.c_int, [ctypes.c_void_p, _dp]),
     'pyl_ctx_stage_totals': (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     'pyl_ma': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
@@ -196,8 +197,15 @@ class Context:
         check(lib().pyl_ctx_memory(self.handle, ctypes.byref(v)))
         return v.value
 
-    def enable_timing(self, on=True):
-        check(lib().pyl_ctx_enable_timing(self.handle, 1 if on else 0))
+    def enable_timing(self, on=True, stages=None):
+        """Time every stage (stages=None) or only the named ones; clears the totals."""
+        if stages is None:
+            check(lib().pyl_ctx_enable_timing(self.handle, 1 if on else 0))
+        else:
+            mask = 0
+            for s in stages:
+                mask |= 1 << STAGES.index(s)
+            check(lib().pyl_ctx_enable_stage_timing(self.handle, mask if on else 0))
 
     def stage_times(self):
         """Stage times (ms) of the latest call."""

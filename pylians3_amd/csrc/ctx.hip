@@ -72,7 +72,7 @@ static int harvest(Ctx *c, int stage, int slot) {
 }
 
 int stage_begin(Ctx *c, int stage) {
-    if (!c->timing || c->ev_used[stage]) return PYL_OK;
+    if (!c->timing || !((c->timing_mask >> stage) & 1u) || c->ev_used[stage]) return PYL_OK;
     const int slot = (c->ev_slot[stage] + 1) % Ctx::kEvRing;
     PYL_TRY(harvest(c, stage, slot));             // recorded kEvRing calls ago
     PYL_HIP(hipEventRecord(c->ev[stage][slot][0], c->stream));
@@ -317,10 +317,15 @@ int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out) {
 }
 
 int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable) {
+    return pyl_ctx_enable_stage_timing(ctx, enable ? ~0u : 0u);
+}
+
+int pyl_ctx_enable_stage_timing(pyl_ctx *ctx, unsigned stage_mask) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
     Ctx *c = &ctx->c;
     PYL_HIP(hipStreamSynchronize(c->stream));   // no recorded pair is still in flight
-    c->timing = enable != 0;
+    c->timing = stage_mask != 0;
+    c->timing_mask = stage_mask;
     for (int s = 0; s < PYL_STAGE_COUNT; ++s) {
         c->ev_used[s] = false;
         c->ev_total_ms[s] = 0.0;

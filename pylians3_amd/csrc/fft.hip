@@ -242,7 +242,7 @@ int cols(Ctx *c, int n, float2 *data, long long nstride, long long ostride, long
 template <int N>
 int launch_rows(Ctx *c, const float *in, float2 *out, long long nrows, const float2 *twh,
                 const float2 *twf) {
-    constexpr int R = (N >= 1024) ? 8 : (N >= 512 ? 16 : 32);
+    constexpr int R = (N >= 512) ? 8 : 32;
     const long long nb = (nrows + R - 1) / R;
     fft_r2c_rows_kernel<N, R><<<(unsigned)nb, kFftBlock, 0, c->stream>>>(in, out, nrows, twh, twf);
     PYL_HIP(hipGetLastError());

@@ -647,6 +647,11 @@ __device__ __forceinline__ void deposit_record(unsigned long long *a, const floa
 template <int MAS, int AXES>
 constexpr int dep_kr() { return (MAS == 4 && AXES == 3) ? 2 : 4; }
 
+// Dealing stages record indices (2 B; the records are reloaded, from cache)
+// instead of the records themselves (16 B): the TSC 3D workgroup then needs
+// 51.8 KB of LDS and 3 fit one CU instead of 2 (1.56 -> 1.46 ms at 512^3).
+constexpr bool kDepStageIdx = true;
+
 // One workgroup per tile.  LDS box of BX*BY*BZ 64-bit integers (the tile plus
 // the upper halo) holding multiples of 2^q; every record adds its whole
 // footprint at fixed offsets from its first node.  WT = false: unit weights
@@ -662,7 +667,7 @@ constexpr int dep_kr() { return (MAS == 4 && AXES == 3) ? 2 : 4; }
 // offsets this halves the cost of a ds_add_u64 (12.6 -> 6.2 clk per wave in
 // a microbenchmark of the two extremes).  The sums are integers, so the
 // order of the atomics does not change any bit of the result.
-template <int MAS, int AXES, bool WT, bool RW>
+template <int MAS, int AXES, bool WT, bool RW, bool UNI>
 __global__ void __launch_bounds__(kDepBlock)
 tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                     const unsigned long long *__restrict__ offsets, TileGeom g,
@@ -679,7 +684,9 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     constexpr int NW = kDepBlock / 64;                 // waves
     constexpr int WB = KR * 64;                        // records per wave round set
     __shared__ unsigned long long acc[NB];
-    __shared__ float4 stage[BAL ? NW * WB : 1];
+    constexpr bool SIDX = BAL && kDepStageIdx;
+    __shared__ float4 stage[(BAL && !SIDX) ? NW * WB : 1];
+    __shared__ unsigned short sidx[SIDX ? NW * WB : 1];
     __shared__ unsigned whist[BAL ? NW * 32 : 1];
     const int key = blockIdx.x;                       // r * ntiles + tile
     const int real = key / (int)g.ntiles;
@@ -687,7 +694,8 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     grid += real * g.ncell;
     // record range, clamped to the buffer (a miscount is reported by the host)
     const unsigned long long r1 = min(offsets[key + 1], rcap), r0 = min(offsets[key], r1);
-    const int ex = tb.ex, ey = tb.ey, ez = tb.ez;
+    // UNI: every tile has extent T (dims % T == 0, two tiles or more per axis)
+    const int ex = UNI ? T : tb.ex, ey = UNI ? T : tb.ey, ez = UNI ? (AXES == 3 ? T : 1) : tb.ez;
 
     if (r0 == r1) {
         // Empty tile: its interior is zero; the finalize reads no halo from it.
@@ -703,6 +711,14 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
             }
         }
         return;
+    }
+    // the first round set of every wave is loaded before the box is cleared
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    float4 nxt[KR];
+    if constexpr (BAL) {
+        const unsigned long long b0 = r0 + (unsigned long long)wv * WB;
+#pragma unroll
+        for (int k = 0; k < KR; ++k) nxt[k] = load_record<RW>(records, min(b0 + ln + 64 * k, r1 - 1));
     }
     for (int c = threadIdx.x; c < NB; c += blockDim.x) acc[c] = 0ull;
 
@@ -733,15 +749,11 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                                           weight_factor);
         }
     } else {
-        const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
         float4 *st = stage + wv * WB;
         unsigned *wh = whist + wv * 32;
         // wave wv takes the round sets starting at r0 + (wv + NW*j)*WB; the next
         // set's records are loaded (clamped into the tile) while this one runs
         unsigned long long b0 = r0 + (unsigned long long)wv * WB;
-        float4 nxt[KR];
-#pragma unroll
-        for (int k = 0; k < KR; ++k) nxt[k] = load_record<RW>(records, min(b0 + ln + 64 * k, r1 - 1));
         for (; b0 < r1; b0 += (unsigned long long)NW * WB) {
             float4 rec[KR];
 #pragma unroll
@@ -765,14 +777,16 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                 rank[k] = (ln + 64 * k < nb) ? atomicAdd(&wh[cls[k]], 1u) : 0u;
             }
             __builtin_amdgcn_wave_barrier();
-            // exclusive prefix of the 32 class counts
+            // exclusive prefix of the 32 class counts: inclusive scan of lanes 0..31
+            // with DPP row shifts (1, 2, 4, 8 within rows of 16 lanes) and
+            // row_bcast:15 (lane 15 added to row 1); lanes 32..63 hold 0
             const unsigned h = (ln < 32) ? wh[ln] : 0u;
             unsigned incl = h;
-#pragma unroll
-            for (int o = 1; o < 32; o <<= 1) {
-                const unsigned t = __shfl_up(incl, o);
-                if (ln >= o) incl += t;
-            }
+            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x111, 0xf, 0xf, false);
+            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x112, 0xf, 0xf, false);
+            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x114, 0xf, 0xf, false);
+            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x118, 0xf, 0xf, false);
+            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x142, 0xa, 0xf, false);
             __builtin_amdgcn_wave_barrier();
             if (ln < 32) wh[ln] = incl - h;
             __builtin_amdgcn_wave_barrier();
@@ -782,8 +796,11 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                 if (ln + 64 * k < nb) {
                     const unsigned p = wh[cls[k]] + rank[k];
                     const unsigned slot = (p % KR) * 64 + ((p / KR) & 1u) * 32 + p / (2 * KR);
-                    st[slot] = make_float4(rec[k].x, rec[k].y, rec[k].z,
-                                           WT ? rec[k].w : __int_as_float(base[k]));
+                    if constexpr (SIDX)
+                        sidx[wv * WB + slot] = (unsigned short)(ln + 64 * k);
+                    else
+                        st[slot] = make_float4(rec[k].x, rec[k].y, rec[k].z,
+                                               WT ? rec[k].w : __int_as_float(base[k]));
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -791,9 +808,15 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
             for (int r = 0; r < KR; ++r) {
                 const int p = (ln & 31) * (2 * KR) + (ln >> 5) * KR + r;
                 if (p < nb) {
-                    const float4 rr = st[r * 64 + ln];
-                    const int bb = WT ? box_base<MAS, AXES>(rr, g, tb) : __float_as_int(rr.w);
-                    deposit_record<MAS, AXES, WT>(acc + bb, rr, to_q, weight_factor);
+                    if constexpr (SIDX) {
+                        const float4 rr = load_record<RW>(records, b0 + sidx[wv * WB + r * 64 + ln]);
+                        deposit_record<MAS, AXES, WT>(acc + box_base<MAS, AXES>(rr, g, tb), rr, to_q,
+                                                      weight_factor);
+                    } else {
+                        const float4 rr = st[r * 64 + ln];
+                        const int bb = WT ? box_base<MAS, AXES>(rr, g, tb) : __float_as_int(rr.w);
+                        deposit_record<MAS, AXES, WT>(acc + bb, rr, to_q, weight_factor);
+                    }
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -803,7 +826,8 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     float *part = partials + (long long)key * g.pslot;
     for (int c = threadIdx.x; c < NB; c += blockDim.x) {
         const int bz = c % BZ, by = (c / BZ) % BY, bx = c / (BZ * BY);
-        if (bx >= ex + F - 1 || by >= ey + F - 1 || (AXES == 3 && bz >= ez + F - 1)) continue;
+        if (!UNI && (bx >= ex + F - 1 || by >= ey + F - 1 || (AXES == 3 && bz >= ez + F - 1)))
+            continue;
         const float v = (float)((double)(long long)acc[c] * from_q);
         const bool inner = bx >= F - 1 && bx < ex && by >= F - 1 && by < ey &&
                            (AXES == 2 || (bz >= F - 1 && bz < ez));
@@ -1123,7 +1147,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
     // Chunks of consecutive particles, never straddling two realisations.
     const long long target = std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
-    std::vector<Chunk> chunks;
+    static thread_local std::vector<Chunk> chunks;   // reused: no fresh pages per call
+    chunks.clear();
     for (int r = 0; r < nreal; ++r) {
         const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
         for (long long s = b; s < e; s += target) chunks.push_back({s, std::min(e, s + target), r, 0});
@@ -1151,7 +1176,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     Chunk *dchunks = (Chunk *)(tb + cnt_bytes + off_bytes + bs_bytes);
     int *dorig = (int *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes);
     unsigned *dump = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes);
-    std::vector<int> orig(g.nt + 1);
+    static thread_local std::vector<int> orig;
+    orig.assign(g.nt + 1, 0);
     for (int i = 0; i <= g.nt; ++i) orig[i] = tile_origin(i, g.nt, dims);
     g.orig = dorig;
     PYL_TRY(ensure(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float)));
@@ -1213,11 +1239,20 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     }
     const bool unit = W == nullptr && weight_factor == 1.0f;
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_DEPOSIT));
+    const bool uni = g.nt >= 2 && dims % (axes == 3 ? kT3 : kT2) == 0;
     auto kd = (axes == 3)
-        ? (unit ? tile_deposit_kernel<MAS, 3, false, false>
-                : (W ? tile_deposit_kernel<MAS, 3, true, true> : tile_deposit_kernel<MAS, 3, true, false>))
-        : (unit ? tile_deposit_kernel<MAS, 2, false, false>
-                : (W ? tile_deposit_kernel<MAS, 2, true, true> : tile_deposit_kernel<MAS, 2, true, false>));
+        ? (uni ? (unit ? tile_deposit_kernel<MAS, 3, false, false, true>
+                       : (W ? tile_deposit_kernel<MAS, 3, true, true, true>
+                            : tile_deposit_kernel<MAS, 3, true, false, true>))
+               : (unit ? tile_deposit_kernel<MAS, 3, false, false, false>
+                       : (W ? tile_deposit_kernel<MAS, 3, true, true, false>
+                            : tile_deposit_kernel<MAS, 3, true, false, false>)))
+        : (uni ? (unit ? tile_deposit_kernel<MAS, 2, false, false, true>
+                       : (W ? tile_deposit_kernel<MAS, 2, true, true, true>
+                            : tile_deposit_kernel<MAS, 2, true, false, true>))
+               : (unit ? tile_deposit_kernel<MAS, 2, false, false, false>
+                       : (W ? tile_deposit_kernel<MAS, 2, true, true, false>
+                            : tile_deposit_kernel<MAS, 2, true, false, false>)));
     kd<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(records, rcap, offsets, g, weight_factor,
                                                  W ? wmax : nullptr, grid, partials, om);
     PYL_HIP(hipGetLastError());

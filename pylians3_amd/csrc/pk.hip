@@ -795,7 +795,9 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
     std::vector<const double *> tab(F);
     for (int f = 0; f < F; ++f) PYL_TRY(get_mas_tab(c, dims, mas[f], &tab[f]));
     const int S = F + F * (F - 1) / 2;
-    std::vector<HostAcc> accs(nreal);
+    // host accumulators reused across calls on this thread (no fresh pages per call)
+    static thread_local std::vector<HostAcc> accs;
+    accs.resize(nreal);
     for (auto &acc : accs) {
         acc.S_total = S;
         acc.P3.assign((size_t)(p.kmax + 1) * 3 * S, 0.0);
@@ -824,6 +826,10 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
             }
     }
     for (int r = 0; r < nreal; ++r) PYL_TRY(finalize(mp, p, BoxSize, accs[r], phase, &outs[r]));
+    if (nreal > 16) {                            // do not keep large batch accumulators
+        accs.clear();
+        accs.shrink_to_fit();
+    }
     return PYL_OK;
 }
 

@@ -129,6 +129,7 @@ struct Ctx {
     // ring slot comes round again (kEvRing calls later) or on request.
     static constexpr int kEvRing = 64;
     bool timing = false;
+    unsigned timing_mask = 0;                          // stages recorded when timing
     hipEvent_t ev[PYL_STAGE_COUNT][kEvRing][2] = {};
     int ev_slot[PYL_STAGE_COUNT] = {};                 // slot of the latest call
     bool ev_used[PYL_STAGE_COUNT] = {};                // stage ran in the latest call

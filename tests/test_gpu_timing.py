@@ -40,6 +40,14 @@ def test_stage_totals_count_every_call():
         assert last['pk_fft'] > 0.0 and last['ma_atomic'] == 0.0
         ctx.enable_timing(True)             # clears the totals
         assert all(c == 0 for _, c in ctx.stage_totals().values())
+        # a stage mask records only the named stages
+        ctx.enable_timing(True, stages=['ma_deposit'])
+        for _ in range(3):
+            N.check(N.lib().pyl_ma(ctx.handle, N.device_ptr(pos), n, 3, None, N.device_ptr(grid), d,
+                                   box, N.CIC, N.PTR_DEVICE | N.MA_OVERWRITE | N.NO_SYNC))
+        tot = ctx.stage_totals()
+        assert tot['ma_deposit'][1] == 3 and tot['ma_deposit'][0] > 0.0
+        assert all(c == 0 for st, (_, c) in tot.items() if st != 'ma_deposit')
         del arrs
     finally:
         ctx.close()
