@@ -283,8 +283,10 @@ def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
 def serving_leg(N, nreq=120, seed=5):
     """Config 5: mixed MA+Pk / MA+XPk requests (grid 64^3-512^3, N log-uniform
     1e5-1.3e8, MAS and axis drawn uniformly) against the continuous-batching server.
-    Plain form: one request at a time through the drop-in calls.  Served form:
-    Poisson arrivals at 0.9x the plain capacity, then a saturated burst."""
+    Plain form: one request at a time, one slot, arrival order, no fusion; its
+    service latency alone, then under Poisson arrivals at 0.9x its capacity.
+    Served form (4 slots, shortest job first with aging, fusion of small
+    requests): the same arrival schedule, then a saturated burst."""
     import numpy as np
     import torch
     from pylians3_amd import workloads
@@ -310,7 +312,7 @@ def serving_leg(N, nreq=120, seed=5):
             reqs.append(Request(pool[o:o + n], d, box, mas, axis, pos2=pool[o2:o2 + n2],
                                 W2=poolw[o2:o2 + n2]))
     # plain form: one request after the other, one slot, no fusion
-    with Server(slots=1, max_batch=1) as srv:
+    with Server(slots=1, max_batch=1, policy='fifo') as srv:
         srv.submit(reqs[0]).result()
         t0 = time.perf_counter()
         plain_lat = np.array([srv.submit(r).result().latency_s for r in reqs])
@@ -319,18 +321,25 @@ def serving_leg(N, nreq=120, seed=5):
     out = {'requests': nreq, 'plain_throughput_rps': round(cap, 2),
            'plain_p50_ms': round(1e3 * np.percentile(plain_lat, 50), 3),
            'plain_p99_ms': round(1e3 * np.percentile(plain_lat, 99), 3)}
-    with Server(slots=4, max_batch=32) as srv:
+    # Poisson arrivals at 0.9x the plain capacity, the same schedule for both forms
+    gaps = rng.exponential(1.0 / (0.9 * cap), nreq)
+
+    def loaded(srv):
         srv.submit(reqs[1]).result()
         srv.completed.clear()
-        # Poisson arrivals at 0.9x the plain capacity
-        gaps = rng.exponential(1.0 / (0.9 * cap), nreq)
         futs = []
         t0 = time.perf_counter()
         for r, gap in zip(reqs, gaps):
             time.sleep(gap)
             futs.append(srv.submit(r))
-        lat = np.array([f.result().latency_s for f in futs])
-        t_arr = time.perf_counter() - t0
+        return np.array([f.result().latency_s for f in futs]), time.perf_counter() - t0
+
+    with Server(slots=1, max_batch=1, policy='fifo') as srv:
+        lat_p, _ = loaded(srv)
+    out.update({'plain_loaded_p50_ms': round(1e3 * np.percentile(lat_p, 50), 3),
+                'plain_loaded_p99_ms': round(1e3 * np.percentile(lat_p, 99), 3)})
+    with Server(slots=4, max_batch=32) as srv:
+        lat, t_arr = loaded(srv)
         srv.completed.clear()
         t0 = time.perf_counter()
         futs = [srv.submit(r) for r in reqs]

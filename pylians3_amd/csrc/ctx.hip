@@ -226,6 +226,7 @@ int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
     std::unique_ptr<pyl_ctx> h(new pyl_ctx());
     Ctx *c = &h->c;
     c->device = device;
+    PYL_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     PYL_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
     c->stream = c->own_stream;
     if (hbm_budget == 0) {

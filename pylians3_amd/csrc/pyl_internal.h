@@ -108,6 +108,7 @@ enum BufId {
 
 struct Ctx {
     int device = 0;
+    int ncu = 0;                                             // compute units of the device
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     size_t budget = 0;

@@ -10,7 +10,9 @@ resident FFT plans and its scratch memory.  A dispatcher admits queued requests
 while a slot is free and the estimated HBM need of all in-flight work stays
 within the budget; requests that arrive while others run are admitted as
 slots and memory free up, and their kernels overlap on the device (one stream
-per slot).  Small requests with the same (dims, MAS, axis, kind) that are
+per slot).  The next request is the shortest waiting job (policy='sjf', work =
+particles + grid nodes) unless one has waited longer than max_wait_s, which
+then goes first; policy='fifo' serves in arrival order.  Small requests with the same (dims, MAS, axis, kind) that are
 waiting together are fused into one batched launch sequence
 (pyl_ma_batch + pyl_pk_batch).  Each result is bitwise the same whether the
 request runs alone, in a batch or alongside other requests: the scatter and
@@ -52,6 +54,10 @@ class Request:
         n = self.pos.shape[0]
         return n + (0 if self.pos2 is None else self.pos2.shape[0])
 
+    def work(self):
+        """Relative cost: particles deposited plus grid nodes transformed and binned."""
+        return self.particles() + 4 * self.nfields * self.dims ** 3
+
     def hbm_bytes(self):
         """Estimated device bytes while running (scratch + FFT buffers)."""
         d = self.dims
@@ -70,9 +76,14 @@ def _is_small(req):
 
 
 class Server:
-    def __init__(self, hbm_budget=None, slots=4, max_batch=32, device=0):
+    def __init__(self, hbm_budget=None, slots=4, max_batch=32, device=0, policy='sjf',
+                 max_wait_s=2.0):
         import torch
         self.device = device
+        if policy not in ('sjf', 'fifo'):
+            raise ValueError('policy must be "sjf" or "fifo"')
+        self.policy = policy
+        self.max_wait_s = float(max_wait_s)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(device)
             hbm_budget = int(free * 0.6)
@@ -123,20 +134,34 @@ class Server:
         return np.array([r.latency_s for r in self.completed])
 
     # ------------------------------------------------------------- scheduling
+    def _pick(self):
+        """Index of the next request: shortest job first (work ~ particles + grid
+        nodes), except that a request waiting longer than `max_wait_s` goes first
+        (oldest first), so that large requests are not starved."""
+        now = time.perf_counter()
+        if self.policy == 'fifo':
+            return 0
+        oldest = min(range(len(self.queue)), key=lambda i: self.queue[i][0].t_submit)
+        if now - self.queue[oldest][0].t_submit > self.max_wait_s:
+            return oldest
+        return min(range(len(self.queue)), key=lambda i: self.queue[i][0].work())
+
     def _dispatch(self):
         while True:
             with self.cv:
-                while not self.stop and (not self.queue or not self.free_slots or
-                                         self.queue[0][0].hbm_bytes() + self.in_flight_bytes > self.budget):
+                while not self.stop and (not self.queue or not self.free_slots):
                     self.cv.wait(0.05)
                 if self.stop and not self.queue:
                     return
                 if not self.queue or not self.free_slots:
                     continue
-                head = self.queue[0][0]
+                i = self._pick()
+                head = self.queue[i][0]
                 if head.hbm_bytes() + self.in_flight_bytes > self.budget:
+                    self.cv.wait(0.05)              # wait for memory to free up
                     continue
-                group = [self.queue.popleft()]
+                group = [self.queue[i]]
+                del self.queue[i]
                 need = head.hbm_bytes()
                 if _is_small(head):
                     # fuse waiting requests of the same shape into one batched launch
