@@ -427,7 +427,16 @@ def main():
     # dominant kernel: the longest stage; its duration is from the timed steps
     # (HIP events on the library's stream)
     ach = kernel_bytes[dom] / (dom_ms * 1e-3) / 1e9
-    roofline = {'bound': 'hbm', 'kernel': dom, 'kernel_ms': round(dom_ms, 4),
+    lds = None
+    if dom == 'ma_deposit':
+        # the deposit is LDS-atomic bound: 27 ds_add_u64 per TSC particle against the
+        # measured ceiling of 64-bit LDS atomics with conflict-free lanes (6.2 clk per
+        # wave instruction per CU: 64 lanes / 6.2 clk * 2.4 GHz * 256 CUs)
+        atomics = 27.0 * n
+        peak = 64.0 / 6.2 * 2.4e9 * 256
+        lds = {'atomics_per_s': round(atomics / (dom_ms * 1e-3), -6), 'peak_per_s': round(peak, -9),
+               'frac': round(atomics / (dom_ms * 1e-3) / peak, 4)}
+    roofline = {'bound': 'hbm', 'kernel': dom, 'kernel_ms': round(dom_ms, 4), 'lds_atomics': lds,
                 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
                 'algorithmic_bytes': int(kernel_bytes[dom]),

@@ -950,6 +950,8 @@ tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
     }
 }
 
+constexpr int kFinGroup = 1;     // band cells per thread whose loads are in flight together
+
 // The same finalize for 3D grids cut into equal tiles of kT3 nodes with at
 // least two tiles per axis (dims % kT3 == 0, dims > kT3).  Every lower
 // neighbour then has extent kT3, so each band cell (lx, ly, lz) takes image 1
@@ -984,35 +986,53 @@ tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, Til
     __syncthreads();
     grid += real * g.ncell;
     const long long ox = (long long)tx * T, oy = (long long)ty * T, oz = (long long)tz * T;
-    for (int t = threadIdx.x; t < NBAND; t += kBlock) {
-        int lx, ly, lz;
-        if (t < NA) {
-            lz = t % T; ly = (t / T) % T; lx = t / (T * T);
-        } else if (t < NA + NB) {
-            const int u = t - NA;
-            lz = u % T; ly = (u / T) % L; lx = L + u / (T * L);
-        } else {
-            const int u = t - NA - NB;
-            lz = u % L; ly = L + (u / L) % (T - L); lx = L + u / (L * (T - L));
-        }
-        float x[8];
+    // the loads of kFinGroup cells of this thread are issued before their sums
+    constexpr int SLOTS = NBAND > 0 ? (NBAND + kBlock - 1) / kBlock : 1;
+    constexpr int G = kFinGroup < SLOTS ? kFinGroup : SLOTS;
+    long long pb[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
-            const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
-            const long long pb = s_pbase[i];
-            x[i] = 0.0f;
-            if (on && pb >= 0)
-                x[i] = partials[pb + halo_index<F, 3>(lx + a * T, ly + b * T, lz + c * T, T, T, T)];
-        }
-        double v = 0.0;
+    for (int i = 0; i < 8; ++i) pb[i] = s_pbase[i];
+    for (int j0 = 0; j0 < SLOTS; j0 += G) {
+        int cl[G][3];
+        float x[G][8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
-            const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
-            if (on && s_pbase[i] >= 0) v += (double)x[i];
+        for (int jj = 0; jj < G; ++jj) {
+            const int t0 = threadIdx.x + (j0 + jj) * kBlock;
+            const int t = t0 < NBAND ? t0 : 0;
+            int lx, ly, lz;
+            if (t < NA) {
+                lz = t % T; ly = (t / T) % T; lx = t / (T * T);
+            } else if (t < NA + NB) {
+                const int u = t - NA;
+                lz = u % T; ly = (u / T) % L; lx = L + u / (T * L);
+            } else {
+                const int u = t - NA - NB;
+                lz = u % L; ly = L + (u / L) % (T - L); lx = L + u / (L * (T - L));
+            }
+            cl[jj][0] = lx; cl[jj][1] = ly; cl[jj][2] = lz;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+                const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
+                x[jj][i] = 0.0f;
+                if (t0 < NBAND && on && pb[i] >= 0)
+                    x[jj][i] = partials[pb[i] + halo_index<F, 3>(lx + a * T, ly + b * T, lz + c * T,
+                                                                 T, T, T)];
+            }
         }
-        store_node(grid, ((ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz), (float)v, om);
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) {
+            if (threadIdx.x + (j0 + jj) * kBlock >= NBAND) continue;
+            const int lx = cl[jj][0], ly = cl[jj][1], lz = cl[jj][2];
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+                const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
+                if (on && pb[i] >= 0) v += (double)x[jj][i];
+            }
+            store_node(grid, ((ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz), (float)v, om);
+        }
     }
 }
 

@@ -270,7 +270,11 @@ __device__ __forceinline__ int isqrt_i(int v) {
 // reweighted least squares on Chebyshev nodes (max error 1.7e-7 in float,
 // 2.3e-7 relative).
 __device__ __forceinline__ float atan_halfplane(float a, float b) {
-    const float t = (b > 0.0f) ? a / b : 0.0f;
+    // hardware reciprocal (1 ulp) for b in the normal range; exact division below
+    float t = 0.0f;
+    if (b >= 0x1p-100f) t = a * __builtin_amdgcn_rcpf(b);
+    else if (b > 0.0f) t = a / b;
+    t = fminf(fmaxf(t, -1.0f), 1.0f);       // |a| <= b: only rounding can leave [-1, 1]
     const float u = t * t;
     float p = -0.00478044f;
     p = p * u + 0.02455707f;
@@ -386,10 +390,11 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
                 im[f] = (double)xi;
             }
             double d2[S];
-            d2[0] = re[0] * re[0] + im[0] * im[0];
+            // the products of floats are exact in double: one rounding, as re*re + im*im
+            d2[0] = fma(re[0], re[0], im[0] * im[0]);
             if (NFL == 2) {
-                d2[1] = re[1] * re[1] + im[1] * im[1];
-                d2[2] = re[0] * re[1] + im[0] * im[1];
+                d2[1] = fma(re[1], re[1], im[1] * im[1]);
+                d2[2] = fma(re[0], re[1], im[0] * im[1]);
             }
             // 3D: merge equal bins in registers, flush on change
             if (kidx != cur3) {
@@ -408,7 +413,7 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
                 r3[3 * sp + 2] += d2[sp] * (double)L4;
             }
             if (PHASE) {
-                const float ph = atan_halfplane((float)re[0], sqrtf((float)d2[0]));
+                const float ph = atan_halfplane((float)re[0], __builtin_amdgcn_sqrtf((float)d2[0]));
                 r3[NA3 - 1] += (double)(ph * ph);
             }
             // 2D line: axes 0/1 index k_per (slowly varying), axis 2 index kz
@@ -723,9 +728,9 @@ int finalize(const ModePlan *mp, const PkDims &p, double BoxSize, const HostAcc 
                 out->Pk1D[(size_t)(i - 1) * S + s] = v;
             }
     }
-    // 2D: keep every bin, bin centres
-    for (int kpar = 0; kpar <= p.kmax_par; ++kpar)
-        for (int kper = 0; kper <= p.kmax_per; ++kper) {
+    // 2D: keep every bin, bin centres (kpar fastest: contiguous writes)
+    for (int kper = 0; kper <= p.kmax_per; ++kper)
+        for (int kpar = 0; kpar <= p.kmax_par; ++kpar) {
             const long long idx = (long long)(p.kmax_par + 1) * kper + kpar;
             if (out->kpar) out->kpar[idx] = 0.5 * (kpar + kpar + 1) * kF;
             if (out->kper) out->kper[idx] = 0.5 * (kper + kper + 1) * kF;
@@ -798,12 +803,12 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
     // host accumulators reused across calls on this thread (no fresh pages per call)
     static thread_local std::vector<HostAcc> accs;
     accs.resize(nreal);
-    for (auto &acc : accs) {
+    for (auto &acc : accs) {      // every column is written by exactly one launch below
         acc.S_total = S;
-        acc.P3.assign((size_t)(p.kmax + 1) * 3 * S, 0.0);
+        acc.P3.resize((size_t)(p.kmax + 1) * 3 * S);
         acc.phase.assign(p.kmax + 1, 0.0);
-        acc.P2.assign((size_t)p.nb2 * S, 0.0);
-        acc.P1.assign((size_t)(p.m + 1) * S, 0.0);
+        acc.P2.resize((size_t)p.nb2 * S);
+        acc.P1.resize((size_t)(p.m + 1) * S);
     }
     const bool phase = (F == 1) && !(flags & PYL_PK_NO_PHASE);
     if (F == 1) {
