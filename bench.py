@@ -440,7 +440,7 @@ def main():
                 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
                 'algorithmic_bytes': int(kernel_bytes[dom]),
-                'traffic': committed_traffic(dom)}
+                'traffic': committed_traffic(dom) if dims == 512 else None}
     per_kernel_gbps = {k: round(kernel_bytes[k] / (t * 1e-3) / 1e9, 1)
                        for k, t in st.items() if k in kernel_bytes and t > 0}
 

@@ -152,6 +152,7 @@ struct TileGeom {
     long long ntiles;  // tiles per grid
     long long ncell;   // nodes per grid
     int pslot;         // partial values stored per tile
+    int wmode;         // sort-chunk key window (kWinDirect / kWinBox / kWinLinear)
     const int *orig;   // device table of tile_origin(i), i = 0..nt (same on every axis)
 };
 
@@ -215,10 +216,22 @@ constexpr int kWin = 16384;                    // 64 KB of LDS
 struct Chunk {
     long long c0, c1;   // particle range
     int r;              // realisation
-    int wbase;          // window start, relative to the realisation's first key (count pass)
+    int wbase;          // window placement (count pass): w0 (linear) or wx * nt + wy (box)
     int smin, smax;     // first and last non-empty window slot (count pass; smin > smax: none)
     int pad0, pad1;
 };
+
+// Key windows of a sort chunk (TileGeom::wmode, chosen on the host):
+//  kWinDirect  ntiles <= kWin: slot = key - rbase, every key counted in LDS;
+//  kWinBox     the 8 x 8 columns of tiles (all z) starting 4 tiles below the
+//              chunk's middle particle along x and y, periodic: slot =
+//              (dx * 8 + dy) * ntz + tz, 64 * ntz <= kWin slots.  A lattice or
+//              ID-ordered chunk spans a few tiles in x and y whatever the grid
+//              size, so its displaced particles stay inside;
+//  kWinLinear  (key - rbase - w0) mod ntiles < kWin, centred on the middle
+//              particle's key (grids too thin or too deep for the box).
+constexpr int kWinDirect = 0, kWinBox = 1, kWinLinear = 2;
+constexpr int kBoxW = 8;
 
 // Home tile along one axis; POW2: every tile has extent 2^shift (or nt == 1).
 template <int MAS, bool POW2>
@@ -260,24 +273,72 @@ __device__ __forceinline__ float3 load_dist(const float *__restrict__ pos, const
 }
 
 template <int MAS, int AXES, bool POW2>
-__device__ __forceinline__ unsigned dist_key(const float3 &d, const TileGeom &g, long long rbase) {
-    const int tx = home_tile<MAS, POW2>(d.x, g);
-    const int ty = home_tile<MAS, POW2>(d.y, g);
-    const int tz = (AXES == 3) ? home_tile<MAS, POW2>(d.z, g) : 0;
-    return (unsigned)(rbase + ((long long)tx * g.nt + ty) * g.ntz + tz);
+__device__ __forceinline__ int3 dist_tiles(const float3 &d, const TileGeom &g) {
+    int3 t;
+    t.x = home_tile<MAS, POW2>(d.x, g);
+    t.y = home_tile<MAS, POW2>(d.y, g);
+    t.z = (AXES == 3) ? home_tile<MAS, POW2>(d.z, g) : 0;
+    return t;
 }
 
-// Window slot of a key: (key - rbase - w0) mod ntiles; slots >= kWin are outside.
-__device__ __forceinline__ unsigned win_slot(unsigned key, long long rbase, int w0,
-                                             long long ntiles) {
-    long long s = (long long)key - rbase - w0;
-    if (s < 0) s += ntiles;
+__device__ __forceinline__ unsigned tiles_key(const int3 &t, const TileGeom &g, long long rbase) {
+    return (unsigned)(rbase + ((long long)t.x * g.nt + t.y) * g.ntz + t.z);
+}
+
+template <int MAS, int AXES, bool POW2>
+__device__ __forceinline__ unsigned dist_key(const float3 &d, const TileGeom &g, long long rbase) {
+    return tiles_key(dist_tiles<MAS, AXES, POW2>(d, g), g, rbase);
+}
+
+// Chunk window: mode, placement (w0 or wx * nt + wy) and number of slots.
+struct Win {
+    int mode, a, b, slots;
+};
+
+__device__ __forceinline__ Win make_win(const TileGeom &g, const int3 &t0, long long k0, long long rbase) {
+    Win w{g.wmode, 0, 0, (int)g.ntiles};
+    if (g.wmode == kWinBox) {
+        w.a = t0.x - kBoxW / 2;
+        if (w.a < 0) w.a += g.nt;
+        w.b = t0.y - kBoxW / 2;
+        if (w.b < 0) w.b += g.nt;
+        w.slots = kBoxW * kBoxW * g.ntz;
+    } else if (g.wmode == kWinLinear) {
+        // taken cyclically so that particles wrapped across the periodic x
+        // boundary stay inside
+        long long s = (k0 - rbase - kWin / 2) % g.ntiles;
+        w.a = (int)(s < 0 ? s + g.ntiles : s);
+        w.slots = kWin;
+    }
+    return w;
+}
+
+// Window slot of a particle's tiles (>= w.slots: outside, counted globally).
+__device__ __forceinline__ unsigned win_slot(const int3 &t, unsigned key, const Win &w,
+                                             const TileGeom &g, long long rbase) {
+    if (w.mode == kWinBox) {
+        int dx = t.x - w.a, dy = t.y - w.b;
+        if (dx < 0) dx += g.nt;
+        if (dy < 0) dy += g.nt;
+        if (dx >= kBoxW || dy >= kBoxW) return 0xffffffffu;
+        return (unsigned)((dx * kBoxW + dy) * g.ntz + t.z);
+    }
+    long long s = (long long)key - rbase - (w.mode == kWinLinear ? w.a : 0);
+    if (s < 0) s += g.ntiles;
     return (unsigned)s;
 }
 
-__device__ __forceinline__ long long win_key(int s, long long rbase, int w0, long long ntiles) {
-    long long k = (long long)w0 + s;
-    if (k >= ntiles) k -= ntiles;
+__device__ __forceinline__ long long win_key(int s, const Win &w, const TileGeom &g,
+                                             long long rbase) {
+    if (w.mode == kWinBox) {
+        const int tz = s % g.ntz, c = s / g.ntz;
+        int tx = w.a + c / kBoxW, ty = w.b + c % kBoxW;
+        if (tx >= g.nt) tx -= g.nt;
+        if (ty >= g.nt) ty -= g.nt;
+        return rbase + ((long long)tx * g.nt + ty) * g.ntz + tz;
+    }
+    long long k = (long long)(w.mode == kWinLinear ? w.a : 0) + s;
+    if (k >= g.ntiles) k -= g.ntiles;
     return rbase + k;
 }
 
@@ -295,18 +356,13 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
     __shared__ unsigned hist[kWin];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
-    // every thread derives the window from the chunk's first particle
-    long long k0 = rbase;
-    if (ch.c1 > ch.c0) k0 = dist_key<MAS, AXES, POW2>(load_dist<AXES>(pos, rsd, ch.c0, inv), g, rbase);
-    // window start relative to the realisation's first key, taken cyclically so
-    // that particles wrapped across the periodic x boundary stay inside it
-    int w0 = 0;
-    if (g.ntiles > kWin) {
-        long long w = (k0 - rbase - kWin / 2) % g.ntiles;
-        w0 = (int)(w < 0 ? w + g.ntiles : w);
-    }
-    if (threadIdx.x == 0) chunks[blockIdx.x].wbase = w0;
-    for (int s = threadIdx.x; s < kWin; s += blockDim.x) hist[s] = 0u;
+    // every thread derives the window from the chunk's middle particle
+    int3 t0 = make_int3(0, 0, 0);
+    if (ch.c1 > ch.c0)
+        t0 = dist_tiles<MAS, AXES, POW2>(load_dist<AXES>(pos, rsd, ch.c0 + (ch.c1 - ch.c0) / 2, inv), g);
+    const Win w = make_win(g, t0, tiles_key(t0, g, rbase), rbase);
+    if (threadIdx.x == 0) chunks[blockIdx.x].wbase = (w.mode == kWinBox) ? w.a * g.nt + w.b : w.a;
+    for (int s = threadIdx.x; s < w.slots; s += blockDim.x) hist[s] = 0u;
     __syncthreads();
     const long long step = (long long)blockDim.x * kUnroll;
     const long long last = ch.c1 - 1;
@@ -318,9 +374,10 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             if (base + (long long)u * blockDim.x < ch.c1) {
-                const unsigned key = dist_key<MAS, AXES, POW2>(d[u], g, rbase);
-                const unsigned s = win_slot(key, rbase, w0, g.ntiles);
-                if (s < (unsigned)kWin) atomicAdd(&hist[s], 1u);
+                const int3 t = dist_tiles<MAS, AXES, POW2>(d[u], g);
+                const unsigned key = tiles_key(t, g, rbase);
+                const unsigned s = win_slot(t, key, w, g, rbase);
+                if (s < (unsigned)w.slots) atomicAdd(&hist[s], 1u);
                 else atomicAdd(&ocounts[key], 1u);
             }
         }
@@ -328,10 +385,10 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
     __syncthreads();
     unsigned *dmp = dump + (long long)blockIdx.x * kWin;
     int smin = kWin, smax = -1;
-    for (int s = threadIdx.x; s < kWin; s += blockDim.x) {
+    for (int s = threadIdx.x; s < w.slots; s += blockDim.x) {
         const unsigned v = hist[s];
         if (v) {
-            dmp[s] = atomicAdd(&counts[win_key(s, rbase, w0, g.ntiles)], v);
+            dmp[s] = atomicAdd(&counts[win_key(s, w, g, rbase)], v);
             smin = min(smin, s);
             smax = max(smax, s);
         }
@@ -374,7 +431,16 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
     __shared__ unsigned cur[kWin];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
-    const int w0 = ch.wbase;
+    // the count pass's window of this chunk
+    Win win{g.wmode, 0, 0, (int)g.ntiles};
+    if (g.wmode == kWinBox) {
+        win.a = ch.wbase / g.nt;
+        win.b = ch.wbase - win.a * g.nt;
+        win.slots = kBoxW * kBoxW * g.ntz;
+    } else if (g.wmode == kWinLinear) {
+        win.a = ch.wbase;
+        win.slots = kWin;
+    }
     const unsigned *dmp = dump + (long long)blockIdx.x * kWin;
     // only slots in [smin, smax] were counted; no other slot is looked up
     for (int s = ch.smin + threadIdx.x; s <= ch.smax; s += blockDim.x) cur[s] = dmp[s];
@@ -395,11 +461,12 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
         }
 #pragma unroll
         for (int u = 0; u < kUnrollS; ++u) {
-            key[u] = dist_key<MAS, AXES, POW2>(d[u], g, rbase);
+            const int3 t = dist_tiles<MAS, AXES, POW2>(d[u], g);
+            key[u] = tiles_key(t, g, rbase);
             slot[u] = 0;
             if (base + (long long)u * blockDim.x < ch.c1) {
-                const unsigned s = win_slot(key[u], rbase, w0, g.ntiles);
-                if (s < (unsigned)kWin) slot[u] = atomicAdd(&cur[s], 1u);
+                const unsigned s = win_slot(t, key[u], win, g, rbase);
+                if (s < (unsigned)win.slots) slot[u] = atomicAdd(&cur[s], 1u);
                 else slot[u] = (unsigned long long)counts[key[u]] + atomicAdd(&ocursors[key[u]], 1u);
             }
         }
@@ -1153,6 +1220,11 @@ TileGeom make_geom(int dims, int axes, int mas) {
         default: ps = axes == 3 ? halo_count<4, 3>(emax, emax, ez) : halo_count<4, 2>(emax, emax, ez); break;
     }
     g.pslot = ps > 0 ? ps : 1;
+    // sort window: every key when they fit, else the box of 8 x 8 tile columns
+    // (all z) when it fits and the grid has at least 8 tiles per axis
+    if (g.ntiles <= kWin) g.wmode = kWinDirect;
+    else if (g.nt >= kBoxW && (long long)kBoxW * kBoxW * g.ntz <= kWin) g.wmode = kWinBox;
+    else g.wmode = kWinLinear;
     return g;
 }
 

@@ -362,21 +362,35 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
 #pragma unroll
         for (int s = 0; s < S; ++s) r2[s] = 0.0;
 
+        // |k|^2 and k_per^2 of the run's first mode; along the run kz grows by one,
+        // so each square grows by 2kz-1 <= 2*isqrt+1 and each integer root by at
+        // most one: one test per mode instead of a square root
+        int k2 = kxy2 + kz0 * kz0;
+        int kidx = isqrt_i(k2);
+        int p2 = (AXIS == 0 ? ky * ky : kx * kx) + kz0 * kz0;
+        int kpr = (AXIS == 2) ? key : isqrt_i(p2);
 #pragma unroll
         for (int s = 0; s < kRun; ++s) {
             const int kz = kz0 + s;
+            if (s > 0) {
+                k2 += 2 * kz - 1;
+                kidx += ((kidx + 1) * (kidx + 1) <= k2) ? 1 : 0;
+                if (AXIS != 2) {
+                    p2 += 2 * kz - 1;
+                    kpr += ((kpr + 1) * (kpr + 1) <= p2) ? 1 : 0;
+                }
+            }
             bool ok = kz < a.zlen;
             if (kz == 0 || (a.even && kz == a.m)) {     // Pk_library.pyx:324-327
                 if (kx_neg) ok = false;
                 else if (kx_edge && ky_neg) ok = false;
             }
             if (!ok) continue;
-            const int k2 = kxy2 + kz * kz;
-            const int kidx = isqrt_i(k2);
-            int kpar, kper;
-            if (AXIS == 0) { kpar = kx < 0 ? -kx : kx; kper = isqrt_i(ky * ky + kz * kz); }
-            else if (AXIS == 1) { kpar = ky < 0 ? -ky : ky; kper = isqrt_i(kx * kx + kz * kz); }
-            else { kpar = kz; kper = key; }
+            int kpar;
+            const int kper = kpr;
+            if (AXIS == 0) kpar = kx < 0 ? -kx : kx;
+            else if (AXIS == 1) kpar = ky < 0 ? -ky : ky;
+            else kpar = kz;
             const float mu2 = (k2 == 0) ? 0.0f : __fdividef((float)(kpar * kpar), (float)k2);
             const float L2 = 1.5f * mu2 - 0.5f;
             const float L4 = (35.0f * mu2 * mu2 - 30.0f * mu2 + 3.0f) * 0.125f;

@@ -273,3 +273,37 @@ def test_ma_delta_equals_ma_then_normalise(oracle, mas):
                                      N.PTR_DEVICE | N.MA_ATOMIC))
     torch.cuda.synchronize()
     assert_cells(c.cpu().numpy() + 1.0, expect + 1.0, what='atomic delta')
+
+
+def test_ma_box_window_lattice_equals_shuffled():
+    # 448^3 grid (28^3 tiles, more than the LDS window holds): lattice-ordered
+    # chunks count their keys in the 8 x 8 tile-column box around them, shuffled
+    # input takes the global overflow counters; the integer tile sums make both
+    # grids bitwise equal, and the total is the particle count
+    import torch
+    from pylians3_amd import _native as N
+    print('case box window 448', flush=True)
+    d, box = 448, 1000.0
+    g = torch.Generator(device='cuda').manual_seed(31)
+    q = (torch.arange(d, device='cuda', dtype=torch.float32) + 0.5) * (box / d)
+    pos = torch.stack(torch.meshgrid(q, q, q, indexing='ij'), -1).reshape(-1, 3)
+    pos = pos + torch.randn(pos.shape, generator=g, device='cuda') * (4.0 * box / d)
+    pos = torch.remainder(pos, box).contiguous()
+    pos[pos >= box] = 0.0
+    n = pos.shape[0]
+    ctx = N.default_context()
+    a = torch.empty((d,) * 3, dtype=torch.float32, device='cuda')
+    b = torch.empty_like(a)
+    with N.torch_stream(ctx, True):
+        N.check(N.lib().pyl_ma(ctx.handle, N.device_ptr(pos), n, 3, None, N.device_ptr(a), d, box,
+                               N.TSC, N.PTR_DEVICE | N.MA_OVERWRITE))
+    perm = torch.randperm(n, generator=g, device='cuda')
+    pos_s = pos[perm].contiguous()
+    del perm, pos
+    with N.torch_stream(ctx, True):
+        N.check(N.lib().pyl_ma(ctx.handle, N.device_ptr(pos_s), n, 3, None, N.device_ptr(b), d, box,
+                               N.TSC, N.PTR_DEVICE | N.MA_OVERWRITE))
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    tot = a.double().sum().item()
+    assert abs(tot - n) <= 1e-6 * n, (tot, n)
