@@ -714,12 +714,10 @@ int launch_bin_axis(Ctx *c, int axis, const BinPlan *bp, const PkDims &p, const 
 }
 
 // Normalisation, Pk_library.pyx:384-418 (and :735-791 for XPk).
-int finalize(const ModePlan *mp, const PkDims &p, double BoxSize, const HostAcc &acc,
-             bool want_phase, pyl_pk_out *out) {
-    const int S = acc.S_total;
+// The outputs that depend only on the mode plan (k, bin centres, mode counts):
+// filled while the GPU still transforms and bins.
+int finalize_modes(const ModePlan *mp, const PkDims &p, double BoxSize, pyl_pk_out *out) {
     const double kF = 2.0 * M_PI / BoxSize;
-    const double kN = p.m * kF;
-    const double fact = std::pow(BoxSize / ((double)p.dims * (double)p.dims), 3.0);
     double sumN = 0.0;
     for (double v : mp->N3D) sumN += v;
     const long long own = (p.dims % 2 == 1) ? 1 : 8;
@@ -728,39 +726,52 @@ int finalize(const ModePlan *mp, const PkDims &p, double BoxSize, const HostAcc 
     PYL_CHECK((long long)sumN == expected, PYL_EMODES,
               "WARNING: Not all modes counted (counted %lld, expected %lld)", (long long)sumN,
               expected);
-    // 1D: drop the DC bin
     for (int i = 1; i <= p.m; ++i) {
-        const double N = mp->N1D[i];
-        const double k1 = (mp->k1D[i] / N) * kF;
-        if (out->k1D) out->k1D[i - 1] = k1;
-        if (out->Nmodes1D) out->Nmodes1D[i - 1] = N;
-        const double kmaxper = std::sqrt(kN * kN - k1 * k1);
-        if (out->Pk1D)
-            for (int s = 0; s < S; ++s) {
-                double v = acc.P1[(size_t)i * S + s] * fact;
-                v = v * (M_PI * (kmaxper * kmaxper) / N) / ((2.0 * M_PI) * (2.0 * M_PI));
-                out->Pk1D[(size_t)(i - 1) * S + s] = v;
-            }
+        if (out->k1D) out->k1D[i - 1] = (mp->k1D[i] / mp->N1D[i]) * kF;
+        if (out->Nmodes1D) out->Nmodes1D[i - 1] = mp->N1D[i];
     }
-    // 2D: keep every bin, bin centres (kpar fastest: contiguous writes)
+    // bin centres (kpar fastest: contiguous writes)
     for (int kper = 0; kper <= p.kmax_per; ++kper)
         for (int kpar = 0; kpar <= p.kmax_par; ++kpar) {
             const long long idx = (long long)(p.kmax_par + 1) * kper + kpar;
             if (out->kpar) out->kpar[idx] = 0.5 * (kpar + kpar + 1) * kF;
             if (out->kper) out->kper[idx] = 0.5 * (kper + kper + 1) * kF;
         }
-    for (long long i = 0; i < p.nb2; ++i) {
-        const double N = mp->N2D[i];
-        if (out->Nmodes2D) out->Nmodes2D[i] = N;
-        if (out->Pk2D)
+    if (out->Nmodes2D) std::memcpy(out->Nmodes2D, mp->N2D.data(), sizeof(double) * p.nb2);
+    for (int b = 1; b <= p.kmax; ++b) {
+        if (out->k3D) out->k3D[b - 1] = (mp->k3D[b] / mp->N3D[b]) * kF;
+        if (out->Nmodes3D) out->Nmodes3D[b - 1] = mp->N3D[b];
+    }
+    return PYL_OK;
+}
+
+// The spectra (Pk_library.pyx:384-418): 1D drops the DC bin, 2D keeps every bin,
+// 3D drops the DC bin with the multipoles times (2l+1).
+void finalize_values(const ModePlan *mp, const PkDims &p, double BoxSize, const HostAcc &acc,
+                     bool want_phase, pyl_pk_out *out) {
+    const int S = acc.S_total;
+    const double kF = 2.0 * M_PI / BoxSize;
+    const double kN = p.m * kF;
+    const double fact = std::pow(BoxSize / ((double)p.dims * (double)p.dims), 3.0);
+    if (out->Pk1D)
+        for (int i = 1; i <= p.m; ++i) {
+            const double N = mp->N1D[i];
+            const double k1 = (mp->k1D[i] / N) * kF;
+            const double kmaxper = std::sqrt(kN * kN - k1 * k1);
+            for (int s = 0; s < S; ++s) {
+                double v = acc.P1[(size_t)i * S + s] * fact;
+                v = v * (M_PI * (kmaxper * kmaxper) / N) / ((2.0 * M_PI) * (2.0 * M_PI));
+                out->Pk1D[(size_t)(i - 1) * S + s] = v;
+            }
+        }
+    if (out->Pk2D)
+        for (long long i = 0; i < p.nb2; ++i) {
+            const double N = mp->N2D[i];
             for (int s = 0; s < S; ++s)
                 out->Pk2D[(size_t)i * S + s] = acc.P2[(size_t)i * S + s] * fact / N;
-    }
-    // 3D: drop the DC bin, multipoles times (2l+1)
+        }
     for (int b = 1; b <= p.kmax; ++b) {
         const double N = mp->N3D[b];
-        if (out->k3D) out->k3D[b - 1] = (mp->k3D[b] / N) * kF;
-        if (out->Nmodes3D) out->Nmodes3D[b - 1] = N;
         if (out->Pk3D)
             for (int s = 0; s < S; ++s) {
                 const size_t o = (size_t)(b - 1) * 3 * S;
@@ -770,7 +781,6 @@ int finalize(const ModePlan *mp, const PkDims &p, double BoxSize, const HostAcc 
             }
         if (want_phase && out->Pkphase) out->Pkphase[b - 1] = (acc.phase[b] / N) * fact;
     }
-    return PYL_OK;
 }
 
 }  // namespace
@@ -813,6 +823,8 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
     PYL_TRY(get_bin_plan(c, dims, axis, nreal, &bp));
     std::vector<const double *> tab(F);
     for (int f = 0; f < F; ++f) PYL_TRY(get_mas_tab(c, dims, mas[f], &tab[f]));
+    // the mode-plan outputs are written while the FFT already queued runs
+    for (int r = 0; r < nreal; ++r) PYL_TRY(finalize_modes(mp, p, BoxSize, &outs[r]));
     const int S = F + F * (F - 1) / 2;
     // host accumulators reused across calls on this thread (no fresh pages per call)
     static thread_local std::vector<HostAcc> accs;
@@ -844,7 +856,7 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
                 PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, nreal, cp, accs, so)));
             }
     }
-    for (int r = 0; r < nreal; ++r) PYL_TRY(finalize(mp, p, BoxSize, accs[r], phase, &outs[r]));
+    for (int r = 0; r < nreal; ++r) finalize_values(mp, p, BoxSize, accs[r], phase, &outs[r]);
     if (nreal > 16) {                            // do not keep large batch accumulators
         accs.clear();
         accs.shrink_to_fit();
