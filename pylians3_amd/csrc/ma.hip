@@ -872,12 +872,26 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                 }
             }
             __builtin_amdgcn_wave_barrier();
+            // SIDX: the dealt records of all rounds are reloaded before the first
+            // atomic, so the reloads overlap each other and not the LDS work
+            float4 dealt[SIDX ? KR : 1];
+            if constexpr (SIDX) {
+                unsigned short di[KR];
+#pragma unroll
+                for (int r = 0; r < KR; ++r) di[r] = sidx[wv * WB + r * 64 + ln];
+#pragma unroll
+                for (int r = 0; r < KR; ++r) {
+                    const int p = (ln & 31) * (2 * KR) + (ln >> 5) * KR + r;
+                    // positions >= nb hold no record: reload a valid one, unused
+                    dealt[r] = load_record<RW>(records, b0 + (p < nb ? di[r] : 0));
+                }
+            }
 #pragma unroll
             for (int r = 0; r < KR; ++r) {
                 const int p = (ln & 31) * (2 * KR) + (ln >> 5) * KR + r;
                 if (p < nb) {
                     if constexpr (SIDX) {
-                        const float4 rr = load_record<RW>(records, b0 + sidx[wv * WB + r * 64 + ln]);
+                        const float4 rr = dealt[r];
                         deposit_record<MAS, AXES, WT>(acc + box_base<MAS, AXES>(rr, g, tb), rr, to_q,
                                                       weight_factor);
                     } else {
