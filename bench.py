@@ -38,7 +38,7 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--grid', type=int, default=512, help='grid and lattice size (512 = config 2)')
+    ap.add_argument('--grid', type=int, default=512, help='grid and particle-lattice size per axis (default 512: 512^3 particles onto 512^3)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain GPU (atomic) leg')
     ap.add_argument('--cpu-seconds', type=float, default=20.0, help='target CPU sample time')
@@ -188,7 +188,7 @@ def cpu_baseline(pos_host, grid_host, dims, box, mas_name, axis, target_s):
 
 
 def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
-    """Config 4: nreal independent 128^3 realisations (box 500): a Zel'dovich matter
+    """Batched leg: nreal independent 128^3 realisations (box 500): a Zel'dovich matter
     field (128^3 particles) and Poisson(lambda) weighted tracers, lambda log-uniform
     in [1e5, 2e6], CIC MA of both, then Pk of both and their cross spectrum (XPk).
     Batched form (pyl_ma_batch x2 + pyl_pk_batch) against the plain per-call form
@@ -281,7 +281,7 @@ def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
 
 
 def serving_leg(N, nreq=120, seed=5):
-    """Config 5: mixed MA+Pk / MA+XPk requests (grid 64^3-512^3, N log-uniform
+    """Serving leg: mixed MA+Pk / MA+XPk requests (grid 64^3-512^3, N log-uniform
     1e5-1.3e8, MAS and axis drawn uniformly) against the continuous-batching server.
     Plain form: one request at a time, one slot, arrival order, no fusion; its
     service latency alone, then under Poisson arrivals at 0.9x its capacity.

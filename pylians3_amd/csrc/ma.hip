@@ -1417,7 +1417,7 @@ int run_atomic(Ctx *c, const float *pos, const RSD &rsd, long long n, const long
 
 }  // namespace
 
-// Device-pointer MA used by the C ABI and by the batched driver.
+// Device-pointer MA used by the C ABI entry points, single and batched.
 // h_offs: host particle ranges of nreal grids (nreal+1 entries, NULL when nreal == 1);
 // grid holds nreal contiguous grids.  With PYL_MA_OVERWRITE the grid receives
 // deposit/scale - shift (scale = 1, shift = 0: the plain deposit).
