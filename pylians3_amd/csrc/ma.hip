@@ -633,10 +633,18 @@ __device__ __forceinline__ float4 load_record(const float *__restrict__ records,
     return make_float4(p[0], p[1], p[2], 1.0f);
 }
 
+// Record j of a wave-uniform set starting at `set` (32-bit lane offsets).
+template <bool RW>
+__device__ __forceinline__ float4 load_set_record(const float *__restrict__ set, unsigned j) {
+    if (RW) return reinterpret_cast<const float4 *>(set)[j];
+    const float *p = set + 3u * j;
+    return make_float4(p[0], p[1], p[2], 1.0f);
+}
+
 // Box offset of a record's first footprint node (records of this tile satisfy
 // 0 <= l < e on every axis; the clamp only guards the LDS box).
 template <int MAS, int AXES>
-__device__ __forceinline__ int box_base(const float4 &rec, const TileGeom &g, const TileBox &tb) {
+__host__ __device__ __forceinline__ int box_base(const float4 &rec, const TileGeom &g, const TileBox &tb) {
     constexpr int F = MAS;
     constexpr int T = (AXES == 3) ? kT3 : kT2;
     constexpr int BY = T + F - 1;
@@ -646,6 +654,28 @@ __device__ __forceinline__ int box_base(const float4 &rec, const TileGeom &g, co
     int lz = 0;
     if (AXES == 3) lz = min(max(wrap_node(first_node<MAS>(rec.z), g.dims) - tb.oz, 0), tb.ez - 1);
     return (lx * BY + ly) * BZ + lz;
+}
+
+// Uniform 3D tiles (dims % 16 == 0, 16 nodes per tile): tile origins and dims
+// are multiples of 16, so the box coordinate of a record of this tile, (first
+// node wrapped into [0, dims)) - origin, equals the unwrapped first node mod 16
+// on every axis.  The mask also keeps any record inside the box.
+template <int MAS>
+__host__ __device__ __forceinline__ int box_base_uni(const float4 &rec) {
+    constexpr int B = kT3 + MAS - 1;
+    const int lx = first_node<MAS>(rec.x) & (kT3 - 1);
+    const int ly = first_node<MAS>(rec.y) & (kT3 - 1);
+    const int lz = first_node<MAS>(rec.z) & (kT3 - 1);
+    return (lx * B + ly) * B + lz;
+}
+
+// Bank class (box offset mod 16) of a record of a uniform 3D tile, from the
+// unwrapped first nodes (B*B and B taken mod 16).
+template <int MAS>
+__host__ __device__ __forceinline__ unsigned box_class_uni(const float4 &rec) {
+    constexpr unsigned B = kT3 + MAS - 1;
+    return ((unsigned)first_node<MAS>(rec.x) * ((B * B) & 15u) +
+            (unsigned)first_node<MAS>(rec.y) * (B & 15u) + (unsigned)first_node<MAS>(rec.z)) & 15u;
 }
 
 // Adds one record's footprint to the LDS box at compile-time offsets from `a`.
@@ -715,10 +745,54 @@ __device__ __forceinline__ void deposit_record(unsigned long long *a, const floa
 template <int MAS, int AXES>
 constexpr int dep_kr() { return (MAS == 4 && AXES == 3) ? 2 : 4; }
 
-// Dealing stages record indices (2 B; the records are reloaded, from cache)
-// instead of the records themselves (16 B): the TSC 3D workgroup then needs
-// 51.8 KB of LDS and 3 fit one CU instead of 2 (1.56 -> 1.46 ms at 512^3).
-constexpr bool kDepStageIdx = true;
+// ---- bank-balanced dealing of a wave's records to atomic rounds -------------
+// A wave instruction ds_add_u64 is served as four 16-lane passes (lanes 0-15,
+// 16-31, 32-47, 48-63).  An 8-byte word at box offset e sits in the bank pair
+// e mod 16, so a pass costs the largest number of its lanes whose offsets agree
+// mod 16 (more when they hit the same word), and the instruction costs about
+// max(6 clk, sum of its four passes) on MI355X: one 2-way pass in each of two
+// groups is free, a 2-way conflict in all four costs 8.3 clk, random offsets
+// 12.6 clk.
+//
+// A wave deals KR*64 records to KR rounds, i.e. to G = 4*KR "layers" of 16
+// lanes.  Records are classed by e mod 16 (c_j records in class j).  Copy k < G
+// of class j goes to layer k, lane j: every layer holds at most one record per
+// class.  Layer k lacks class j when c_j <= k < G; those slots are holes, and
+// the copies k >= G (excess) fill them, holes listed class by class (class j's
+// holes lie in distinct layers, so one excess class rarely meets itself twice
+// in a layer).  Layer L runs in round L % KR as lane group L / KR: the high,
+// hole-bearing layers of a round are in different groups.  Holes left empty
+// (a set of fewer than KR*64 records) are marked kNoRecord.
+constexpr unsigned short kNoRecord = 0xffff;
+
+template <int KR>
+__host__ __device__ __forceinline__ int deal_slot(unsigned layer, unsigned lane) {
+    return (int)((layer % KR) * 64 + (layer / KR) * 16 + lane);
+}
+
+// Class j's holes: layers cj..G-1, hole indices H, H+1, ...; the first E holes of
+// the wave (E = number of excess copies) receive an excess copy through the hole
+// table ht (entry layer*16 + lane), the others are marked empty in sidx.
+template <int KR>
+__host__ __device__ __forceinline__ void deal_holes(unsigned j, unsigned cj, unsigned H, unsigned E,
+                                                    unsigned char *ht, unsigned short *sidx) {
+    constexpr unsigned G = 4 * KR;
+    for (unsigned L = cj; L < G; ++L) {
+        const unsigned i = H + (L - cj);
+        if (i < E) ht[i] = (unsigned char)((L << 4) | j);
+        else sidx[deal_slot<KR>(L, j)] = kNoRecord;
+    }
+}
+
+// Slot of copy `rank` of class `cls`; X = index of the class's first excess copy.
+template <int KR>
+__host__ __device__ __forceinline__ int deal_record_slot(unsigned cls, unsigned rank, unsigned X,
+                                                         const unsigned char *ht) {
+    constexpr unsigned G = 4 * KR;
+    if (rank < G) return deal_slot<KR>(rank, cls);
+    const unsigned e = ht[X + rank - G];
+    return deal_slot<KR>(e >> 4, e & 15u);
+}
 
 // One workgroup per tile.  LDS box of BX*BY*BZ 64-bit integers (the tile plus
 // the upper halo) holding multiples of 2^q; every record adds its whole
@@ -728,13 +802,11 @@ constexpr bool kDepStageIdx = true;
 //
 // Bank balancing (F >= 2): a footprint's F^AXES atomics all share the lanes'
 // box offsets up to a constant, so their LDS bank conflicts are fixed by the
-// offsets' residues mod 32 (32 banks of 8-byte words).  Each wave takes KR*64
-// consecutive records, ranks them by residue (LDS counters), and deals them
-// out in residue order so that every half-wave of every round takes every
-// (2*KR)-th record of that order: about one record per residue.  On random
-// offsets this halves the cost of a ds_add_u64 (12.6 -> 6.2 clk per wave in
-// a microbenchmark of the two extremes).  The sums are integers, so the
-// order of the atomics does not change any bit of the result.
+// offsets mod 16.  Each wave takes KR*64 consecutive records, ranks them by
+// class (LDS counters), deals them to rounds as described at deal_slot, stages
+// the dealt record indices (2 B each) in LDS and reloads the records (from
+// cache).  The sums are integers, so the order of the atomics does not change
+// any bit of the result.
 template <int MAS, int AXES, bool WT, bool RW, bool UNI>
 __global__ void __launch_bounds__(kDepBlock)
 tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
@@ -752,10 +824,9 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     constexpr int NW = kDepBlock / 64;                 // waves
     constexpr int WB = KR * 64;                        // records per wave round set
     __shared__ unsigned long long acc[NB];
-    constexpr bool SIDX = BAL && kDepStageIdx;
-    __shared__ float4 stage[(BAL && !SIDX) ? NW * WB : 1];
-    __shared__ unsigned short sidx[SIDX ? NW * WB : 1];
-    __shared__ unsigned whist[BAL ? NW * 32 : 1];
+    __shared__ unsigned short sidx[BAL ? NW * WB : 1];    // dealt record indices
+    __shared__ unsigned char htab[BAL ? NW * WB : 1];      // hole table
+    __shared__ unsigned whist[BAL ? NW * 16 : 1];          // class counters
     const int key = blockIdx.x;                       // r * ntiles + tile
     const int real = key / (int)g.ntiles;
     const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
@@ -781,13 +852,19 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
         return;
     }
     // the first round set of every wave is loaded before the box is cleared
-    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
-    float4 nxt[KR];
-    if constexpr (BAL) {
-        const unsigned long long b0 = r0 + (unsigned long long)wv * WB;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
+    constexpr int RS = RW ? 4 : 3;                     // floats per record
+    // records b .. b+WB-1 of a set, clamped to the tile's last record, through a
+    // wave-uniform pointer and 32-bit lane offsets
+    auto load_set = [&](unsigned long long b, float4 *dst) {
+        const unsigned long long sb = min(b, r1 - 1);
+        const float *set = records + (unsigned long long)RS * sb;
+        const unsigned lim = (unsigned)min(r1 - 1 - sb, (unsigned long long)WB);
 #pragma unroll
-        for (int k = 0; k < KR; ++k) nxt[k] = load_record<RW>(records, min(b0 + ln + 64 * k, r1 - 1));
-    }
+        for (int k = 0; k < KR; ++k) dst[k] = load_set_record<RW>(set, min((unsigned)(ln + 64 * k), lim));
+    };
+    float4 nxt[KR];
+    if constexpr (BAL) load_set(r0 + (unsigned long long)wv * WB, nxt);
     for (int c = threadIdx.x; c < NB; c += blockDim.x) acc[c] = 0ull;
 
     // Quantum 2^q: footprint weights are <= 1, so |contribution| <=
@@ -817,8 +894,10 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                                           weight_factor);
         }
     } else {
-        float4 *st = stage + wv * WB;
-        unsigned *wh = whist + wv * 32;
+        unsigned *wh = whist + wv * 16;
+        unsigned short *si = sidx + wv * WB;
+        unsigned char *ht = htab + wv * WB;
+        constexpr unsigned G = 4 * KR;                 // layers of 16 lanes
         // wave wv takes the round sets starting at r0 + (wv + NW*j)*WB; the next
         // set's records are loaded (clamped into the tile) while this one runs
         unsigned long long b0 = r0 + (unsigned long long)wv * WB;
@@ -827,78 +906,78 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
 #pragma unroll
             for (int k = 0; k < KR; ++k) rec[k] = nxt[k];
             const int nb = (int)min((unsigned long long)WB, r1 - b0);   // records in this set
-            const unsigned long long b1 = b0 + (unsigned long long)NW * WB;
-            if (b1 < r1) {
-#pragma unroll
-                for (int k = 0; k < KR; ++k)
-                    nxt[k] = load_record<RW>(records, min(b1 + ln + 64 * k, r1 - 1));
-            }
-            // rank within residue class (LDS ops of one wave run in issue order)
-            if (ln < 32) wh[ln] = 0u;
+            // rank within class (LDS ops of one wave run in issue order)
+            if (ln < 16) wh[ln] = 0u;
             __builtin_amdgcn_wave_barrier();
-            int base[KR];
             unsigned cls[KR], rank[KR];
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
-                base[k] = box_base<MAS, AXES>(rec[k], g, tb);
-                cls[k] = (unsigned)base[k] & 31u;
+                if constexpr (UNI && AXES == 3) cls[k] = box_class_uni<MAS>(rec[k]);
+                else cls[k] = (unsigned)box_base<MAS, AXES>(rec[k], g, tb) & 15u;
                 rank[k] = (ln + 64 * k < nb) ? atomicAdd(&wh[cls[k]], 1u) : 0u;
             }
             __builtin_amdgcn_wave_barrier();
-            // exclusive prefix of the 32 class counts: inclusive scan of lanes 0..31
-            // with DPP row shifts (1, 2, 4, 8 within rows of 16 lanes) and
-            // row_bcast:15 (lane 15 added to row 1); lanes 32..63 hold 0
-            const unsigned h = (ln < 32) ? wh[ln] : 0u;
-            unsigned incl = h;
+            // lanes 0..15 hold class j = ln: c_j records, G - c_j holes or c_j - G
+            // excess copies.  Exclusive prefixes of both (excess in the high half)
+            // by an inclusive DPP scan over row 0 (row_shr 1, 2, 4, 8).
+            const unsigned cj = (ln < 16) ? wh[ln] : 0u;
+            const unsigned v = (ln < 16) ? ((cj > G ? cj - G : 0u) << 16) | (cj < G ? G - cj : 0u) : 0u;
+            unsigned incl = v;
             incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x111, 0xf, 0xf, false);
             incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x112, 0xf, 0xf, false);
             incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x114, 0xf, 0xf, false);
             incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x118, 0xf, 0xf, false);
-            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x142, 0xa, 0xf, false);
+            const unsigned excl = incl - v;
+            const unsigned E = (unsigned)__builtin_amdgcn_readlane((int)incl, 15) >> 16;
             __builtin_amdgcn_wave_barrier();
-            if (ln < 32) wh[ln] = incl - h;
+            if (ln < 16) {
+                wh[ln] = excl >> 16;                   // first excess index of class ln
+                deal_holes<KR>((unsigned)ln, cj, excl & 0xffffu, E, ht, si);
+            }
             __builtin_amdgcn_wave_barrier();
-            // position p in residue order -> round p % KR, lane ((p/KR)&1)*32 + p/(2KR)
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
                 if (ln + 64 * k < nb) {
-                    const unsigned p = wh[cls[k]] + rank[k];
-                    const unsigned slot = (p % KR) * 64 + ((p / KR) & 1u) * 32 + p / (2 * KR);
-                    if constexpr (SIDX)
-                        sidx[wv * WB + slot] = (unsigned short)(ln + 64 * k);
-                    else
-                        st[slot] = make_float4(rec[k].x, rec[k].y, rec[k].z,
-                                               WT ? rec[k].w : __int_as_float(base[k]));
+                    const unsigned X = rank[k] < G ? 0u : wh[cls[k]];
+                    si[deal_record_slot<KR>(cls[k], rank[k], X, ht)] = (unsigned short)(ln + 64 * k);
                 }
             }
             __builtin_amdgcn_wave_barrier();
-            // SIDX: the dealt records of all rounds are reloaded before the first
-            // atomic, so the reloads overlap each other and not the LDS work
-            float4 dealt[SIDX ? KR : 1];
-            if constexpr (SIDX) {
-                unsigned short di[KR];
+            // the dealt records of all rounds are reloaded (from cache) before the
+            // first atomic, and the next set is loaded only after them: loads
+            // complete in issue order, so the atomics wait for the reloads alone
+            // and the next set's memory latency overlaps this set's atomics.  The
+            // next-set loads are issued unconditionally (past the tile's end they
+            // all read its last record) so that the number of loads in flight is
+            // the same on every path and the waits before the atomics stay exact.
+            unsigned short di[KR];
+            float4 dealt[KR];
 #pragma unroll
-                for (int r = 0; r < KR; ++r) di[r] = sidx[wv * WB + r * 64 + ln];
+            for (int r = 0; r < KR; ++r) di[r] = si[r * 64 + ln];
+            const float *set = records + (unsigned long long)RS * b0;
 #pragma unroll
-                for (int r = 0; r < KR; ++r) {
-                    const int p = (ln & 31) * (2 * KR) + (ln >> 5) * KR + r;
-                    // positions >= nb hold no record: reload a valid one, unused
-                    dealt[r] = load_record<RW>(records, b0 + (p < nb ? di[r] : 0));
-                }
+            for (int r = 0; r < KR; ++r)
+                dealt[r] = load_set_record<RW>(set, di[r] != kNoRecord ? di[r] : 0u);
+            __builtin_amdgcn_sched_barrier(0);         // keep the reloads ahead of the next set
+            load_set(b0 + (unsigned long long)NW * WB, nxt);
+            __builtin_amdgcn_sched_barrier(0);
+            // every reloaded record is consumed here, on all lanes: the reloads
+            // are not sunk into the per-round branches below (where they would be
+            // issued after the next set's loads), and one wait covers them
+#pragma unroll
+            for (int r = 0; r < KR; ++r) {
+                if constexpr (RW)
+                    asm volatile("" ::"v"(dealt[r].x), "v"(dealt[r].y), "v"(dealt[r].z), "v"(dealt[r].w));
+                else
+                    asm volatile("" ::"v"(dealt[r].x), "v"(dealt[r].y), "v"(dealt[r].z));
             }
 #pragma unroll
             for (int r = 0; r < KR; ++r) {
-                const int p = (ln & 31) * (2 * KR) + (ln >> 5) * KR + r;
-                if (p < nb) {
-                    if constexpr (SIDX) {
-                        const float4 rr = dealt[r];
-                        deposit_record<MAS, AXES, WT>(acc + box_base<MAS, AXES>(rr, g, tb), rr, to_q,
-                                                      weight_factor);
-                    } else {
-                        const float4 rr = st[r * 64 + ln];
-                        const int bb = WT ? box_base<MAS, AXES>(rr, g, tb) : __float_as_int(rr.w);
-                        deposit_record<MAS, AXES, WT>(acc + bb, rr, to_q, weight_factor);
-                    }
+                if (di[r] != kNoRecord) {
+                    int bb;
+                    if constexpr (UNI && AXES == 3) bb = box_base_uni<MAS>(dealt[r]);
+                    else bb = box_base<MAS, AXES>(dealt[r], g, tb);
+                    deposit_record<MAS, AXES, WT>(acc + bb, dealt[r], to_q, weight_factor);
                 }
             }
             __builtin_amdgcn_wave_barrier();

@@ -251,8 +251,111 @@ static int all_dims() {
     return bad;
 }
 
+// Dealing of one wave's round set (deal_holes / deal_record_slot): for uniform,
+// skewed, few-class and one-class record sets, full and partial, every record
+// gets its own slot, exactly the unused slots are marked kNoRecord, and with no
+// excess copies every 16-lane group holds at most one record per class.
+template <int KR>
+static int check_dealing(unsigned seed) {
+    constexpr int WB = KR * 64;
+    constexpr unsigned G = 4 * KR;
+    std::mt19937 rng(seed);
+    std::normal_distribution<float> Nd(0.0f, 1.0f);
+    int bad = 0;
+    long long sets = 0, excess = 0;
+    for (int trial = 0; trial < 4000; ++trial) {
+        const int nb = (trial % 5 == 0) ? 1 + (int)(rng() % WB) : WB;
+        const int mode = trial % 4;
+        std::vector<unsigned> cls(nb);
+        for (int i = 0; i < nb; ++i) {
+            if (mode == 0) cls[i] = rng() % 16;
+            else if (mode == 1) cls[i] = std::min(15u, (unsigned)(std::fabs(Nd(rng)) * 4.0f));
+            else if (mode == 2) cls[i] = rng() % 3;
+            else cls[i] = 7;
+        }
+        unsigned cnt[16] = {0};
+        std::vector<unsigned> rank(nb);
+        for (int i = 0; i < nb; ++i) rank[i] = cnt[cls[i]]++;
+        unsigned H[16], X[16], E = 0, holes = 0;
+        for (int j = 0; j < 16; ++j) {
+            H[j] = holes; X[j] = E;
+            holes += cnt[j] < G ? G - cnt[j] : 0u;
+            E += cnt[j] > G ? cnt[j] - G : 0u;
+        }
+        std::vector<unsigned short> si(WB, 0x1234);
+        std::vector<unsigned char> ht(WB, 0xee);
+        for (unsigned j = 0; j < 16; ++j) deal_holes<KR>(j, cnt[j], H[j], E, ht.data(), si.data());
+        std::vector<int> owner(WB, -1);
+        for (int i = 0; i < nb; ++i) {
+            const int sl = deal_record_slot<KR>(cls[i], rank[i], rank[i] < G ? 0u : X[cls[i]], ht.data());
+            if (sl < 0 || sl >= WB || owner[sl] != -1 || si[sl] == kNoRecord) { ++bad; continue; }
+            owner[sl] = i;
+            si[sl] = (unsigned short)i;
+        }
+        int empty = 0;
+        for (int sl = 0; sl < WB; ++sl) {
+            if (owner[sl] == -1) { ++empty; if (si[sl] != kNoRecord) ++bad; }
+        }
+        if (empty != WB - nb) ++bad;
+        if (E == 0) {
+            for (int gq = 0; gq < WB / 16; ++gq) {
+                unsigned seen = 0;
+                for (int l = 0; l < 16; ++l) {
+                    const int o = owner[gq * 16 + l];
+                    if (o < 0) continue;
+                    if (seen & (1u << cls[o])) ++bad;
+                    seen |= 1u << cls[o];
+                }
+            }
+        }
+        ++sets; excess += E;
+    }
+    printf("dealing KR %d: %lld sets, %lld excess copies: %s\n", KR, sets, excess, bad ? "FAIL" : "ok");
+    return bad;
+}
+
+// Uniform 3D tiles: box_base_uni / box_class_uni (unwrapped first node mod 16)
+// equal the general box_base for every record of its home tile, also at the
+// periodic edges.
+template <int MAS>
+static int check_box_uni(unsigned seed) {
+    int bad = 0;
+    long long n = 0;
+    std::mt19937 rng(seed);
+    const int dims_list[] = {32, 48, 64, 96, 128, 512};
+    for (int dims : dims_list) {
+        TileGeom g = make_geom(dims, 3, MAS);
+        std::vector<int> orig(g.nt + 1);
+        for (int i = 0; i <= g.nt; ++i) orig[i] = tile_origin(i, g.nt, dims);
+        g.orig = orig.data();
+        if (!(g.nt >= 2 && dims % kT3 == 0)) { ++bad; continue; }
+        for (int i = 0; i < 20000; ++i) {
+            float d[3];
+            for (int a = 0; a < 3; ++a) {
+                const int k = (int)(rng() % (dims + 1));
+                const float fr[7] = {0.0f, 0.5f, 0.25f, 0.999999f, 0.5000001f, 0.4999999f, 0.75f};
+                float x = k + fr[rng() % 7];
+                if (rng() % 5 == 0) x = std::nextafter((float)dims, 0.0f);
+                if (x >= (float)dims) x = 0.0f;
+                d[a] = x;
+            }
+            const float4 rec = make_float4(d[0], d[1], d[2], 1.0f);
+            const int tx = home_tile<MAS, true>(d[0], g), ty = home_tile<MAS, true>(d[1], g),
+                      tz = home_tile<MAS, true>(d[2], g);
+            const TileBox tb = tile_box<3>((tx * g.nt + ty) * g.ntz + tz, g);
+            const int bb = box_base<MAS, 3>(rec, g, tb);
+            if (bb != box_base_uni<MAS>(rec) || (unsigned)(bb & 15) != box_class_uni<MAS>(rec)) ++bad;
+            ++n;
+        }
+    }
+    printf("uniform box MAS %d: %lld records: %s\n", MAS, n, bad ? "FAIL" : "ok");
+    return bad;
+}
+
 int main() {
-    int bad = all_dims<PYL_NGP>() + all_dims<PYL_CIC>() + all_dims<PYL_TSC>() + all_dims<PYL_PCS>();
+    int bad = check_dealing<4>(11) + check_dealing<2>(12);
+    bad += check_box_uni<PYL_CIC>(21) + check_box_uni<PYL_TSC>(22) + check_box_uni<PYL_PCS>(23);
+    bad += all_dims<PYL_NGP>() + all_dims<PYL_CIC>() + all_dims<PYL_TSC>() + all_dims<PYL_PCS>();
     printf("uniform finalize cells checked: %lld\n", uni_checked);
     printf(bad ? "FAILURES: %d\n" : "all ok\n", bad);
     return bad != 0;
