@@ -770,18 +770,18 @@ __host__ __device__ __forceinline__ int deal_slot(unsigned layer, unsigned lane)
     return (int)((layer % KR) * 64 + (layer / KR) * 16 + lane);
 }
 
-// Class j's holes: layers cj..G-1, hole indices H, H+1, ...; the first E holes of
-// the wave (E = number of excess copies) receive an excess copy through the hole
-// table ht (entry layer*16 + lane), the others are marked empty in sidx.
+// Slot (layer L, lane j) when class j has cj records: a hole if cj <= L < G, with
+// hole index H + L - cj (H = holes of the classes before j).  The first E holes
+// of the wave (E = number of excess copies) receive an excess copy through the
+// hole table ht (entry layer*16 + lane), the others are marked empty in sidx.
 template <int KR>
-__host__ __device__ __forceinline__ void deal_holes(unsigned j, unsigned cj, unsigned H, unsigned E,
-                                                    unsigned char *ht, unsigned short *sidx) {
+__host__ __device__ __forceinline__ void deal_hole(unsigned L, unsigned j, unsigned cj, unsigned H,
+                                                   unsigned E, unsigned char *ht, unsigned short *sidx) {
     constexpr unsigned G = 4 * KR;
-    for (unsigned L = cj; L < G; ++L) {
-        const unsigned i = H + (L - cj);
-        if (i < E) ht[i] = (unsigned char)((L << 4) | j);
-        else sidx[deal_slot<KR>(L, j)] = kNoRecord;
-    }
+    if (L < cj || L >= G) return;
+    const unsigned i = H + (L - cj);
+    if (i < E) ht[i] = (unsigned char)((L << 4) | j);
+    else sidx[deal_slot<KR>(L, j)] = kNoRecord;
 }
 
 // Slot of copy `rank` of class `cls`; X = index of the class's first excess copy.
@@ -930,15 +930,22 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
             const unsigned excl = incl - v;
             const unsigned E = (unsigned)__builtin_amdgcn_readlane((int)incl, 15) >> 16;
             __builtin_amdgcn_wave_barrier();
-            if (ln < 16) {
-                wh[ln] = excl >> 16;                   // first excess index of class ln
-                deal_holes<KR>((unsigned)ln, cj, excl & 0xffffu, E, ht, si);
+            // class j's count, first hole and first excess index (9 bits each)
+            if (ln < 16) wh[ln] = cj | (excl & 0xffffu) << 9 | (excl >> 16) << 18;
+            __builtin_amdgcn_wave_barrier();
+            {
+                // the G*16 (layer, lane) slots, KR per lane: lane ln takes class ln % 16
+                const unsigned cw = wh[ln & 15];
+#pragma unroll
+                for (int k = 0; k < KR; ++k)
+                    deal_hole<KR>((unsigned)(ln >> 4) + 4u * k, (unsigned)(ln & 15), cw & 511u,
+                                  (cw >> 9) & 511u, E, ht, si);
             }
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
                 if (ln + 64 * k < nb) {
-                    const unsigned X = rank[k] < G ? 0u : wh[cls[k]];
+                    const unsigned X = rank[k] < G ? 0u : wh[cls[k]] >> 18;
                     si[deal_record_slot<KR>(cls[k], rank[k], X, ht)] = (unsigned short)(ln + 64 * k);
                 }
             }

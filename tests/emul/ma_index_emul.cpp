@@ -251,7 +251,7 @@ static int all_dims() {
     return bad;
 }
 
-// Dealing of one wave's round set (deal_holes / deal_record_slot): for uniform,
+// Dealing of one wave's round set (deal_hole / deal_record_slot): for uniform,
 // skewed, few-class and one-class record sets, full and partial, every record
 // gets its own slot, exactly the unused slots are marked kNoRecord, and with no
 // excess copies every 16-lane group holds at most one record per class.
@@ -284,7 +284,8 @@ static int check_dealing(unsigned seed) {
         }
         std::vector<unsigned short> si(WB, 0x1234);
         std::vector<unsigned char> ht(WB, 0xee);
-        for (unsigned j = 0; j < 16; ++j) deal_holes<KR>(j, cnt[j], H[j], E, ht.data(), si.data());
+        for (unsigned L = 0; L < G; ++L)
+            for (unsigned j = 0; j < 16; ++j) deal_hole<KR>(L, j, cnt[j], H[j], E, ht.data(), si.data());
         std::vector<int> owner(WB, -1);
         for (int i = 0; i < nb; ++i) {
             const int sl = deal_record_slot<KR>(cls[i], rank[i], rank[i] < G ? 0u : X[cls[i]], ht.data());
