@@ -669,13 +669,21 @@ __host__ __device__ __forceinline__ int box_base_uni(const float4 &rec) {
     return (lx * B + ly) * B + lz;
 }
 
-// Bank class (box offset mod 16) of a record of a uniform 3D tile, from the
-// unwrapped first nodes (B*B and B taken mod 16).
-template <int MAS>
-__host__ __device__ __forceinline__ unsigned box_class_uni(const float4 &rec) {
-    constexpr unsigned B = kT3 + MAS - 1;
-    return ((unsigned)first_node<MAS>(rec.x) * ((B * B) & 15u) +
-            (unsigned)first_node<MAS>(rec.y) * (B & 15u) + (unsigned)first_node<MAS>(rec.z)) & 15u;
+// Bank class of a record for dealing it to an atomic round: its box offset mod
+// 16 up to a constant of the tile, from its first nodes up to a constant
+// ((int)(d + 0.5f) for NGP and TSC, (int)d for CIC and PCS; a record whose
+// approximation is off, or that wraps around the periodic edge, only lands in
+// another class).  The class steers bank balance alone: every record is
+// deposited at its exact box offset whatever class it was dealt with.
+template <int MAS, int AXES>
+__host__ __device__ __forceinline__ unsigned bank_class(const float4 &rec) {
+    constexpr unsigned T = (AXES == 3) ? kT3 : kT2;
+    constexpr unsigned BY = T + MAS - 1, BZ = (AXES == 3) ? T + MAS - 1 : 1;
+    constexpr float h = (MAS == PYL_NGP || MAS == PYL_TSC) ? 0.5f : 0.0f;
+    const unsigned fx = (unsigned)(int)(rec.x + h), fy = (unsigned)(int)(rec.y + h);
+    if (AXES == 2) return (fx * (BY & 15u) + fy) & 15u;
+    const unsigned fz = (unsigned)(int)(rec.z + h);
+    return (fx * ((BY * BZ) & 15u) + fy * (BZ & 15u) + fz) & 15u;
 }
 
 // Adds one record's footprint to the LDS box at compile-time offsets from `a`.
@@ -912,8 +920,7 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
             unsigned cls[KR], rank[KR];
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
-                if constexpr (UNI && AXES == 3) cls[k] = box_class_uni<MAS>(rec[k]);
-                else cls[k] = (unsigned)box_base<MAS, AXES>(rec[k], g, tb) & 15u;
+                cls[k] = bank_class<MAS, AXES>(rec[k]);
                 rank[k] = (ln + 64 * k < nb) ? atomicAdd(&wh[cls[k]], 1u) : 0u;
             }
             __builtin_amdgcn_wave_barrier();
@@ -992,6 +999,40 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     }
     __syncthreads();
     float *part = partials + (long long)key * g.pslot;
+    if constexpr (UNI && AXES == 3) {
+        // Uniform tile: the interior nodes go to the grid, the halo to the
+        // partial slots in halo_index order, region by region (x outside the
+        // interior: whole planes; x inside, y outside: rows of B; x, y inside, z
+        // outside: H nodes), so that every loop has constant extents and the
+        // partial stores are contiguous.
+        constexpr int B = BX, H = 2 * (F - 1), I = T - (F - 1);
+        const unsigned long long node0 =
+            ((unsigned long long)tb.ox * g.dims + tb.oy) * g.dims + tb.oz;
+        const unsigned dd = (unsigned)g.dims * (unsigned)g.dims;
+        for (int i = threadIdx.x; i < I * I * I; i += blockDim.x) {
+            const int x = F - 1 + i / (I * I), y = F - 1 + (i / I) % I, z = F - 1 + i % I;
+            const float v = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
+            store_node(grid, (long long)(node0 + (unsigned)x * dd + (unsigned)(y * g.dims + z)), v, om);
+        }
+        for (int i = threadIdx.x; i < H * B * B; i += blockDim.x) {
+            const int xi = i / (B * B), yz = i - xi * (B * B);
+            const int x = xi < F - 1 ? xi : xi + I;
+            part[i] = (float)((double)(long long)acc[x * B * B + yz] * from_q);
+        }
+        constexpr int SA = H * B * B;
+        for (int i = threadIdx.x; i < I * H * B; i += blockDim.x) {
+            const int x = F - 1 + i / (H * B), r = i % (H * B), yi = r / B, z = r - yi * B;
+            const int y = yi < F - 1 ? yi : yi + I;
+            part[SA + i] = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
+        }
+        constexpr int SB = SA + I * H * B;
+        for (int i = threadIdx.x; i < I * I * H; i += blockDim.x) {
+            const int x = F - 1 + i / (I * H), r = i % (I * H), y = F - 1 + r / H, zi = r % H;
+            const int z = zi < F - 1 ? zi : zi + I;
+            part[SB + i] = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
+        }
+        return;
+    }
     for (int c = threadIdx.x; c < NB; c += blockDim.x) {
         const int bz = c % BZ, by = (c / BZ) % BY, bx = c / (BZ * BY);
         if (!UNI && (bx >= ex + F - 1 || by >= ey + F - 1 || (AXES == 3 && bz >= ez + F - 1)))

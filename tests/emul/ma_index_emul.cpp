@@ -315,8 +315,8 @@ static int check_dealing(unsigned seed) {
     return bad;
 }
 
-// Uniform 3D tiles: box_base_uni / box_class_uni (unwrapped first node mod 16)
-// equal the general box_base for every record of its home tile, also at the
+// Uniform 3D tiles: box_base_uni (unwrapped first node mod 16) equals the
+// general box_base for every record of its home tile, also at the
 // periodic edges.
 template <int MAS>
 static int check_box_uni(unsigned seed) {
@@ -345,7 +345,7 @@ static int check_box_uni(unsigned seed) {
                       tz = home_tile<MAS, true>(d[2], g);
             const TileBox tb = tile_box<3>((tx * g.nt + ty) * g.ntz + tz, g);
             const int bb = box_base<MAS, 3>(rec, g, tb);
-            if (bb != box_base_uni<MAS>(rec) || (unsigned)(bb & 15) != box_class_uni<MAS>(rec)) ++bad;
+            if (bb != box_base_uni<MAS>(rec)) ++bad;
             ++n;
         }
     }
@@ -353,8 +353,46 @@ static int check_box_uni(unsigned seed) {
     return bad;
 }
 
+// Uniform 3D tiles: the deposit's region-by-region halo loops (restated from
+// tile_deposit_kernel) visit every halo node of the box once, at its
+// halo_index slot, and the interior loop the remaining nodes.
+template <int MAS>
+static int check_uni_final() {
+    constexpr int F = MAS, T = kT3, B = T + F - 1, H = 2 * (F - 1), I = T - (F - 1);
+    std::vector<int> seen(B * B * B, 0);
+    int bad = 0;
+    for (int i = 0; i < I * I * I; ++i) {
+        const int x = F - 1 + i / (I * I), y = F - 1 + (i / I) % I, z = F - 1 + i % I;
+        ++seen[(x * B + y) * B + z];
+    }
+    auto halo = [&](int slot, int x, int y, int z) {
+        ++seen[(x * B + y) * B + z];
+        if (halo_index<F, 3>(x, y, z, T, T, T) != slot) ++bad;
+    };
+    for (int i = 0; i < H * B * B; ++i) {
+        const int xi = i / (B * B), yz = i - xi * (B * B);
+        halo(i, xi < F - 1 ? xi : xi + I, yz / B, yz % B);
+    }
+    constexpr int SA = H * B * B;
+    for (int i = 0; i < I * H * B; ++i) {
+        const int x = F - 1 + i / (H * B), r = i % (H * B), yi = r / B, z = r - yi * B;
+        halo(SA + i, x, yi < F - 1 ? yi : yi + I, z);
+    }
+    constexpr int SB = SA + I * H * B;
+    for (int i = 0; i < I * I * H; ++i) {
+        const int x = F - 1 + i / (I * H), r = i % (I * H), y = F - 1 + r / H, zi = r % H;
+        halo(SB + i, x, y, zi < F - 1 ? zi : zi + I);
+    }
+    for (int c = 0; c < B * B * B; ++c) if (seen[c] != 1) ++bad;
+    if (SB + I * I * H != halo_count<F, 3>(T, T, T)) ++bad;
+    printf("uniform final loops MAS %d: %s\n", MAS, bad ? "FAIL" : "ok");
+    return bad;
+}
+
 int main() {
     int bad = check_dealing<4>(11) + check_dealing<2>(12);
+    bad += check_uni_final<PYL_NGP>() + check_uni_final<PYL_CIC>() + check_uni_final<PYL_TSC>() +
+           check_uni_final<PYL_PCS>();
     bad += check_box_uni<PYL_CIC>(21) + check_box_uni<PYL_TSC>(22) + check_box_uni<PYL_PCS>(23);
     bad += all_dims<PYL_NGP>() + all_dims<PYL_CIC>() + all_dims<PYL_TSC>() + all_dims<PYL_PCS>();
     printf("uniform finalize cells checked: %lld\n", uni_checked);
