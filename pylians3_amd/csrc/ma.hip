@@ -81,6 +81,20 @@ __host__ __device__ __forceinline__ int first_node(float dist) {
     }
 }
 
+// TSC weights of the nodes raw, raw+1, raw+2 as (1.5-d0)^2, 0.75-d1^2, (1.5-d2)^2:
+// the outer two still take the factor 0.5 (MAS_library.pyx:392-399).
+// (float)(raw + j) == (float)raw + j exactly for |raw| < 2^24.
+__host__ __device__ __forceinline__ void tsc_parts(float dist, int raw, float &q0, float &c1, float &q2) {
+    const float fr = (float)raw;
+    const float d0 = fabsf(fr - dist);
+    const float d1 = fabsf((fr + 1.0f) - dist);
+    const float d2 = fabsf((fr + 2.0f) - dist);
+    const float a0 = 1.5f - d0, a2 = 1.5f - d2;
+    q0 = a0 * a0;
+    c1 = 0.75f - d1 * d1;
+    q2 = a2 * a2;
+}
+
 // Footprint weights of the nodes raw .. raw+F-1.
 template <int MAS>
 __host__ __device__ __forceinline__ void foot_weights(float dist, int raw, float *C) {
@@ -99,13 +113,10 @@ __host__ __device__ __forceinline__ void foot_weights(float dist, int raw, float
         // same float).  diff*diff is a float product, then 0.75 - it;
         // 0.5*(1.5-d)^2 rounds once like the reference's double expression
         // (1.5 - d is exact, its square fits in a double).
-        const float d0 = fabsf((float)raw - dist);
-        const float d1 = fabsf((float)(raw + 1) - dist);
-        const float d2 = fabsf((float)(raw + 2) - dist);
-        const float a0 = 1.5f - d0, a2 = 1.5f - d2;
-        C[0] = 0.5f * (a0 * a0);
-        C[1] = 0.75f - d1 * d1;
-        C[2] = 0.5f * (a2 * a2);
+        float q0, q2;
+        tsc_parts(dist, raw, q0, C[1], q2);
+        C[0] = 0.5f * q0;
+        C[2] = 0.5f * q2;
     } else {
         // PCS, MAS_library.pyx:485-492 (double polynomials, rounded to float).
         // With raw = floor(dist) - 1, d0 and d3 lie in [1, 2] and d1, d2 in
@@ -113,7 +124,7 @@ __host__ __device__ __forceinline__ void foot_weights(float dist, int raw, float
         // of the reference give the same double.
         float d[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) d[j] = fabsf((float)(raw + j) - dist);
+        for (int j = 0; j < 4; ++j) d[j] = fabsf(((float)raw + (float)j) - dist);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const double dd = (double)d[j];
@@ -726,8 +737,19 @@ __device__ __forceinline__ void deposit_record(unsigned long long *a, const floa
         }
     } else {
         float Cs[4];                         // last axis' weights in quanta (>= 0)
+        if constexpr (MAS == PYL_TSC && AXES == 3) {
+            // 0.5*q*to_q == q*(0.5*to_q) exactly (power-of-two scaling; a
+            // subnormal q is far below one quantum either way)
+            float q0, q2;
+            tsc_parts(rec.z, first_node<MAS>(rec.z), q0, Cs[1], q2);
+            const float h_q = 0.5f * to_q;
+            Cs[0] = q0 * h_q;
+            Cs[1] *= to_q;
+            Cs[2] = q2 * h_q;
+        } else {
 #pragma unroll
-        for (int k = 0; k < F; ++k) Cs[k] = (AXES == 3 ? Cz[k] : Cy[k]) * to_q;
+            for (int k = 0; k < F; ++k) Cs[k] = (AXES == 3 ? Cz[k] : Cy[k]) * to_q;
+        }
 #pragma unroll
         for (int l = 0; l < F; ++l) {
             if (AXES == 3) {
