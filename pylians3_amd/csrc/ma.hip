@@ -1180,15 +1180,47 @@ tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
     }
 }
 
-constexpr int kFinGroup = 1;     // band cells per thread whose loads are in flight together
+// Band cell t of an equal tile of T = kT3 nodes (t < NA: lx < L; then lx >= L,
+// ly < L; then lx, ly >= L, lz < L; L = F-1) and the partial slot of its image
+// (a, b, c), i.e. halo_index of (lx + a*T, ly + b*T, lz + c*T) in equal tiles,
+// as base + a*dA + b*dB + c*dC (B = T + L box nodes, I = T - L interior and
+// H = 2L halo nodes per axis, SA = H*B*B, SB = I*H*B):
+//   lx < L:           ((lx*B + ly)*B + lz) + a*L*B*B + b*T*B + c*T
+//   lx >= L, ly < L:  SA + ((lx-L)*H + ly)*B + lz + b*L*B + c*T    (a = 0)
+//   lx, ly >= L:      SA + SB + ((lx-L)*I + ly-L)*H + lz + c*L     (a = b = 0)
+// Image (a, b, c) contributes when each of a, b, c is 0 or its coordinate is
+// below L.
+template <int MAS>
+__host__ __device__ __forceinline__ void band_cell(int t, int &lx, int &ly, int &lz, int &base, int &dA,
+                                                   int &dB, int &dC) {
+    constexpr int F = MAS, T = kT3, L = F - 1;
+    constexpr int B = T + L, I = T - L, H = 2 * L, SA = H * B * B, SB = I * H * B;
+    constexpr int NA = L * T * T, NB = (T - L) * L * T;
+    constexpr int LL = L > 0 ? L : 1;                    // NGP has no band cells
+    if (t < NA) {
+        lz = t % T; ly = (t / T) % T; lx = t / (T * T);
+        base = (lx * B + ly) * B + lz;
+        dA = L * B * B; dB = T * B; dC = T;
+    } else if (t < NA + NB) {
+        const int u = t - NA;
+        lz = u % T; ly = (u / T) % LL; lx = L + u / (T * LL);
+        base = SA + ((lx - L) * H + ly) * B + lz;
+        dA = 0; dB = L * B; dC = T;
+    } else {
+        const int u = t - NA - NB;
+        lz = u % LL; ly = L + (u / LL) % (T - L); lx = L + u / (LL * (T - L));
+        base = SA + SB + ((lx - L) * I + (ly - L)) * H + lz;
+        dA = 0; dB = 0; dC = L;
+    }
+}
 
 // The same finalize for 3D grids cut into equal tiles of kT3 nodes with at
 // least two tiles per axis (dims % kT3 == 0, dims > kT3).  Every lower
 // neighbour then has extent kT3, so each band cell (lx, ly, lz) takes image 1
 // along an axis exactly when its coordinate there is below F-1, and the
-// partial indices fold to constants.  The up to 8 loads of a cell are issued
-// before any is summed; the sum runs in the image order of the general kernel
-// (a, b, c nested), so both kernels give the same bits.
+// partial slots take the closed forms of band_cell.  The up to 8 loads of a
+// cell are issued before any is summed; the sum runs in the image order of the
+// general kernel (a, b, c nested), so both kernels give the same bits.
 template <int MAS>
 __global__ void __launch_bounds__(kBlock)
 tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
@@ -1215,54 +1247,41 @@ tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, Til
     }
     __syncthreads();
     grid += real * g.ncell;
-    const long long ox = (long long)tx * T, oy = (long long)ty * T, oz = (long long)tz * T;
-    // the loads of kFinGroup cells of this thread are issued before their sums
-    constexpr int SLOTS = NBAND > 0 ? (NBAND + kBlock - 1) / kBlock : 1;
-    constexpr int G = kFinGroup < SLOTS ? kFinGroup : SLOTS;
-    long long pb[8];
+    const float *pim[8];
+    bool has[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) pb[i] = s_pbase[i];
-    for (int j0 = 0; j0 < SLOTS; j0 += G) {
-        int cl[G][3];
-        float x[G][8];
+    for (int i = 0; i < 8; ++i) {
+        const long long v = s_pbase[i];
+        has[i] = v >= 0;
+        pim[i] = partials + (v >= 0 ? v : 0);
+    }
+    const unsigned long long node0 =
+        (((unsigned long long)tx * T * g.dims + (unsigned long long)ty * T) * g.dims) + (unsigned long long)tz * T;
+    const unsigned dd = (unsigned)g.dims * (unsigned)g.dims;
+    for (int t = threadIdx.x; t < NBAND; t += kBlock) {
+        int lx, ly, lz, base, dA, dB, dC;
+        band_cell<MAS>(t, lx, ly, lz, base, dA, dB, dC);
+        float x[8];
 #pragma unroll
-        for (int jj = 0; jj < G; ++jj) {
-            const int t0 = threadIdx.x + (j0 + jj) * kBlock;
-            const int t = t0 < NBAND ? t0 : 0;
-            int lx, ly, lz;
-            if (t < NA) {
-                lz = t % T; ly = (t / T) % T; lx = t / (T * T);
-            } else if (t < NA + NB) {
-                const int u = t - NA;
-                lz = u % T; ly = (u / T) % L; lx = L + u / (T * L);
-            } else {
-                const int u = t - NA - NB;
-                lz = u % L; ly = L + (u / L) % (T - L); lx = L + u / (L * (T - L));
-            }
-            cl[jj][0] = lx; cl[jj][1] = ly; cl[jj][2] = lz;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
-                const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
-                x[jj][i] = 0.0f;
-                if (t0 < NBAND && on && pb[i] >= 0)
-                    x[jj][i] = partials[pb[i] + halo_index<F, 3>(lx + a * T, ly + b * T, lz + c * T,
-                                                                 T, T, T)];
-            }
+        for (int i = 0; i < 8; ++i) {
+            const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+            const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L) && has[i];
+            x[i] = 0.0f;
+            if (on) x[i] = pim[i][base + a * dA + b * dB + c * dC];
         }
+        // all loads are issued before the first wait: the values are consumed
+        // here, not inside the per-image branches
+        asm volatile("" ::"v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]),
+                     "v"(x[7]));
+        // the sum runs in the image order of the general kernel (a, b, c nested)
+        double v = 0.0;
 #pragma unroll
-        for (int jj = 0; jj < G; ++jj) {
-            if (threadIdx.x + (j0 + jj) * kBlock >= NBAND) continue;
-            const int lx = cl[jj][0], ly = cl[jj][1], lz = cl[jj][2];
-            double v = 0.0;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
-                const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L);
-                if (on && pb[i] >= 0) v += (double)x[jj][i];
-            }
-            store_node(grid, ((ox + lx) * g.dims + (oy + ly)) * g.dims + (oz + lz), (float)v, om);
+        for (int i = 0; i < 8; ++i) {
+            const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+            const bool on = (a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L) && has[i];
+            if (on) v += (double)x[i];
         }
+        store_node(grid, (long long)(node0 + (unsigned)lx * dd + (unsigned)(ly * g.dims + lz)), (float)v, om);
     }
 }
 

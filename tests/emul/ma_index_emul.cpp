@@ -355,7 +355,8 @@ static int check_box_uni(unsigned seed) {
 
 // Uniform 3D tiles: the deposit's region-by-region halo loops (restated from
 // tile_deposit_kernel) visit every halo node of the box once, at its
-// halo_index slot, and the interior loop the remaining nodes.
+// halo_index slot, and the interior loop the remaining nodes; the finalize's
+// band_cell enumerates the band and its partial slots.
 template <int MAS>
 static int check_uni_final() {
     constexpr int F = MAS, T = kT3, B = T + F - 1, H = 2 * (F - 1), I = T - (F - 1);
@@ -385,6 +386,28 @@ static int check_uni_final() {
     }
     for (int c = 0; c < B * B * B; ++c) if (seen[c] != 1) ++bad;
     if (SB + I * I * H != halo_count<F, 3>(T, T, T)) ++bad;
+    // finalize side: band_cell visits every band cell (some coordinate below
+    // F-1) once, and its closed-form slots equal halo_index for every image
+    constexpr int L = F - 1;
+    const int nband = T * T * T - I * I * I;
+    std::vector<int> cell(T * T * T, 0);
+    for (int t = 0; t < nband; ++t) {
+        int lx, ly, lz, base, dA, dB, dC;
+        band_cell<MAS>(t, lx, ly, lz, base, dA, dB, dC);
+        if (lx < 0 || lx >= T || ly < 0 || ly >= T || lz < 0 || lz >= T) { ++bad; continue; }
+        if (!(lx < L || ly < L || lz < L)) ++bad;
+        ++cell[(lx * T + ly) * T + lz];
+        for (int i = 0; i < 8; ++i) {
+            const int a = i >> 2, b = (i >> 1) & 1, c = i & 1;
+            if (!((a == 0 || lx < L) && (b == 0 || ly < L) && (c == 0 || lz < L))) continue;
+            if (base + a * dA + b * dB + c * dC != halo_index<F, 3>(lx + a * T, ly + b * T, lz + c * T, T, T, T))
+                ++bad;
+        }
+    }
+    for (int c = 0; c < T * T * T; ++c) {
+        const int lx = c / (T * T), ly = (c / T) % T, lz = c % T;
+        if (cell[c] != ((lx < L || ly < L || lz < L) ? 1 : 0)) ++bad;
+    }
     printf("uniform final loops MAS %d: %s\n", MAS, bad ? "FAIL" : "ok");
     return bad;
 }
