@@ -842,7 +842,8 @@ __global__ void __launch_bounds__(kDepBlock)
 tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                     const unsigned long long *__restrict__ offsets, TileGeom g,
                     float weight_factor, const unsigned *__restrict__ wmax_bits,
-                    float *__restrict__ grid, float *__restrict__ partials, OutMode om) {
+                    float *__restrict__ grid, float *__restrict__ partials, OutMode om,
+                    const unsigned *__restrict__ order) {
     constexpr int F = MAS;
     constexpr int T = (AXES == 3) ? kT3 : kT2;
     constexpr int BY = T + F - 1;
@@ -857,7 +858,7 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     __shared__ unsigned short sidx[BAL ? NW * WB : 1];    // dealt record indices
     __shared__ unsigned char htab[BAL ? NW * WB : 1];      // hole table
     __shared__ unsigned whist[BAL ? NW * 16 : 1];          // class counters
-    const int key = blockIdx.x;                       // r * ntiles + tile
+    const int key = order ? (int)order[blockIdx.x] : (int)blockIdx.x;   // r * ntiles + tile
     const int real = key / (int)g.ntiles;
     const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
     grid += real * g.ncell;
@@ -1069,6 +1070,79 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
             store_node(grid, node, v, om);
         } else {
             part[halo_index<F, AXES>(bx, by, bz, ex, ey, ez)] = v;
+        }
+    }
+}
+
+// Deposit order: keys by decreasing record count (quarter-octave buckets), so
+// that the largest tiles start first and the smallest fill the end; in key order
+// the big tiles of a dense region can start last and leave most CUs idle while
+// they finish.  One workgroup; the order within a bucket is arbitrary, and no
+// result depends on it (every tile's sums are its own integers).
+constexpr int kOrderBlock = 1024;
+constexpr int kOrderBuckets = 512;
+constexpr long long kOrderMinKeys = 4096;
+
+// 16 buckets per octave of the record count (0 alone in bucket 0); distinct
+// buckets also spread the LDS histogram's atomics over more addresses
+__device__ __forceinline__ int count_bucket(unsigned long long n) {
+    if (n == 0) return 0;
+    const int lg = 63 - __clzll((long long)n);
+    const int q = lg >= 4 ? (int)((n >> (lg - 4)) & 15ull) : (int)((n << (4 - lg)) & 15ull);
+    return min(kOrderBuckets - 1, 1 + 16 * lg + q);
+}
+
+__global__ void __launch_bounds__(kOrderBlock)
+tile_order_kernel(const unsigned long long *__restrict__ offsets, long long nt, unsigned *__restrict__ order) {
+    constexpr int U = 8;                                // keys per thread in flight
+    __shared__ unsigned hist[kOrderBuckets];
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) hist[b] = 0u;
+    __syncthreads();
+    const long long step = (long long)blockDim.x * U;
+    for (long long k0 = 0; k0 < nt; k0 += step) {
+        unsigned long long o0[U], o1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = min(k0 + threadIdx.x + (long long)u * blockDim.x, nt - 1);
+            o0[u] = offsets[k];
+            o1[u] = offsets[k + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = k0 + threadIdx.x + (long long)u * blockDim.x;
+            if (k < nt) atomicAdd(&hist[count_bucket(o1[u] - o0[u])], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {                             // starts, largest bucket first
+        constexpr int PB = kOrderBuckets / 64;          // buckets per lane
+        const int top = kOrderBuckets - 1 - (int)threadIdx.x * PB;
+        unsigned h[PB], sum = 0;
+#pragma unroll
+        for (int j = 0; j < PB; ++j) { h[j] = hist[top - j]; sum += h[j]; }
+        unsigned incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned t = __shfl_up(incl, o);
+            if ((int)threadIdx.x >= o) incl += t;
+        }
+        unsigned run = incl - sum;
+#pragma unroll
+        for (int j = 0; j < PB; ++j) { hist[top - j] = run; run += h[j]; }
+    }
+    __syncthreads();
+    for (long long k0 = 0; k0 < nt; k0 += step) {
+        unsigned long long o0[U], o1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = min(k0 + threadIdx.x + (long long)u * blockDim.x, nt - 1);
+            o0[u] = offsets[k];
+            o1[u] = offsets[k + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = k0 + threadIdx.x + (long long)u * blockDim.x;
+            if (k < nt) order[atomicAdd(&hist[count_bucket(o1[u] - o0[u])], 1u)] = (unsigned)k;
         }
     }
 }
@@ -1438,9 +1512,10 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const size_t bs_bytes = ((size_t)nscan_blocks * sizeof(unsigned long long) + 255) & ~(size_t)255;
     const size_t ch_bytes = ((size_t)std::max(nchunk, 1) * sizeof(Chunk) + 255) & ~(size_t)255;
     const size_t org_bytes = ((size_t)(g.nt + 1) * sizeof(int) + 255) & ~(size_t)255;
+    const size_t ord_bytes = ((size_t)nt * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t dump_bytes = (size_t)std::max(nchunk, 1) * kWin * sizeof(unsigned);
     PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
-                                        dump_bytes));
+                                        ord_bytes + dump_bytes));
     char *tb = (char *)c->bufs[BUF_TILE_CNT].ptr;
     unsigned *counts = (unsigned *)tb;
     unsigned *ocounts = counts + nt;
@@ -1450,7 +1525,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     unsigned long long *bsums = (unsigned long long *)(tb + cnt_bytes + off_bytes);
     Chunk *dchunks = (Chunk *)(tb + cnt_bytes + off_bytes + bs_bytes);
     int *dorig = (int *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes);
-    unsigned *dump = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes);
+    unsigned *order = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes);
+    unsigned *dump = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes + ord_bytes);
     static thread_local std::vector<int> orig;
     orig.assign(g.nt + 1, 0);
     for (int i = 0; i <= g.nt; ++i) orig[i] = tile_origin(i, g.nt, dims);
@@ -1482,6 +1558,9 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
     scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
                                                                    offsets);
+    // largest tiles first, when there are enough tiles for the dispatch tail to matter
+    const bool ordered = nt >= kOrderMinKeys;
+    if (ordered) tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
     // offsets[nt] must equal n; it is read back asynchronously and checked at the
@@ -1529,7 +1608,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                        : (W ? tile_deposit_kernel<MAS, 2, true, true, false>
                             : tile_deposit_kernel<MAS, 2, true, false, false>)));
     kd<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(records, rcap, offsets, g, weight_factor,
-                                                 W ? wmax : nullptr, grid, partials, om);
+                                                 W ? wmax : nullptr, grid, partials, om,
+                                                 ordered ? order : nullptr);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));
     if (MAS > 1) {
