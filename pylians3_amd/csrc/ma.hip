@@ -1082,6 +1082,7 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
 constexpr int kOrderBlock = 1024;
 constexpr int kOrderBuckets = 512;
 constexpr long long kOrderMinKeys = 4096;
+constexpr long long kOrderMinRecords = 512;      // mean records per key
 
 // 16 buckets per octave of the record count (0 alone in bucket 0); distinct
 // buckets also spread the LDS histogram's atomics over more addresses
@@ -1558,8 +1559,9 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
     scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
                                                                    offsets);
-    // largest tiles first, when there are enough tiles for the dispatch tail to matter
-    const bool ordered = nt >= kOrderMinKeys;
+    // largest tiles first, when there are enough tiles, and records per tile, for the
+    // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys)
+    const bool ordered = nt >= kOrderMinKeys && n >= kOrderMinRecords * nt;
     if (ordered) tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
