@@ -292,7 +292,10 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
     constexpr int S = Spec<NFL>::S;
     constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);   // per 3D bin: P0,P2,P4 per spectrum [+phase]
     extern __shared__ double lds[];
-    double *acc3 = lds;                               // [(kmax+1)*NA3]
+    // acc3 is [NA3][kmax+1] in LDS (consecutive bins in consecutive banks: lanes
+    // flushing neighbouring bins do not collide), [kmax+1][NA3] in the partials
+    double *acc3 = lds;                               // [NA3*(kmax+1)]
+    const int K1 = a.kmax + 1;
     double *line2 = acc3 + (a.kmax + 1) * NA3;        // [L2*S]
     double *line1 = line2 + a.L2 * S;                 // [L1*S]
     double *corrz = line1 + a.L1 * S;                 // [zlen*NFL]
@@ -414,7 +417,7 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
             if (kidx != cur3) {
                 if (cur3 >= 0) {
 #pragma unroll
-                    for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[cur3 * NA3 + i], r3[i]);
+                    for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[i * K1 + cur3], r3[i]);
                 }
                 cur3 = kidx;
 #pragma unroll
@@ -461,7 +464,7 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
         }
         if (cur3 >= 0) {
 #pragma unroll
-            for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[cur3 * NA3 + i], r3[i]);
+            for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[i * K1 + cur3], r3[i]);
         }
         if (AXIS != 2 && cur2 >= 0) {
 #pragma unroll
@@ -478,7 +481,7 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
     }
     __syncthreads();
     double *out = a.partials + ((long long)blockIdx.y * a.nitems + blockIdx.x) * a.P;
-    for (int i = tid; i < n3; i += blockDim.x) out[i] = acc3[i];
+    for (int i = tid; i < n3; i += blockDim.x) out[i] = acc3[(i % NA3) * K1 + i / NA3];
     for (int i = tid; i < a.L2 * S; i += blockDim.x) out[a.off2 + i] = line2[i];
     for (int i = tid; i < a.L1 * S; i += blockDim.x) out[a.off1 + i] = line1[i];
 }
