@@ -836,7 +836,8 @@ __host__ __device__ __forceinline__ int deal_record_slot(unsigned cls, unsigned 
 // class (LDS counters), deals them to rounds as described at deal_slot, stages
 // the dealt record indices (2 B each) in LDS and reloads the records (from
 // cache).  The sums are integers, so the order of the atomics does not change
-// any bit of the result.
+// any bit of the result.  With `order` (tile_order_kernel), workgroup i takes
+// key order[i] instead of key i.
 template <int MAS, int AXES, bool WT, bool RW, bool UNI>
 __global__ void __launch_bounds__(kDepBlock)
 tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
