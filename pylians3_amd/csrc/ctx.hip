@@ -43,10 +43,12 @@ int ensure(Ctx *c, BufId id, size_t bytes) {
     hipError_t e = hipMalloc(&b.ptr, need);
     if (e != hipSuccess) {
         b.ptr = nullptr;
+        b.gen = 0;
         set_error("hipMalloc(%zu) failed: %s", need, hipGetErrorString(e));
         return PYL_ENOMEM;
     }
     b.cap = need;
+    b.gen = ++c->alloc_gen;
     return PYL_OK;
 }
 

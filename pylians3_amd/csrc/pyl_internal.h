@@ -51,6 +51,7 @@ void set_error(const char *fmt, ...);
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
+    unsigned long long gen = 0;   // allocation number (Ctx::alloc_gen) of ptr; 0: none
 };
 
 // Pinned host staging (hipHostMalloc): small per-call uploads and result
@@ -118,6 +119,15 @@ struct Ctx {
     std::map<std::tuple<int, int, int>, BinPlan> bin_plans; // (dims, axis, rows per item)
     std::map<int, void *> twiddles;                          // exp(-2 pi i k/N), per N
     std::map<std::pair<int, int>, void *> mas_tabs;          // MAS correction table per (dims, p)
+    unsigned long long alloc_gen = 0;                        // device allocations so far
+    // The chunk table and tile origins last uploaded into BUF_TILE_CNT, and what they
+    // were built from: a call with the same particle ranges and grid reuses them.
+    struct TileTables {
+        unsigned long long gen = 0;          // BUF_TILE_CNT allocation they live in
+        long long n = -1, nchunk = -1;
+        int nreal = 0, dims = 0, axes = 0, mas = 0;
+        std::vector<long long> offs;         // realisation offsets (empty: one realisation)
+    } tile_tables;
     PinnedBuf pin_up;                                        // MA chunk table + tile origins
     PinnedBuf pin_down;                                      // Pk bin sums
     // Tile-sort consistency check of the latest tiled MA, read back without a
