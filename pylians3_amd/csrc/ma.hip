@@ -1532,7 +1532,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     PYL_TRY(ensure(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float)));
     float *partials = (float *)c->bufs[BUF_PARTIAL].ptr;
 
-    PYL_HIP(hipMemsetAsync(counts, 0, nwords * sizeof(unsigned), c->stream));
+    // the whole 256-byte-rounded region: one aligned fill instead of a body and a tail
+    PYL_HIP(hipMemsetAsync(counts, 0, cnt_bytes, c->stream));
     // Chunk table and tile origins: the kernels only read their particle ranges and
     // origins (the count pass rewrites each chunk's window fields), so a call with
     // the same ranges and grid in the same allocation reuses the uploaded tables.
