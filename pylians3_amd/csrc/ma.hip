@@ -1087,9 +1087,9 @@ constexpr long long kOrderMinRecords = 512;      // mean records per key
 
 // 16 buckets per octave of the record count (0 alone in bucket 0); distinct
 // buckets also spread the LDS histogram's atomics over more addresses
-__device__ __forceinline__ int count_bucket(unsigned long long n) {
+__host__ __device__ __forceinline__ int count_bucket(unsigned long long n) {
     if (n == 0) return 0;
-    const int lg = 63 - __clzll((long long)n);
+    const int lg = 63 - __builtin_clzll(n);           // floor(log2 n), n > 0
     const int q = lg >= 4 ? (int)((n >> (lg - 4)) & 15ull) : (int)((n << (4 - lg)) & 15ull);
     return min(kOrderBuckets - 1, 1 + 16 * lg + q);
 }

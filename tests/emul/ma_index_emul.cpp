@@ -412,8 +412,26 @@ static int check_uni_final() {
     return bad;
 }
 
+// Deposit order buckets: non-decreasing in the record count (so larger tiles
+// never sort after smaller ones), 0 only for empty tiles, within the table.
+static int check_count_bucket() {
+    int bad = 0, prev = 0;
+    for (unsigned long long n = 0; n < (1ull << 22); ++n) {
+        const int b = count_bucket(n);
+        if (b < prev || b < 0 || b >= kOrderBuckets || (b == 0) != (n == 0)) ++bad;
+        prev = b;
+    }
+    for (int s = 22; s < 64; ++s) {
+        const int b = count_bucket(1ull << s);
+        if (b < prev || b >= kOrderBuckets) ++bad;
+        prev = b;
+    }
+    printf("count buckets: %s\n", bad ? "FAIL" : "ok");
+    return bad;
+}
+
 int main() {
-    int bad = check_dealing<4>(11) + check_dealing<2>(12);
+    int bad = check_dealing<4>(11) + check_dealing<2>(12) + check_count_bucket();
     bad += check_uni_final<PYL_NGP>() + check_uni_final<PYL_CIC>() + check_uni_final<PYL_TSC>() +
            check_uni_final<PYL_PCS>();
     bad += check_box_uni<PYL_CIC>(21) + check_box_uni<PYL_TSC>(22) + check_box_uni<PYL_PCS>(23);
