@@ -824,6 +824,92 @@ __host__ __device__ __forceinline__ int deal_record_slot(unsigned cls, unsigned 
     return deal_slot<KR>(e >> 4, e & 15u);
 }
 
+// Quantum 2^q of a realisation's box sums: footprint weights are <= 1, so
+// |contribution| <= weight_factor * max finite |W| of the realisation < 2^e.
+// With q = e - 30 every rounded contribution is an int32 and every box sum an
+// int64 for any count below 2^33: exact integer atomics, independent of their
+// order.
+template <bool WT>
+__device__ __forceinline__ void box_quantum(const unsigned *__restrict__ wmax_bits, int real,
+                                            float weight_factor, float &to_q, double &from_q) {
+    const float wmax = (WT && wmax_bits) ? __uint_as_float(wmax_bits[real]) : 1.0f;
+    const double bound = (double)wmax * (double)weight_factor;
+    int e = -1000;
+    if (bound > 0.0) frexp(bound, &e);
+    int q = e - 30;
+    if (q < -127) q = -127;                 // keep 2^-q a normal float
+    if (q > 127) q = 127;
+    to_q = ldexpf(1.0f, -q);
+    from_q = ldexp(1.0, q);
+}
+
+// Writes a tile's box of integer sums (LDS or global): the nodes no other tile
+// touches go to the grid (grid already offset to the realisation), the band and
+// halo to the tile's partial slots (halo_index order).
+template <int MAS, int AXES, bool UNI>
+__device__ __forceinline__ void flush_box(const unsigned long long *acc, int key, const TileBox &tb,
+                                          const TileGeom &g, float *__restrict__ grid,
+                                          float *__restrict__ partials, const OutMode &om,
+                                          double from_q, int ex, int ey, int ez) {
+    constexpr int F = MAS;
+    constexpr int T = (AXES == 3) ? kT3 : kT2;
+    constexpr int BY = T + F - 1;
+    constexpr int BZ = (AXES == 3) ? T + F - 1 : 1;
+    constexpr int BX = T + F - 1;
+    constexpr int NB = BX * BY * BZ;
+    float *part = partials + (long long)key * g.pslot;
+    if constexpr (UNI && AXES == 3) {
+        // Uniform tile: the interior nodes go to the grid, the halo to the
+        // partial slots in halo_index order, region by region (x outside the
+        // interior: whole planes; x inside, y outside: rows of B; x, y inside, z
+        // outside: H nodes), so that every loop has constant extents and the
+        // partial stores are contiguous.
+        constexpr int B = BX, H = 2 * (F - 1), I = T - (F - 1);
+        const unsigned long long node0 =
+            ((unsigned long long)tb.ox * g.dims + tb.oy) * g.dims + tb.oz;
+        const unsigned dd = (unsigned)g.dims * (unsigned)g.dims;
+        for (int i = threadIdx.x; i < I * I * I; i += blockDim.x) {
+            const int x = F - 1 + i / (I * I), y = F - 1 + (i / I) % I, z = F - 1 + i % I;
+            const float v = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
+            store_node(grid, (long long)(node0 + (unsigned)x * dd + (unsigned)(y * g.dims + z)), v, om);
+        }
+        for (int i = threadIdx.x; i < H * B * B; i += blockDim.x) {
+            const int xi = i / (B * B), yz = i - xi * (B * B);
+            const int x = xi < F - 1 ? xi : xi + I;
+            part[i] = (float)((double)(long long)acc[x * B * B + yz] * from_q);
+        }
+        constexpr int SA = H * B * B;
+        for (int i = threadIdx.x; i < I * H * B; i += blockDim.x) {
+            const int x = F - 1 + i / (H * B), r = i % (H * B), yi = r / B, z = r - yi * B;
+            const int y = yi < F - 1 ? yi : yi + I;
+            part[SA + i] = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
+        }
+        constexpr int SB = SA + I * H * B;
+        for (int i = threadIdx.x; i < I * I * H; i += blockDim.x) {
+            const int x = F - 1 + i / (I * H), r = i % (I * H), y = F - 1 + r / H, zi = r % H;
+            const int z = zi < F - 1 ? zi : zi + I;
+            part[SB + i] = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
+        }
+        return;
+    }
+    for (int c = threadIdx.x; c < NB; c += blockDim.x) {
+        const int bz = c % BZ, by = (c / BZ) % BY, bx = c / (BZ * BY);
+        if (!UNI && (bx >= ex + F - 1 || by >= ey + F - 1 || (AXES == 3 && bz >= ez + F - 1)))
+            continue;
+        const float v = (float)((double)(long long)acc[c] * from_q);
+        const bool inner = bx >= F - 1 && bx < ex && by >= F - 1 && by < ey &&
+                           (AXES == 2 || (bz >= F - 1 && bz < ez));
+        if (inner) {
+            const long long node = AXES == 3
+                ? ((long long)(tb.ox + bx) * g.dims + (tb.oy + by)) * g.dims + (tb.oz + bz)
+                : (long long)(tb.ox + bx) * g.dims + (tb.oy + by);
+            store_node(grid, node, v, om);
+        } else {
+            part[halo_index<F, AXES>(bx, by, bz, ex, ey, ez)] = v;
+        }
+    }
+}
+
 // One workgroup per tile.  LDS box of BX*BY*BZ 64-bit integers (the tile plus
 // the upper halo) holding multiples of 2^q; every record adds its whole
 // footprint at fixed offsets from its first node.  WT = false: unit weights
@@ -844,7 +930,8 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                     const unsigned long long *__restrict__ offsets, TileGeom g,
                     float weight_factor, const unsigned *__restrict__ wmax_bits,
                     float *__restrict__ grid, float *__restrict__ partials, OutMode om,
-                    const unsigned *__restrict__ order) {
+                    const unsigned *__restrict__ order, int nsplit,
+                    unsigned long long *__restrict__ gbox) {
     constexpr int F = MAS;
     constexpr int T = (AXES == 3) ? kT3 : kT2;
     constexpr int BY = T + F - 1;
@@ -859,12 +946,22 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     __shared__ unsigned short sidx[BAL ? NW * WB : 1];    // dealt record indices
     __shared__ unsigned char htab[BAL ? NW * WB : 1];      // hole table
     __shared__ unsigned whist[BAL ? NW * 16 : 1];          // class counters
-    const int key = order ? (int)order[blockIdx.x] : (int)blockIdx.x;   // r * ntiles + tile
+    // key r * ntiles + tile; with nsplit > 1, workgroup i takes part i % nsplit of
+    // key i / nsplit's records and adds its box into gbox (tile_box_flush_kernel
+    // writes the grid)
+    const int key = nsplit > 1 ? (int)(blockIdx.x / (unsigned)nsplit)
+                               : (order ? (int)order[blockIdx.x] : (int)blockIdx.x);
     const int real = key / (int)g.ntiles;
     const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
     grid += real * g.ncell;
     // record range, clamped to the buffer (a miscount is reported by the host)
-    const unsigned long long r1 = min(offsets[key + 1], rcap), r0 = min(offsets[key], r1);
+    unsigned long long r1 = min(offsets[key + 1], rcap), r0 = min(offsets[key], r1);
+    if (nsplit > 1) {
+        const unsigned long long part = (r1 - r0 + nsplit - 1) / nsplit;
+        r0 = min(r0 + part * (blockIdx.x % (unsigned)nsplit), r1);
+        r1 = min(r0 + part, r1);
+        if (r0 == r1) return;                     // no records in this part
+    }
     // UNI: every tile has extent T (dims % T == 0, two tiles or more per axis)
     const int ex = UNI ? T : tb.ex, ey = UNI ? T : tb.ey, ez = UNI ? (AXES == 3 ? T : 1) : tb.ez;
 
@@ -899,19 +996,9 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     if constexpr (BAL) load_set(r0 + (unsigned long long)wv * WB, nxt);
     for (int c = threadIdx.x; c < NB; c += blockDim.x) acc[c] = 0ull;
 
-    // Quantum 2^q: footprint weights are <= 1, so |contribution| <=
-    // weight_factor * max finite |W| of the realisation < 2^e.  With q = e - 30
-    // every rounded contribution is an int32 and every box sum an int64 for
-    // any count below 2^33: exact integer atomics, independent of their order.
-    const float wmax = (WT && wmax_bits) ? __uint_as_float(wmax_bits[real]) : 1.0f;
-    const double bound = (double)wmax * (double)weight_factor;
-    int e = -1000;
-    if (bound > 0.0) frexp(bound, &e);
-    int q = e - 30;
-    if (q < -127) q = -127;                 // keep 2^-q a normal float
-    if (q > 127) q = 127;
-    const float to_q = ldexpf(1.0f, -q);
-    const double from_q = ldexp(1.0, q);
+    float to_q;
+    double from_q;
+    box_quantum<WT>(wmax_bits, real, weight_factor, to_q, from_q);
     __syncthreads();
 
     if constexpr (!BAL) {
@@ -1022,57 +1109,36 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
         }
     }
     __syncthreads();
-    float *part = partials + (long long)key * g.pslot;
-    if constexpr (UNI && AXES == 3) {
-        // Uniform tile: the interior nodes go to the grid, the halo to the
-        // partial slots in halo_index order, region by region (x outside the
-        // interior: whole planes; x inside, y outside: rows of B; x, y inside, z
-        // outside: H nodes), so that every loop has constant extents and the
-        // partial stores are contiguous.
-        constexpr int B = BX, H = 2 * (F - 1), I = T - (F - 1);
-        const unsigned long long node0 =
-            ((unsigned long long)tb.ox * g.dims + tb.oy) * g.dims + tb.oz;
-        const unsigned dd = (unsigned)g.dims * (unsigned)g.dims;
-        for (int i = threadIdx.x; i < I * I * I; i += blockDim.x) {
-            const int x = F - 1 + i / (I * I), y = F - 1 + (i / I) % I, z = F - 1 + i % I;
-            const float v = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
-            store_node(grid, (long long)(node0 + (unsigned)x * dd + (unsigned)(y * g.dims + z)), v, om);
-        }
-        for (int i = threadIdx.x; i < H * B * B; i += blockDim.x) {
-            const int xi = i / (B * B), yz = i - xi * (B * B);
-            const int x = xi < F - 1 ? xi : xi + I;
-            part[i] = (float)((double)(long long)acc[x * B * B + yz] * from_q);
-        }
-        constexpr int SA = H * B * B;
-        for (int i = threadIdx.x; i < I * H * B; i += blockDim.x) {
-            const int x = F - 1 + i / (H * B), r = i % (H * B), yi = r / B, z = r - yi * B;
-            const int y = yi < F - 1 ? yi : yi + I;
-            part[SA + i] = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
-        }
-        constexpr int SB = SA + I * H * B;
-        for (int i = threadIdx.x; i < I * I * H; i += blockDim.x) {
-            const int x = F - 1 + i / (I * H), r = i % (I * H), y = F - 1 + r / H, zi = r % H;
-            const int z = zi < F - 1 ? zi : zi + I;
-            part[SB + i] = (float)((double)(long long)acc[(x * B + y) * B + z] * from_q);
+    if (nsplit > 1) {
+        // integer sums: the parts of a tile add up exactly in any order
+        unsigned long long *gb = gbox + (long long)key * NB;
+        for (int c = threadIdx.x; c < NB; c += blockDim.x) {
+            const unsigned long long v = acc[c];
+            if (v) atomicAdd(&gb[c], v);
         }
         return;
     }
-    for (int c = threadIdx.x; c < NB; c += blockDim.x) {
-        const int bz = c % BZ, by = (c / BZ) % BY, bx = c / (BZ * BY);
-        if (!UNI && (bx >= ex + F - 1 || by >= ey + F - 1 || (AXES == 3 && bz >= ez + F - 1)))
-            continue;
-        const float v = (float)((double)(long long)acc[c] * from_q);
-        const bool inner = bx >= F - 1 && bx < ex && by >= F - 1 && by < ey &&
-                           (AXES == 2 || (bz >= F - 1 && bz < ez));
-        if (inner) {
-            const long long node = AXES == 3
-                ? ((long long)(tb.ox + bx) * g.dims + (tb.oy + by)) * g.dims + (tb.oz + bz)
-                : (long long)(tb.ox + bx) * g.dims + (tb.oy + by);
-            store_node(grid, node, v, om);
-        } else {
-            part[halo_index<F, AXES>(bx, by, bz, ex, ey, ez)] = v;
-        }
-    }
+    flush_box<MAS, AXES, UNI>(acc, key, tb, g, grid, partials, om, from_q, ex, ey, ez);
+}
+
+// Split deposit, second pass: one workgroup per key writes the summed box of its
+// parts (all keys, also empty ones: their box is zero).
+template <int MAS, int AXES, bool WT, bool UNI>
+__global__ void __launch_bounds__(kDepBlock)
+tile_box_flush_kernel(const unsigned long long *__restrict__ gbox, TileGeom g, float weight_factor,
+                      const unsigned *__restrict__ wmax_bits, float *__restrict__ grid,
+                      float *__restrict__ partials, OutMode om) {
+    constexpr int T = (AXES == 3) ? kT3 : kT2;
+    constexpr int NB = (T + MAS - 1) * (T + MAS - 1) * ((AXES == 3) ? T + MAS - 1 : 1);
+    const int key = (int)blockIdx.x;
+    const int real = key / (int)g.ntiles;
+    const TileBox tb = tile_box<AXES>(key - real * (int)g.ntiles, g);
+    const int ex = UNI ? T : tb.ex, ey = UNI ? T : tb.ey, ez = UNI ? (AXES == 3 ? T : 1) : tb.ez;
+    float to_q;
+    double from_q;
+    box_quantum<WT>(wmax_bits, real, weight_factor, to_q, from_q);
+    flush_box<MAS, AXES, UNI>(gbox + (long long)key * NB, key, tb, g, grid + real * g.ncell, partials, om,
+                              from_q, ex, ey, ez);
 }
 
 // Deposit order: keys by decreasing record count (quarter-octave buckets), so
@@ -1084,6 +1150,12 @@ constexpr int kOrderBlock = 1024;
 constexpr int kOrderBuckets = 512;
 constexpr long long kOrderMinKeys = 4096;
 constexpr long long kOrderMinRecords = 512;      // mean records per key
+// Split deposit: below kSplitMaxKeys tiles, a tile holding more than
+// kSplitRecords records per workgroup on average is shared by up to kSplitMax
+// workgroups.
+constexpr long long kSplitMaxKeys = 2048;
+constexpr long long kSplitRecords = 16384;
+constexpr long long kSplitMax = 256;
 
 // 16 buckets per octave of the record count (0 alone in bucket 0); distinct
 // buckets also spread the LDS histogram's atomics over more addresses
@@ -1585,9 +1657,15 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
     scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
                                                                    offsets);
-    // largest tiles first, when there are enough tiles, and records per tile, for the
-    // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys)
-    const bool ordered = nt >= kOrderMinKeys && n >= kOrderMinRecords * nt;
+    // Few tiles with many records each (a small grid and many particles): each
+    // tile's records are split over nsplit workgroups, which add their boxes into
+    // per-tile int64 boxes, and a second pass writes the grid.  Otherwise, largest
+    // tiles first, when there are enough tiles, and records per tile, for the
+    // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys).
+    int nsplit = 1;
+    if (nt < kSplitMaxKeys && n > kSplitRecords * nt)
+        nsplit = (int)std::min<long long>(kSplitMax, (n + kSplitRecords * nt - 1) / (kSplitRecords * nt));
+    const bool ordered = nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt;
     if (ordered) tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
@@ -1635,10 +1713,28 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                : (unit ? tile_deposit_kernel<MAS, 2, false, false, false>
                        : (W ? tile_deposit_kernel<MAS, 2, true, true, false>
                             : tile_deposit_kernel<MAS, 2, true, false, false>)));
-    kd<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(records, rcap, offsets, g, weight_factor,
-                                                 W ? wmax : nullptr, grid, partials, om,
-                                                 ordered ? order : nullptr);
+    unsigned long long *gbox = nullptr;
+    if (nsplit > 1) {
+        const int T = axes == 3 ? kT3 : kT2;
+        const size_t nbox = (size_t)(T + MAS - 1) * (T + MAS - 1) * (axes == 3 ? T + MAS - 1 : 1);
+        PYL_TRY(ensure(c, BUF_TILE_BOX, (size_t)nt * nbox * sizeof(unsigned long long)));
+        gbox = (unsigned long long *)c->bufs[BUF_TILE_BOX].ptr;
+        PYL_HIP(hipMemsetAsync(gbox, 0, (size_t)nt * nbox * sizeof(unsigned long long), c->stream));
+    }
+    kd<<<(unsigned)(nt * nsplit), kDepBlock, 0, c->stream>>>(records, rcap, offsets, g, weight_factor,
+                                                            W ? wmax : nullptr, grid, partials, om,
+                                                            ordered ? order : nullptr, nsplit, gbox);
     PYL_HIP(hipGetLastError());
+    if (nsplit > 1) {
+        auto kf = (axes == 3)
+            ? (uni ? (unit ? tile_box_flush_kernel<MAS, 3, false, true> : tile_box_flush_kernel<MAS, 3, true, true>)
+                   : (unit ? tile_box_flush_kernel<MAS, 3, false, false> : tile_box_flush_kernel<MAS, 3, true, false>))
+            : (uni ? (unit ? tile_box_flush_kernel<MAS, 2, false, true> : tile_box_flush_kernel<MAS, 2, true, true>)
+                   : (unit ? tile_box_flush_kernel<MAS, 2, false, false> : tile_box_flush_kernel<MAS, 2, true, false>));
+        kf<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(gbox, g, weight_factor, W ? wmax : nullptr, grid,
+                                                      partials, om);
+        PYL_HIP(hipGetLastError());
+    }
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));
     if (MAS > 1) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_FINAL));

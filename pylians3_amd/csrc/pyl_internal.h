@@ -104,6 +104,7 @@ enum BufId {
     BUF_MAS_TAB,     // MAS correction tables
     BUF_VEL,
     BUF_PARTIAL,     // per-tile halo partial sums of the tiled deposit
+    BUF_TILE_BOX,    // per-tile int64 boxes of a split deposit
     BUF_COUNT_
 };
 

@@ -205,6 +205,26 @@ def test_ma_ordered_tiles_vs_oracle(oracle, mas):
     assert np.array_equal(g, _gpu_ma(pos, dims, box, mas))
 
 
+@pytest.mark.parametrize('mas', SCHEMES)
+@pytest.mark.parametrize('dims,coord', [(32, 3), (40, 3), (128, 2)])
+def test_ma_split_tiles_vs_oracle(oracle, mas, dims, coord):
+    # few tiles with many records each (32^3: 8 tiles, 40^3: 27 unequal tiles, 128^2:
+    # 4 tiles of 64^2): each tile's records are shared by several workgroups whose
+    # integer boxes add up in a second pass; every cell matches the oracle, with and
+    # without weights, and the result is bitwise reproducible
+    print('case split tiles', mas, dims, coord, flush=True)
+    box, n = 100.0, 1_000_000
+    pos = _pos(n, coord, box, 91 + dims)
+    W = np.random.default_rng(5).lognormal(0.0, 1.0, n).astype(np.float32)
+    for w in (None, W):
+        ref = np.zeros((dims,) * coord, np.float32)
+        oracle.ma_f32(pos, ref, box, mas, w)
+        g = _gpu_ma(pos, dims, box, mas, w)
+        assert_cells(g, ref, what='split %s %d %s' % (mas, dims, w is not None),
+                     nadds=contrib_counts(pos, dims, box, mas))
+        assert np.array_equal(g, _gpu_ma(pos, dims, box, mas, w))
+
+
 def test_cic_interp_vs_oracle(oracle):
     import torch
     from pylians3_amd import MAS_library as MASL
