@@ -282,18 +282,31 @@ def device_ptr(t):
 
 
 class torch_stream:
-    """Run library work on torch's current stream for the duration of a call."""
+    """Order library work after torch's queued work for the duration of a call.
+
+    A side stream of torch is handed to the context, so the call is queued on it.
+    Torch's default stream has the handle 0, which the context reads as "own
+    stream" (a non-blocking stream, not ordered with it): then torch's queued work
+    is waited for on entry, and the context's on exit, so that inputs written just
+    before the call are complete and outputs are complete after it."""
 
     def __init__(self, ctx, active):
         self.ctx, self.active = ctx, active
+        self.legacy = False
 
     def __enter__(self):
         if self.active:
             import torch
-            self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+            s = torch.cuda.current_stream()
+            self.legacy = not s.cuda_stream
+            if self.legacy:
+                s.synchronize()
+            self.ctx.set_stream(s.cuda_stream)
         return self
 
     def __exit__(self, *exc):
         if self.active:
             self.ctx.set_stream(None)
+            if self.legacy and exc[0] is None:
+                self.ctx.synchronize()
         return False
