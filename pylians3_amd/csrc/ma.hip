@@ -223,6 +223,8 @@ __host__ __device__ __forceinline__ int halo_index(int x, int y, int z, int ex, 
 // atomics only.
 constexpr int kSortBlock = 1024;
 constexpr int kWin = 16384;                    // 64 KB of LDS
+constexpr int kWinWide = 32768;                // 128 KB of LDS (a gfx950 workgroup may hold 160 KB)
+constexpr bool kUseWide = true;
 
 struct Chunk {
     long long c0, c1;   // particle range
@@ -234,6 +236,9 @@ struct Chunk {
 
 // Key windows of a sort chunk (TileGeom::wmode, chosen on the host):
 //  kWinDirect  ntiles <= kWin: slot = key - rbase, every key counted in LDS;
+//              also the wide window (kWinWide slots, 3D grids of up to 32768
+//              tiles), taken with chunks of 4-16 particles per tile so that
+//              particles in any order are counted in LDS;
 //  kWinBox     the 8 x 8 columns of tiles (all z) starting 4 tiles below the
 //              chunk's middle particle along x and y, periodic: slot =
 //              (dx * 8 + dy) * ntz + tz, 64 * ntz <= kWin slots.  A lattice or
@@ -359,12 +364,12 @@ __device__ __forceinline__ long long win_key(int s, const Win &w, const TileGeom
 constexpr int kUnroll = 4;       // count pass (measured best of 2, 4, 8 at 512^3)
 constexpr int kUnrollS = 8;      // scatter pass (8: 0.80 -> 0.71 ms at 512^3)
 
-template <int MAS, int AXES, bool POW2>
+template <int MAS, int AXES, bool POW2, int WS>
 __global__ void __launch_bounds__(kSortBlock)
 tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
                   Chunk *__restrict__ chunks, unsigned *__restrict__ counts,
                   unsigned *__restrict__ ocounts, unsigned *__restrict__ dump) {
-    __shared__ unsigned hist[kWin];
+    __shared__ unsigned hist[WS];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
     // every thread derives the window from the chunk's middle particle
@@ -394,8 +399,8 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
         }
     }
     __syncthreads();
-    unsigned *dmp = dump + (long long)blockIdx.x * kWin;
-    int smin = kWin, smax = -1;
+    unsigned *dmp = dump + (long long)blockIdx.x * WS;
+    int smin = WS, smax = -1;
     for (int s = threadIdx.x; s < w.slots; s += blockDim.x) {
         const unsigned v = hist[s];
         if (v) {
@@ -417,7 +422,7 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int a = kWin, b = -1;
+        int a = WS, b = -1;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
             a = min(a, red[2 * k]);
             b = max(b, red[2 * k + 1]);
@@ -431,7 +436,7 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
 // window keys take slot = chunk base + LDS cursor, overflow keys slot =
 // counts[key] + global cursor.  Weighted input also records max|W| per
 // realisation (float bits of the largest finite |W|).
-template <int MAS, int AXES, bool POW2, bool WT>
+template <int MAS, int AXES, bool POW2, bool WT, int WS>
 __global__ void __launch_bounds__(kSortBlock)
 tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restrict__ W, float inv,
                     TileGeom g, const Chunk *__restrict__ chunks,
@@ -439,7 +444,7 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
                     const unsigned *__restrict__ counts, unsigned *__restrict__ ocursors,
                     unsigned *__restrict__ wmax_bits, float *__restrict__ records,
                     unsigned long long rcap) {
-    __shared__ unsigned cur[kWin];
+    __shared__ unsigned cur[WS];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
     // the count pass's window of this chunk
@@ -452,7 +457,7 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
         win.a = ch.wbase;
         win.slots = kWin;
     }
-    const unsigned *dmp = dump + (long long)blockIdx.x * kWin;
+    const unsigned *dmp = dump + (long long)blockIdx.x * WS;
     // only slots in [smin, smax] were counted; no other slot is looked up
     for (int s = ch.smin + threadIdx.x; s <= ch.smax; s += blockDim.x) cur[s] = dmp[s];
     __syncthreads();
@@ -698,14 +703,32 @@ __host__ __device__ __forceinline__ unsigned bank_class(const float4 &rec) {
 }
 
 // Adds one record's footprint to the LDS box at compile-time offsets from `a`.
+// 64-bit fixed point of s (|s| < 2^62, in quanta): hf = s * 2^-32 rounded to an
+// integer and lf = s - hf * 2^32 are exact floats (|lf| <= 2^31: s and hf * 2^32
+// are multiples of s's ulp, and the difference has at most 23 significant bits), lf
+// is truncated toward zero, so the result is s to within one quantum.  A tie (s an
+// odd multiple of 2^31 whose hf rounds down to even) gives lf = 2^31, which int32
+// cannot hold: it is taken as 2^31 - 128, 2^-24 of |s| (s's own float resolution).
+// Six instructions (mul, rndne, fma, min, two conversions) and a 64-bit add of the
+// sign of lf.
+__host__ __device__ __forceinline__ unsigned long long fix64(float s) {
+    const float hf = rintf(s * 0x1p-32f);
+    const float lf = fminf(fmaf(hf, -0x1p32f, s), 0x1.fffffep30f);
+    const int lo = (int)lf;
+    return ((unsigned long long)(unsigned)((int)hf + (lo >> 31)) << 32) | (unsigned)lo;
+}
+
 // ((Cx*Cy)*Cz)*(W*2^-q) == 2^-q * (((Cx*Cy)*Cz)*W) exactly (power-of-two
 // scaling commutes with float rounding), so the scaled product is the
 // reference's float contribution in units of the quantum; it is truncated
 // toward zero to an integer (|error| < 1 quantum = 2^-30 of the weight bound,
-// one conversion instruction).  Without weights (Cx*Cy)*(Cz*2^-q) is the same
+// one conversion instruction).  With per-particle weights (FX) the quantum is
+// 2^-61 of the bound times the realisation's record count, rounded up to a power of two, and
+// each contribution is converted to 64 bits (fix64), so a cell's error stays
+// below 1e-5 of the mean cell value even for weights spanning 5 decades.  Without weights (Cx*Cy)*(Cz*2^-q) is the same
 // number: it can differ only where (Cx*Cy)*Cz is subnormal, far below one
 // quantum, and both truncate to 0.
-template <int MAS, int AXES, bool WT>
+template <int MAS, int AXES, bool WT, bool FX>
 __device__ __forceinline__ void deposit_record(unsigned long long *a, const float4 &rec,
                                                float to_q, float weight_factor) {
     constexpr int F = MAS;
@@ -727,11 +750,12 @@ __device__ __forceinline__ void deposit_record(unsigned long long *a, const floa
 #pragma unroll
                     for (int k = 0; k < F; ++k) {
                         const float c = (cxy * Cz[k]) * w;
-                        atomicAdd(a + (l * BY + m) * BZ + k, (unsigned long long)(long long)(int)c);
+                        atomicAdd(a + (l * BY + m) * BZ + k,
+                                  FX ? fix64(c) : (unsigned long long)(long long)(int)c);
                     }
                 } else {
                     const float c = cxy * w;   // the reference multiplies by C[2][n] = 1.0
-                    atomicAdd(a + l * BY + m, (unsigned long long)(long long)(int)c);
+                    atomicAdd(a + l * BY + m, FX ? fix64(c) : (unsigned long long)(long long)(int)c);
                 }
             }
         }
@@ -831,12 +855,17 @@ __host__ __device__ __forceinline__ int deal_record_slot(unsigned cls, unsigned 
 // order.
 template <bool WT>
 __device__ __forceinline__ void box_quantum(const unsigned *__restrict__ wmax_bits, int real,
-                                            float weight_factor, float &to_q, double &from_q) {
+                                            float weight_factor, unsigned long long nrec,
+                                            float &to_q, double &from_q) {
     const float wmax = (WT && wmax_bits) ? __uint_as_float(wmax_bits[real]) : 1.0f;
     const double bound = (double)wmax * (double)weight_factor;
     int e = -1000;
     if (bound > 0.0) frexp(bound, &e);
+    // contributions below 2^30 quanta (int32); with per-particle weights (records
+    // carry W, wmax_bits set) below 2^(61 - lg) quanta, lg = ceil(log2(nrec)), so
+    // that any nrec of them sum below 2^61 in magnitude (int64)
     int q = e - 30;
+    if (WT && wmax_bits) q = e - 61 + (nrec > 1 ? 64 - __clzll((long long)(nrec - 1)) : 0);
     if (q < -127) q = -127;                 // keep 2^-q a normal float
     if (q > 127) q = 127;
     to_q = ldexpf(1.0f, -q);
@@ -998,7 +1027,9 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
 
     float to_q;
     double from_q;
-    box_quantum<WT>(wmax_bits, real, weight_factor, to_q, from_q);
+    // the realisation's record count, so that a batch and single calls agree bitwise
+    const unsigned long long nrec = offsets[(long long)(real + 1) * g.ntiles] - offsets[(long long)real * g.ntiles];
+    box_quantum<WT>(wmax_bits, real, weight_factor, nrec, to_q, from_q);
     __syncthreads();
 
     if constexpr (!BAL) {
@@ -1009,8 +1040,8 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
         for (; r < r1; r += blockDim.x) {
             const float4 rec = nxt;
             if (r + blockDim.x < r1) nxt = load_record<RW>(records, r + blockDim.x);
-            deposit_record<MAS, AXES, WT>(acc + box_base<MAS, AXES>(rec, g, tb), rec, to_q,
-                                          weight_factor);
+            deposit_record<MAS, AXES, WT, RW>(acc + box_base<MAS, AXES>(rec, g, tb), rec, to_q,
+                                              weight_factor);
         }
     } else {
         unsigned *wh = whist + wv * 16;
@@ -1102,7 +1133,7 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                     int bb;
                     if constexpr (UNI && AXES == 3) bb = box_base_uni<MAS>(dealt[r]);
                     else bb = box_base<MAS, AXES>(dealt[r], g, tb);
-                    deposit_record<MAS, AXES, WT>(acc + bb, dealt[r], to_q, weight_factor);
+                    deposit_record<MAS, AXES, WT, RW>(acc + bb, dealt[r], to_q, weight_factor);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -1126,7 +1157,8 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
 template <int MAS, int AXES, bool WT, bool UNI>
 __global__ void __launch_bounds__(kDepBlock)
 tile_box_flush_kernel(const unsigned long long *__restrict__ gbox, TileGeom g, float weight_factor,
-                      const unsigned *__restrict__ wmax_bits, float *__restrict__ grid,
+                      const unsigned *__restrict__ wmax_bits,
+                      const unsigned long long *__restrict__ offsets, float *__restrict__ grid,
                       float *__restrict__ partials, OutMode om) {
     constexpr int T = (AXES == 3) ? kT3 : kT2;
     constexpr int NB = (T + MAS - 1) * (T + MAS - 1) * ((AXES == 3) ? T + MAS - 1 : 1);
@@ -1136,7 +1168,8 @@ tile_box_flush_kernel(const unsigned long long *__restrict__ gbox, TileGeom g, f
     const int ex = UNI ? T : tb.ex, ey = UNI ? T : tb.ey, ez = UNI ? (AXES == 3 ? T : 1) : tb.ez;
     float to_q;
     double from_q;
-    box_quantum<WT>(wmax_bits, real, weight_factor, to_q, from_q);
+    const unsigned long long nrec = offsets[(long long)(real + 1) * g.ntiles] - offsets[(long long)real * g.ntiles];
+    box_quantum<WT>(wmax_bits, real, weight_factor, nrec, to_q, from_q);
     flush_box<MAS, AXES, UNI>(gbox + (long long)key * NB, key, tb, g, grid + real * g.ncell, partials, om,
                               from_q, ex, ey, ez);
 }
@@ -1568,8 +1601,16 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const int nscan_blocks = (int)((nt + kScanTile - 1) / kScanTile);
     PYL_CHECK(nscan_blocks <= kScanTile, PYL_EINVAL, "too many tiles (%lld)", nt);
     PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
+    // Wide window: a 3D realisation of kWin < ntiles <= kWinWide tiles is counted
+    // over all its keys in LDS, with chunks of 4-16 particles per tile, so that
+    // particles in any order (not only lattice or ID order) are counted and
+    // placed without per-particle global atomics.
+    const bool wide = kUseWide && axes == 3 && g.ntiles > kWin && g.ntiles <= kWinWide;
+    if (wide) g.wmode = kWinDirect;
+    const int ws = wide ? kWinWide : kWin;
     // Chunks of consecutive particles, never straddling two realisations.
-    const long long target = std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
+    const long long target = wide ? std::max(4 * g.ntiles, std::min(16 * g.ntiles, (n + 255) / 256))
+                                  : std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
     long long nchunk_ll = 0;
     for (int r = 0; r < nreal; ++r) {
         const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
@@ -1586,7 +1627,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const size_t ch_bytes = ((size_t)std::max(nchunk, 1) * sizeof(Chunk) + 255) & ~(size_t)255;
     const size_t org_bytes = ((size_t)(g.nt + 1) * sizeof(int) + 255) & ~(size_t)255;
     const size_t ord_bytes = ((size_t)nt * sizeof(unsigned) + 255) & ~(size_t)255;
-    const size_t dump_bytes = (size_t)std::max(nchunk, 1) * kWin * sizeof(unsigned);
+    const size_t dump_bytes = (size_t)std::max(nchunk, 1) * ws * sizeof(unsigned);
     PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
                                         ord_bytes + dump_bytes));
     char *tb = (char *)c->bufs[BUF_TILE_CNT].ptr;
@@ -1611,6 +1652,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     // the same ranges and grid in the same allocation reuses the uploaded tables.
     Ctx::TileTables &tt = c->tile_tables;
     bool same = tt.gen == c->bufs[BUF_TILE_CNT].gen && tt.n == n && tt.nchunk == nchunk &&
+                tt.target == target &&
                 tt.nreal == nreal && tt.dims == dims && tt.axes == axes && tt.mas == MAS &&
                 tt.offs.size() == (h_offs ? (size_t)nreal + 1 : 0);
     for (size_t i = 0; same && i < tt.offs.size(); ++i) same = tt.offs[i] == h_offs[i];
@@ -1637,6 +1679,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         PYL_TRY(pinned_done(c, c->pin_up));
         tt.n = n;
         tt.nchunk = nchunk;
+        tt.target = target;
         tt.nreal = nreal;
         tt.dims = dims;
         tt.axes = axes;
@@ -1646,8 +1689,12 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     }
     if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
-        auto kc = (axes == 3) ? (pow2 ? tile_count_kernel<MAS, 3, true> : tile_count_kernel<MAS, 3, false>)
-                              : (pow2 ? tile_count_kernel<MAS, 2, true> : tile_count_kernel<MAS, 2, false>);
+        auto kc = wide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
+                               : tile_count_kernel<MAS, 3, false, kWinWide>)
+                : (axes == 3) ? (pow2 ? tile_count_kernel<MAS, 3, true, kWin>
+                                      : tile_count_kernel<MAS, 3, false, kWin>)
+                              : (pow2 ? tile_count_kernel<MAS, 2, true, kWin>
+                                      : tile_count_kernel<MAS, 2, false, kWin>);
         kc<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, inv, g, dchunks, counts, ocounts, dump);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
@@ -1684,14 +1731,19 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const unsigned long long rcap = (unsigned long long)n;
     if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
-        auto ks = W ? ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true>
-                                           : tile_scatter_kernel<MAS, 3, false, true>)
-                                   : (pow2 ? tile_scatter_kernel<MAS, 2, true, true>
-                                           : tile_scatter_kernel<MAS, 2, false, true>))
-                    : ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, false>
-                                           : tile_scatter_kernel<MAS, 3, false, false>)
-                                   : (pow2 ? tile_scatter_kernel<MAS, 2, true, false>
-                                           : tile_scatter_kernel<MAS, 2, false, false>));
+        auto ks = wide
+            ? (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWinWide>
+                         : tile_scatter_kernel<MAS, 3, false, true, kWinWide>)
+                 : (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWinWide>
+                         : tile_scatter_kernel<MAS, 3, false, false, kWinWide>))
+            : W ? ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWin>
+                                       : tile_scatter_kernel<MAS, 3, false, true, kWin>)
+                               : (pow2 ? tile_scatter_kernel<MAS, 2, true, true, kWin>
+                                       : tile_scatter_kernel<MAS, 2, false, true, kWin>))
+                : ((axes == 3) ? (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWin>
+                                       : tile_scatter_kernel<MAS, 3, false, false, kWin>)
+                               : (pow2 ? tile_scatter_kernel<MAS, 2, true, false, kWin>
+                                       : tile_scatter_kernel<MAS, 2, false, false, kWin>));
         ks<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, W, inv, g, dchunks, dump, offsets, counts,
                                                  ocursors, wmax, records, rcap);
         PYL_HIP(hipGetLastError());
@@ -1731,8 +1783,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                    : (unit ? tile_box_flush_kernel<MAS, 3, false, false> : tile_box_flush_kernel<MAS, 3, true, false>))
             : (uni ? (unit ? tile_box_flush_kernel<MAS, 2, false, true> : tile_box_flush_kernel<MAS, 2, true, true>)
                    : (unit ? tile_box_flush_kernel<MAS, 2, false, false> : tile_box_flush_kernel<MAS, 2, true, false>));
-        kf<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(gbox, g, weight_factor, W ? wmax : nullptr, grid,
-                                                      partials, om);
+        kf<<<(unsigned)nt, kDepBlock, 0, c->stream>>>(gbox, g, weight_factor, W ? wmax : nullptr, offsets,
+                                                      grid, partials, om);
         PYL_HIP(hipGetLastError());
     }
     PYL_TRY(stage_end(c, PYL_STAGE_MA_DEPOSIT));

@@ -125,7 +125,7 @@ struct Ctx {
     // were built from: a call with the same particle ranges and grid reuses them.
     struct TileTables {
         unsigned long long gen = 0;          // BUF_TILE_CNT allocation they live in
-        long long n = -1, nchunk = -1;
+        long long n = -1, nchunk = -1, target = -1;
         int nreal = 0, dims = 0, axes = 0, mas = 0;
         std::vector<long long> offs;         // realisation offsets (empty: one realisation)
     } tile_tables;

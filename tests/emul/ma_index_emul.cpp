@@ -430,8 +430,38 @@ static int check_count_bucket() {
     return bad;
 }
 
+// fix64 (64-bit fixed point of a float count of quanta) against exact arithmetic:
+// within one quantum everywhere below 2^62 except at ties (odd multiples of 2^31,
+// within 128 quanta = 2^-24 of |s|), truncation toward zero below 2^31, and exact
+// for integers that are not ties.
+static int check_fix64() {
+    std::mt19937_64 rng(29);
+    std::uniform_real_distribution<double> ex(-30.0, 62.0);
+    int bad = 0;
+    auto one = [&](float s) {
+        const long long r = (long long)fix64(s);
+        const long double err = (long double)r - (long double)s;
+        const bool tie = std::fabs(std::fmod((double)s, 4294967296.0)) == 2147483648.0;
+        if (!(std::fabs((double)err) < 1.0 || (tie && std::fabs((double)err) <= 128.0))) ++bad;
+        if (std::fabs(s) < 2147483648.0f && r != (long long)std::trunc((double)s)) ++bad;
+        if (!tie && s == std::trunc(s) && (long double)r != (long double)s) ++bad;
+    };
+    for (int i = 0; i < 2000000; ++i) {
+        float s = (float)std::ldexp(1.0 + (double)(rng() >> 11) * 0x1p-53, 0);
+        s = std::ldexp(s, (int)std::floor(ex(rng)));
+        if (std::fabs(s) >= 0x1p62f) continue;
+        one((rng() & 1) ? s : -s);
+    }
+    const float edge[] = {0.0f, -0.0f, 0.5f, -0.5f, 1.0f, -1.0f, 0x1p31f, -0x1p31f, 0x1p32f, -0x1p32f,
+                          3 * 0x1p31f, -3 * 0x1p31f, 5 * 0x1p31f, -5 * 0x1p31f, 0x1.fffffep61f,
+                          -0x1.fffffep61f, 0x1.fffffep30f, -0x1.fffffep30f, 1e-30f, -1e-30f};
+    for (float s : edge) one(s);
+    printf("fix64: %s\n", bad ? "FAIL" : "ok");
+    return bad;
+}
+
 int main() {
-    int bad = check_dealing<4>(11) + check_dealing<2>(12) + check_count_bucket();
+    int bad = check_dealing<4>(11) + check_dealing<2>(12) + check_count_bucket() + check_fix64();
     bad += check_uni_final<PYL_NGP>() + check_uni_final<PYL_CIC>() + check_uni_final<PYL_TSC>() +
            check_uni_final<PYL_PCS>();
     bad += check_box_uni<PYL_CIC>(21) + check_box_uni<PYL_TSC>(22) + check_box_uni<PYL_PCS>(23);

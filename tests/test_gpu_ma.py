@@ -206,6 +206,53 @@ def test_ma_ordered_tiles_vs_oracle(oracle, mas):
 
 
 @pytest.mark.parametrize('mas', SCHEMES)
+def test_ma_mass_weighted_sparse_vs_oracle(oracle, mas):
+    # halo-catalogue shape: 200k positions on 256^3 (one per 84 cells) weighted by
+    # masses spanning five decades (1e10-1e15, log-uniform), and signed weights
+    # (velocities): with per-particle weights each contribution is held in 64-bit
+    # fixed point scaled by n * max|W|, so even cells far below max|W| match
+    print('case mass weighted sparse', mas, flush=True)
+    box, dims, n = 1000.0, 256, 200_000
+    rng = np.random.default_rng(41)
+    pos = (rng.random((n, 3)) * box).astype(np.float32)
+    pos[pos >= box] = 0.0
+    nadds = contrib_counts(pos, dims, box, mas)
+    for w in ((10.0 ** rng.uniform(10.0, 15.0, n)).astype(np.float32),
+              rng.normal(0.0, 300.0, n).astype(np.float32)):
+        ref = np.zeros((dims,) * 3, np.float32)
+        oracle.ma_f32(pos, ref, box, mas, w)
+        g = _gpu_ma(pos, dims, box, mas, w)
+        assert_cells(g, ref, what='mass weighted %s' % mas, nadds=nadds)
+        assert np.array_equal(g, _gpu_ma(pos, dims, box, mas, w))
+
+
+@pytest.mark.parametrize('mas', ['CIC', 'PCS'])
+@pytest.mark.parametrize('dims', [512, 480])
+def test_ma_wide_window_any_order(oracle, mas, dims):
+    # 3D grids of 16384 < tiles <= 32768 (512^3: 32^3 tiles; 480^3: 30^3 unequal
+    # tiles) count every tile of a sort chunk in LDS, so particles in random order
+    # take no per-particle global atomics: every cell matches the oracle, with and
+    # without weights, and the same particles sorted by cell give the same bits
+    print('case wide window', mas, dims, flush=True)
+    box, n = 1000.0, 1 << 21
+    rng = np.random.default_rng(dims + 3)
+    pos = (rng.random((n, 3)) * box).astype(np.float32)
+    pos[pos >= box] = 0.0
+    W = rng.lognormal(0.0, 1.0, n).astype(np.float32)
+    nadds = contrib_counts(pos, dims, box, mas)
+    cell = np.minimum((pos.astype(np.float64) * (dims / box)).astype(np.int64), dims - 1)
+    srt = np.argsort((cell[:, 0] * dims + cell[:, 1]) * dims + cell[:, 2], kind='stable')
+    for w in (None, W):
+        ref = np.zeros((dims,) * 3, np.float32)
+        oracle.ma_f32(pos, ref, box, mas, w)
+        g = _gpu_ma(pos, dims, box, mas, w)
+        assert_cells(g, ref, what='wide %s %d %s' % (mas, dims, w is not None), nadds=nadds)
+        gs = _gpu_ma(np.ascontiguousarray(pos[srt]), dims, box, mas,
+                     None if w is None else np.ascontiguousarray(w[srt]))
+        assert np.array_equal(g, gs)
+
+
+@pytest.mark.parametrize('mas', SCHEMES)
 @pytest.mark.parametrize('dims,coord', [(32, 3), (40, 3), (128, 2)])
 def test_ma_split_tiles_vs_oracle(oracle, mas, dims, coord):
     # few tiles with many records each (32^3: 8 tiles, 40^3: 27 unequal tiles, 128^2:

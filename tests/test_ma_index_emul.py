@@ -37,7 +37,8 @@ def test_tiled_index_logic(tmp_path):
     out = r.stdout
     assert r.returncode == 0, out[-4000:] + r.stderr[-2000:]
     assert 'all ok' in out
-    assert out.count(": ok") >= 379
+    assert out.count(": ok") >= 380
     assert "dealing KR 4" in out and "dealing KR 2" in out and "count buckets: ok" in out
+    assert "fix64: ok" in out
     checked = [l for l in out.splitlines() if l.startswith('uniform finalize cells checked')]
     assert checked and int(checked[0].split(':')[1]) > 0
