@@ -246,7 +246,8 @@ int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
     PYL_HIP(hipEventCreateWithFlags(&c->pin_up.done, hipEventDisableTiming));
     PYL_HIP(hipEventCreateWithFlags(&c->pin_down.done, hipEventDisableTiming));
     PYL_HIP(hipEventCreateWithFlags(&c->ev_total, hipEventDisableTiming));
-    PYL_HIP(hipHostMalloc((void **)&c->h_total, sizeof(unsigned long long), hipHostMallocDefault));
+    PYL_HIP(hipHostMalloc((void **)&c->h_total, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    c->h_total[1] = 0;
     *out = h.release();
     return PYL_OK;
 }

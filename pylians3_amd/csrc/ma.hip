@@ -368,7 +368,8 @@ template <int MAS, int AXES, bool POW2, int WS>
 __global__ void __launch_bounds__(kSortBlock)
 tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
                   Chunk *__restrict__ chunks, unsigned *__restrict__ counts,
-                  unsigned *__restrict__ ocounts, unsigned *__restrict__ dump) {
+                  unsigned *__restrict__ ocounts, unsigned *__restrict__ dump,
+                  unsigned *__restrict__ box_miss) {
     __shared__ unsigned hist[WS];
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
@@ -378,6 +379,12 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
         t0 = dist_tiles<MAS, AXES, POW2>(load_dist<AXES>(pos, rsd, ch.c0 + (ch.c1 - ch.c0) / 2, inv), g);
     const Win w = make_win(g, t0, tiles_key(t0, g, rbase), rbase);
     if (threadIdx.x == 0) chunks[blockIdx.x].wbase = (w.mode == kWinBox) ? w.a * g.nt + w.b : w.a;
+    // particles the 8 x 8 tile-column box around t0 would miss (box_miss: the
+    // host's choice between the box and the wide window for the next call)
+    TileGeom gb = g;
+    gb.wmode = kWinBox;
+    const Win wb = make_win(gb, t0, 0, rbase);
+    unsigned miss = 0;
     for (int s = threadIdx.x; s < w.slots; s += blockDim.x) hist[s] = 0u;
     __syncthreads();
     const long long step = (long long)blockDim.x * kUnroll;
@@ -395,6 +402,9 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
                 const unsigned s = win_slot(t, key, w, g, rbase);
                 if (s < (unsigned)w.slots) atomicAdd(&hist[s], 1u);
                 else atomicAdd(&ocounts[key], 1u);
+                // in the box window itself the miss is the overflow just taken
+                if (box_miss) miss += WS == kWinWide ? win_slot(t, key, wb, gb, rbase) >= (unsigned)wb.slots
+                                                     : s >= (unsigned)w.slots;
             }
         }
     }
@@ -413,7 +423,9 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
     for (int off = 32; off > 0; off >>= 1) {
         smin = min(smin, __shfl_xor(smin, off));
         smax = max(smax, __shfl_xor(smax, off));
+        if (box_miss) miss += (unsigned)__shfl_xor((int)miss, off);
     }
+    if (box_miss && (threadIdx.x & 63) == 0 && miss) atomicAdd(box_miss, miss);
     __syncthreads();                           // hist is reused for the wave results
     int *red = (int *)hist;
     if ((threadIdx.x & 63) == 0) {
@@ -1602,14 +1614,30 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     PYL_CHECK(nscan_blocks <= kScanTile, PYL_EINVAL, "too many tiles (%lld)", nt);
     PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
     // Wide window: a 3D realisation of kWin < ntiles <= kWinWide tiles is counted
-    // over all its keys in LDS, with chunks of 4-16 particles per tile, so that
+    // over all its keys in LDS, with chunks of 2-16 particles per tile, so that
     // particles in any order (not only lattice or ID order) are counted and
     // placed without per-particle global atomics.
-    const bool wide = kUseWide && axes == 3 && g.ntiles > kWin && g.ntiles <= kWinWide;
+    // Taken when the previous eligible call on this context found more than a
+    // quarter of its particles outside their chunk's box (read back without
+    // waiting: until the copy has landed the previous choice stands).  Both
+    // windows give the same bits (integer tile sums).
+    const bool eligible = kUseWide && axes == 3 && g.ntiles > kWin && g.ntiles <= kWinWide;
+    if (c->box_miss_n > 0) {
+        const hipError_t q = hipEventQuery(c->ev_total);
+        if (q == hipSuccess) {
+            c->wide_hint = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
+            c->box_miss_n = -1;
+        } else if (q == hipErrorNotReady) {
+            (void)hipGetLastError();          // pending, not an error: keep it out of later checks
+        } else {
+            PYL_HIP(q);
+        }
+    }
+    const bool wide = eligible && c->wide_hint;
     if (wide) g.wmode = kWinDirect;
     const int ws = wide ? kWinWide : kWin;
     // Chunks of consecutive particles, never straddling two realisations.
-    const long long target = wide ? std::max(4 * g.ntiles, std::min(16 * g.ntiles, (n + 255) / 256))
+    const long long target = wide ? std::max(2 * g.ntiles, std::min(16 * g.ntiles, (n + 2047) / 2048))
                                   : std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
     long long nchunk_ll = 0;
     for (int r = 0; r < nreal; ++r) {
@@ -1620,7 +1648,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const int nchunk = (int)nchunk_ll;
     // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | wmax[nreal] | pad |
     //              offsets[nt+1] | block sums | chunk table | per-chunk window bases
-    const size_t nwords = (size_t)nt * 3 + (size_t)nreal;
+    const size_t nwords = (size_t)nt * 3 + (size_t)nreal + 1;     // + box_miss
     const size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t off_bytes = (((size_t)nt + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255;
     const size_t bs_bytes = ((size_t)nscan_blocks * sizeof(unsigned long long) + 255) & ~(size_t)255;
@@ -1635,6 +1663,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     unsigned *ocounts = counts + nt;
     unsigned *ocursors = ocounts + nt;
     unsigned *wmax = ocursors + nt;
+    unsigned *box_miss = wmax + nreal;
     unsigned long long *offsets = (unsigned long long *)(tb + cnt_bytes);
     unsigned long long *bsums = (unsigned long long *)(tb + cnt_bytes + off_bytes);
     Chunk *dchunks = (Chunk *)(tb + cnt_bytes + off_bytes + bs_bytes);
@@ -1695,7 +1724,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                                       : tile_count_kernel<MAS, 3, false, kWin>)
                               : (pow2 ? tile_count_kernel<MAS, 2, true, kWin>
                                       : tile_count_kernel<MAS, 2, false, kWin>);
-        kc<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, inv, g, dchunks, counts, ocounts, dump);
+        kc<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, inv, g, dchunks, counts, ocounts, dump,
+                                                 eligible ? box_miss : nullptr);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
     }
@@ -1723,6 +1753,12 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     PYL_TRY(check_total(c));                    // the previous call's check, if still pending
     PYL_HIP(hipMemcpyAsync(c->h_total, offsets + nt, sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, c->stream));
+    c->box_miss_n = -1;
+    if (eligible && nchunk > 0) {
+        PYL_HIP(hipMemcpyAsync(c->h_total + 1, box_miss, sizeof(unsigned), hipMemcpyDeviceToHost,
+                               c->stream));
+        c->box_miss_n = n;
+    }
     PYL_HIP(hipEventRecord(c->ev_total, c->stream));
     c->total_expect = n;
     const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);

@@ -133,7 +133,9 @@ struct Ctx {
     PinnedBuf pin_down;                                      // Pk bin sums
     // Tile-sort consistency check of the latest tiled MA, read back without a
     // synchronisation of its own: checked at the next synchronising call.
-    unsigned long long *h_total = nullptr;                   // pinned
+    unsigned long long *h_total = nullptr;                   // pinned: [0] record total, [1] box misses
+    long long box_miss_n = -1;       // particles of the call whose box misses h_total[1] will hold
+    bool wide_hint = false;          // next eligible MA takes the wide sort window
     long long total_expect = -1;
     hipEvent_t ev_total = nullptr;
     // Stage timing: a ring of event pairs per stage, so that timing many calls

@@ -361,20 +361,22 @@ def test_ma_delta_equals_ma_then_normalise(oracle, mas):
     assert_cells(c.cpu().numpy() + 1.0, expect + 1.0, what='atomic delta')
 
 
-def test_ma_box_window_lattice_equals_shuffled():
-    # 448^3 grid (28^3 tiles, more than the LDS window holds): lattice-ordered
-    # chunks count their keys in the 8 x 8 tile-column box around them, shuffled
-    # input takes the global overflow counters; the integer tile sums make both
-    # grids bitwise equal, and the total is the particle count
+@pytest.mark.parametrize('d,sub', [(448, 1), (576, 4)])
+def test_ma_box_window_lattice_equals_shuffled(d, sub):
+    # 448^3 grid (28^3 tiles): every tile has an LDS counter in the wide window;
+    # 576^3 (36^3 tiles, more than it holds): lattice-ordered chunks count their
+    # keys in the 8 x 8 tile-column box around them and shuffled input takes the
+    # global overflow counters.  The integer tile sums make lattice-ordered and
+    # shuffled grids bitwise equal, and the total is the particle count
     import torch
     from pylians3_amd import _native as N
-    print('case box window 448', flush=True)
-    d, box = 448, 1000.0
+    print('case box window', d, sub, flush=True)
+    box = 1000.0
     g = torch.Generator(device='cuda').manual_seed(31)
     q = (torch.arange(d, device='cuda', dtype=torch.float32) + 0.5) * (box / d)
     pos = torch.stack(torch.meshgrid(q, q, q, indexing='ij'), -1).reshape(-1, 3)
     pos = pos + torch.randn(pos.shape, generator=g, device='cuda') * (4.0 * box / d)
-    pos = torch.remainder(pos, box).contiguous()
+    pos = torch.remainder(pos, box)[::sub].contiguous()
     pos[pos >= box] = 0.0
     n = pos.shape[0]
     ctx = N.default_context()
