@@ -230,8 +230,10 @@ def test_ma_mass_weighted_sparse_vs_oracle(oracle, mas):
 @pytest.mark.parametrize('dims', [512, 480])
 def test_ma_wide_window_any_order(oracle, mas, dims):
     # 3D grids of 16384 < tiles <= 32768 (512^3: 32^3 tiles; 480^3: 30^3 unequal
-    # tiles) count every tile of a sort chunk in LDS, so particles in random order
-    # take no per-particle global atomics: every cell matches the oracle, with and
+    # tiles): a call whose chunks' tile boxes miss over a quarter of the particles
+    # (random order) makes the next call on the context count every tile in LDS (the
+    # wide window), and a cell-sorted call switches it back, so the calls below
+    # alternate between the two windows.  Every cell matches the oracle, with and
     # without weights, and the same particles sorted by cell give the same bits
     print('case wide window', mas, dims, flush=True)
     box, n = 1000.0, 1 << 21
