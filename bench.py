@@ -44,6 +44,8 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=20.0, help='target CPU sample time')
     ap.add_argument('--no-batched', action='store_true', help='skip the batched-realisations leg')
     ap.add_argument('--no-serving', action='store_true', help='skip the serving leg')
+    ap.add_argument('--no-precision', action='store_true',
+                    help='skip the 1024^3 PCS precision study (tiled vs atomic vs FP64)')
     ap.add_argument('--nreal', type=int, default=256, help='realisations in the batched leg')
     ap.add_argument('--calib-copy', action='store_true',
                     help='one untimed 512 MiB device-to-device copy before the steps: the '
@@ -139,52 +141,239 @@ def time_steps(pipe, steps, warmup, stages=None):
     return ms
 
 
-def cpu_baseline(pos_host, grid_host, dims, box, mas_name, axis, target_s):
-    """The CPU path on this host's cores: the reference's MAS_c.c (OpenMP build from
-    oracle/_ref) on a particle sample scaled to the full count, then the FFT and
-    the reference binning loop (oracle restatement, serial as in Pk_library.pyx)."""
+def host_cores():
+    """Cores this process may use: its affinity set, capped by a cgroup CPU quota."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open('/sys/fs/cgroup/cpu.max') as f:
+            q, per = f.read().split()[:2]
+        if q != 'max':
+            n = max(1, min(n, int(-(-int(q) // int(per)))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def _runs(fn, warm=3, runs=20):
+    """Median and p99 seconds of `runs` calls after `warm` untimed ones."""
     import numpy as np
-    from oracle import oracle as O
-    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get('OMP_NUM_THREADS', '16') or 16))
+    for _ in range(warm):
+        fn()
+    t = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        t.append(time.perf_counter() - t0)
+    return float(np.median(t)), float(np.percentile(t, 99))
+
+
+def _ma_sample(run, pos_host, dims, target_s):
+    """Seconds for all particles, scaled from a sample grown to ~target_s."""
+    import numpy as np
     n_total = pos_host.shape[0]
-    ns = min(n_total, 1 << 22)
-    parts = {}
+    ns = min(n_total, 1 << 20)
     g = np.zeros((dims,) * 3, np.float32)
-    if O.ref_available('omp'):
-        kind_ma = 'reference MAS_c.c (OpenMP build, %d threads)' % cores
-        run = lambda p, gg: O.ma_reference(p, gg, box, mas_name, None, flavour='omp', threads=cores)
-    else:
-        kind_ma = 'oracle restatement (serial)'
-        run = lambda p, gg: O.ma_f32(p, gg, box, mas_name)
-    # grow the sample until it takes ~1/3 of the budget
     while True:
         t0 = time.perf_counter()
         run(pos_host[:ns], g)
         dt = time.perf_counter() - t0
-        if dt > target_s / 3 or ns == n_total:
+        if dt > target_s or ns == n_total:
             break
         ns = min(n_total, ns * 4)
-    t_ma = dt * n_total / ns
-    parts['ma_s_scaled'] = t_ma
+    return dt * n_total / ns, ns, dt
+
+
+def cpu_baseline(pos_host, grid_host, dims, box, mas_name, axis, target_s):
+    """The CPU path on this host's cores (count from the affinity set and cgroup quota):
+      MA     the reference's MAS_c.c built with OpenMP (oracle/_ref, all cores), and the
+             reference's own Cython MA() (serial loops, MAS_library.pyx:57-545, -O3
+             -ffast-math as shipped), each on a sample scaled to the full count;
+      FFT    scipy.fft.rfftn float32 with all cores (FFTW is not installed here);
+      bins   the reference mode loop (Pk_library.pyx:311-378) restated in C, serial as
+             the reference runs it and split over all cores.
+    FFT and the threaded loop: 3 warm-ups, 20 timed runs, median and p99.  `value`
+    is particles / (MA OpenMP + FFT + threaded bins); `as_shipped_step_s` is what
+    the reference's MA() + Pk() path takes (serial MA, threaded FFT, serial bins)."""
+    import numpy as np
+    from oracle import oracle as O
+    cores = host_cores()
+    n_total = pos_host.shape[0]
+    out = {'cores': cores}
+    if O.ref_available('omp'):
+        kind = 'reference'
+        t_ma, ns, dt = _ma_sample(lambda p, g: O.ma_reference(p, g, box, mas_name, None, flavour='omp',
+                                                              threads=cores), pos_host, dims, target_s / 4)
+        ma_desc = 'reference MAS_c.c (OpenMP, %d threads) on %d of %d particles (%.2f s)' % (cores, ns, n_total, dt)
+    else:
+        kind = 'port'
+        t_ma, ns, dt = _ma_sample(lambda p, g: O.ma_f32(p, g, box, mas_name), pos_host, dims, target_s / 4)
+        ma_desc = 'oracle restatement (serial) on %d of %d particles (%.2f s)' % (ns, n_total, dt)
+    if O.cython_available('shipped'):
+        t_ser, ns2, dt2 = _ma_sample(lambda p, g: O.ma_cython(p, g, box, mas_name, flavour='shipped'),
+                                     pos_host, dims, target_s / 4)
+        ser_desc = 'reference Cython MA() (serial) on %d particles (%.2f s)' % (ns2, dt2)
+    else:
+        t_ser, ns2, dt2 = _ma_sample(lambda p, g: O.ma_f32(p, g, box, mas_name), pos_host, dims,
+                                     target_s / 4)
+        ser_desc = 'oracle restatement of MA() (serial) on %d particles (%.2f s)' % (ns2, dt2)
     delta = grid_host / np.float32(grid_host.mean()) - np.float32(1.0)
-    t0 = time.perf_counter()
+    fft_med, fft_p99 = _runs(lambda: O.fft_f32(delta, workers=cores))
     dk = O.fft_f32(delta, workers=cores)
-    t_fft = time.perf_counter() - t0
+    bin_med, bin_p99 = _runs(lambda: O.raw_bins([dk], dims, axis, [mas_name], threads=cores))
     t0 = time.perf_counter()
     O.raw_bins([dk], dims, axis, [mas_name])
-    t_bin = time.perf_counter() - t0
-    total = t_ma + t_fft + t_bin
-    return {
+    t_bin_serial = time.perf_counter() - t0
+    total = t_ma + fft_med + bin_med
+    out.update({
         'value': round(n_total / total / 1e6, 3),
         'unit': 'Mparticles/s',
-        'cores': cores,
-        'kind': 'port' if 'oracle' in kind_ma else 'reference',
-        'sample': ('MA: %s on %d of %d particles (%.2f s), scaled linearly; FFT: scipy.fft.rfftn '
-                   'float32, %d workers, full %d^3 (%.2f s); binning: reference mode loop restated '
-                   'in C, serial, full grid (%.2f s)' % (kind_ma, ns, n_total, dt, cores, dims, t_fft,
-                                                         t_bin)),
+        'kind': kind,
+        'sample': ('MA: %s, scaled linearly; MA() serial: %s; FFT: scipy.fft.rfftn float32, %d '
+                   'workers, full %d^3, median of 20; bins: reference mode loop restated in C over '
+                   '%d threads, full grid, median of 20; serial bins once' % (
+                       ma_desc, ser_desc, cores, dims, cores)),
         'step_s': round(total, 3),
-    }
+        'grid_cells_per_s': round(dims ** 3 / total, 1),
+        'ma_omp_s': round(t_ma, 4),
+        'ma_serial_s': round(t_ser, 3),
+        'fft_s': {'median': round(fft_med, 4), 'p99': round(fft_p99, 4)},
+        'bins_threads_s': {'median': round(bin_med, 4), 'p99': round(bin_p99, 4)},
+        'bins_serial_s': round(t_bin_serial, 3),
+        'as_shipped_step_s': round(t_ser + fft_med + t_bin_serial, 3),
+    })
+    return out
+
+
+def _errs(x, ref, floor=None):
+    import numpy as np
+    x = np.asarray(x, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    fl = np.mean(np.abs(ref)) if floor is None else np.abs(np.asarray(floor, np.float64)).ravel()
+    den = np.maximum(np.abs(ref), fl)
+    num = np.abs(x - ref)
+    # bins that are zero in both (Pk1D at k_par = middle) carry no error
+    rel = np.where(den > 0, num / np.where(den > 0, den, 1.0), np.where(num > 0, np.inf, 0.0))
+    return {'max': float('%.3g' % rel.max()), 'rms': float('%.3g' % np.sqrt(np.mean(rel * rel)))}
+
+
+def cpu_check_errors(pos_host, grid_gpu, pk_gpu, dims, box, mas_name, axis, mean, threads):
+    """Errors of the GPU results against the FP64 oracle (MA, FFT and bins in double,
+    on all particles) and against the float32 oracle on the GPU's own delta.
+    pk_gpu: dict of the GPU's finalized arrays (Pk3D, Pk1D, Pk2D).  The oracle is
+    the checker here, run on the host cores."""
+    import numpy as np
+    from oracle import oracle as O
+    g64 = O.ma_f64(pos_host, dims, box, mas_name, threads=threads)
+    d64 = g64 / g64.mean() - 1.0
+    r64 = O.pk(d64, box, axis=axis, MAS=mas_name, f64=True, workers=threads, threads=threads)
+    del d64
+    delta = grid_gpu / np.float32(mean) - np.float32(1.0)
+    r32 = O.pk(delta, box, axis=axis, MAS=mas_name, workers=threads, threads=threads)
+    del delta
+    P = pk_gpu['Pk3D'][:, :, 0]
+    P1, P2 = pk_gpu['Pk1D'][:, 0], pk_gpu['Pk2D'][:, 0]
+
+    def block(ref, grid_ref):
+        b = {} if grid_ref is None else {'grid': _errs(grid_gpu, grid_ref)}
+        P0 = ref.Pk[:, 0]
+        b.update({'Pk0': _errs(P[:, 0], P0, P0), 'Pk2': _errs(P[:, 1], ref.Pk[:, 1], P0),
+                  'Pk4': _errs(P[:, 2], ref.Pk[:, 2], P0), 'Pk1D': _errs(P1, ref.Pk1D, ref.Pk1D),
+                  'Pk2D': _errs(P2, ref.Pk2D, np.median(np.abs(ref.Pk2D)))})
+        return b
+
+    g32 = np.zeros((dims,) * 3, np.float32)
+    O.ma_f32(pos_host, g32, box, mas_name)       # the reference's serial float32 arithmetic
+    out = {'vs_fp64': block(r64, g64), 'vs_f32_oracle_same_delta': block(r32, None),
+           'f32_oracle_vs_fp64': {'grid': _errs(g32, g64),
+                                  'Pk0': _errs(r32.Pk[:, 0], r64.Pk[:, 0], r64.Pk[:, 0])},
+           'gpu_vs_f32_oracle_grid': _errs(grid_gpu, g32),
+           'floors': 'grid: mean cell; Pk2, Pk4: |Pk0| of the bin; Pk2D: median |Pk2D|'}
+    return out
+
+
+def precision_leg(N, reps=10, steps=5):
+    """Large single field (1024^3 Zel'dovich particles, box 2000 Mpc/h, seed 3, PCS onto
+    1024^3, Pk axis 0): the tiled (sorted, exact integer sums) and the atomic float32
+    scatters.  Times the full MA + delta + Pk step, checks `reps` repeated tiled MAs
+    and 3 atomic ones for bitwise reproducibility, and keeps both grids and their
+    spectra for the CPU leg's FP64 comparison."""
+    import numpy as np
+    import torch
+    from pylians3_amd import workloads
+    dims, box, mas = 1024, 2000.0, N.PCS
+    pos = workloads.config_c3()
+    n = pos.shape[0]
+    ctx = N.Context(0)
+    L = N.lib()
+    out = {'workload': '1024^3 Zel\'dovich particles (seed 3, box 2000 Mpc/h), PCS onto 1024^3, '
+                       'Pk axis 0', 'particles': n}
+    pipe = Pipeline(N, ctx, pos, dims, box, mas, 0)
+    ms = time_steps(pipe, steps, 2)
+    st = pipe.stage_avg()
+    out['ms_per_step'] = round(ms, 3)
+    out['mparticles_per_s'] = round(n / ms / 1e3, 1)
+    out['grid_cells_per_s'] = round(dims ** 3 / (ms * 1e-3), 1)
+    out['kernels_ms'] = {k: round(v, 4) for k, v in st.items()}
+    ctx.enable_timing(False)
+    grid = pipe.grid
+    keep = {}
+    for name, flag, nrep in (('tiled', 0, reps), ('atomic', N.MA_ATOMIC, 3)):
+        first = None
+        differ = 0
+        t = []
+        for r in range(nrep):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(pos.data_ptr()), n, 3, None,
+                             ctypes.c_void_p(grid.data_ptr()), dims, box, mas,
+                             N.PTR_DEVICE | N.MA_OVERWRITE | flag))
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+            if first is None:
+                first = grid.clone()
+            elif not torch.equal(grid, first):
+                differ += 1
+        arrs = N.alloc_pk_out(dims, 1)
+        mean = float(np.float32(n / dims ** 3))
+        N.check(L.pyl_pk(ctx.handle, (ctypes.c_void_p * 1)(first.data_ptr()), 1,
+                         (ctypes.c_int * 1)(mas), dims, box, 0, mean, 1.0, N.PTR_DEVICE,
+                         ctypes.byref(arrs[0])))
+        keep[name] = (first.cpu().numpy(), arrs[1])
+        del first
+        out[name] = {'runs': nrep, 'runs_differing_from_first': differ,
+                     'bitwise_reproducible': differ == 0,
+                     'ma_ms_median': round(1e3 * float(np.median(t)), 3)}
+    del pipe, grid
+    ctx.close()
+    pos_host = pos.cpu().numpy()
+    del pos
+    torch.cuda.empty_cache()
+    return out, keep, pos_host
+
+
+def cpu_precision_errors(pos_host, keep, threads):
+    """FP64 oracle (MA, FFT, bins in double) of the 1024^3 field: per-cell and per-bin
+    relative errors of the tiled and atomic GPU results."""
+    import numpy as np
+    from oracle import oracle as O
+    dims, box = 1024, 2000.0
+    g64 = O.ma_f64(pos_host, dims, box, 'PCS', threads=threads)
+    d64 = g64 / g64.mean() - 1.0
+    r64 = O.pk(d64, box, axis=0, MAS='PCS', f64=True, workers=threads, threads=threads)
+    del d64
+    res = {}
+    P0 = r64.Pk[:, 0]
+    for name, (g, a) in keep.items():
+        P = a['Pk3D'][:, :, 0]
+        rel0 = np.abs(P[:, 0] - P0) / np.abs(P0)
+        res[name] = {'grid': _errs(g, g64), 'Pk0': _errs(P[:, 0], P0, P0),
+                     'Pk2': _errs(P[:, 1], r64.Pk[:, 1], P0), 'Pk4': _errs(P[:, 2], r64.Pk[:, 2], P0),
+                     'Pk1D': _errs(a['Pk1D'][:, 0], r64.Pk1D, r64.Pk1D),
+                     'Pk0_rel_err_by_k': {'k_h_per_mpc': [round(float(r64.k3D[i]), 4) for i in (0, 9, 99, 499, 885)],
+                                          'rel': [float('%.3g' % rel0[i]) for i in (0, 9, 99, 499, 885)]}}
+    g_t, g_a = keep['tiled'][0], keep['atomic'][0]
+    res['tiled_vs_atomic_grid'] = _errs(g_a, g_t)
+    return res
 
 
 def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
@@ -196,26 +385,10 @@ def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
     import numpy as np
     import torch
     from pylians3_amd import workloads
-    rng = np.random.default_rng(seed0)
-    lam = np.exp(rng.uniform(np.log(1e5), np.log(2e6), nreal))
-    counts = np.clip(rng.poisson(lam), 100000, 2000000).astype(np.int64)
-    matter, tr_pos, tr_w = [], [], []
-    for r in range(nreal):
-        psi = workloads.displacement_field(dims, box, seed0 + r)
-        q = (torch.arange(dims, device='cuda', dtype=torch.float32) + 0.5) * (box / dims)
-        lat = torch.stack(torch.meshgrid(q, q, q, indexing='ij'), -1).view(-1, 3)
-        matter.append(torch.remainder(lat + psi.view(3, -1).t(), box))
-        p, w = workloads.tracers(int(counts[r]), psi, box, seed0 + 5000 + r)
-        tr_pos.append(p)
-        tr_w.append(w)
-        del psi, lat
-    mpos = torch.cat(matter).contiguous()
-    mpos[mpos >= box] = 0.0
-    tpos = torch.cat(tr_pos).contiguous()
-    tw = torch.cat(tr_w).contiguous()
-    del matter, tr_pos, tr_w
-    moffs = np.arange(nreal + 1, dtype=np.int64) * dims ** 3
-    toffs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    wl = workloads.batched_realisations(nreal, dims, box, seed0)
+    mpos, moffs, tpos, tw, toffs, counts = (wl['mpos'], wl['moffs'], wl['tpos'], wl['tw'],
+                                            wl['toffs'], wl['counts'])
+    del wl
     ctx = N.Context(0)
     grids = [torch.empty((nreal, dims, dims, dims), dtype=torch.float32, device='cuda')
              for _ in range(2)]
@@ -436,7 +609,11 @@ def main():
         peak = 64.0 / 6.2 * 2.4e9 * 256
         lds = {'atomics_per_s': round(atomics / (dom_ms * 1e-3), -6), 'peak_per_s': round(peak, -9),
                'frac': round(atomics / (dom_ms * 1e-3) / peak, 4)}
-    roofline = {'bound': 'hbm', 'kernel': dom, 'kernel_ms': round(dom_ms, 4), 'lds_atomics': lds,
+    # The roofline is HBM bandwidth (no kernel here is a dense contraction).  What
+    # limits the kernel below it: the deposit and the binning are bound by their LDS
+    # atomics and VALU work, the sort and FFT passes by HBM.
+    limiter = {'ma_deposit': 'lds_atomics+valu', 'pk_bin': 'lds_atomics+valu'}.get(dom, 'hbm')
+    roofline = {'bound': 'hbm', 'limiter': limiter, 'kernel': dom, 'kernel_ms': round(dom_ms, 4), 'lds_atomics': lds,
                 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
                 'algorithmic_bytes': int(kernel_bytes[dom]),
@@ -445,15 +622,29 @@ def main():
                        for k, t in st.items() if k in kernel_bytes and t > 0}
 
     cpu = None
+    errors = None
     if not args.no_cpu:
         log('CPU baseline leg')
         pos_host = pos.cpu().numpy()
         cpu = cpu_baseline(pos_host, grid_host, dims, box, 'TSC', axis, args.cpu_seconds)
+        log('CPU leg: errors of the timed step against the FP64 oracle')
+        errors = cpu_check_errors(pos_host, grid_host, pipe.arrs, dims, box, 'TSC', axis, pipe.mean,
+                                  cpu['cores'])
         del pos_host
     del pipe
     ctx.close()
     del pos
     torch.cuda.empty_cache()
+
+    precision = None
+    if not args.no_precision and dims == 512:
+        log('precision leg: 1024^3 PCS, tiled vs atomic')
+        precision, keep, pos3 = precision_leg(N)
+        if not args.no_cpu:
+            log('CPU leg: 1024^3 FP64 oracle')
+            precision['errors_vs_fp64'] = cpu_precision_errors(pos3, keep, cpu['cores'])
+        del keep, pos3
+        torch.cuda.empty_cache()
 
     batched = None
     if not args.no_batched:
@@ -487,6 +678,9 @@ def main():
         'pk_ms': round(pk_ms, 4),
         'kernel_GBps': per_kernel_gbps,
         'ma_mparticles_per_s': round(n / ma_ms / 1e3, 2) if ma_ms > 0 else None,
+        'grid_cells_per_s': round(dims ** 3 / (ms * 1e-3), 1),
+        'errors_vs_fp64': errors,
+        'precision_1024': precision,
         'kernels_ms': kernels,                 # from the instrumented steps (events on every stage)
         'instrumented_ms_per_step': round(ms_all, 4),
         'stages': stages,

@@ -64,6 +64,7 @@ extern "C" {
 #define PYL_NO_SYNC        0x0020  /* device pointers: return without synchronising the stream */
 #define PYL_PK_DESTROY_INPUT 0x0040 /* Pk, device delta: the FFT may overwrite delta */
 #define PYL_PK_NO_PHASE    0x0080  /* Pk: skip the Pkphase accumulator */
+#define PYL_PK_SCALES_DEVICE 0x0100 /* pyl_pk_batch: scales and shifts are device arrays */
 
 /* ---- context ---------------------------------------------------------- */
 typedef struct pyl_ctx pyl_ctx;
@@ -73,7 +74,9 @@ int pyl_version(void);
 /* hbm_budget: bytes of device scratch the context may hold (0 = 90% of free memory). */
 int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget);
 int pyl_ctx_destroy(pyl_ctx *ctx);
-/* Use an external HIP stream (hipStream_t) for all work; NULL restores the own stream. */
+/* Use an external HIP stream (hipStream_t) for all work; NULL restores the own stream.
+ * When the stream changes, the new stream first waits for the work already queued on
+ * the old one (the context's scratch is shared by its calls). */
 int pyl_ctx_set_stream(pyl_ctx *ctx, void *hip_stream);
 int pyl_ctx_synchronize(pyl_ctx *ctx);
 /* Release scratch, FFT plans and mode-count plans. */
@@ -205,7 +208,9 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
  * contiguous (dims,dims,dims) float32 cubes of field f (field-major, so each field can
  * live in its own array).  One batched rocFFT and one binning launch for the whole
  * batch.  scales/shifts: nreal*nfields values, [r*nfields + f], for the prologue
- * (NULL = none).  outs: nreal result structs (each sized as for pyl_pk with nfields
+ * (NULL = none); host arrays, or device arrays with PYL_PK_SCALES_DEVICE (then both
+ * are given and read on the device, so they may come from device work queued just
+ * before the call).  outs: nreal result structs (each sized as for pyl_pk with nfields
  * fields).  Each realisation's result is bitwise the same as a single pyl_pk call on
  * it.  Replaces a Python loop of Pk/XPk calls over realisations
  * (Pk_library.pyx:263-420, :529-793). */

@@ -23,7 +23,11 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORA_NGP 1
 #define ORA_CIC 2
@@ -144,67 +148,172 @@ void ora_ma_f32(const float *pos, long n, int coord, const float *W,
  * double (positions and weights are the float32 inputs, widened exactly).
  * 2D is the native single deposit.
  */
+/* Float64 footprint of one coordinate: the B-spline definitions evaluated in
+ * double from the float32 position (MAS_library.pyx:123-545 without roundings). */
+static int f64_axis(double dist, int dims, int mas, int *idx, double *C)
+{
+    int mn, j;
+    switch (mas) {
+    case ORA_NGP:
+        idx[0] = wrap_index((long)floor(dist + 0.5), dims);
+        C[0] = 1.0;
+        return 1;
+    case ORA_CIC:
+        mn = (int)floor(dist);
+        C[1] = dist - mn; C[0] = 1.0 - C[1];
+        idx[0] = wrap_index(mn, dims);
+        idx[1] = wrap_index((long)mn + 1, dims);
+        return 2;
+    case ORA_TSC:
+        mn = (int)floor(dist - 1.5);
+        for (j = 0; j < 3; j++) {
+            double diff = fabs(mn + j + 1 - dist);
+            idx[j] = wrap_index((long)mn + j + 1, dims);
+            C[j] = diff < 0.5 ? 0.75 - diff * diff
+                 : diff < 1.5 ? 0.5 * (1.5 - diff) * (1.5 - diff) : 0.0;
+        }
+        return 3;
+    case ORA_PCS:
+        mn = (int)floor(dist - 2.0);
+        for (j = 0; j < 4; j++) {
+            double diff = fabs(mn + j + 1 - dist);
+            idx[j] = wrap_index((long)mn + j + 1, dims);
+            C[j] = diff < 1.0 ? (4.0 - 6.0 * diff * diff + 3.0 * diff * diff * diff) / 6.0
+                 : diff < 2.0 ? (2.0 - diff) * (2.0 - diff) * (2.0 - diff) / 6.0 : 0.0;
+        }
+        return 4;
+    default:
+        return 0;
+    }
+}
+
+/* Adds particle i's float64 footprint into number (atomically when `atomic`). */
+static void f64_particle(const float *pos, long i, int coord, const float *W, double *number,
+                         int dims, double inv, int mas, int atomic)
+{
+    int idx[3][4];
+    double C[3][4];
+    int F = 1, Fz, l, m, q, a, b;
+    for (a = 0; a < 3; a++)
+        for (b = 0; b < 4; b++) { idx[a][b] = 0; C[a][b] = 1.0; }
+    for (a = 0; a < coord; a++)
+        F = f64_axis((double)pos[(long)coord * i + a] * inv, dims, mas, idx[a], C[a]);
+    Fz = (coord == 3) ? F : 1;
+    for (l = 0; l < F; l++)
+        for (m = 0; m < F; m++)
+            for (q = 0; q < Fz; q++) {
+                long cell;
+                double w = C[0][l] * C[1][m] * C[2][q];
+                if (W) w *= (double)W[i];
+                if (coord == 3)
+                    cell = ((long)idx[0][l] * dims + idx[1][m]) * dims + idx[2][q];
+                else
+                    cell = (long)idx[0][l] * dims + idx[1][m];
+                if (atomic) {
+#pragma omp atomic
+                    number[cell] += w;
+                } else {
+                    number[cell] += w;
+                }
+            }
+}
+
 void ora_ma_f64(const float *pos, long n, int coord, const float *W,
                 double *number, int dims, double BoxSize, int mas)
 {
     long i;
-    double inv = (double)dims / BoxSize;
-    int idx[3][4];
-    double C[3][4];
-    int F = 1, Fz, l, m, q, a, b, j;
-    for (a = 0; a < 3; a++)
-        for (b = 0; b < 4; b++) { idx[a][b] = 0; C[a][b] = 1.0; }
+    const double inv = (double)dims / BoxSize;
+    if (mas < ORA_NGP || mas > ORA_PCS) return;
+    for (i = 0; i < n; i++) f64_particle(pos, i, coord, W, number, dims, inv, mas, 0);
+}
 
-    for (i = 0; i < n; i++) {
-        int axis;
-        for (axis = 0; axis < coord; axis++) {
-            double dist = (double)pos[(long)coord * i + axis] * inv;
-            int mn;
-            switch (mas) {
-            case ORA_NGP:
-                idx[axis][0] = wrap_index((long)floor(dist + 0.5), dims);
-                C[axis][0] = 1.0; F = 1; break;
-            case ORA_CIC:
-                mn = (int)floor(dist);
-                C[axis][1] = dist - mn; C[axis][0] = 1.0 - C[axis][1];
-                idx[axis][0] = wrap_index(mn, dims);
-                idx[axis][1] = wrap_index((long)mn + 1, dims);
-                F = 2; break;
-            case ORA_TSC:
-                mn = (int)floor(dist - 1.5);
-                for (j = 0; j < 3; j++) {
-                    double diff = fabs(mn + j + 1 - dist);
-                    idx[axis][j] = wrap_index((long)mn + j + 1, dims);
-                    C[axis][j] = diff < 0.5 ? 0.75 - diff * diff
-                               : diff < 1.5 ? 0.5 * (1.5 - diff) * (1.5 - diff) : 0.0;
-                }
-                F = 3; break;
-            case ORA_PCS:
-                mn = (int)floor(dist - 2.0);
-                for (j = 0; j < 4; j++) {
-                    double diff = fabs(mn + j + 1 - dist);
-                    idx[axis][j] = wrap_index((long)mn + j + 1, dims);
-                    C[axis][j] = diff < 1.0 ? (4.0 - 6.0 * diff * diff + 3.0 * diff * diff * diff) / 6.0
-                               : diff < 2.0 ? (2.0 - diff) * (2.0 - diff) * (2.0 - diff) / 6.0 : 0.0;
-                }
-                F = 4; break;
-            default: return;
+/* The same sums with `threads` OpenMP threads, deterministic and without atomics.
+ * Particles are grouped by the x slab of their first footprint node (2T slabs of
+ * at least F-1 planes, kept in input order within a slab); even slabs run in
+ * parallel, then odd slabs.  A particle of slab s writes only planes of slabs s
+ * and s+1 (periodically), so no two threads of a phase touch the same cell.  Per
+ * cell, the additions come in a fixed order that differs from the serial one
+ * only at the 1e-16 level of double rounding.  Falls back to the serial loop
+ * when the grid is too thin for two slabs per thread. */
+void ora_ma_f64_threads(const float *pos, long n, int coord, const float *W,
+                        double *number, int dims, double BoxSize, int mas, int threads)
+{
+    const double inv = (double)dims / BoxSize;
+    const int F = mas;                      /* footprint width: NGP 1 .. PCS 4 */
+    int T = threads < 1 ? 1 : threads;
+    int nslab, s;
+    long i, *cnt, *start;
+    unsigned *order;
+    if (mas < ORA_NGP || mas > ORA_PCS) return;
+    nslab = 2 * T;
+    while (nslab > 2 && (dims / nslab < F - 1 || dims / nslab < 1)) nslab -= 2;
+    if (T == 1 || nslab < 2 || dims / nslab < F - 1 || dims / nslab < 1 || n >= 4294967295L) {
+        ora_ma_f64(pos, n, coord, W, number, dims, BoxSize, mas);
+        return;
+    }
+    {
+        unsigned char *slab_of = (unsigned char *)malloc((size_t)n);
+        long *tcnt = (long *)calloc((size_t)T * nslab, sizeof(long));
+        cnt = (long *)calloc((size_t)nslab + 1, sizeof(long));
+        start = (long *)calloc((size_t)nslab + 1, sizeof(long));
+        order = (unsigned *)malloc(sizeof(unsigned) * (size_t)(n > 0 ? n : 1));
+        if (!slab_of || !tcnt || !cnt || !start || !order) {
+            free(slab_of); free(tcnt); free(cnt); free(start); free(order);
+            ora_ma_f64(pos, n, coord, W, number, dims, BoxSize, mas);
+            return;
+        }
+        /* slab of each particle, per-thread counts over contiguous particle ranges */
+#pragma omp parallel num_threads(T)
+        {
+            int t = 0, k;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            long a = n * t / T, b = n * (t + 1) / T, j;
+            for (j = a; j < b; j++) {
+                int idx[4];
+                double C[4];
+                f64_axis((double)pos[(long)coord * j] * inv, dims, mas, idx, C);
+                k = (int)((long)idx[0] * nslab / dims);
+                slab_of[j] = (unsigned char)k;
+                tcnt[(long)t * nslab + k]++;
             }
         }
-        Fz = (coord == 3) ? F : 1;
-        for (l = 0; l < F; l++)
-            for (m = 0; m < F; m++)
-                for (q = 0; q < Fz; q++) {
-                    long cell;
-                    double w = C[0][l] * C[1][m] * C[2][q];
-                    if (W) w *= (double)W[i];
-                    if (coord == 3)
-                        cell = ((long)idx[0][l] * dims + idx[1][m]) * dims + idx[2][q];
-                    else
-                        cell = (long)idx[0][l] * dims + idx[1][m];
-                    number[cell] += w;
+        /* stable order: slab, then thread range, then particle index */
+        {
+            long run = 0;
+            int t;
+            for (s = 0; s < nslab; s++) {
+                start[s] = run;
+                for (t = 0; t < T; t++) {
+                    long c = tcnt[(long)t * nslab + s];
+                    tcnt[(long)t * nslab + s] = run;
+                    run += c;
                 }
+            }
+            start[nslab] = run;
+        }
+#pragma omp parallel num_threads(T)
+        {
+            int t = 0;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            long a = n * t / T, b = n * (t + 1) / T, j;
+            for (j = a; j < b; j++) order[tcnt[(long)t * nslab + slab_of[j]]++] = (unsigned)j;
+        }
+        free(slab_of);
+        free(tcnt);
     }
+    for (int phase = 0; phase < 2; phase++) {
+#pragma omp parallel for num_threads(T) schedule(dynamic, 1)
+        for (s = phase; s < nslab; s += 2) {
+            long j;
+            for (j = start[s]; j < start[s + 1]; j++)
+                f64_particle(pos, (long)order[j], coord, W, number, dims, inv, mas, 0);
+        }
+    }
+    free(cnt); free(start); free(order);
 }
 
 /*

@@ -54,11 +54,15 @@ def lib():
                                  ctypes.c_float, ctypes.c_int, ctypes.c_int]
         L.ora_ma_f64.argtypes = [fp, ctypes.c_long, ctypes.c_int, fp, dp, ctypes.c_int,
                                  ctypes.c_double, ctypes.c_int]
+        L.ora_ma_f64_threads.argtypes = [fp, ctypes.c_long, ctypes.c_int, fp, dp, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.ora_axis_weights.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                        ctypes.c_int, ip, fp]
         L.ora_axis_weights.restype = ctypes.c_int
         L.ora_pk_loop.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ip] + [dp] * 9
+        L.ora_pk_loop_threads.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ip] + [dp] * 9 + [ctypes.c_int]
         L.ora_mas_correction.argtypes = [ctypes.c_double, ctypes.c_int]
         L.ora_mas_correction.restype = ctypes.c_double
         L.ora_pos_redshift_space.argtypes = [fp, fp, ctypes.c_long, ctypes.c_float,
@@ -96,14 +100,19 @@ def ma_f32(pos, number, BoxSize, MAS='CIC', W=None, cython2d=True, renormalize_2
     return number
 
 
-def ma_f64(pos, dims, BoxSize, MAS='CIC', W=None):
-    """Float64 oracle grid (positions/weights widened from float32)."""
+def ma_f64(pos, dims, BoxSize, MAS='CIC', W=None, threads=1):
+    """Float64 oracle grid (positions/weights widened from float32).  threads > 1
+    adds with OpenMP atomics (order varies at the 1e-16 level)."""
     pos = np.ascontiguousarray(pos, dtype=np.float32)
     coord = pos.shape[1]
     number = np.zeros((dims,) * coord, dtype=np.float64)
     Wc = None if W is None else np.ascontiguousarray(W, dtype=np.float32)
-    lib().ora_ma_f64(_p(pos, ctypes.c_float), pos.shape[0], coord, _p(Wc, ctypes.c_float),
-                     _p(number, ctypes.c_double), dims, float(BoxSize), MAS_INDEX[MAS])
+    args = (_p(pos, ctypes.c_float), pos.shape[0], coord, _p(Wc, ctypes.c_float),
+            _p(number, ctypes.c_double), dims, float(BoxSize), MAS_INDEX[MAS])
+    if threads > 1:
+        lib().ora_ma_f64_threads(*args, int(threads))
+    else:
+        lib().ora_ma_f64(*args)
     return number
 
 
@@ -135,6 +144,35 @@ def ma_reference(pos, number, BoxSize, MAS='CIC', W=None, flavour='strict', thre
     getattr(_REF[flavour], MAS)(_p(pos, ctypes.c_float), _p(number, ctypes.c_float),
                                 _p(Wc, ctypes.c_float), pos.shape[0], number.shape[0],
                                 pos.shape[1], float(BoxSize), int(threads))
+    return number
+
+
+_CY = {}
+
+
+def cython_available(flavour='strict'):
+    import glob
+    return bool(glob.glob(os.path.join(HERE, '_ref', 'cy_%s' % flavour, 'MAS_library*.so')))
+
+
+def cython_module(flavour='strict'):
+    """The reference's own Cython MAS_library (MAS_library.pyx + MAS_c.c), translated
+    and compiled from the reference sources into oracle/_ref by `make cython`."""
+    if flavour not in _CY:
+        import glob
+        import importlib.util
+        path = glob.glob(os.path.join(HERE, '_ref', 'cy_%s' % flavour, 'MAS_library*.so'))[0]
+        spec = importlib.util.spec_from_file_location('MAS_library', path)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _CY[flavour] = mod
+    return _CY[flavour]
+
+
+def ma_cython(pos, number, BoxSize, MAS='CIC', W=None, renormalize_2D=True, flavour='strict'):
+    """MA() of the reference module itself (MAS_library.pyx:57-112): serial loops."""
+    cython_module(flavour).MA(pos, number, BoxSize, MAS=MAS, W=W, verbose=False,
+                              renormalize_2D=renormalize_2D)
     return number
 
 
@@ -188,8 +226,10 @@ def expected_modes(dims):
     return (dims ** 3 - own) // 2 + own
 
 
-def raw_bins(dk_list, dims, axis, mas_list, is_f64=False):
-    """Run the reference mode loop on F half-complex fields; return raw sums."""
+def raw_bins(dk_list, dims, axis, mas_list, is_f64=False, threads=1):
+    """Run the reference mode loop on F half-complex fields; return raw sums.
+    threads > 1 splits the loop over kx into threads with private sums added in
+    thread order (deterministic; sums differ from the serial loop at 1e-16)."""
     F = len(dk_list)
     S = F + F * (F - 1) // 2
     middle = dims // 2
@@ -198,7 +238,10 @@ def raw_bins(dk_list, dims, axis, mas_list, is_f64=False):
     kmax = int(np.sqrt(3 * middle ** 2))
     nb2 = (kmax_par + 1) * (kmax_per + 1)
     ct = np.complex128 if is_f64 else np.complex64
-    dk = np.ascontiguousarray(np.stack([np.asarray(d, dtype=ct) for d in dk_list]))
+    if F == 1:      # no stacked copy (a 1024^3 complex128 field is 8.6 GB)
+        dk = np.ascontiguousarray(np.asarray(dk_list[0], dtype=ct))[None]
+    else:
+        dk = np.ascontiguousarray(np.stack([np.asarray(d, dtype=ct) for d in dk_list]))
     assert dk.shape[1:] == (dims, dims, middle + 1)
     out = dict(
         k1D=np.zeros(kmax_par + 1), Pk1D=np.zeros((kmax_par + 1, S)), N1D=np.zeros(kmax_par + 1),
@@ -207,14 +250,16 @@ def raw_bins(dk_list, dims, axis, mas_list, is_f64=False):
         N3D=np.zeros(kmax + 1))
     mas = (ctypes.c_int * F)(*[MAS_INDEX.get(m, 0) if isinstance(m, str) else int(m)
                                for m in mas_list])
-    d = ctypes.POINTER(ctypes.c_double)
-    lib().ora_pk_loop(dk.ctypes.data, 1 if is_f64 else 0, F, dims, axis, mas,
-                      _p(out['k1D'], ctypes.c_double), _p(out['Pk1D'], ctypes.c_double),
-                      _p(out['N1D'], ctypes.c_double), _p(out['Pk2D'], ctypes.c_double),
-                      _p(out['N2D'], ctypes.c_double), _p(out['k3D'], ctypes.c_double),
-                      _p(out['Pk3D'], ctypes.c_double), _p(out['Pkphase'], ctypes.c_double),
-                      _p(out['N3D'], ctypes.c_double))
-    del d
+    args = (dk.ctypes.data, 1 if is_f64 else 0, F, dims, axis, mas,
+            _p(out['k1D'], ctypes.c_double), _p(out['Pk1D'], ctypes.c_double),
+            _p(out['N1D'], ctypes.c_double), _p(out['Pk2D'], ctypes.c_double),
+            _p(out['N2D'], ctypes.c_double), _p(out['k3D'], ctypes.c_double),
+            _p(out['Pk3D'], ctypes.c_double), _p(out['Pkphase'], ctypes.c_double),
+            _p(out['N3D'], ctypes.c_double))
+    if threads > 1:
+        lib().ora_pk_loop_threads(*args, int(threads))
+    else:
+        lib().ora_pk_loop(*args)
     return out
 
 
@@ -262,7 +307,7 @@ def finalize_pk(raw, dims, BoxSize, spectra):
     return r
 
 
-def pk(delta, BoxSize, axis=2, MAS='CIC', f64=False, dk=None):
+def pk(delta, BoxSize, axis=2, MAS='CIC', f64=False, dk=None, workers=None, threads=1):
     """Pk_library.Pk restated (float32 FFT + reference loop) or the FP64 oracle.
 
     Returns an object with the reference attributes (k3D, Pk (kmax,3), Nmodes3D,
@@ -270,8 +315,8 @@ def pk(delta, BoxSize, axis=2, MAS='CIC', f64=False, dk=None):
     """
     dims = delta.shape[0]
     if dk is None:
-        dk = fft_f64(delta) if f64 else fft_f32(delta)
-    raw = raw_bins([dk], dims, axis, [MAS], is_f64=f64)
+        dk = fft_f64(delta, workers) if f64 else fft_f32(delta, workers)
+    raw = raw_bins([dk], dims, axis, [MAS], is_f64=f64, threads=threads)
     r = finalize_pk(raw, dims, BoxSize, [0])
     r.Pk = r.Pk[:, :, 0]
     r.Pk1D = r.Pk1D[:, 0]
@@ -280,13 +325,13 @@ def pk(delta, BoxSize, axis=2, MAS='CIC', f64=False, dk=None):
     return r
 
 
-def xpk(deltas, BoxSize, axis=2, MAS=None, f64=False, dks=None):
+def xpk(deltas, BoxSize, axis=2, MAS=None, f64=False, dks=None, workers=None, threads=1):
     """Pk_library.XPk restated.  Pk (kmax,3,F), XPk (kmax,3,F(F-1)/2), etc."""
     F = len(deltas)
     dims = deltas[0].shape[0]
     if dks is None:
-        dks = [fft_f64(d) if f64 else fft_f32(d) for d in deltas]
-    raw = raw_bins(dks, dims, axis, MAS, is_f64=f64)
+        dks = [fft_f64(d, workers) if f64 else fft_f32(d, workers) for d in deltas]
+    raw = raw_bins(dks, dims, axis, MAS, is_f64=f64, threads=threads)
     autos = list(range(F))
     crosses = list(range(F, F + F * (F - 1) // 2))
     a = finalize_pk(raw, dims, BoxSize, autos)
@@ -306,13 +351,14 @@ def xpk(deltas, BoxSize, axis=2, MAS=None, f64=False, dks=None):
 
 
 # ----------------------------------------------------------------------------- brute force
-def brute_mode_counts(dims, axis):
-    """Enumerate all dims^3 modes, keep one of each Hermitian pair (plus the
-    self-conjugate ones), and histogram them into the reference's 1D/2D/3D bins.
-    Independent of the reference loop's half-space skip rule."""
+def brute_modes(dims, axis):
+    """Enumerate all dims^3 modes and keep one of each Hermitian pair (plus the
+    self-conjugate ones): the independent modes, found without the reference
+    loop's half-space skip rule.  Returns a dict of per-mode integer arrays
+    kx, ky, kz (signed; a component equal to middle stays +middle for even dims),
+    k2, kidx = floor(|k|), kpar = |k_par|, kper = floor(|k_perp|), i2 (2D bin)."""
     middle = dims // 2
-    kmax_par, kmax_per = middle, int(np.sqrt(2 * middle ** 2))
-    kmax = int(np.sqrt(3 * middle ** 2))
+    kmax_par = middle
     k = np.arange(dims)
     k = np.where(k > middle, k - dims, k)
     KX, KY, KZ = np.meshgrid(k, k, k, indexing='ij')
@@ -321,19 +367,36 @@ def brute_mode_counts(dims, axis):
     fj = ((-KX) % dims) * dims * dims + ((-KY) % dims) * dims + ((-KZ) % dims)
     keep = fi <= fj
     kx, ky, kz = KX[keep], KY[keep], KZ[keep]
-    # sign convention of the reference: component == middle stays +middle (even dims)
     k2 = kx * kx + ky * ky + kz * kz
     kidx = np.floor(np.sqrt(k2)).astype(np.int64)
     par = [kx, ky, kz][axis]
     perp2 = k2 - par * par
     kper = np.floor(np.sqrt(perp2)).astype(np.int64)
     kpar = np.abs(par)
-    N3D = np.bincount(kidx, minlength=kmax + 1).astype(np.float64)
-    N2D = np.bincount((kmax_par + 1) * kper + kpar,
-                      minlength=(kmax_par + 1) * (kmax_per + 1)).astype(np.float64)
-    m1 = k2 <= middle * middle
-    N1D = np.bincount(kpar[m1], minlength=kmax_par + 1).astype(np.float64)
+    return dict(kx=kx, ky=ky, kz=kz, k2=k2, kidx=kidx, kpar=kpar, kper=kper,
+                i2=(kmax_par + 1) * kper + kpar)
+
+
+def brute_mode_counts(dims, axis):
+    """Histogram of the independent modes (brute_modes) into the reference's
+    1D/2D/3D bins: (N1D, N2D, N3D)."""
+    middle = dims // 2
+    kmax_par, kmax_per = middle, int(np.sqrt(2 * middle ** 2))
+    kmax = int(np.sqrt(3 * middle ** 2))
+    m = brute_modes(dims, axis)
+    N3D = np.bincount(m['kidx'], minlength=kmax + 1).astype(np.float64)
+    N2D = np.bincount(m['i2'], minlength=(kmax_par + 1) * (kmax_per + 1)).astype(np.float64)
+    m1 = m['k2'] <= middle * middle
+    N1D = np.bincount(m['kpar'][m1], minlength=kmax_par + 1).astype(np.float64)
     return N1D, N2D, N3D
+
+
+def brute_k3d_sums(dims, axis=0):
+    """Sum of |k| (grid units, float64 sqrt of the integer k^2) over the
+    independent modes of each 3D bin."""
+    kmax = int(np.sqrt(3 * (dims // 2) ** 2))
+    m = brute_modes(dims, axis)
+    return np.bincount(m['kidx'], weights=np.sqrt(m['k2'].astype(np.float64)), minlength=kmax + 1)
 
 
 def direct_dft_r2c(x):
@@ -376,7 +439,10 @@ def raw_plane_bins(dk_list, grid, mas_list, is_f64=False):
     S = 1 if F == 1 else 3
     kmax = int(np.sqrt(2 * (grid // 2) ** 2))
     ct = np.complex128 if is_f64 else np.complex64
-    dk = np.ascontiguousarray(np.stack([np.asarray(d, dtype=ct) for d in dk_list]))
+    if F == 1:      # no stacked copy (a 1024^3 complex128 field is 8.6 GB)
+        dk = np.ascontiguousarray(np.asarray(dk_list[0], dtype=ct))[None]
+    else:
+        dk = np.ascontiguousarray(np.stack([np.asarray(d, dtype=ct) for d in dk_list]))
     assert dk.shape[1:] == (grid, grid // 2 + 1)
     out = dict(k=np.zeros(kmax + 1), Pk=np.zeros((kmax + 1, S)), N=np.zeros(kmax + 1))
     mas = (ctypes.c_int * F)(*[MAS_INDEX.get(m, 0) if isinstance(m, str) or m is None else int(m)

@@ -23,6 +23,7 @@ MA_OVERWRITE = 0x0010
 NO_SYNC = 0x0020
 PK_DESTROY_INPUT = 0x0040
 PK_NO_PHASE = 0x0080
+PK_SCALES_DEVICE = 0x0100
 
 STAGES = ['ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit', 'ma_atomic', 'pk_prep', 'pk_fft',
           'pk_bin', 'ma_final']
@@ -307,6 +308,15 @@ class torch_stream:
     def __exit__(self, *exc):
         if self.active:
             self.ctx.set_stream(None)
-            if self.legacy and exc[0] is None:
-                self.ctx.synchronize()
+            if self.legacy:
+                if exc[0] is None:
+                    self.ctx.synchronize()
+                else:
+                    # Kernels already queued may still write torch-owned outputs:
+                    # finish them before the caller sees the error.  The error
+                    # being raised is the one that matters.
+                    try:
+                        self.ctx.synchronize()
+                    except Exception:
+                        pass
         return False

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libpylians_hip.so')
 OBJDIR = os.path.join(HERE, '_build')
-SOURCES = ['ctx.hip', 'ma.hip', 'pk.hip', 'batch.hip', 'plane.hip', 'fft.hip']
+SOURCES = ['ctx.hip', 'ma.hip', 'pk.hip', 'plane.hip', 'fft.hip']
 ARCH = os.environ.get('PYLIANS_HIP_ARCH', 'gfx950')
 
 HIPCC = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'

@@ -245,7 +245,9 @@ int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
     c->ev_ready = true;
     PYL_HIP(hipEventCreateWithFlags(&c->pin_up.done, hipEventDisableTiming));
     PYL_HIP(hipEventCreateWithFlags(&c->pin_down.done, hipEventDisableTiming));
+    PYL_HIP(hipEventCreateWithFlags(&c->pin_scales.done, hipEventDisableTiming));
     PYL_HIP(hipEventCreateWithFlags(&c->ev_total, hipEventDisableTiming));
+    PYL_HIP(hipEventCreateWithFlags(&c->ev_switch, hipEventDisableTiming));
     PYL_HIP(hipHostMalloc((void **)&c->h_total, 2 * sizeof(unsigned long long), hipHostMallocDefault));
     c->h_total[1] = 0;
     *out = h.release();
@@ -274,7 +276,7 @@ int pyl_ctx_trim(pyl_ctx *ctx) {
     c->twiddles.clear();
     for (auto &kv : c->mas_tabs) (void)hipFree(kv.second);
     c->mas_tabs.clear();
-    for (PinnedBuf *b : {&c->pin_up, &c->pin_down}) {
+    for (PinnedBuf *b : {&c->pin_up, &c->pin_down, &c->pin_scales}) {
         if (b->ptr) (void)hipHostFree(b->ptr);
         b->ptr = nullptr;
         b->cap = 0;
@@ -293,7 +295,7 @@ int pyl_ctx_destroy(pyl_ctx *ctx) {
                 if (c->ev[i][k][0]) (void)hipEventDestroy(c->ev[i][k][0]);
                 if (c->ev[i][k][1]) (void)hipEventDestroy(c->ev[i][k][1]);
             }
-    for (hipEvent_t e : {c->pin_up.done, c->pin_down.done, c->ev_total})
+    for (hipEvent_t e : {c->pin_up.done, c->pin_down.done, c->pin_scales.done, c->ev_total, c->ev_switch})
         if (e) (void)hipEventDestroy(e);
     if (c->h_total) (void)hipHostFree(c->h_total);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -303,7 +305,16 @@ int pyl_ctx_destroy(pyl_ctx *ctx) {
 
 int pyl_ctx_set_stream(pyl_ctx *ctx, void *hip_stream) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
-    ctx->c.stream = hip_stream ? (hipStream_t)hip_stream : ctx->c.own_stream;
+    Ctx *c = &ctx->c;
+    hipStream_t next = hip_stream ? (hipStream_t)hip_stream : c->own_stream;
+    if (next != c->stream) {
+        // The scratch buffers are shared by every call on the context: the new
+        // stream must not start on them while the old stream's kernels still run.
+        PYL_HIP(hipSetDevice(c->device));
+        PYL_HIP(hipEventRecord(c->ev_switch, c->stream));
+        PYL_HIP(hipStreamWaitEvent(next, c->ev_switch, 0));
+    }
+    c->stream = next;
     return PYL_OK;
 }
 

@@ -617,6 +617,7 @@ scan_final_kernel(const unsigned *__restrict__ in, const unsigned *__restrict__ 
 struct OutMode {
     int overwrite;
     float scale, shift;
+    float prescale;     // > 0: the grid is first divided by it (2D renormalisation), host side only
 };
 
 __device__ __forceinline__ void store_node(float *__restrict__ grid, long long node, float v,
@@ -1574,6 +1575,72 @@ int grid_blocks(long long n, int block) {
     return (int)b;
 }
 
+// Clears nwords 4-byte words at p (a kernel rather than hipMemsetAsync, so that
+// a caller may capture the call in a HIP graph); 16-byte stores when p allows.
+template <bool V16>
+__global__ void zero_words_kernel(unsigned *__restrict__ p, long long nwords) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V16) {
+        const long long n16 = nwords >> 2;
+        for (long long i = t; i < n16; i += stride) ((uint4 *)p)[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (t < (nwords & 3)) p[n16 * 4 + t] = 0u;
+    } else {
+        for (long long i = t; i < nwords; i += stride) p[i] = 0u;
+    }
+}
+
+// The 2D renormalisation's division of the existing grid (OutMode::prescale).
+int prescale_grid(Ctx *c, float *grid, long long ncell, const OutMode &om) {
+    if (om.prescale <= 0.0f) return PYL_OK;
+    scale_grid_kernel<<<grid_blocks(ncell, kBlock), kBlock, 0, c->stream>>>(grid, ncell, om.prescale);
+    PYL_HIP(hipGetLastError());
+    return PYL_OK;
+}
+
+int zero_bytes(Ctx *c, void *p, size_t bytes) {
+    const long long nwords = (long long)(bytes / 4);      // every region here is whole words
+    if (nwords == 0) return PYL_OK;
+    if (((uintptr_t)p & 15) == 0)
+        zero_words_kernel<true><<<grid_blocks((nwords + 3) / 4, kBlock), kBlock, 0, c->stream>>>(
+            (unsigned *)p, nwords);
+    else
+        zero_words_kernel<false><<<grid_blocks(nwords, kBlock), kBlock, 0, c->stream>>>(
+            (unsigned *)p, nwords);
+    PYL_HIP(hipGetLastError());
+    return PYL_OK;
+}
+
+// Chunk table and tile origins of one realisation, written on the device (the
+// fixed-scratch path has no pinned staging to upload them from).
+__global__ void chunk_table_kernel(Chunk *__restrict__ ch, long long n, long long target, int nchunk,
+                                   int *__restrict__ orig, int nt, int dims) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nchunk) {
+        const long long s = (long long)i * target;
+        ch[i] = Chunk{s, s + target < n ? s + target : n, 0, 0, 0, 0, 0, 0};
+    }
+    if (i <= nt) orig[i] = tile_origin(i, nt, dims);
+}
+
+// Scratch piece `id` of a call: the context's grow-only buffer, or, while
+// pyl_ma_noalloc runs (c->arena set), the next 256-byte-aligned piece of its
+// fixed device array.
+int take(Ctx *c, BufId id, size_t bytes, void **out) {
+    if (c->arena) {
+        const size_t at = (c->arena_used + 255) & ~(size_t)255;
+        PYL_CHECK(at + bytes <= c->arena_cap, PYL_EBUDGET,
+                  "pyl_ma_noalloc: the call needs more than the %zu bytes of fixed scratch "
+                  "(%zu bytes at least)", c->arena_cap, at + bytes);
+        *out = c->arena + at;
+        c->arena_used = at + bytes;
+        return PYL_OK;
+    }
+    PYL_TRY(ensure(c, id, bytes));
+    *out = c->bufs[id].ptr;
+    return PYL_OK;
+}
+
 TileGeom make_geom(int dims, int axes, int mas) {
     TileGeom g{};
     const int T = axes == 3 ? kT3 : kT2;
@@ -1656,9 +1723,27 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const size_t org_bytes = ((size_t)(g.nt + 1) * sizeof(int) + 255) & ~(size_t)255;
     const size_t ord_bytes = ((size_t)nt * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t dump_bytes = (size_t)std::max(nchunk, 1) * ws * sizeof(unsigned);
-    PYL_TRY(ensure(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
-                                        ord_bytes + dump_bytes));
-    char *tb = (char *)c->bufs[BUF_TILE_CNT].ptr;
+    // Few tiles with many records each (a small grid and many particles): each
+    // tile's records are split over nsplit workgroups, which add their boxes into
+    // per-tile int64 boxes, and a second pass writes the grid.  Otherwise, largest
+    // tiles first, when there are enough tiles, and records per tile, for the
+    // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys).
+    int nsplit = 1;
+    if (nt < kSplitMaxKeys && n > kSplitRecords * nt)
+        nsplit = (int)std::min<long long>(kSplitMax, (n + kSplitRecords * nt - 1) / (kSplitRecords * nt));
+    const bool ordered = nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt;
+    const int T = axes == 3 ? kT3 : kT2;
+    const size_t nbox = (size_t)(T + MAS - 1) * (T + MAS - 1) * (axes == 3 ? T + MAS - 1 : 1);
+    const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
+    // all scratch of the call is taken before the first launch
+    void *tbv = nullptr, *pv = nullptr, *rv = nullptr, *bv = nullptr;
+    PYL_TRY(take(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
+                                      ord_bytes + dump_bytes, &tbv));
+    PYL_TRY(take(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float), &pv));
+    PYL_TRY(take(c, BUF_RECORDS, (size_t)(n ? n : 1) * rec_bytes, &rv));
+    if (nsplit > 1)
+        PYL_TRY(take(c, BUF_TILE_BOX, (size_t)nt * nbox * sizeof(unsigned long long), &bv));
+    char *tb = (char *)tbv;
     unsigned *counts = (unsigned *)tb;
     unsigned *ocounts = counts + nt;
     unsigned *ocursors = ocounts + nt;
@@ -1671,21 +1756,31 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     unsigned *order = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes);
     unsigned *dump = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes + ord_bytes);
     g.orig = dorig;
-    PYL_TRY(ensure(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float)));
-    float *partials = (float *)c->bufs[BUF_PARTIAL].ptr;
+    float *partials = (float *)pv;
+    float *records = (float *)rv;
+    unsigned long long *gbox = (unsigned long long *)bv;
 
+    PYL_TRY(prescale_grid(c, grid, g.ncell * nreal, om));
     // the whole 256-byte-rounded region: one aligned fill instead of a body and a tail
-    PYL_HIP(hipMemsetAsync(counts, 0, cnt_bytes, c->stream));
+    PYL_TRY(zero_bytes(c, counts, cnt_bytes));
     // Chunk table and tile origins: the kernels only read their particle ranges and
     // origins (the count pass rewrites each chunk's window fields), so a call with
     // the same ranges and grid in the same allocation reuses the uploaded tables.
+    // The fixed-scratch path writes them on the device every call.
     Ctx::TileTables &tt = c->tile_tables;
-    bool same = tt.gen == c->bufs[BUF_TILE_CNT].gen && tt.n == n && tt.nchunk == nchunk &&
+    if (c->arena) {
+        PYL_CHECK(nreal == 1, PYL_EINVAL, "fixed-scratch MA takes one grid");
+        const int nfill = std::max(nchunk, g.nt + 1);
+        chunk_table_kernel<<<ceil_div(nfill, kBlock), kBlock, 0, c->stream>>>(dchunks, n, target, nchunk,
+                                                                           dorig, g.nt, dims);
+        PYL_HIP(hipGetLastError());
+    }
+    bool same = !c->arena && tt.gen == c->bufs[BUF_TILE_CNT].gen && tt.n == n && tt.nchunk == nchunk &&
                 tt.target == target &&
                 tt.nreal == nreal && tt.dims == dims && tt.axes == axes && tt.mas == MAS &&
                 tt.offs.size() == (h_offs ? (size_t)nreal + 1 : 0);
     for (size_t i = 0; same && i < tt.offs.size(); ++i) same = tt.offs[i] == h_offs[i];
-    if (!same) {
+    if (!same && !c->arena) {
         tt.gen = 0;                               // valid again once both are queued
         static thread_local std::vector<Chunk> chunks;   // reused: no fresh pages per call
         chunks.clear();
@@ -1734,15 +1829,6 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
     scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
                                                                    offsets);
-    // Few tiles with many records each (a small grid and many particles): each
-    // tile's records are split over nsplit workgroups, which add their boxes into
-    // per-tile int64 boxes, and a second pass writes the grid.  Otherwise, largest
-    // tiles first, when there are enough tiles, and records per tile, for the
-    // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys).
-    int nsplit = 1;
-    if (nt < kSplitMaxKeys && n > kSplitRecords * nt)
-        nsplit = (int)std::min<long long>(kSplitMax, (n + kSplitRecords * nt - 1) / (kSplitRecords * nt));
-    const bool ordered = nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt;
     if (ordered) tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order);
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
@@ -1761,9 +1847,6 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     }
     PYL_HIP(hipEventRecord(c->ev_total, c->stream));
     c->total_expect = n;
-    const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
-    PYL_TRY(ensure(c, BUF_RECORDS, (size_t)(n ? n : 1) * rec_bytes));
-    float *records = (float *)c->bufs[BUF_RECORDS].ptr;
     const unsigned long long rcap = (unsigned long long)n;
     if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
@@ -1801,14 +1884,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                : (unit ? tile_deposit_kernel<MAS, 2, false, false, false>
                        : (W ? tile_deposit_kernel<MAS, 2, true, true, false>
                             : tile_deposit_kernel<MAS, 2, true, false, false>)));
-    unsigned long long *gbox = nullptr;
-    if (nsplit > 1) {
-        const int T = axes == 3 ? kT3 : kT2;
-        const size_t nbox = (size_t)(T + MAS - 1) * (T + MAS - 1) * (axes == 3 ? T + MAS - 1 : 1);
-        PYL_TRY(ensure(c, BUF_TILE_BOX, (size_t)nt * nbox * sizeof(unsigned long long)));
-        gbox = (unsigned long long *)c->bufs[BUF_TILE_BOX].ptr;
-        PYL_HIP(hipMemsetAsync(gbox, 0, (size_t)nt * nbox * sizeof(unsigned long long), c->stream));
-    }
+    if (nsplit > 1) PYL_TRY(zero_bytes(c, gbox, (size_t)nt * nbox * sizeof(unsigned long long)));
     kd<<<(unsigned)(nt * nsplit), kDepBlock, 0, c->stream>>>(records, rcap, offsets, g, weight_factor,
                                                             W ? wmax : nullptr, grid, partials, om,
                                                             ordered ? order : nullptr, nsplit, gbox);
@@ -1846,8 +1922,8 @@ int run_atomic(Ctx *c, const float *pos, const RSD &rsd, long long n, const long
                int nreal, int axes, const float *W, float *grid, int dims, float inv,
                float weight_factor, const OutMode &om) {
     const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1);
-    if (om.overwrite)
-        PYL_HIP(hipMemsetAsync(grid, 0, (size_t)ncell * nreal * sizeof(float), c->stream));
+    PYL_TRY(prescale_grid(c, grid, ncell * nreal, om));
+    if (om.overwrite) PYL_TRY(zero_bytes(c, grid, (size_t)ncell * nreal * sizeof(float)));
     PYL_TRY(stage_begin(c, PYL_STAGE_MA_ATOMIC));
     for (int r = 0; r < nreal; ++r) {
         const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
@@ -1892,17 +1968,14 @@ int ma_device(Ctx *c, const float *pos, long long n, const long long *h_offs, in
     const float inv = (float)dims / BoxSize;
     float weight_factor = 1.0f;
     const long long ncell = (long long)dims * dims * (axes == 3 ? dims : 1) * nreal;
-    OutMode om{(flags & PYL_MA_OVERWRITE) ? 1 : 0, scale, shift};
+    OutMode om{(flags & PYL_MA_OVERWRITE) ? 1 : 0, scale, shift, 0.0f};
     if (axes == 2 && (flags & PYL_MA_CYTHON_2D) && mas != PYL_NGP) {
         // MA() runs the 3D kernel on a (d,d,1) view, so each deposit lands F times
         // in the same cell; with renormalize_2D the whole grid is then divided by F
         // (MAS_library.pyx:84-110).  Equivalent: grid/F + deposit, or grid + F*deposit.
+        // The division is launched by the algorithm once its scratch is in hand.
         if (flags & PYL_MA_RENORM_2D) {
-            if (!om.overwrite) {
-                scale_grid_kernel<<<grid_blocks(ncell, kBlock), kBlock, 0, c->stream>>>(
-                    grid, ncell, (float)mas);
-                PYL_HIP(hipGetLastError());
-            }
+            if (!om.overwrite) om.prescale = (float)mas;
         } else {
             weight_factor = (float)mas;
         }

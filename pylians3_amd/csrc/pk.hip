@@ -565,12 +565,12 @@ __global__ void pk_reduce1_kernel(ReduceArgs a) {
 __global__ void prologue_kernel(const float *__restrict__ in, float *__restrict__ out,
                                 long long n, const float *__restrict__ scales,
                                 const float *__restrict__ shifts, float scale, float shift,
-                                int apply) {
+                                int apply, int sstride = 1) {
     const long long cube = blockIdx.y;
     in += cube * n;
     out += cube * n;
-    if (scales) scale = scales[cube];
-    if (shifts) shift = shifts[cube];
+    if (scales) scale = scales[cube * sstride];
+    if (shifts) shift = shifts[cube * sstride];
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         float x = in[i];
@@ -1041,21 +1041,33 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfield
         PYL_TRY(ensure(c, BUF_FFT_IN, (size_t)n * ncube * sizeof(float)));
         in = (float *)c->bufs[BUF_FFT_IN].ptr;
     }
-    float *dsc = nullptr, *dsh = nullptr;
-    if (apply) {
-        // caller order [r][f] -> cube order [f][r]
-        std::vector<float> sc(ncube, 1.0f), sh(ncube, 0.0f);
+    // Per-cube scale/shift: caller order [r][f].  Device arrays are read in place
+    // (stride nfields from field f); host arrays are uploaded through pinned
+    // staging in cube order [f][r] (no wait for the work already queued).
+    const float *dsc = nullptr, *dsh = nullptr;
+    int sstride = 1;
+    const bool sdev = (flags & PYL_PK_SCALES_DEVICE) != 0;
+    if (apply && sdev) {
+        PYL_CHECK(scales != nullptr && shifts != nullptr, PYL_EINVAL,
+                  "PYL_PK_SCALES_DEVICE needs both scales and shifts");
+        dsc = scales;
+        dsh = shifts;
+        sstride = nfields;
+    } else if (apply) {
+        PYL_TRY(ensure(c, BUF_SCAN_TMP, sizeof(float) * 2 * (size_t)ncube));
+        float *d = (float *)c->bufs[BUF_SCAN_TMP].ptr;
+        void *stage = nullptr;
+        PYL_TRY(pinned(c, c->pin_scales, sizeof(float) * 2 * (size_t)ncube, &stage));
+        float *sc = (float *)stage, *sh = sc + ncube;
         for (int r = 0; r < nreal; ++r)
             for (int f = 0; f < nfields; ++f) {
-                if (scales) sc[(size_t)f * nreal + r] = scales[(size_t)r * nfields + f];
-                if (shifts) sh[(size_t)f * nreal + r] = shifts[(size_t)r * nfields + f];
+                sc[(size_t)f * nreal + r] = scales ? scales[(size_t)r * nfields + f] : 1.0f;
+                sh[(size_t)f * nreal + r] = shifts ? shifts[(size_t)r * nfields + f] : 0.0f;
             }
-        PYL_TRY(ensure(c, BUF_SCAN_TMP, sizeof(float) * 2 * (size_t)ncube));
-        dsc = (float *)c->bufs[BUF_SCAN_TMP].ptr;
-        dsh = dsc + ncube;
-        PYL_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(float) * ncube, hipMemcpyHostToDevice, c->stream));
-        PYL_HIP(hipMemcpyAsync(dsh, sh.data(), sizeof(float) * ncube, hipMemcpyHostToDevice, c->stream));
-        PYL_HIP(hipStreamSynchronize(c->stream));   // sc/sh are host temporaries
+        PYL_HIP(hipMemcpyAsync(d, stage, sizeof(float) * 2 * ncube, hipMemcpyHostToDevice, c->stream));
+        PYL_TRY(pinned_done(c, c->pin_scales));
+        dsc = d;
+        dsh = d + ncube;
     }
     const int nbx = (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048));
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_PREP));
@@ -1065,8 +1077,9 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfield
             PYL_HIP(hipMemcpyAsync(dst, deltas[f], (size_t)n * nreal * sizeof(float),
                                    dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
         if (apply) {
+            const size_t off = sdev ? (size_t)f : (size_t)f * nreal;
             prologue_kernel<<<dim3(nbx, nreal), 256, 0, c->stream>>>(
-                dst, dst, n, dsc + (size_t)f * nreal, dsh + (size_t)f * nreal, 1.0f, 0.0f, 1);
+                dst, dst, n, dsc + off, dsh + off, 1.0f, 0.0f, 1, sstride);
             PYL_HIP(hipGetLastError());
         }
     }

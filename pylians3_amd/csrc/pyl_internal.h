@@ -131,6 +131,7 @@ struct Ctx {
     } tile_tables;
     PinnedBuf pin_up;                                        // MA chunk table + tile origins
     PinnedBuf pin_down;                                      // Pk bin sums
+    PinnedBuf pin_scales;                                    // Pk batch scale/shift upload
     // Tile-sort consistency check of the latest tiled MA, read back without a
     // synchronisation of its own: checked at the next synchronising call.
     unsigned long long *h_total = nullptr;                   // pinned: [0] record total, [1] box misses
@@ -138,6 +139,14 @@ struct Ctx {
     bool wide_hint = false;          // next eligible MA takes the wide sort window
     long long total_expect = -1;
     hipEvent_t ev_total = nullptr;
+    // Recorded on the outgoing stream when pyl_ctx_set_stream changes streams; the
+    // incoming stream waits for it, so work of the next call (which reuses the same
+    // scratch buffers) starts only after the previous call's work is done.
+    hipEvent_t ev_switch = nullptr;
+    // Caller-provided device scratch: while set, MA scratch is cut from `arena`
+    // (arena_used bytes so far) instead of the grow-only buffers.
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
     // Stage timing: a ring of event pairs per stage, so that timing many calls
     // needs no host synchronisation between them.  A pair is read back when its
     // ring slot comes round again (kEvRing calls later) or on request.
@@ -154,6 +163,7 @@ struct Ctx {
 };
 
 size_t held_bytes(const Ctx *c);
+
 int ensure(Ctx *c, BufId id, size_t bytes);
 void reset_stages(Ctx *c);
 int stage_begin(Ctx *c, int stage);

@@ -24,6 +24,7 @@ the binning are order-independent/deterministic.
 """
 import collections
 import ctypes
+import queue
 import threading
 import time
 from concurrent.futures import Future
@@ -41,6 +42,7 @@ class Request:
         self.W, self.pos2, self.W2 = W, pos2, W2
         self.t_submit = None
         self.t_start = None
+        self.ready = None      # event on the submitter's stream after which the inputs are complete
 
     @property
     def nfields(self):
@@ -100,6 +102,12 @@ class Server:
             c.set_stream(s.cuda_stream)
         self.stop = False
         self.completed = []
+        # one resident worker thread per slot, fed by the dispatcher
+        self.work = [queue.SimpleQueue() for _ in range(slots)]
+        self.workers = [threading.Thread(target=self._worker, args=(i,), daemon=True)
+                        for i in range(slots)]
+        for w in self.workers:
+            w.start()
         self.dispatcher = threading.Thread(target=self._dispatch, daemon=True)
         self.dispatcher.start()
 
@@ -109,6 +117,12 @@ class Server:
             raise N.BudgetError(N.PYL_EBUDGET, 'request needs %d bytes, budget is %d'
                                 % (req.hbm_bytes(), self.budget))
         fut = Future()
+        if any(N.is_device_array(a) for a in (req.pos, req.W, req.pos2, req.W2)):
+            # the slot stream must not read the inputs before the work that wrote
+            # them on the submitter's stream is done
+            import torch
+            req.ready = torch.cuda.Event()
+            req.ready.record(torch.cuda.current_stream())
         req.t_submit = time.perf_counter()
         with self.cv:
             self.queue.append((req, fut))
@@ -120,6 +134,10 @@ class Server:
             self.stop = True
             self.cv.notify_all()
         self.dispatcher.join()
+        for q in self.work:
+            q.put(None)
+        for w in self.workers:
+            w.join()
         for c in self.ctxs:
             c.close()
 
@@ -179,8 +197,14 @@ class Server:
                     self.queue = keep
                 slot = self.free_slots.pop()
                 self.in_flight_bytes += need
-            t = threading.Thread(target=self._run, args=(slot, group, need), daemon=True)
-            t.start()
+            self.work[slot].put((group, need))
+
+    def _worker(self, slot):
+        while True:
+            item = self.work[slot].get()
+            if item is None:
+                return
+            self._run(slot, *item)
 
     def _run(self, slot, group, need):
         ctx = self.ctxs[slot]
@@ -212,11 +236,16 @@ class Server:
         return N.device_ptr(a) if N.is_device_array(a) else a.ctypes.data
 
     def _grid_fields(self, ctx, reqs, grids):
-        """MA of field f of every request into grids[f] (nreal, d, d, d), overwrite."""
+        """MA of field f of every request into grids[f] (nreal, d, d, d), overwrite.
+        Returns the mean density of every (request, field) as a device float32
+        (nreal, F) tensor, as the reference caller computes it (Pk_snapshot.py:77-83):
+        len(pos)/dims**3, or sum(W)/dims**3 (float64 sum), rounded to float32; a
+        zero mean is replaced by 1.  Nothing here waits for the device."""
         import torch
         d = reqs[0].dims
         F = reqs[0].nfields
         mas = _MAS_IDS[reqs[0].MAS]
+        means = torch.empty((len(reqs), F), dtype=torch.float64, device='cuda')
         for f in range(F):
             pos = [r.pos if f == 0 else r.pos2 for r in reqs]
             Ws = [r.W if f == 0 else r.W2 for r in reqs]
@@ -227,44 +256,47 @@ class Server:
             P = P.contiguous()
             Wc = None
             if Ws[0] is not None:
-                Wc = torch.cat([torch.as_tensor(w, device='cuda') for w in Ws]) if len(reqs) > 1 \
-                    else torch.as_tensor(Ws[0], device='cuda')
+                Wd = [torch.as_tensor(w, device='cuda') for w in Ws]
+                Wc = torch.cat(Wd) if len(reqs) > 1 else Wd[0]
                 Wc = Wc.contiguous()
+                for i, w in enumerate(Wd):
+                    means[i, f] = w.double().sum() / float(d) ** 3
+            else:
+                for i, c in enumerate(counts):
+                    means[i, f] = float(c) / float(d) ** 3
             N.check(N.lib().pyl_ma_batch(ctx.handle, N.device_ptr(P), offs.ctypes.data, len(reqs), 3,
                                          None if Wc is None else N.device_ptr(Wc),
                                          N.device_ptr(grids[f]), d, reqs[0].BoxSize, mas,
-                                         N.PTR_DEVICE | N.MA_OVERWRITE))
-        # mean density per field as the reference caller computes it (Pk_snapshot.py:77-83):
-        # len(pos)/dims**3, or sum(M)/dims**3 with weights (float64 sum per request)
-        means = np.zeros((len(reqs), F), np.float64)
-        for i, r in enumerate(reqs):
-            for f in range(F):
-                p, w = (r.pos, r.W) if f == 0 else (r.pos2, r.W2)
-                tot = float(p.shape[0]) if w is None else \
-                    float(torch.as_tensor(w, device='cuda').double().sum().item())
-                means[i, f] = tot / float(d) ** 3
-        return means.astype(np.float32)
+                                         N.PTR_DEVICE | N.MA_OVERWRITE | N.NO_SYNC))
+        means = means.float()
+        return torch.where(means == 0, torch.ones_like(means), means).contiguous()
 
     def _run_batch(self, ctx, stream, reqs):
         import torch
         d = reqs[0].dims
         F = reqs[0].nfields
         with torch.cuda.stream(stream):
+            for r in reqs:
+                if r.ready is not None:
+                    stream.wait_event(r.ready)
+                    for a in (r.pos, r.W, r.pos2, r.W2):
+                        if N.is_device_array(a):
+                            a.record_stream(stream)
             grids = [torch.empty((len(reqs), d, d, d), dtype=torch.float32, device='cuda')
                      for _ in range(F)]
-            means = self._grid_fields(ctx, reqs, grids)
-            means[means == 0] = 1.0
+            scales = self._grid_fields(ctx, reqs, grids)
+            shifts = torch.ones_like(scales)
             mas = (ctypes.c_int * F)(*([MAS_function(reqs[0].MAS)] * F))
             allocs = [N.alloc_pk_out(d, F) for _ in reqs]
             outs = (N.PkOut * len(reqs))(*[a[0] for a in allocs])
-            sc = np.ascontiguousarray(means, dtype=np.float32)
-            sh = np.ones_like(sc)
             ptrs = (ctypes.c_void_p * F)(*[N.device_ptr(g) for g in grids])
+            # returns once the spectra are on the host (the call waits for its own work)
             N.check(N.lib().pyl_pk_batch(ctx.handle, ptrs, len(reqs), F, mas, d,
-                                         reqs[0].BoxSize, reqs[0].axis, sc.ctypes.data,
-                                         sh.ctypes.data, N.PTR_DEVICE | N.PK_DESTROY_INPUT, outs))
-            stream.synchronize()
-            del grids
+                                         reqs[0].BoxSize, reqs[0].axis, N.device_ptr(scales),
+                                         N.device_ptr(shifts),
+                                         N.PTR_DEVICE | N.PK_DESTROY_INPUT | N.PK_SCALES_DEVICE,
+                                         outs))
+            del grids, scales, shifts
         results = []
         X = F * (F - 1) // 2
         for _, arrs, S in allocs:

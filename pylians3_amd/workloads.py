@@ -145,6 +145,42 @@ def config_c2(device='cuda'):
     return zeldovich(512, 1000.0, 2, f_rsd=0.5, rsd_axis=0, device=device)
 
 
+def batched_realisations(nreal=256, dims=128, box=500.0, seed0=1000, device='cuda'):
+    """Batched workload: nreal independent realisations (box 500 Mpc/h, 128^3).
+    Realisation r (seed seed0 + r): a Zel'dovich-displaced dims^3 matter lattice
+    and N_r ~ Poisson(lambda_r) tracers (lambda_r log-uniform in [1e5, 2e6], N_r
+    clipped to that range) at q ~ U[0, box)^3 moved by the CIC-interpolated
+    displacement, with lognormal(0, 1) weights.  Returns a dict with the matter
+    positions and per-realisation offsets (mpos, moffs), the tracer positions,
+    weights and offsets (tpos, tw, toffs) and the tracer counts."""
+    import torch
+    rng = np.random.default_rng(seed0)
+    lam = np.exp(rng.uniform(np.log(1e5), np.log(2e6), nreal))
+    counts = np.clip(rng.poisson(lam), 100000, 2000000).astype(np.int64)
+    matter, tr_pos, tr_w = [], [], []
+    q = (torch.arange(dims, device=device, dtype=torch.float32) + 0.5) * (box / dims)
+    lat = torch.stack(torch.meshgrid(q, q, q, indexing='ij'), -1).view(-1, 3)
+    for r in range(nreal):
+        psi = displacement_field(dims, box, seed0 + r, device=device)
+        matter.append(torch.remainder(lat + psi.view(3, -1).t(), box))
+        p, w = tracers(int(counts[r]), psi, box, seed0 + 5000 + r, device=device)
+        tr_pos.append(p)
+        tr_w.append(w)
+        del psi
+    mpos = torch.cat(matter).contiguous()
+    mpos[mpos >= box] = 0.0
+    tpos = torch.cat(tr_pos).contiguous()
+    tw = torch.cat(tr_w).contiguous()
+    return dict(mpos=mpos, moffs=np.arange(nreal + 1, dtype=np.int64) * dims ** 3,
+                tpos=tpos, tw=tw, toffs=np.concatenate([[0], np.cumsum(counts)]).astype(np.int64),
+                counts=counts, nreal=nreal, dims=dims, box=box)
+
+
+def config_c3(device='cuda'):
+    """Large single field: 1024^3 ZA particles, box 2000 Mpc/h, no RSD, seed 3."""
+    return zeldovich(1024, 2000.0, 3, device=device)
+
+
 def config_c1(device='cuda'):
     """128^3 uniform particles in a 1000 Mpc/h box, seed 1."""
     return uniform(128 ** 3, 1000.0, 1, device=device)

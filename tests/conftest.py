@@ -22,28 +22,31 @@ BIN_RTOL = 1e-5    # per bin:  |P - P_ref| <= 1e-5 * max(|P_ref|, floor)
 U32 = 2.0 ** -24    # float32 unit roundoff
 
 
-def contrib_counts(pos, dims, box, mas, cython2d=True):
+def contrib_counts(pos, dims, box, mas, cython2d=True, chunk=1 << 22):
     """Number of float32 additions the reference makes into each cell (its serial
     loop adds every footprint weight, zero or not; the 2D expand_dims path adds
     each (l, m) weight F times).  Bounds the reference's own summation error:
-    |fl(sum) - sum| <= n_adds * u * sum|x_i|."""
+    |fl(sum) - sum| <= n_adds * u * sum|x_i|.  Particles are taken in chunks so
+    that 10^8-particle inputs fit in memory."""
     F = {'NGP': 1, 'CIC': 2, 'TSC': 3, 'PCS': 4}[mas]
-    x = pos.astype(np.float64) * (dims / box)
-    if mas == 'NGP':
-        base = np.floor(x + 0.5)
-    elif mas == 'CIC':
-        base = np.floor(x)
-    else:
-        base = np.floor(x - (F / 2.0)) + 1
-    base = base.astype(np.int64)
     coord = pos.shape[1]
-    idx = [(base[:, ax, None] + np.arange(F)[None, :]) % dims for ax in range(coord)]  # (n, F)
-    if coord == 3:
-        cells = ((idx[0][:, :, None, None] * dims + idx[1][:, None, :, None]) * dims
-                 + idx[2][:, None, None, :])
-    else:
-        cells = idx[0][:, :, None] * dims + idx[1][:, None, :]
-    cnt = np.bincount(cells.ravel(), minlength=dims ** coord).astype(np.float64)
+    cnt = np.zeros(dims ** coord, np.float64)
+    for s in range(0, pos.shape[0], chunk):
+        x = pos[s:s + chunk].astype(np.float64) * (dims / box)
+        if mas == 'NGP':
+            base = np.floor(x + 0.5)
+        elif mas == 'CIC':
+            base = np.floor(x)
+        else:
+            base = np.floor(x - (F / 2.0)) + 1
+        base = base.astype(np.int64)
+        idx = [(base[:, ax, None] + np.arange(F)[None, :]) % dims for ax in range(coord)]  # (n, F)
+        if coord == 3:
+            cells = ((idx[0][:, :, None, None] * dims + idx[1][:, None, :, None]) * dims
+                     + idx[2][:, None, None, :])
+        else:
+            cells = idx[0][:, :, None] * dims + idx[1][:, None, :]
+        cnt += np.bincount(cells.ravel(), minlength=dims ** coord)
     if coord == 2 and cython2d:
         cnt *= F
     return cnt.reshape((dims,) * coord)
@@ -112,3 +115,15 @@ def assert_bins_fp64(P, P_ref32, P_ref64, floor=None, rtol=BIN_RTOL, what='Pk'):
         raise AssertionError('%s: %d/%d bins out of tolerance; worst %d: got %r f32 ref %r f64 %r tol %r'
                              % (what, int(bad.sum()), P.size, i, P.ravel()[i], P32.ravel()[i],
                                 P64.ravel()[i], tol.ravel()[i]))
+
+
+def err_stats(x, ref, floor=None):
+    """Max and RMS of |x - ref| / max(|ref|, floor) (floor defaults to mean |ref|)."""
+    x = np.asarray(x, dtype=np.float64).ravel()
+    ref = np.asarray(ref, dtype=np.float64).ravel()
+    fl = np.mean(np.abs(ref)) if floor is None else np.abs(np.asarray(floor, dtype=np.float64)).ravel()
+    den = np.maximum(np.abs(ref), fl)
+    num = np.abs(x - ref)
+    # a bin that is exactly zero in both (e.g. Pk1D at k_par = middle) has no error
+    rel = np.where(den > 0, num / np.where(den > 0, den, 1.0), np.where(num > 0, np.inf, 0.0))
+    return float(rel.max()), float(np.sqrt(np.mean(rel * rel)))

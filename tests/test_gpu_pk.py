@@ -50,7 +50,7 @@ def test_pk_vs_oracle(oracle, dims, axis):
     from pylians3_amd import Pk_library as PKL
     box = 500.0
     delta = _field(dims, box, dims + axis)
-    for mas in ('CIC', 'PCS', 'None'):
+    for mas in ('NGP', 'CIC', 'TSC', 'PCS', 'None'):
         r = PKL.Pk(delta, box, axis=axis, MAS=mas, verbose=False)
         ref = oracle.pk(delta, box, axis=axis, MAS=mas)
         _compare(r, ref, 'Pk d=%d axis=%d %s' % (dims, axis, mas))
@@ -195,3 +195,37 @@ def test_fft_power_of_two_vs_scipy(oracle, dims):
     assert abs(g[3, 1, 5] - dims ** 3 / 2) < 1e-4 * dims ** 3
     g[3, 1, 5] = 0
     assert np.abs(g).max() < 1e-4 * dims ** 3
+
+
+def test_pkphase_closed_form_angle_sweep():
+    """Pkphase on exact Fourier data: one mode per 3D bin, at angles swept over
+    (-pi, pi] and magnitudes over 60 decades; each bin holds atan2(Re, |delta|)^2
+    of its mode (Pk_library.pyx:358,377) / Nmodes3D * (L/d^2)^3 (:418)."""
+    from oracle import oracle as O
+    from pylians3_amd import Pk_library as PKL
+    dims, L = 64, 300.0
+    m = dims // 2
+    kmax = int(np.sqrt(3 * m * m))
+    spots = {}
+    for kx in range(-m + 1, m + 1):
+        for ky in range(-m + 1, m + 1):
+            K = int(np.floor(np.sqrt(kx * kx + ky * ky + 1)))
+            spots.setdefault(K, (kx % dims, ky % dims, 1))
+    bins = sorted(k for k in spots if 1 <= k <= kmax)
+    _, _, N3 = O.brute_mode_counts(dims, 0)
+    rng = np.random.default_rng(5)
+    for rep in range(8):
+        theta = np.linspace(-np.pi, np.pi, len(bins), endpoint=False) + rep * 0.37
+        mag = 10.0 ** rng.uniform(-30, 30, len(bins))
+        dk = np.zeros((dims, dims, m + 1), np.complex64)
+        raw = np.zeros(kmax + 1)
+        for K, t, a in zip(bins, theta, mag):
+            v = np.complex64(a * np.exp(1j * t))
+            dk[spots[K]] = v
+            re, im = float(v.real), float(v.imag)
+            raw[K] = np.arctan2(re, np.sqrt(re * re + im * im)) ** 2
+        exp = raw[1:] / N3[1:] * (L / dims ** 2) ** 3
+        r = PKL.Pk_from_fourier([dk], L, dims, axis=0, MAS=['None'])
+        got = r['Pkphase']
+        np.testing.assert_allclose(got, exp, rtol=2e-6, atol=1e-12 * exp.max(),
+                                   err_msg='angle sweep %d' % rep)

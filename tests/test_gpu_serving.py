@@ -67,6 +67,12 @@ def test_matches_drop_in_api(oracle):
     ref = PKL.Pk(delta, 500.0, axis=1, MAS='TSC', verbose=False)
     np.testing.assert_allclose(res.Pk, ref.Pk, rtol=1e-6, atol=1e-9 * np.abs(ref.Pk[:, 0]).max())
     np.testing.assert_array_equal(res.Nmodes3D, ref.Nmodes3D)
+    # and against the oracle on the same float32 delta
+    from conftest import assert_bins
+    ora = oracle.pk(delta, 500.0, axis=1, MAS='TSC')
+    np.testing.assert_array_equal(res.Nmodes3D, ora.Nmodes3D)
+    for l in range(3):
+        assert_bins(res.Pk[:, l], ora.Pk[:, l], ora.Pk[:, 0], what='served vs oracle l=%d' % (2 * l))
 
 
 def test_budget_admission_and_rejection():

@@ -219,3 +219,44 @@ def test_cic_interp_oracle_adjoint_and_closed_form(oracle):
     lin = np.tile(np.arange(dims, dtype=np.float32)[:, None, None], (1, dims, dims))
     v = oracle.cic_interp(lin, box, np.array([[2.5 * box / dims, 1.0, 1.0]], np.float32))
     assert abs(v[0] - 2.5) < 1e-5
+
+
+@pytest.mark.parametrize('mas', ['NGP', 'CIC', 'TSC', 'PCS'])
+@pytest.mark.parametrize('coord', [2, 3])
+def test_threaded_f64_equals_serial(oracle, mas, coord):
+    """The slab-phased threaded FP64 MA equals the serial one to double rounding
+    and is deterministic, including grids too thin for two slabs per thread."""
+    rng = np.random.default_rng(21)
+    for dims in (3, 8, 31, 64):
+        pos = (rng.random((150000, coord)) * 77.0).astype(np.float32)
+        W = rng.lognormal(0, 1, 150000).astype(np.float32)
+        a = oracle.ma_f64(pos, dims, 77.0, mas, W)
+        b = oracle.ma_f64(pos, dims, 77.0, mas, W, threads=6)
+        c = oracle.ma_f64(pos, dims, 77.0, mas, W, threads=6)
+        np.testing.assert_array_equal(b, c)
+        np.testing.assert_allclose(b, a, rtol=1e-13, atol=1e-13 * np.abs(a).max())
+
+
+@pytest.mark.skipif(not __import__('oracle.oracle', fromlist=['x']).cython_available(),
+                    reason='oracle/_ref/cy_strict not built (needs the reference sources: make -C oracle cython)')
+@pytest.mark.parametrize('mas', ['NGP', 'CIC', 'TSC', 'PCS'])
+@pytest.mark.parametrize('coord', [2, 3])
+@pytest.mark.parametrize('weighted', [False, True])
+@pytest.mark.parametrize('renorm', [True, False])
+def test_oracle_equals_reference_cython_ma(oracle, mas, coord, weighted, renorm):
+    """The float32 restatement is bitwise the reference's own MA() (MAS_library.pyx
+    translated by Cython and compiled with strict IEEE flags into oracle/_ref): the
+    serial loops MA() runs, the 2D expand_dims path and its renormalisation of a
+    pre-filled grid (MAS_library.pyx:57-545)."""
+    rng = np.random.default_rng(7 + coord)
+    box, dims, n = 90.0, 20, 30000
+    pos = (rng.random((n, coord)) * box).astype(np.float32)
+    pos[:5] = [[0.0] * coord, [box] * coord, [box * (1 - 1e-7)] * coord,
+               [box / dims * 3.5] * coord, [box / dims * 7.0] * coord]
+    W = rng.lognormal(0, 1, n).astype(np.float32) if weighted else None
+    init = rng.random((dims,) * coord).astype(np.float32)
+    a = init.copy()
+    oracle.ma_f32(pos, a, box, mas, W, renormalize_2D=renorm)
+    b = init.copy()
+    oracle.ma_cython(pos, b, box, mas, W, renormalize_2D=renorm)
+    np.testing.assert_array_equal(a, b)

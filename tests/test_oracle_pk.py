@@ -142,3 +142,102 @@ def test_f32_path_close_to_fp64_oracle(oracle):
     b = oracle.pk(g64 / g64.mean() - 1.0, 500.0, axis=0, MAS='CIC', f64=True)
     np.testing.assert_allclose(a.Pk[:, 0], b.Pk[:, 0], rtol=1e-4)
     np.testing.assert_array_equal(a.Nmodes3D, b.Nmodes3D)
+
+
+@pytest.mark.parametrize('dims', [5, 8, 12, 16])
+@pytest.mark.parametrize('axis', [0, 2])
+def test_k3d_sums_brute_force(oracle, dims, axis):
+    """raw k3D[k_index] += k (Pk_library.pyx:373): the sum of |k| over the bin's
+    independent modes, from the brute-force enumeration."""
+    x = np.random.default_rng(dims).standard_normal((dims,) * 3).astype(np.float32)
+    r = oracle.pk(x, 80.0, axis=axis)
+    np.testing.assert_allclose(r.raw['k3D'], oracle.brute_k3d_sums(dims, axis), rtol=1e-13)
+    # finalized k3D is the mean |k| of the bin in units of kF (Pk_library.pyx:411)
+    K = oracle.brute_k3d_sums(dims, axis)[1:]
+    _, N2, N3 = oracle.brute_mode_counts(dims, axis)
+    np.testing.assert_allclose(r.k3D, K / N3[1:] * (2 * np.pi / 80.0), rtol=1e-13)
+
+
+@pytest.mark.parametrize('dims', [6, 9, 16])
+@pytest.mark.parametrize('axis', [0, 1, 2])
+def test_kpar_kper_bin_centres(oracle, dims, axis):
+    """Every independent mode's |k_par| and |k_perp| fall inside the half-open
+    bin [centre - kF/2, centre + kF/2) of the 2D bin it is counted in
+    (index_2D = (kmax_par+1) k_per + k_par, Pk_library.pyx:367,399-400)."""
+    L = 150.0
+    kF = 2 * np.pi / L
+    x = np.random.default_rng(3).standard_normal((dims,) * 3).astype(np.float32)
+    r = oracle.pk(x, L, axis=axis)
+    m = oracle.brute_modes(dims, axis)
+    par = m['kpar'] * kF
+    perp = np.sqrt((m['k2'] - m['kpar'] ** 2).astype(np.float64)) * kF
+    cpar, cper = r.kpar[m['i2']], r.kper[m['i2']]
+    assert np.all(par >= cpar - 0.5 * kF * (1 + 1e-12)) and np.all(par < cpar + 0.5 * kF)
+    assert np.all(perp >= cper - 0.5 * kF * (1 + 1e-12)) and np.all(perp < cper + 0.5 * kF)
+    # bins nobody reaches still have their centres: a full (kmax_par+1) x (kmax_per+1) table
+    middle = dims // 2
+    kmax_per = int(np.sqrt(2 * middle ** 2))
+    assert r.kpar.shape == ((middle + 1) * (kmax_per + 1),)
+    np.testing.assert_allclose(np.unique(r.kpar), (np.arange(middle + 1) + 0.5) * kF, rtol=1e-14)
+    np.testing.assert_allclose(np.unique(r.kper), (np.arange(kmax_per + 1) + 0.5) * kF, rtol=1e-14)
+
+
+def phase_field(dims, modes):
+    """Half-complex field that is zero except at the given (kxx, kyy, kzz) -> value
+    entries (0 < kzz < middle, so each is visited once by the mode loop)."""
+    dk = np.zeros((dims, dims, dims // 2 + 1), np.complex64)
+    for (i, j, l), v in modes.items():
+        assert 0 < l < dims // 2
+        dk[i, j, l] = v
+    return dk
+
+
+PHASE_MODES = {(1, 2, 3): 3 - 4j, (5, 0, 1): -1 + 0j, (0, 0, 7): 2j, (14, 9, 4): -2.5 + 0.5j}
+
+
+def expected_phase(dims, L, modes):
+    """Pkphase closed form: phase = atan2(Re, |delta_k|) per mode (Pk_library.pyx:358),
+    phase^2 summed per 3D bin (:377), then / Nmodes3D * (L/d^2)^3 (:418).  A zero
+    mode has atan2(0, 0) = 0."""
+    kmax = int(np.sqrt(3 * (dims // 2) ** 2))
+    raw = np.zeros(kmax + 1)
+    for (i, j, l), v in modes.items():
+        k = [a - dims if a > dims // 2 else a for a in (i, j, l)]
+        kidx = int(np.floor(np.sqrt(sum(a * a for a in k))))
+        raw[kidx] += np.arctan2(v.real, abs(v)) ** 2
+    _, _, N3 = oracle_counts(dims)
+    return raw[1:] / N3[1:] * (L / dims ** 2) ** 3
+
+
+def oracle_counts(dims):
+    from oracle import oracle as O
+    return O.brute_mode_counts(dims, 0)
+
+
+def test_pkphase_closed_form(oracle):
+    dims, L = 16, 120.0
+    dk = phase_field(dims, PHASE_MODES)
+    r = oracle.pk(np.zeros((dims,) * 3, np.float32), L, axis=0, MAS='None', dk=dk)
+    exp = expected_phase(dims, L, PHASE_MODES)
+    np.testing.assert_allclose(r.Pkphase, exp, rtol=1e-15, atol=0)
+    assert np.count_nonzero(r.Pkphase) == len({int(np.sqrt(sum((a - dims if a > dims // 2 else a) ** 2
+                                                               for a in m)))
+                                               for m in PHASE_MODES if PHASE_MODES[m].real != 0})
+
+
+@pytest.mark.parametrize('F', [1, 2])
+def test_threaded_loop_equals_serial(oracle, F):
+    """The threaded mode loop (per-thread sums, added in thread order) gives the
+    serial loop's counts exactly and its sums to double rounding, run after run."""
+    rng = np.random.default_rng(9)
+    for dims in (7, 16, 34):
+        dks = [oracle.fft_f32(rng.standard_normal((dims,) * 3).astype(np.float32)) for _ in range(F)]
+        for axis in (0, 2):
+            a = oracle.raw_bins(dks, dims, axis, ['TSC', 'CIC'][:F])
+            b = oracle.raw_bins(dks, dims, axis, ['TSC', 'CIC'][:F], threads=5)
+            c = oracle.raw_bins(dks, dims, axis, ['TSC', 'CIC'][:F], threads=5)
+            for k in a:
+                np.testing.assert_array_equal(b[k], c[k])
+                np.testing.assert_allclose(b[k], a[k], rtol=1e-13, atol=1e-13 * np.abs(a[k]).max())
+            for k in ('N1D', 'N2D', 'N3D', 'k1D'):
+                np.testing.assert_array_equal(a[k], b[k])
