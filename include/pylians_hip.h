@@ -119,6 +119,21 @@ int pyl_ctx_stage_totals(pyl_ctx *ctx, double *ms_out /* [PYL_STAGE_COUNT] */,
 int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
            float *grid, int dims, float BoxSize, int mas, int flags);
 
+/* MA that allocates nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, algorithm (tiled) and
+ * bits as pyl_ma with PYL_PTR_DEVICE, one grid, device pointers only; every piece
+ * of scratch comes from one fixed per-device array of PYL_MA_NOALLOC_SCRATCH bytes
+ * that the library holds from load until the process ends, and counters in it are
+ * cleared by fill kernels (no hipMalloc, hipFree, hipMemsetAsync or staging upload).
+ * A call that would need more scratch (about 12 B per particle, 16 with W, plus the
+ * sort tables and tile partials) returns PYL_EBUDGET before launching anything.
+ * All calls on one device share the array: calls that overlap on one GPU (other
+ * contexts, other streams without ordering) race and must be serialised by the caller. */
+#define PYL_MA_NOALLOC_SCRATCH ((size_t)128 << 20)
+int pyl_ma_noalloc(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
+                   float *grid, int dims, float BoxSize, int mas, int flags);
+
 /* MA fused with the density contrast: delta = deposit/scale - shift in float32,
  * written once (overwrite).  The caller's two steps after MA in the reference
  * pipeline (library/Pk_library/Pk_snapshot.py:80-88, "delta /= mean; delta -= 1.0")

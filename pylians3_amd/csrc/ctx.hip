@@ -250,6 +250,7 @@ int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
     PYL_HIP(hipEventCreateWithFlags(&c->ev_switch, hipEventDisableTiming));
     PYL_HIP(hipHostMalloc((void **)&c->h_total, 2 * sizeof(unsigned long long), hipHostMallocDefault));
     c->h_total[1] = 0;
+    PYL_TRY(ma_noalloc_setup(c));
     *out = h.release();
     return PYL_OK;
 }

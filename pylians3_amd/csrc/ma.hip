@@ -1998,6 +1998,29 @@ int ma_device(Ctx *c, const float *pos, long long n, const long long *h_offs, in
     return PYL_EINVAL;
 }
 
+// ---- fixed scratch of pyl_ma_noalloc ------------------------------------------
+// One array per device, resident from the library's load until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: only the
+// address from hipGetSymbolAddress (kept per device below) is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[PYL_MA_NOALLOC_SCRATCH];
+
+static std::mutex g_noalloc_mu;
+static char *g_noalloc_ptr[64] = {};
+
+int ma_noalloc_setup(Ctx *c) {
+    PYL_CHECK(c->device >= 0 && c->device < 64, PYL_EINVAL, "device %d out of range", c->device);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[c->device]) {
+        void *ptr = nullptr;
+        PYL_HIP(hipSetDevice(c->device));
+        PYL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PYL_CHECK(ptr != nullptr, PYL_EHIP, "no device address for the fixed MA scratch");
+        g_noalloc_ptr[c->device] = (char *)ptr;
+    }
+    c->noalloc_base = g_noalloc_ptr[c->device];
+    return PYL_OK;
+}
+
 }  // namespace pyl
 
 using namespace pyl;
@@ -2052,6 +2075,22 @@ static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *
 extern "C" int pyl_ma(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
                       float *grid, int dims, float BoxSize, int mas, int flags) {
     return ma_entry(ctx, pos, n, nullptr, 1, axes, W, grid, dims, BoxSize, mas, flags);
+}
+
+extern "C" int pyl_ma_noalloc(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,
+                              float *grid, int dims, float BoxSize, int mas, int flags) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    Ctx *c = &ctx->c;
+    PYL_CHECK(flags & PYL_PTR_DEVICE, PYL_EINVAL, "pyl_ma_noalloc takes device pointers only");
+    PYL_CHECK(!(flags & PYL_MA_ATOMIC), PYL_EINVAL, "pyl_ma_noalloc runs the tiled algorithm only");
+    PYL_CHECK(c->noalloc_base != nullptr, PYL_EINVAL, "context has no fixed scratch");
+    c->arena = c->noalloc_base;
+    c->arena_cap = PYL_MA_NOALLOC_SCRATCH;
+    c->arena_used = 0;
+    const int s = ma_entry(ctx, pos, n, nullptr, 1, axes, W, grid, dims, BoxSize, mas, flags);
+    c->arena = nullptr;
+    c->arena_cap = c->arena_used = 0;
+    return s;
 }
 
 extern "C" int pyl_ma_delta(pyl_ctx *ctx, const float *pos, int64_t n, int axes, const float *W,

@@ -143,10 +143,12 @@ struct Ctx {
     // incoming stream waits for it, so work of the next call (which reuses the same
     // scratch buffers) starts only after the previous call's work is done.
     hipEvent_t ev_switch = nullptr;
-    // Caller-provided device scratch: while set, MA scratch is cut from `arena`
-    // (arena_used bytes so far) instead of the grow-only buffers.
+    // Fixed device scratch of pyl_ma_noalloc: while such a call runs, `arena` is
+    // this device's address of the library's fixed array and MA scratch is cut
+    // from it (arena_used bytes so far) instead of the grow-only buffers.
     char *arena = nullptr;
     size_t arena_cap = 0, arena_used = 0;
+    char *noalloc_base = nullptr;    // the array's address on this context's device (set-up)
     // Stage timing: a ring of event pairs per stage, so that timing many calls
     // needs no host synchronisation between them.  A pair is read back when its
     // ring slot comes round again (kEvRing calls later) or on request.
@@ -163,7 +165,9 @@ struct Ctx {
 };
 
 size_t held_bytes(const Ctx *c);
-
+// Looks up this device's address of the fixed MA scratch array (once per device,
+// with the device current); called by pyl_ctx_create.
+int ma_noalloc_setup(Ctx *c);
 int ensure(Ctx *c, BufId id, size_t bytes);
 void reset_stages(Ctx *c);
 int stage_begin(Ctx *c, int stage);
