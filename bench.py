@@ -380,8 +380,9 @@ def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
     """Batched leg: nreal independent 128^3 realisations (box 500): a Zel'dovich matter
     field (128^3 particles) and Poisson(lambda) weighted tracers, lambda log-uniform
     in [1e5, 2e6], CIC MA of both, then Pk of both and their cross spectrum (XPk).
-    Batched form (pyl_ma_batch x2 + pyl_pk_batch) against the plain per-call form
-    (pyl_ma x2 + pyl_pk per realisation), both device-resident."""
+    Forms, all device-resident: batched (pyl_ma_batch x2 + pyl_pk_batch), the same
+    work as one tuned HIP graph (batch.BatchGraph), and one call per realisation
+    (pyl_ma x2 + pyl_pk) with the tiled and with the atomic scatter."""
     import numpy as np
     import torch
     from pylians3_amd import workloads
@@ -415,23 +416,26 @@ def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
 
     one_out, one_arrs, _ = N.alloc_pk_out(dims, 2)   # one_arrs keeps the result arrays alive
 
-    def per_call():
-        for r in range(nreal):
-            g0 = grids[0][r]
-            g1 = grids[1][r]
-            N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(tpos[toffs[r]:].data_ptr()),
-                             int(counts[r]), 3, ctypes.c_void_p(tw[toffs[r]:].data_ptr()),
-                             ctypes.c_void_p(g0.data_ptr()), dims, box, N.CIC,
-                             N.PTR_DEVICE | N.MA_OVERWRITE | N.MA_ATOMIC))
-            N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(mpos[moffs[r]:].data_ptr()),
-                             dims ** 3, 3, None, ctypes.c_void_p(g1.data_ptr()), dims, box, N.CIC,
-                             N.PTR_DEVICE | N.MA_OVERWRITE | N.MA_ATOMIC))
-            ptrs = (ctypes.c_void_p * 2)(g0.data_ptr(), g1.data_ptr())
-            N.check(L.pyl_pk(ctx.handle, ptrs, 2, mas, dims, box, 0, 1.0, 0.0, N.PTR_DEVICE,
-                             ctypes.byref(one_out)))
+    def per_call(flag):
+        def run():
+            for r in range(nreal):
+                g0 = grids[0][r]
+                g1 = grids[1][r]
+                N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(tpos[toffs[r]:].data_ptr()),
+                                 int(counts[r]), 3, ctypes.c_void_p(tw[toffs[r]:].data_ptr()),
+                                 ctypes.c_void_p(g0.data_ptr()), dims, box, N.CIC,
+                                 N.PTR_DEVICE | N.MA_OVERWRITE | flag))
+                N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(mpos[moffs[r]:].data_ptr()),
+                                 dims ** 3, 3, None, ctypes.c_void_p(g1.data_ptr()), dims, box, N.CIC,
+                                 N.PTR_DEVICE | N.MA_OVERWRITE | flag))
+                ptrs = (ctypes.c_void_p * 2)(g0.data_ptr(), g1.data_ptr())
+                N.check(L.pyl_pk(ctx.handle, ptrs, 2, mas, dims, box, 0, 1.0, 0.0, N.PTR_DEVICE,
+                                 ctypes.byref(one_out)))
+        return run
 
     res = {}
-    for name, fn, reps in (('batched', batched, 3), ('per_call', per_call, 2)):
+    for name, fn, reps in (('batched', batched, 3), ('per_call_tiled', per_call(0), 2),
+                           ('per_call_atomic', per_call(N.MA_ATOMIC), 2)):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -439,18 +443,45 @@ def batched_leg(N, nreal, dims=128, box=500.0, seed0=1000):
             fn()
         torch.cuda.synchronize()
         res[name] = (time.perf_counter() - t0) / reps
-    npart = int(counts.sum()) + nreal * dims ** 3
     ctx.close()
+    del grids
+    torch.cuda.empty_cache()
+    # the same work as one captured HIP graph, launch parameters from the tuner
+    from pylians3_amd import batch, tune
+    fields = [dict(pos=tpos, offs=torch.from_numpy(toffs).cuda(), W=tw, mas='CIC'),
+              dict(pos=mpos, offs=torch.from_numpy(moffs).cuda(), W=None, mas='CIC')]
+    t0 = time.perf_counter()
+    best = tune.tune_batch(fields, nreal, dims, box, 0, use_cache=False, reps=2,
+                           candidates={'chunks_per_real': (4, 16, 64), 'bin_rows': (0, 32),
+                                       'deposit_order': (-1,)})
+    t_tune = time.perf_counter() - t0
+    tuning = {k: best[k] for k in ('chunks_per_real', 'bin_rows', 'deposit_order')}
+    g = batch.BatchGraph(fields, nreal, dims, box, 0, tuning=tuning)
+    try:
+        res['graph'] = tune.time_graph(g, reps=5)
+    finally:
+        g.close()
+    npart = int(counts.sum()) + nreal * dims ** 3
     return {'workload': '%d x %d^3 realisations: ZA matter + Poisson lognormal-weighted tracers '
                         '(lambda log-uniform 1e5-2e6), CIC MA of both, Pk + XPk, axis 0'
                         % (nreal, dims),
             'particles': npart,
             'batched_s': round(res['batched'], 4),
-            'per_call_s': round(res['per_call'], 4),
-            'realisations_per_s': round(nreal / res['batched'], 2),
-            'per_call_realisations_per_s': round(nreal / res['per_call'], 2),
-            'speedup_vs_per_call': round(res['per_call'] / res['batched'], 2),
-            'per_call_form': 'plain atomic MA x2 + single-field-set Pk call per realisation'}
+            'graph_s': round(res['graph'], 4),
+            'per_call_tiled_s': round(res['per_call_tiled'], 4),
+            'per_call_atomic_s': round(res['per_call_atomic'], 4),
+            'realisations_per_s': round(nreal / min(res['batched'], res['graph']), 2),
+            'speedup_vs_per_call_tiled': round(res['per_call_tiled'] / min(res['batched'], res['graph']), 2),
+            'speedup_vs_per_call_atomic': round(res['per_call_atomic'] / min(res['batched'], res['graph']), 2),
+            'graph_tuning': tuning,
+            'tuner_s': round(t_tune, 2),
+            'tuner_timings_ms': [dict({k: t[k] for k in tuning}, ms=round(1e3 * t['seconds'], 2))
+                                 for t in best.get('timings', [])],
+            'forms': {'batched': 'pyl_ma_batch x2 + pyl_pk_batch (launch sequence per call)',
+                      'graph': 'the batched work captured once as a HIP graph, replayed (host '
+                               'normalisation after each replay), tuned launch parameters',
+                      'per_call_tiled': 'pyl_ma x2 (tiled) + pyl_pk per realisation',
+                      'per_call_atomic': 'pyl_ma x2 (per-particle atomics) + pyl_pk per realisation'}}
 
 
 def serving_leg(N, nreq=120, seed=5):
@@ -484,16 +515,25 @@ def serving_leg(N, nreq=120, seed=5):
             o2 = int(rng.integers(0, nmax - n2 + 1))
             reqs.append(Request(pool[o:o + n], d, box, mas, axis, pos2=pool[o2:o2 + n2],
                                 W2=poolw[o2:o2 + n2]))
-    # plain form: one request after the other, one slot, no fusion
+    # plain forms: one request after the other, one slot, no fusion; the tiled
+    # scatter, and the per-particle atomic scatter (the plain GPU form)
     with Server(slots=1, max_batch=1, policy='fifo') as srv:
         srv.submit(reqs[0]).result()
         t0 = time.perf_counter()
         plain_lat = np.array([srv.submit(r).result().latency_s for r in reqs])
         t_plain = time.perf_counter() - t0
+    with Server(slots=1, max_batch=1, policy='fifo', atomic=True) as srv:
+        srv.submit(reqs[0]).result()
+        t0 = time.perf_counter()
+        atomic_lat = np.array([srv.submit(r).result().latency_s for r in reqs])
+        t_atomic = time.perf_counter() - t0
     cap = nreq / t_plain
     out = {'requests': nreq, 'plain_throughput_rps': round(cap, 2),
            'plain_p50_ms': round(1e3 * np.percentile(plain_lat, 50), 3),
-           'plain_p99_ms': round(1e3 * np.percentile(plain_lat, 99), 3)}
+           'plain_p99_ms': round(1e3 * np.percentile(plain_lat, 99), 3),
+           'plain_atomic_throughput_rps': round(nreq / t_atomic, 2),
+           'plain_atomic_p50_ms': round(1e3 * np.percentile(atomic_lat, 50), 3),
+           'plain_atomic_p99_ms': round(1e3 * np.percentile(atomic_lat, 99), 3)}
     # Poisson arrivals at 0.9x the plain capacity, the same schedule for both forms
     gaps = rng.exponential(1.0 / (0.9 * cap), nreq)
 
@@ -524,7 +564,13 @@ def serving_leg(N, nreq=120, seed=5):
                 'p99_ms': round(1e3 * np.percentile(lat, 99), 3),
                 'served_window_s': round(t_arr, 3),
                 'saturated_throughput_rps': round(nreq / t_sat, 2),
-                'speedup_vs_plain': round((nreq / t_sat) / cap, 2)})
+                'speedup_vs_plain': round((nreq / t_sat) / cap, 2),
+                'speedup_vs_plain_atomic': round(t_atomic / t_sat, 2),
+                'forms': {'plain': '1 slot, arrival order, no fusion, tiled scatter',
+                          'plain_atomic': '1 slot, arrival order, no fusion, per-particle '
+                                          'atomic scatter (the plain per-call GPU form)',
+                          'served': '4 slots, shortest job first with aging, fusion of '
+                                    'small requests, tiled scatter'}})
     return out
 
 

@@ -161,6 +161,17 @@ int pyl_ma_rsd(pyl_ctx *ctx, const float *pos, const float *vel, int64_t n, cons
 int pyl_ma_batch(pyl_ctx *ctx, const float *pos, const int64_t *offsets, int nreal, int axes,
                  const float *W, float *grids, int dims, float BoxSize, int mas, int flags);
 
+/* Batched MA with the particle ranges in device memory: realisation r holds particles
+ * d_offsets[r] .. d_offsets[r+1] (int64, device, nreal+1 entries, d_offsets[0] = 0 and
+ * d_offsets[nreal] <= n_cap).  Each realisation is cut into chunks_per_real sort
+ * chunks on the device, so the launch sequence depends only on (nreal, n_cap,
+ * chunks_per_real, dims, MAS) and not on the counts: the form a HIP graph can replay
+ * with new particles.  Device pointers only, tiled algorithm; grids as pyl_ma_batch.
+ * Replaces the same Python loop of MA calls as pyl_ma_batch (MAS_library.pyx:57-112). */
+int pyl_ma_batch_dev(pyl_ctx *ctx, const float *pos, const int64_t *d_offsets, int nreal,
+                     int64_t n_cap, int chunks_per_real, int axes, const float *W, float *grids,
+                     int dims, float BoxSize, int mas, int flags);
+
 /* Reference C kernels, same names and signatures as MAS_c.h:3-10 (host pointers,
  * threads ignored, += semantics, native 2D as in MAS_c.c:28-34).  They run on
  * a process-wide context on device 0.  On error they print pyl_last_error() to
@@ -273,6 +284,32 @@ int pyl_pk_plane(pyl_ctx *ctx, const float *const *deltas, int nfields, const in
  * are (dims,dims,dims) float32 and may alias. */
 int pyl_correct_mas(pyl_ctx *ctx, const float *delta, float *out, int dims, double BoxSize,
                     int mas, int flags);
+
+/* ---- HIP graphs of a context's work --------------------------------------
+ * pyl_graph_begin starts capturing the context's stream; the library calls that
+ * follow (device pointers: pyl_ma_batch_dev, pyl_ma with PYL_PTR_DEVICE, pyl_pk,
+ * pyl_pk_batch with PYL_PK_SCALES_DEVICE, ...) only enqueue their kernels and
+ * copies, and pyl_graph_end turns them into a graph.  Host steps that need the
+ * device's results (the spectra's normalisation into the caller's result arrays,
+ * the sort's particle-count check) run after every replay, in pyl_graph_launch,
+ * which returns when the results are in the arrays given at capture.  Run the same
+ * work once on the context before capturing it: scratch, plans and tables may not
+ * be created or grow while capturing (such a call fails).  A context that captured
+ * a graph serves that graph: other calls on it may grow its scratch and invalidate
+ * the graph.  Inputs are read, and outputs written, at the addresses given at capture.
+ * The reference has no counterpart: replaces a Python loop of MA and Pk/XPk calls
+ * over realisations (MAS_library.pyx:57-112, Pk_library.pyx:263-420, :529-793). */
+typedef struct pyl_graph pyl_graph;
+int pyl_graph_begin(pyl_ctx *ctx);
+int pyl_graph_end(pyl_ctx *ctx, pyl_graph **out);
+int pyl_graph_abort(pyl_ctx *ctx);          /* ends a failed capture, discarding it */
+int pyl_graph_launch(pyl_graph *graph);     /* replay, wait, run the host steps */
+int pyl_graph_destroy(pyl_graph *graph);
+
+/* Launch parameters for a tuner (value 0 / -1: the built-in choice). */
+#define PYL_TUNE_BIN_ROWS      1   /* binning rows per work item, 1..256 */
+#define PYL_TUNE_DEPOSIT_ORDER 2   /* MA deposit in largest-tile-first order: 1 on, 0 off */
+int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value);
 
 #ifdef __cplusplus
 }

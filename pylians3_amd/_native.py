@@ -24,6 +24,7 @@ NO_SYNC = 0x0020
 PK_DESTROY_INPUT = 0x0040
 PK_NO_PHASE = 0x0080
 PK_SCALES_DEVICE = 0x0100
+TUNE_BIN_ROWS, TUNE_DEPOSIT_ORDER = 1, 2
 
 STAGES = ['ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit', 'ma_atomic', 'pk_prep', 'pk_fft',
           'pk_bin', 'ma_final']
@@ -85,6 +86,16 @@ EXPORTS = {
     'pyl_ma_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_float, ctypes.c_int, ctypes.c_int]),
+    'pyl_ma_batch_dev': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                        ctypes.c_int]),
+    'pyl_graph_begin': (ctypes.c_int, [ctypes.c_void_p]),
+    'pyl_graph_end': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    'pyl_graph_abort': (ctypes.c_int, [ctypes.c_void_p]),
+    'pyl_graph_launch': (ctypes.c_int, [ctypes.c_void_p]),
+    'pyl_graph_destroy': (ctypes.c_int, [ctypes.c_void_p]),
+    'pyl_ctx_set_tuning': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     'NGP': (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
                    ctypes.c_int, ctypes.c_float, ctypes.c_int]),
     'CIC': (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,

@@ -45,7 +45,7 @@ def build(verbose=False):
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=%s' % ARCH, '-o', tmp] + objs + [
-        '-L/opt/rocm/lib', '-lrocfft', '-Wl,-rpath,/opt/rocm/lib']
+        '-L/opt/rocm/lib', '-lrocfft', '-lrocprofiler-sdk-roctx', '-Wl,-rpath,/opt/rocm/lib']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError('link failed:\n%s\n%s' % (' '.join(cmd), r.stderr))

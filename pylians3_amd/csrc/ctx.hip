@@ -28,6 +28,9 @@ size_t held_bytes(const Ctx *c) {
 int ensure(Ctx *c, BufId id, size_t bytes) {
     DevBuf &b = c->bufs[id];
     if (bytes <= b.cap) return PYL_OK;
+    PYL_CHECK(!c->capturing, PYL_EINVAL,
+              "scratch buffer %d would grow while a graph is captured: run the same work once "
+              "on the context before capturing it", (int)id);
     size_t need = bytes + (bytes >> 4);  // 6% headroom so slowly growing requests do not thrash
     size_t others = held_bytes(c) - b.cap;
     PYL_CHECK(others + need <= c->budget || others + bytes <= c->budget, PYL_EBUDGET,
@@ -91,13 +94,16 @@ int stage_end(Ctx *c, int stage) {
 }
 
 int finish(Ctx *c, int flags) {
+    if (c->capturing) return PYL_OK;
     if ((flags & PYL_PTR_DEVICE) && (flags & PYL_NO_SYNC)) return PYL_OK;
     PYL_HIP(hipStreamSynchronize(c->stream));
     return check_total(c);
 }
 
 int pinned(Ctx *c, PinnedBuf &b, size_t bytes, void **out) {
-    if (b.pending) {
+    PYL_CHECK(!c->capturing || bytes <= b.cap, PYL_EINVAL,
+              "pinned staging would grow while a graph is captured");
+    if (b.pending && !c->capturing) {
         PYL_HIP(hipEventSynchronize(b.done));
         b.pending = false;
     }
@@ -119,18 +125,22 @@ int pinned(Ctx *c, PinnedBuf &b, size_t bytes, void **out) {
 }
 
 int pinned_done(Ctx *c, PinnedBuf &b) {
+    if (c->capturing) return PYL_OK;      // a replay is waited for before its host steps
     PYL_HIP(hipEventRecord(b.done, c->stream));
     b.pending = true;
     return PYL_OK;
 }
 
+// total_expect: -1 nothing pending; >= 0 the expected record total; -2 the total
+// of the device particle ranges, read back into h_total[2] (pyl_ma_batch_dev).
 int check_total(Ctx *c) {
-    if (c->total_expect < 0) return PYL_OK;
-    const long long expect = c->total_expect;
+    if (c->total_expect == -1 || c->capturing) return PYL_OK;
+    const long long mode = c->total_expect;
     c->total_expect = -1;
     PYL_HIP(hipEventSynchronize(c->ev_total));
-    PYL_CHECK(*c->h_total == (unsigned long long)expect, PYL_EHIP,
-              "tile sort lost particles (%llu of %lld)", *c->h_total, expect);
+    const unsigned long long expect = mode == -2 ? c->h_total[2] : (unsigned long long)mode;
+    PYL_CHECK(c->h_total[0] == expect, PYL_EHIP, "tile sort lost particles (%llu of %llu)",
+              c->h_total[0], expect);
     return PYL_OK;
 }
 
@@ -248,8 +258,8 @@ int pyl_ctx_create(pyl_ctx **out, int device, size_t hbm_budget) {
     PYL_HIP(hipEventCreateWithFlags(&c->pin_scales.done, hipEventDisableTiming));
     PYL_HIP(hipEventCreateWithFlags(&c->ev_total, hipEventDisableTiming));
     PYL_HIP(hipEventCreateWithFlags(&c->ev_switch, hipEventDisableTiming));
-    PYL_HIP(hipHostMalloc((void **)&c->h_total, 2 * sizeof(unsigned long long), hipHostMallocDefault));
-    c->h_total[1] = 0;
+    PYL_HIP(hipHostMalloc((void **)&c->h_total, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    c->h_total[1] = c->h_total[2] = c->h_total[3] = 0;
     PYL_TRY(ma_noalloc_setup(c));
     *out = h.release();
     return PYL_OK;
@@ -376,6 +386,124 @@ int pyl_ctx_stage_totals(pyl_ctx *ctx, double *ms_out, long long *count_out) {
         count_out[s] = c->ev_count[s];
     }
     return PYL_OK;
+}
+
+// ---- HIP graph capture -------------------------------------------------------
+
+struct pyl_graph {
+    pyl_ctx *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::function<int()>> deferred;   // host steps after each replay, in order
+    std::vector<void *> host;                     // pinned buffers read or filled by the graph
+};
+
+static void drop_capture_state(Ctx *c) {
+    for (void *p : c->cap_host) (void)hipHostFree(p);
+    c->cap_host.clear();
+    c->deferred.clear();
+    c->capturing = false;
+}
+
+int pyl_graph_begin(pyl_ctx *ctx) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    Ctx *c = &ctx->c;
+    PYL_CHECK(!c->capturing, PYL_EINVAL, "a graph is already being captured on this context");
+    PYL_HIP(hipSetDevice(c->device));
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    PYL_TRY(check_total(c));
+    for (PinnedBuf *b : {&c->pin_up, &c->pin_down, &c->pin_scales}) b->pending = false;
+    c->box_miss_n = -1;
+    c->timing = false;
+    drop_capture_state(c);
+    PYL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    c->capturing = true;
+    return PYL_OK;
+}
+
+int pyl_graph_end(pyl_ctx *ctx, pyl_graph **out) {
+    PYL_CHECK(ctx != nullptr && out != nullptr, PYL_EINVAL, "NULL argument");
+    *out = nullptr;
+    Ctx *c = &ctx->c;
+    PYL_CHECK(c->capturing, PYL_EINVAL, "no graph is being captured on this context");
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (e != hipSuccess || graph == nullptr) {
+        drop_capture_state(c);
+        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        return PYL_EHIP;
+    }
+    std::unique_ptr<pyl_graph> g(new pyl_graph());
+    g->ctx = ctx;
+    g->graph = graph;
+    g->deferred = std::move(c->deferred);
+    g->host = std::move(c->cap_host);
+    c->deferred.clear();
+    c->cap_host.clear();
+    c->capturing = false;
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        g->exec = nullptr;
+        pyl_graph_destroy(g.release());
+        set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        return PYL_EHIP;
+    }
+    *out = g.release();
+    return PYL_OK;
+}
+
+int pyl_graph_abort(pyl_ctx *ctx) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    Ctx *c = &ctx->c;
+    if (!c->capturing) return PYL_OK;
+    hipGraph_t graph = nullptr;
+    (void)hipStreamEndCapture(c->stream, &graph);
+    (void)hipGetLastError();
+    if (graph) (void)hipGraphDestroy(graph);
+    drop_capture_state(c);
+    return PYL_OK;
+}
+
+int pyl_graph_launch(pyl_graph *g) {
+    TraceRange trace_("pyl_graph_launch");
+    PYL_CHECK(g != nullptr && g->exec != nullptr, PYL_EINVAL, "NULL graph");
+    Ctx *c = &g->ctx->c;
+    PYL_CHECK(!c->capturing, PYL_EINVAL, "the context is capturing another graph");
+    PYL_HIP(hipSetDevice(c->device));
+    PYL_HIP(hipGraphLaunch(g->exec, c->stream));
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    for (auto &f : g->deferred) PYL_TRY(f());
+    return PYL_OK;
+}
+
+int pyl_graph_destroy(pyl_graph *g) {
+    if (!g) return PYL_OK;
+    if (g->ctx) {
+        (void)hipSetDevice(g->ctx->c.device);
+        (void)hipStreamSynchronize(g->ctx->c.stream);
+    }
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (void *p : g->host) (void)hipHostFree(p);
+    delete g;
+    return PYL_OK;
+}
+
+int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    Ctx *c = &ctx->c;
+    switch (knob) {
+        case PYL_TUNE_BIN_ROWS:
+            PYL_CHECK(value >= 0 && value <= 256, PYL_EINVAL, "bin rows must be 0..256");
+            c->tune_bin_rows = value;
+            return PYL_OK;
+        case PYL_TUNE_DEPOSIT_ORDER:
+            PYL_CHECK(value >= -1 && value <= 1, PYL_EINVAL, "deposit order must be -1, 0 or 1");
+            c->tune_order = value;
+            return PYL_OK;
+    }
+    set_error("unknown tuning knob %d", knob);
+    return PYL_EINVAL;
 }
 
 }  // extern "C"

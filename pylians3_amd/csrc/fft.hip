@@ -274,6 +274,7 @@ bool fft_own_size(int dims) {
 int get_twiddles(Ctx *c, int n, const float2 **out) {
     auto it = c->twiddles.find(n);
     if (it == c->twiddles.end()) {
+        PYL_CHECK(!c->capturing, PYL_EINVAL, "FFT twiddles created while a graph is captured");
         std::vector<float2> h(n);
         for (int k = 0; k < n; ++k) {
             const double a = -2.0 * M_PI * (double)k / (double)n;
@@ -292,6 +293,8 @@ int get_twiddles(Ctx *c, int n, const float2 **out) {
 // `in` may be overwritten.
 int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch) {
     if (!fft_own_size(dims)) {
+        PYL_CHECK(!c->capturing, PYL_EINVAL,
+                  "graph capture takes the own FFT passes only (power-of-two sides 16..1024)");
         FFTPlan *plan = nullptr;
         PYL_TRY(get_fft_plan(c, dims, batch, &plan, FFT_R2C_3D));
         return run_fft(c, plan, in, out);

@@ -1623,6 +1623,20 @@ __global__ void chunk_table_kernel(Chunk *__restrict__ ch, long long n, long lon
     if (i <= nt) orig[i] = tile_origin(i, nt, dims);
 }
 
+// Chunk table of a batch whose particle ranges are in device memory: realisation r
+// (particles offs[r] .. offs[r+1]) is cut into cpr chunks of near-equal length
+// (empty ones when it is short), so the chunk count does not depend on the counts.
+__global__ void chunk_table_dev_kernel(Chunk *__restrict__ ch, const long long *__restrict__ offs,
+                                       int nreal, int cpr, int *__restrict__ orig, int nt, int dims) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nreal * cpr) {
+        const int r = i / cpr, j = i - r * cpr;
+        const long long b = offs[r], len = offs[r + 1] - b;
+        ch[i] = Chunk{b + len * j / cpr, b + len * (j + 1) / cpr, r, 0, 0, 0, 0, 0};
+    }
+    if (i <= nt) orig[i] = tile_origin(i, nt, dims);
+}
+
 // Scratch piece `id` of a call: the context's grow-only buffer, or, while
 // pyl_ma_noalloc runs (c->arena set), the next 256-byte-aligned piece of its
 // fixed device array.
@@ -1688,8 +1702,13 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     // quarter of its particles outside their chunk's box (read back without
     // waiting: until the copy has landed the previous choice stands).  Both
     // windows give the same bits (integer tile sums).
-    const bool eligible = kUseWide && axes == 3 && g.ntiles > kWin && g.ntiles <= kWinWide;
-    if (c->box_miss_n > 0) {
+    // particle ranges in device memory (pyl_ma_batch_dev): chunks cut on the device
+    const bool devt = c->dev_offs != nullptr;
+    // (the box-miss readback needs the host between calls: not with device ranges
+    // or under graph capture)
+    const bool eligible = kUseWide && axes == 3 && g.ntiles > kWin && g.ntiles <= kWinWide &&
+                          !devt && !c->capturing;
+    if (c->box_miss_n > 0 && !c->capturing) {
         const hipError_t q = hipEventQuery(c->ev_total);
         if (q == hipSuccess) {
             c->wide_hint = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
@@ -1707,10 +1726,12 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const long long target = wide ? std::max(2 * g.ntiles, std::min(16 * g.ntiles, (n + 2047) / 2048))
                                   : std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
     long long nchunk_ll = 0;
-    for (int r = 0; r < nreal; ++r) {
-        const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
-        if (e > b) nchunk_ll += (e - b + target - 1) / target;
-    }
+    if (devt) nchunk_ll = (long long)nreal * c->chunks_per_real;
+    else
+        for (int r = 0; r < nreal; ++r) {
+            const long long b = h_offs ? h_offs[r] : 0, e = h_offs ? h_offs[r + 1] : n;
+            if (e > b) nchunk_ll += (e - b + target - 1) / target;
+        }
     PYL_CHECK(nchunk_ll < (1 << 30), PYL_EINVAL, "too many particle chunks");
     const int nchunk = (int)nchunk_ll;
     // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | wmax[nreal] | pad |
@@ -1731,7 +1752,9 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     int nsplit = 1;
     if (nt < kSplitMaxKeys && n > kSplitRecords * nt)
         nsplit = (int)std::min<long long>(kSplitMax, (n + kSplitRecords * nt - 1) / (kSplitRecords * nt));
-    const bool ordered = nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt;
+    const bool ordered = c->tune_order >= 0
+        ? (nsplit == 1 && c->tune_order == 1)
+        : (nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt);
     const int T = axes == 3 ? kT3 : kT2;
     const size_t nbox = (size_t)(T + MAS - 1) * (T + MAS - 1) * (axes == 3 ? T + MAS - 1 : 1);
     const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
@@ -1768,19 +1791,24 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     // the same ranges and grid in the same allocation reuses the uploaded tables.
     // The fixed-scratch path writes them on the device every call.
     Ctx::TileTables &tt = c->tile_tables;
-    if (c->arena) {
+    if (devt) {
+        const int nfill = std::max(nchunk, g.nt + 1);
+        chunk_table_dev_kernel<<<ceil_div(nfill, kBlock), kBlock, 0, c->stream>>>(
+            dchunks, c->dev_offs, nreal, c->chunks_per_real, dorig, g.nt, dims);
+        PYL_HIP(hipGetLastError());
+    } else if (c->arena) {
         PYL_CHECK(nreal == 1, PYL_EINVAL, "fixed-scratch MA takes one grid");
         const int nfill = std::max(nchunk, g.nt + 1);
         chunk_table_kernel<<<ceil_div(nfill, kBlock), kBlock, 0, c->stream>>>(dchunks, n, target, nchunk,
                                                                            dorig, g.nt, dims);
         PYL_HIP(hipGetLastError());
     }
-    bool same = !c->arena && tt.gen == c->bufs[BUF_TILE_CNT].gen && tt.n == n && tt.nchunk == nchunk &&
+    bool same = !c->arena && !devt && tt.gen == c->bufs[BUF_TILE_CNT].gen && tt.n == n && tt.nchunk == nchunk &&
                 tt.target == target &&
                 tt.nreal == nreal && tt.dims == dims && tt.axes == axes && tt.mas == MAS &&
                 tt.offs.size() == (h_offs ? (size_t)nreal + 1 : 0);
     for (size_t i = 0; same && i < tt.offs.size(); ++i) same = tt.offs[i] == h_offs[i];
-    if (!same && !c->arena) {
+    if (!same && !c->arena && !devt) {
         tt.gen = 0;                               // valid again once both are queued
         static thread_local std::vector<Chunk> chunks;   // reused: no fresh pages per call
         chunks.clear();
@@ -1837,16 +1865,37 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     // deposit keep every record index below n, so a bad count cannot write or
     // read outside the record buffer.
     PYL_TRY(check_total(c));                    // the previous call's check, if still pending
-    PYL_HIP(hipMemcpyAsync(c->h_total, offsets + nt, sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, c->stream));
-    c->box_miss_n = -1;
-    if (eligible && nchunk > 0) {
-        PYL_HIP(hipMemcpyAsync(c->h_total + 1, box_miss, sizeof(unsigned), hipMemcpyDeviceToHost,
+    if (c->capturing) {
+        // each captured call reads its totals into pinned words of its own, compared
+        // after every replay
+        unsigned long long *ht = nullptr;
+        PYL_HIP(hipHostMalloc((void **)&ht, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+        c->cap_host.push_back(ht);
+        ht[1] = (unsigned long long)n;
+        PYL_HIP(hipMemcpyAsync(ht, offsets + nt, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                c->stream));
-        c->box_miss_n = n;
+        if (devt)
+            PYL_HIP(hipMemcpyAsync(ht + 1, c->dev_offs + nreal, sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, c->stream));
+        c->deferred.push_back([ht]() {
+            PYL_CHECK(ht[0] == ht[1], PYL_EHIP, "tile sort lost particles (%llu of %llu)", ht[0], ht[1]);
+            return PYL_OK;
+        });
+    } else {
+        PYL_HIP(hipMemcpyAsync(c->h_total, offsets + nt, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
+        if (devt)
+            PYL_HIP(hipMemcpyAsync(c->h_total + 2, c->dev_offs + nreal, sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, c->stream));
+        c->box_miss_n = -1;
+        if (eligible && nchunk > 0) {
+            PYL_HIP(hipMemcpyAsync(c->h_total + 1, box_miss, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                   c->stream));
+            c->box_miss_n = n;
+        }
+        PYL_HIP(hipEventRecord(c->ev_total, c->stream));
+        c->total_expect = devt ? -2 : n;      // -2: against the device offsets' total
     }
-    PYL_HIP(hipEventRecord(c->ev_total, c->stream));
-    c->total_expect = n;
     const unsigned long long rcap = (unsigned long long)n;
     if (nchunk > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
@@ -2029,6 +2078,7 @@ static int ma_entry(pyl_ctx *ctx, const float *pos, int64_t n, const long long *
                     int axes, const float *W, float *grid, int dims, float BoxSize, int mas,
                     int flags, float scale = 1.0f, float shift = 0.0f,
                     const float *vel = nullptr, float rsd_factor = 0.0f, int rsd_axis = 0) {
+    TraceRange trace_("pyl_ma");
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
     Ctx *c = &ctx->c;
     PYL_CHECK(axes == 2 || axes == 3, PYL_EINVAL, "pos must have 2 or 3 columns (got %d)", axes);
@@ -2128,8 +2178,29 @@ extern "C" int pyl_ma_batch(pyl_ctx *ctx, const float *pos, const int64_t *offse
                     dims, BoxSize, mas, flags);
 }
 
+extern "C" int pyl_ma_batch_dev(pyl_ctx *ctx, const float *pos, const int64_t *d_offsets, int nreal,
+                                int64_t n_cap, int chunks_per_real, int axes, const float *W,
+                                float *grids, int dims, float BoxSize, int mas, int flags) {
+    PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
+    PYL_CHECK(d_offsets != nullptr && nreal >= 1, PYL_EINVAL, "need nreal >= 1 and device offsets");
+    PYL_CHECK(flags & PYL_PTR_DEVICE, PYL_EINVAL, "pyl_ma_batch_dev takes device pointers only");
+    PYL_CHECK(!(flags & PYL_MA_ATOMIC), PYL_EINVAL, "pyl_ma_batch_dev runs the tiled algorithm only");
+    PYL_CHECK(chunks_per_real >= 1 && chunks_per_real <= 4096, PYL_EINVAL,
+              "chunks_per_real must be in 1..4096 (got %d)", chunks_per_real);
+    PYL_CHECK((long long)nreal * chunks_per_real < (1 << 30), PYL_EINVAL, "too many particle chunks");
+    Ctx *c = &ctx->c;
+    c->dev_offs = (const long long *)d_offsets;
+    c->chunks_per_real = chunks_per_real;
+    static_assert(sizeof(int64_t) == sizeof(long long), "int64_t");
+    const int s = ma_entry(ctx, pos, n_cap, nullptr, nreal, axes, W, grids, dims, BoxSize, mas, flags);
+    c->dev_offs = nullptr;
+    c->chunks_per_real = 0;
+    return s;
+}
+
 extern "C" int pyl_cic_interp(pyl_ctx *ctx, const float *density, int dims, float BoxSize,
                               const float *pos, int64_t n, float *den, int flags) {
+    TraceRange trace_("pyl_cic_interp");
     PYL_CHECK(ctx && density && den && (pos || n == 0), PYL_EINVAL, "NULL argument");
     PYL_CHECK(dims >= 1 && n >= 0 && BoxSize > 0.0f, PYL_EINVAL, "bad arguments");
     Ctx *c = &ctx->c;

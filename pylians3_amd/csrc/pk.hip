@@ -23,6 +23,8 @@
 // on the host once per (dims, axis) from Hermitian-pair counting and cached.
 #include "pyl_internal.h"
 
+#include <array>
+
 #include <algorithm>
 #include <cmath>
 #include <thread>
@@ -181,13 +183,16 @@ int get_bin_plan(Ctx *c, int dims, int axis, int nreal, BinPlan **out) {
     // so that the partials stay small
     const long long nrows_total = (long long)dims * dims;
     const long long target = nreal > 1 ? std::max(1LL, 16384LL / nreal) : 2048LL;
-    const int R = (int)std::max(4LL, std::min(256LL, nrows_total / target));
+    const int R = c->tune_bin_rows > 0 ? std::min(256, c->tune_bin_rows)
+                                       : (int)std::max(4LL, std::min(256LL, nrows_total / target));
     auto key = std::make_tuple(dims, axis, R);
     auto it = c->bin_plans.find(key);
     if (it != c->bin_plans.end()) {
         *out = &it->second;
         return PYL_OK;
     }
+    PYL_CHECK(!c->capturing, PYL_EINVAL, "binning plan created while a graph is captured: run the "
+              "same work once on the context before capturing it");
     PkDims p = pk_dims(dims);
     const int nkeys = (axis == 2) ? p.kmax_per + 1 : p.m + 1;
     std::vector<std::vector<int>> by_key(nkeys);
@@ -430,7 +435,17 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
                 r3[3 * sp + 2] += d2[sp] * (double)L4;
             }
             if (PHASE) {
-                const float ph = atan_halfplane((float)re[0], __builtin_amdgcn_sqrtf((float)d2[0]));
+                // atan2(Re, |delta|) from the float components scaled by a power of two
+                // near 1/max(|Re|, |Im|) (exact): |delta|^2 formed in float would
+                // overflow or underflow for |delta| beyond ~1e19 or below ~1e-19
+                float ph = 0.0f;
+                const float xr = (float)re[0], xi = (float)im[0];
+                const float mx = fmaxf(fabsf(xr), fabsf(xi));
+                if (mx > 0.0f) {
+                    const int e = __builtin_amdgcn_frexp_expf(mx);
+                    const float xs = __builtin_ldexpf(xr, -e), ys = __builtin_ldexpf(xi, -e);
+                    ph = atan_halfplane(xs, __builtin_amdgcn_sqrtf(xs * xs + ys * ys));
+                }
                 r3[NA3 - 1] += (double)(ph * ph);
             }
             // 2D line: axes 0/1 index k_per (slowly varying), axis 2 index kz
@@ -611,10 +626,53 @@ struct HostAcc {
     const double *P2v = nullptr, *P1v = nullptr;
 };
 
+using AccsPtr = std::shared_ptr<std::vector<HostAcc>>;
+
+// Copies the downloaded sums of one binning launch (h3 | h2 | h1, per realisation)
+// into the spectrum columns spec_out of the host accumulators.
+template <int NFL, bool PHASE>
+void bin_host_part(const PkDims &p, const double *hp, size_t out3, size_t out2, size_t out1,
+                   int nreal, const int *spec_out, std::vector<HostAcc> &accs) {
+    constexpr int S = Spec<NFL>::S;
+    constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);
+    const double *h3 = hp, *h2 = h3 + out3 * nreal, *h1 = h2 + out2 * nreal;
+    for (int rr = 0; rr < nreal; ++rr) {
+        HostAcc &acc = accs[rr];
+        const double *H3 = h3 + out3 * rr, *H2 = h2 + out2 * rr, *H1 = h1 + out1 * rr;
+        const int ST = acc.S_total;
+        for (int b = 0; b <= p.kmax; ++b) {
+            for (int sp = 0; sp < S; ++sp) {
+                int so = spec_out[sp];
+                if (so < 0) continue;
+                for (int l = 0; l < 3; ++l)
+                    acc.P3[((size_t)b * 3 + l) * ST + so] = H3[(size_t)b * NA3 + 3 * sp + l];
+            }
+            if (PHASE) acc.phase[b] = H3[(size_t)b * NA3 + NA3 - 1];
+        }
+        bool identity = ST == S;
+        for (int sp = 0; sp < S; ++sp) identity = identity && spec_out[sp] == sp;
+        if (identity) {
+            // the only launch of this call: finalize straight from the pinned buffer
+            // (valid until the next download into it)
+            acc.P2v = H2;
+            acc.P1v = H1;
+            continue;
+        }
+        for (long long i = 0; i < p.nb2; ++i)
+            for (int sp = 0; sp < S; ++sp)
+                if (spec_out[sp] >= 0) acc.P2[(size_t)i * ST + spec_out[sp]] = H2[(size_t)i * S + sp];
+        for (int i = 0; i <= p.m; ++i)
+            for (int sp = 0; sp < S; ++sp)
+                if (spec_out[sp] >= 0) acc.P1[(size_t)i * ST + spec_out[sp]] = H1[(size_t)i * S + sp];
+        acc.P2v = acc.P2.data();
+        acc.P1v = acc.P1.data();
+    }
+}
+
 template <int AXIS, int NFL, bool PHASE>
 int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
                long long fstride, int nreal, const double *const *corr,
-               std::vector<HostAcc> &accs, const int *spec_out) {
+               const AccsPtr &accs, const int *spec_out) {
     constexpr int S = Spec<NFL>::S;
     constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);
     const int L2 = (AXIS == 2) ? p.zlen : p.kmax_per + 1;
@@ -682,49 +740,34 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     // d3 | d2 | d1 are contiguous: one DMA into pinned memory
     const size_t nout = (out3 + out2 + out1) * nreal;
     void *hp = nullptr;
-    PYL_TRY(pinned(c, c->pin_down, sizeof(double) * nout, &hp));
-    PYL_HIP(hipMemcpyAsync(hp, d3, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream));
-    PYL_HIP(hipStreamSynchronize(c->stream));
-    const double *h3 = (const double *)hp, *h2 = h3 + out3 * nreal, *h1 = h2 + out2 * nreal;
-    // scatter into the caller's spectrum columns
-    for (int rr = 0; rr < nreal; ++rr) {
-        HostAcc &acc = accs[rr];
-        const double *H3 = h3 + out3 * rr, *H2 = h2 + out2 * rr, *H1 = h1 + out1 * rr;
-        const int ST = acc.S_total;
-        for (int b = 0; b <= p.kmax; ++b) {
-            for (int sp = 0; sp < S; ++sp) {
-                int so = spec_out[sp];
-                if (so < 0) continue;
-                for (int l = 0; l < 3; ++l)
-                    acc.P3[((size_t)b * 3 + l) * ST + so] = H3[(size_t)b * NA3 + 3 * sp + l];
-            }
-            if (PHASE) acc.phase[b] = H3[(size_t)b * NA3 + NA3 - 1];
-        }
-        bool identity = ST == S;
-        for (int sp = 0; sp < S; ++sp) identity = identity && spec_out[sp] == sp;
-        if (identity) {
-            // the only launch of this call: finalize straight from the pinned buffer
-            // (valid until the next download on this context)
-            acc.P2v = H2;
-            acc.P1v = H1;
-            continue;
-        }
-        for (long long i = 0; i < p.nb2; ++i)
-            for (int sp = 0; sp < S; ++sp)
-                if (spec_out[sp] >= 0) acc.P2[(size_t)i * ST + spec_out[sp]] = H2[(size_t)i * S + sp];
-        for (int i = 0; i <= p.m; ++i)
-            for (int sp = 0; sp < S; ++sp)
-                if (spec_out[sp] >= 0) acc.P1[(size_t)i * ST + spec_out[sp]] = H1[(size_t)i * S + sp];
-        acc.P2v = acc.P2.data();
-        acc.P1v = acc.P1.data();
+    if (c->capturing) {
+        // a pinned buffer of the graph's own; the host part runs after each replay
+        PYL_HIP(hipHostMalloc(&hp, sizeof(double) * nout, hipHostMallocDefault));
+        c->cap_host.push_back(hp);
+    } else {
+        PYL_TRY(pinned(c, c->pin_down, sizeof(double) * nout, &hp));
     }
+    PYL_HIP(hipMemcpyAsync(hp, d3, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream));
+    if (c->capturing) {
+        std::array<int, Spec<NFL>::S> so;
+        for (int sp = 0; sp < Spec<NFL>::S; ++sp) so[sp] = spec_out[sp];
+        const double *h = (const double *)hp;
+        AccsPtr keep = accs;
+        c->deferred.push_back([p, h, out3, out2, out1, nreal, so, keep]() {
+            bin_host_part<NFL, PHASE>(p, h, out3, out2, out1, nreal, so.data(), *keep);
+            return PYL_OK;
+        });
+        return PYL_OK;
+    }
+    PYL_HIP(hipStreamSynchronize(c->stream));
+    bin_host_part<NFL, PHASE>(p, (const double *)hp, out3, out2, out1, nreal, spec_out, *accs);
     return PYL_OK;
 }
 
 template <int NFL, bool PHASE>
 int launch_bin_axis(Ctx *c, int axis, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
                     long long fstride, int nreal, const double *const *corr,
-                    std::vector<HostAcc> &accs, const int *spec_out) {
+                    const AccsPtr &accs, const int *spec_out) {
     if (axis == 0) return launch_bin<0, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
     if (axis == 1) return launch_bin<1, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
     return launch_bin<2, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
@@ -823,6 +866,7 @@ int get_mas_tab(Ctx *c, int dims, int p, const double **out) {
     auto key = std::make_pair(dims, p);
     auto it = c->mas_tabs.find(key);
     if (it == c->mas_tabs.end()) {
+        PYL_CHECK(!c->capturing, PYL_EINVAL, "MAS table created while a graph is captured");
         std::vector<double> tab(dims);
         mas_table(dims, p, tab.data());
         void *d = nullptr;
@@ -850,8 +894,17 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
     // the mode-plan outputs are written while the FFT already queued runs
     for (int r = 0; r < nreal; ++r) PYL_TRY(finalize_modes(mp, p, BoxSize, &outs[r]));
     const int S = F + F * (F - 1) / 2;
-    // host accumulators reused across calls on this thread (no fresh pages per call)
-    static thread_local std::vector<HostAcc> accs;
+    // host accumulators reused across calls on this thread (no fresh pages per call);
+    // a captured call keeps its own, used by the host steps of every replay
+    static thread_local AccsPtr tl_accs;
+    AccsPtr accs_p;
+    if (c->capturing) {
+        accs_p = std::make_shared<std::vector<HostAcc>>();
+    } else {
+        if (!tl_accs) tl_accs = std::make_shared<std::vector<HostAcc>>();
+        accs_p = tl_accs;
+    }
+    std::vector<HostAcc> &accs = *accs_p;
     accs.resize(nreal);
     for (auto &acc : accs) {      // every column is written by exactly one launch below
         acc.S_total = S;
@@ -865,8 +918,8 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
         const float2 *f0[1] = {dk[0]};
         const double *c0[1] = {tab[0]};
         int so[1] = {0};
-        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, nreal, c0, accs, so)));
-        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, nreal, c0, accs, so)));
+        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, nreal, c0, accs_p, so)));
+        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, nreal, c0, accs_p, so)));
     } else {
         // pairs i<j in lexicographic order; each launch yields auto i, auto j, cross ij
         std::vector<bool> have(F, false);
@@ -877,8 +930,17 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
                 const double *cp[2] = {tab[i], tab[j]};
                 int so[3] = {have[i] ? -1 : i, have[j] ? -1 : j, F + xi};
                 have[i] = have[j] = true;
-                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, nreal, cp, accs, so)));
+                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, nreal, cp, accs_p, so)));
             }
+    }
+    if (c->capturing) {
+        // the caller's result structs point at host arrays that outlive the graph
+        std::vector<pyl_pk_out> o(outs, outs + nreal);
+        c->deferred.push_back([mp, p, BoxSize, accs_p, phase, o, nreal]() mutable {
+            for (int r = 0; r < nreal; ++r) finalize_values(mp, p, BoxSize, (*accs_p)[r], phase, &o[r]);
+            return PYL_OK;
+        });
+        return PYL_OK;
     }
     for (int r = 0; r < nreal; ++r) finalize_values(mp, p, BoxSize, accs[r], phase, &outs[r]);
     if (nreal > 16) {                            // do not keep large batch accumulators
@@ -922,6 +984,7 @@ int pyl_mode_counts(int dims, int axis, double *N1D, double *N2D, double *N3D, d
 }
 
 int pyl_fft_r2c(pyl_ctx *ctx, const float *delta, float *delta_k, int dims, int flags) {
+    TraceRange trace_("pyl_fft_r2c");
     PYL_CHECK(ctx && delta && delta_k && dims >= 2, PYL_EINVAL, "bad arguments");
     Ctx *c = &ctx->c;
     PYL_HIP(hipSetDevice(c->device));
@@ -948,6 +1011,7 @@ int pyl_fft_r2c(pyl_ctx *ctx, const float *delta, float *delta_k, int dims, int 
 
 int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas, int dims,
            double BoxSize, int axis, float scale, float shift, int flags, pyl_pk_out *out) {
+    TraceRange trace_("pyl_pk");
     PYL_CHECK(ctx && deltas && mas && out, PYL_EINVAL, "NULL argument");
     PYL_CHECK(nfields >= 1, PYL_EINVAL, "need at least one field");
     PYL_CHECK(dims >= 2, PYL_EINVAL, "dims must be >= 2");
@@ -1017,6 +1081,7 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
 int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfields,
                  const int *mas, int dims, double BoxSize, int axis, const float *scales,
                  const float *shifts, int flags, pyl_pk_out *outs) {
+    TraceRange trace_("pyl_pk_batch");
     PYL_CHECK(ctx && deltas && mas && outs, PYL_EINVAL, "NULL argument");
     PYL_CHECK(nreal >= 1 && nfields >= 1 && (long long)nreal * nfields <= 65535, PYL_EINVAL,
               "need 1 <= nreal*nfields <= 65535");
@@ -1098,6 +1163,7 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfield
 
 int pyl_pk_from_fourier(pyl_ctx *ctx, const float *const *delta_k, int nfields, const int *mas,
                         int dims, double BoxSize, int axis, int flags, pyl_pk_out *out) {
+    TraceRange trace_("pyl_pk_from_fourier");
     PYL_CHECK(ctx && delta_k && mas && out, PYL_EINVAL, "NULL argument");
     PYL_CHECK(nfields >= 1 && dims >= 2 && axis >= 0 && axis <= 2 && BoxSize > 0.0, PYL_EINVAL,
               "bad arguments");

@@ -167,6 +167,7 @@ int pyl_pk_plane_nbins(int grid) {
 
 int pyl_pk_plane(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas, int grid,
                  double BoxSize, int flags, double *k, double *Nmodes, double *Pk) {
+    TraceRange trace_("pyl_pk_plane");
     PYL_CHECK(ctx && deltas && mas && k && Nmodes && Pk, PYL_EINVAL, "NULL argument");
     PYL_CHECK(nfields == 1 || nfields == 2, PYL_EINVAL, "nfields must be 1 or 2");
     PYL_CHECK(grid >= 2 && grid <= (1 << 16) && BoxSize > 0.0, PYL_EINVAL, "bad arguments");
@@ -244,6 +245,7 @@ int pyl_pk_plane(pyl_ctx *ctx, const float *const *deltas, int nfields, const in
 
 int pyl_correct_mas(pyl_ctx *ctx, const float *delta, float *out, int dims, double BoxSize, int mas,
                     int flags) {
+    TraceRange trace_("pyl_correct_mas");
     PYL_CHECK(ctx && delta && out, PYL_EINVAL, "NULL argument");
     PYL_CHECK(dims >= 2 && dims <= 4096 && BoxSize > 0.0, PYL_EINVAL, "bad arguments");
     PYL_CHECK(mas >= 0 && mas <= 4, PYL_EINVAL, "bad MAS id %d", mas);

@@ -3,12 +3,14 @@
 
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <cmath>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -46,6 +48,14 @@ void set_error(const char *fmt, ...);
         int _s = (expr);                                                           \
         if (_s != PYL_OK) return _s;                                               \
     } while (0)
+
+// ---- roctx range around a C-ABI call (seen by rocprofv3 --marker-trace) ----
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 
 // ---- grow-only device buffer owned by a context ---------------------------
 struct DevBuf {
@@ -134,7 +144,8 @@ struct Ctx {
     PinnedBuf pin_scales;                                    // Pk batch scale/shift upload
     // Tile-sort consistency check of the latest tiled MA, read back without a
     // synchronisation of its own: checked at the next synchronising call.
-    unsigned long long *h_total = nullptr;                   // pinned: [0] record total, [1] box misses
+    // pinned: [0] record total, [1] box misses, [2] particle total of device offsets
+    unsigned long long *h_total = nullptr;
     long long box_miss_n = -1;       // particles of the call whose box misses h_total[1] will hold
     bool wide_hint = false;          // next eligible MA takes the wide sort window
     long long total_expect = -1;
@@ -149,6 +160,19 @@ struct Ctx {
     char *arena = nullptr;
     size_t arena_cap = 0, arena_used = 0;
     char *noalloc_base = nullptr;    // the array's address on this context's device (set-up)
+    // HIP graph capture (pyl_graph_begin .. pyl_graph_end): calls only enqueue work;
+    // host steps that need the device's results are deferred and run, in order,
+    // after each replay.  Scratch may not grow while capturing.
+    bool capturing = false;
+    std::vector<std::function<int()>> deferred;
+    std::vector<void *> cap_host;    // pinned buffers the captured work reads or fills
+    // Batched MA with particle ranges in device memory (pyl_ma_batch_dev): the sort
+    // chunks are cut on the device, chunks_per_real per realisation.
+    const long long *dev_offs = nullptr;
+    int chunks_per_real = 0;
+    // Launch parameters a tuner may set (0: the built-in choice).
+    int tune_bin_rows = 0;           // binning rows per work item
+    int tune_order = -1;             // largest-first deposit order: -1 auto, 0 off, 1 on
     // Stage timing: a ring of event pairs per stage, so that timing many calls
     // needs no host synchronisation between them.  A pair is read back when its
     // ring slot comes round again (kEvRing calls later) or on request.

@@ -79,9 +79,11 @@ def _is_small(req):
 
 class Server:
     def __init__(self, hbm_budget=None, slots=4, max_batch=32, device=0, policy='sjf',
-                 max_wait_s=2.0):
+                 max_wait_s=2.0, atomic=False):
         import torch
         self.device = device
+        # atomic=True: per-particle atomic scatter (the plain GPU form, for comparison)
+        self.ma_flags = N.MA_ATOMIC if atomic else 0
         if policy not in ('sjf', 'fifo'):
             raise ValueError('policy must be "sjf" or "fifo"')
         self.policy = policy
@@ -267,7 +269,7 @@ class Server:
             N.check(N.lib().pyl_ma_batch(ctx.handle, N.device_ptr(P), offs.ctypes.data, len(reqs), 3,
                                          None if Wc is None else N.device_ptr(Wc),
                                          N.device_ptr(grids[f]), d, reqs[0].BoxSize, mas,
-                                         N.PTR_DEVICE | N.MA_OVERWRITE | N.NO_SYNC))
+                                         N.PTR_DEVICE | N.MA_OVERWRITE | N.NO_SYNC | self.ma_flags))
         means = means.float()
         return torch.where(means == 0, torch.ones_like(means), means).contiguous()
 

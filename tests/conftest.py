@@ -127,3 +127,25 @@ def err_stats(x, ref, floor=None):
     # a bin that is exactly zero in both (e.g. Pk1D at k_par = middle) has no error
     rel = np.where(den > 0, num / np.where(den > 0, den, 1.0), np.where(num > 0, np.inf, 0.0))
     return float(rel.max()), float(np.sqrt(np.mean(rel * rel)))
+
+
+def assert_bins_same_delta(P, P32, P64, floor, rtol=BIN_RTOL, what='Pk'):
+    """Per bin, for spectra of one float32 delta: within rtol of the float32 oracle
+    (the reference's arithmetic, Pk_library.pyx:311-418, with scipy's float32 FFT in
+    place of FFTW's), or, where the two float32 FFTs differ by more than that, no
+    further from the FP64 transform of the same delta than 3x the float32 oracle is.
+    A float32 FFT's error is set by the field's total power, so bins of much lower
+    power than the field's peak (the corner of the cube under a MAS window, 2D bins
+    of one or two such modes) carry more than 1e-5 in any float32 FFT."""
+    P = np.asarray(P, dtype=np.float64)
+    P32 = np.asarray(P32, dtype=np.float64)
+    P64 = np.asarray(P64, dtype=np.float64)
+    fl = np.maximum(np.abs(P32), np.abs(floor))
+    ok32 = np.abs(P - P32) <= rtol * fl
+    ok64 = np.abs(P - P64) <= np.maximum(rtol * fl, 3.0 * np.abs(P32 - P64))
+    bad = ~(ok32 | ok64)
+    if bad.any():
+        i = int(np.argmax(np.where(bad, np.abs(P - P64) / np.maximum(fl, 1e-300), -1.0)))
+        raise AssertionError('%s: %d/%d bins out of tolerance; worst %d: got %r f32 ref %r f64 %r'
+                             % (what, int(bad.sum()), P.size, i, P.ravel()[i], P32.ravel()[i],
+                                P64.ravel()[i]))

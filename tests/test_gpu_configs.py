@@ -17,8 +17,8 @@ stated errors, printed as ``errors_vs_fp64`` lines, and bounds them.
 import numpy as np
 import pytest
 
-from conftest import (assert_bins, assert_bins_fp64, assert_cells, contrib_counts,
-                      err_stats)
+from conftest import (assert_bins, assert_bins_fp64, assert_bins_same_delta, assert_cells,
+                      contrib_counts, err_stats)
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
@@ -39,21 +39,33 @@ def _delta(g):
     return (g / np.float32(np.mean(g, dtype=np.float64)) - np.float32(1.0)).astype(np.float32)
 
 
-def _compare_pk(r, ref, what):
+def _compare_pk(r, ref, what, ref64=None):
+    """GPU spectra against the float32 oracle on the same delta: mode counts and
+    bin centres exact, k sums to double rounding, values within 1e-5 per bin -- or,
+    with ref64 (the FP64 transform of the same delta), within the float32 FFT's own
+    error where that exceeds 1e-5 (assert_bins_same_delta)."""
     np.testing.assert_array_equal(r.Nmodes3D, ref.Nmodes3D)
     np.testing.assert_array_equal(r.Nmodes2D, ref.Nmodes2D)
     np.testing.assert_array_equal(r.Nmodes1D, ref.Nmodes1D)
     np.testing.assert_array_equal(r.kpar, ref.kpar)
     np.testing.assert_array_equal(r.kper, ref.kper)
-    np.testing.assert_allclose(r.k3D, ref.k3D, rtol=1e-13)
-    np.testing.assert_allclose(r.k1D, ref.k1D, rtol=1e-13)
+    # mean |k| per bin: both sum the same square roots in double, in different
+    # orders (5e5 terms per bin at 1024^3)
+    np.testing.assert_allclose(r.k3D, ref.k3D, rtol=1e-11)
+    np.testing.assert_allclose(r.k1D, ref.k1D, rtol=1e-11)
     P0 = ref.Pk[:, 0]
-    for l in range(3):
-        assert_bins(r.Pk[:, l], ref.Pk[:, l], P0, what='%s l=%d' % (what, 2 * l))
-    assert_bins(r.Pk1D, ref.Pk1D, ref.Pk1D, what=what + ' 1D')
     fl2 = np.full_like(ref.Pk2D, np.median(np.abs(ref.Pk2D)))
-    assert_bins(r.Pk2D, ref.Pk2D, fl2, what=what + ' 2D')
-    assert_bins(r.Pkphase, ref.Pkphase, ref.Pkphase, what=what + ' phase')
+    checks = [(r.Pk[:, l], ref.Pk[:, l], None if ref64 is None else ref64.Pk[:, l], P0,
+               '%s l=%d' % (what, 2 * l)) for l in range(3)]
+    checks += [(r.Pk1D, ref.Pk1D, None if ref64 is None else ref64.Pk1D, ref.Pk1D, what + ' 1D'),
+               (r.Pk2D, ref.Pk2D, None if ref64 is None else ref64.Pk2D, fl2, what + ' 2D'),
+               (r.Pkphase, ref.Pkphase, None if ref64 is None else ref64.Pkphase, ref.Pkphase,
+                what + ' phase')]
+    for got, p32, p64, fl, w in checks:
+        if p64 is None:
+            assert_bins(got, p32, fl, what=w)
+        else:
+            assert_bins_same_delta(got, p32, p64, fl, what=w)
 
 
 def _fp64_report(tag, g, g64, r, r64, r32, g32):
@@ -159,8 +171,11 @@ def test_clustered512_pk_vs_oracle_and_fp64(oracle, clustered512):
     delta = _delta(c['grid'])
     # device-resident delta, as the benchmark step runs it
     r = PKL.Pk(torch.from_numpy(delta).cuda(), c['box'], axis=0, MAS='TSC', verbose=False)
-    ref = oracle.pk(delta, c['box'], axis=0, MAS='TSC', workers=16)
-    _compare_pk(r, ref, 'clustered512')
+    ref = oracle.pk(delta, c['box'], axis=0, MAS='TSC', workers=16, threads=16)
+    ref64 = oracle.pk(delta.astype(np.float64), c['box'], axis=0, MAS='TSC', f64=True, workers=16,
+                      threads=16)
+    _compare_pk(r, ref, 'clustered512', ref64)
+    del ref64
     del delta
     print('case clustered512 FP64 chain', flush=True)
     g64 = oracle.ma_f64(c['pos'], c['dims'], c['box'], 'TSC', threads=16)
@@ -247,7 +262,10 @@ def test_precision1024_pk_vs_oracle(oracle, precision1024):
     delta = _delta(c['grid'])
     r = PKL.Pk(torch.from_numpy(delta).cuda(), c['box'], axis=0, MAS='PCS', verbose=False)
     ref = oracle.pk(delta, c['box'], axis=0, MAS='PCS', workers=16, threads=16)
-    _compare_pk(r, ref, 'precision1024')
+    ref64 = oracle.pk(delta.astype(np.float64), c['box'], axis=0, MAS='PCS', f64=True, workers=16,
+                      threads=16)
+    _compare_pk(r, ref, 'precision1024', ref64)
+    del ref64
     c['r_gpu'], c['r_f32'] = r, ref
 
 
@@ -369,12 +387,25 @@ def test_batched256_sample_vs_oracle(oracle, batched256, i):
 
 
 # ---------------------------------------------------------------- served requests at size
+def _drop_in_grid(pos, dims, box, mas, W=None):
+    import torch
+    from pylians3_amd import MAS_library as MASL
+    g = torch.zeros((dims,) * 3, device='cuda')
+    MASL.MA(pos, g, box, mas, W)
+    return g.cpu().numpy()
+
+
 @pytest.mark.parametrize('dims,n,mas,axis,cross', [(256, 20_000_000, 'CIC', 1, True),
                                                    (512, 30_000_000, 'TSC', 2, False)])
 def test_served_large_requests_vs_oracle(oracle, dims, n, mas, axis, cross):
     """Requests of the serving mix at 256^3 and 512^3 with 1e7+ tracers (positions
     from the tracer recipe on a 256^3 displacement field, box 1000 Mpc/h), served
-    next to a small request, against the oracle."""
+    next to a small request.  The served spectra are compared with the oracle's
+    (Pk_library.pyx:263-420, :529-793) on the delta of the drop-in MA grid, which is
+    itself compared per cell with the oracle's grid (MAS_library.pyx:57-545).  (Two
+    float32 grids that agree to 1e-7 per cell can still differ by 1e-5 in the power
+    of the last bins: there the MAS window leaves 1/50 of the signal amplitude while
+    rounding differences are white.)"""
     import torch
     from pylians3_amd import workloads
     from pylians3_amd.serving import Request, Server
@@ -393,28 +424,39 @@ def test_served_large_requests_vs_oracle(oracle, dims, n, mas, axis, cross):
         res = fut.result(timeout=300)
         f_small.result(timeout=300)
     ph = pos.cpu().numpy()
-    g = np.zeros((dims,) * 3, np.float32)
-    oracle.ma_f32(ph, g, box, mas)
+    g = _drop_in_grid(pos, dims, box, mas)
+    ref_g = np.zeros((dims,) * 3, np.float32)
+    oracle.ma_f32(ph, ref_g, box, mas)
+    _cells_vs_oracle(g, ref_g, ph, dims, box, mas, 'served field 1')
     d1 = g / np.float32(n / dims ** 3) - np.float32(1.0)
     if not cross:
         ref = oracle.pk(d1, box, axis=axis, MAS=mas, workers=16, threads=16)
+        ref64 = oracle.pk(d1.astype(np.float64), box, axis=axis, MAS=mas, f64=True, workers=16,
+                          threads=16)
         np.testing.assert_array_equal(res.Nmodes3D, ref.Nmodes3D)
         P0 = ref.Pk[:, 0]
         for l in range(3):
-            assert_bins(res.Pk[:, l], ref.Pk[:, l], P0, what='served l=%d' % (2 * l))
-        assert_bins(res.Pk1D, ref.Pk1D, ref.Pk1D, what='served 1D')
+            assert_bins_same_delta(res.Pk[:, l], ref.Pk[:, l], ref64.Pk[:, l], P0,
+                                   what='served l=%d' % (2 * l))
+        assert_bins_same_delta(res.Pk1D, ref.Pk1D, ref64.Pk1D, ref.Pk1D, what='served 1D')
         return
     p2h, w2h = pos2.cpu().numpy(), W2.cpu().numpy()
-    g2 = np.zeros((dims,) * 3, np.float32)
-    oracle.ma_f32(p2h, g2, box, mas, w2h)
+    g2 = _drop_in_grid(pos2, dims, box, mas, W2)
+    ref_g2 = np.zeros((dims,) * 3, np.float32)
+    oracle.ma_f32(p2h, ref_g2, box, mas, w2h)
+    _cells_vs_oracle(g2, ref_g2, p2h, dims, box, mas, 'served field 2 (weighted)')
     mean2 = np.float32(np.sum(w2h, dtype=np.float64) / dims ** 3)
     d2 = g2 / mean2 - np.float32(1.0)
     ref = oracle.xpk([d1, d2], box, axis=axis, MAS=[mas, mas], workers=16, threads=16)
+    ref64 = oracle.xpk([d1.astype(np.float64), d2.astype(np.float64)], box, axis=axis,
+                       MAS=[mas, mas], f64=True, workers=16, threads=16)
     np.testing.assert_array_equal(res.Nmodes3D, ref.Nmodes3D)
     for f in range(2):
         P0 = ref.Pk[:, 0, f]
         for l in range(3):
-            assert_bins(res.Pk[:, l, f], ref.Pk[:, l, f], P0, what='served auto %d l=%d' % (f, 2 * l))
+            assert_bins_same_delta(res.Pk[:, l, f], ref.Pk[:, l, f], ref64.Pk[:, l, f], P0,
+                                   what='served auto %d l=%d' % (f, 2 * l))
     fl = np.sqrt(np.abs(ref.Pk[:, 0, 0] * ref.Pk[:, 0, 1]))
     for l in range(3):
-        assert_bins(res.XPk[:, l, 0], ref.XPk[:, l, 0], fl, what='served cross l=%d' % (2 * l))
+        assert_bins_same_delta(res.XPk[:, l, 0], ref.XPk[:, l, 0], ref64.XPk[:, l, 0], fl,
+                               what='served cross l=%d' % (2 * l))

@@ -46,16 +46,16 @@ def test_side_stream_device_ma_then_host_ma(oracle):
 
 
 def test_serving_waits_for_submitter_stream(oracle):
-    """Positions written by a long kernel on the default stream, submitted at once."""
+    """Positions written by a long kernel on the default stream, submitted at once:
+    the served spectrum equals the drop-in API's on the finished positions, and
+    the oracle's on the drop-in grid's delta (a read of half-written positions
+    would give neither)."""
     import torch
+    from pylians3_amd import MAS_library as MASL, Pk_library as PKL
     from pylians3_amd.serving import Request, Server
     box, dims, n = 400.0, 64, 200_000
     rng = np.random.default_rng(12)
     host = (rng.random((n, 3)) * box).astype(np.float32)
-    ref = np.zeros((dims,) * 3, np.float32)
-    oracle.ma_f32(host, ref, box, 'TSC')
-    delta = ref / np.float32(n / dims ** 3) - np.float32(1.0)
-    pref = oracle.pk(delta, box, axis=2, MAS='TSC')
     with Server(slots=2) as srv:
         src = torch.from_numpy(host).cuda()
         torch.cuda.synchronize()
@@ -67,7 +67,13 @@ def test_serving_waits_for_submitter_stream(oracle):
         pos = torch.empty_like(src)
         pos.copy_(src + 0.0 * junk[0, 0])
         res = srv.submit(Request(pos, dims, box, 'TSC', 2)).result(timeout=120)
+    grid = np.zeros((dims,) * 3, np.float32)
+    MASL.MA(host, grid, box, 'TSC')
+    delta = grid / np.float32(n / dims ** 3) - np.float32(1.0)
+    ref = PKL.Pk(delta, box, axis=2, MAS='TSC', verbose=False)
+    np.testing.assert_allclose(res.Pk, ref.Pk, rtol=1e-6, atol=1e-9 * np.abs(ref.Pk[:, 0]).max())
+    ora = oracle.pk(delta, box, axis=2, MAS='TSC')
     from conftest import assert_bins
-    np.testing.assert_array_equal(res.Nmodes3D, pref.Nmodes3D)
+    np.testing.assert_array_equal(res.Nmodes3D, ora.Nmodes3D)
     for l in range(3):
-        assert_bins(res.Pk[:, l], pref.Pk[:, l], pref.Pk[:, 0], what='served Pk l=%d' % (2 * l))
+        assert_bins(res.Pk[:, l], ora.Pk[:, l], ora.Pk[:, 0], what='served Pk l=%d' % (2 * l))

@@ -30,7 +30,7 @@ def test_tiled_index_logic(tmp_path):
     cmd = [_hipcc(), '-x', 'hip', '-O2', '-std=c++17', '--offload-arch=gfx950',
            '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
            '-I', csrc, '-I', os.path.join(ROOT, 'include'),
-           src, os.path.join(csrc, 'ctx.hip'), '-o', exe, '-lrocfft']
+           src, os.path.join(csrc, 'ctx.hip'), '-o', exe, '-lrocfft', '-lrocprofiler-sdk-roctx']
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-4000:]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)

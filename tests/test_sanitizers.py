@@ -53,7 +53,7 @@ def test_ma_index_emulator_under_asan_ubsan(tmp_path):
            '-Xarch_host', '-fsanitize=address', '-Xarch_host', '-fsanitize=undefined',
            '-Xarch_host', '-fno-sanitize-recover=all',
            '-DPYL_EMUL_QUICK=1', '-I', csrc, '-I', os.path.join(ROOT, 'include'),
-           src, os.path.join(csrc, 'ctx.hip'), '-o', exe, '-lrocfft']
+           src, os.path.join(csrc, 'ctx.hip'), '-o', exe, '-lrocfft', '-lrocprofiler-sdk-roctx']
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-4000:]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=900, env=SAN_ENV)
