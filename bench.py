@@ -11,7 +11,9 @@ One step = the reference caller's chain (Pk_snapshot.py:77-89):
     delta = grid / mean - 1        (fused into the FFT input pass)
     Pk(delta, axis=0, MAS='TSC')   (own r2c FFT passes + fused binning + finalisation)
 value = particles / step time (Mparticles/s).  The plain GPU form (per-particle
-global atomics) and a CPU baseline are timed in the same run.
+global atomics) and a CPU baseline are timed in the same run, and so are the same
+particles in random order (each MA sort form), a 1024^3 precision study, batched
+realisations and a serving stream.
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-plain]
 """
@@ -44,6 +46,7 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=20.0, help='target CPU sample time')
     ap.add_argument('--no-batched', action='store_true', help='skip the batched-realisations leg')
     ap.add_argument('--no-serving', action='store_true', help='skip the serving leg')
+    ap.add_argument('--no-random', action='store_true', help='skip the random-order sort leg')
     ap.add_argument('--no-precision', action='store_true',
                     help='skip the 1024^3 PCS precision study (tiled vs atomic vs FP64)')
     ap.add_argument('--nreal', type=int, default=256, help='realisations in the batched leg')
@@ -288,6 +291,73 @@ def cpu_check_errors(pos_host, grid_gpu, pk_gpu, dims, box, mas_name, axis, mean
                                   'Pk0': _errs(r32.Pk[:, 0], r64.Pk[:, 0], r64.Pk[:, 0])},
            'gpu_vs_f32_oracle_grid': _errs(grid_gpu, g32),
            'floors': 'grid: mean cell; Pk2, Pk4: |Pk0| of the bin; Pk2D: median |Pk2D|'}
+    return out
+
+
+def random_order_leg(N, pos, dims, box, mas, reps=5):
+    """The timed step's particles in random order (a seeded permutation): one MA per
+    sort form, forced on a context of its own -- the one-pass sort with the grid's
+    default window, the wide window, the two-level sort -- then the adaptive
+    choice (the first call takes one pass and tallies its box misses; the next calls
+    take the two-level sort).  Median of `reps` timed MAs after one warm-up (HIP
+    events around each stage), and whether every form gave the same bits."""
+    import numpy as np
+    import torch
+    L = N.lib()
+    g = torch.Generator(device='cuda').manual_seed(7)
+    perm = torch.randperm(pos.shape[0], generator=g, device='cuda')
+    sp = pos[perm].contiguous()
+    del perm
+    n = sp.shape[0]
+    grid = torch.empty((dims,) * 3, device='cuda')
+    ref = None
+    out = {'workload': 'the timed step\'s %d particles in random order, %s onto %d^3'
+                       % (n, {N.NGP: 'NGP', N.CIC: 'CIC', N.TSC: 'TSC', N.PCS: 'PCS'}[mas], dims),
+           'forms': {}}
+    same = True
+    for name, force in (('one_pass', N.SORT_ONEPASS), ('wide_window', N.SORT_WIDE),
+                        ('two_level', N.SORT_TWOLEVEL), ('adaptive', -1)):
+        ctx = N.Context(0)
+        N.check(L.pyl_ctx_set_tuning(ctx.handle, N.TUNE_SORT_WINDOW, force))
+
+        def one():
+            N.check(L.pyl_ma(ctx.handle, ctypes.c_void_p(sp.data_ptr()), n, 3, None,
+                             ctypes.c_void_p(grid.data_ptr()), dims, box, mas,
+                             N.PTR_DEVICE | N.MA_OVERWRITE))
+        one()
+        torch.cuda.synchronize()
+        first_form = ctypes.c_int()
+        N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(first_form)))
+        ctx.enable_timing(True)
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            one()
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+        tot = ctx.stage_totals()
+        form = ctypes.c_int()
+        N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(form)))
+        if ref is None:
+            ref = grid.clone()
+        else:
+            same = same and bool(torch.equal(grid, ref))
+        names = {N.WINDOW_DIRECT: 'direct', N.WINDOW_BOX: 'box', N.WINDOW_LINEAR: 'linear',
+                 N.WINDOW_WIDE: 'wide', N.WINDOW_TWOLEVEL: 'two_level'}
+        out['forms'][name] = {
+            'ma_ms_median': round(1e3 * float(np.median(t)), 3),
+            'form_taken': names.get(form.value, form.value),
+            'first_call_form': names.get(first_form.value, first_form.value),
+            'stages_ms': {k: round(v[0] / v[1], 3) for k, v in tot.items() if v[1]}}
+        ctx.close()
+    out['bitwise_equal_across_forms'] = same
+    f = out['forms']
+    out['two_level_speedup_vs_one_pass'] = round(f['one_pass']['ma_ms_median'] /
+                                                 f['two_level']['ma_ms_median'], 2)
+    out['two_level_speedup_vs_wide'] = round(f['wide_window']['ma_ms_median'] /
+                                             f['two_level']['ma_ms_median'], 2)
+    del sp, grid, ref
+    torch.cuda.empty_cache()
     return out
 
 
@@ -632,8 +702,13 @@ def main():
                  'ma_ms': round(sp.get('ma_atomic', 0.0), 4)}
         del pp
 
-    ma_ms = sum(st.get(k, 0.0) for k in ('ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit',
-                                          'ma_final'))
+    random_order = None
+    if not args.no_random:
+        log('random-order leg: the same particles shuffled, each sort form')
+        random_order = random_order_leg(N, pos, dims, box, mas)
+
+    ma_ms = sum(st.get(k, 0.0) for k in ('ma_bucket', 'ma_count', 'ma_scan', 'ma_scatter',
+                                          'ma_deposit', 'ma_final'))
     pk_ms = sum(st.get(k, 0.0) for k in ('pk_prep', 'pk_fft', 'pk_bin'))
     nk = dims * dims * (dims // 2 + 1)
     B_ma = 12.0 * n + 4.0 * dims ** 3            # positions once + grid written once
@@ -731,6 +806,7 @@ def main():
         'instrumented_ms_per_step': round(ms_all, 4),
         'stages': stages,
         'plain_gpu': plain,
+        'random_order': random_order,
         'batched': batched,
         'serving': serving,
         'roofline': roofline,

@@ -24,10 +24,12 @@ NO_SYNC = 0x0020
 PK_DESTROY_INPUT = 0x0040
 PK_NO_PHASE = 0x0080
 PK_SCALES_DEVICE = 0x0100
-TUNE_BIN_ROWS, TUNE_DEPOSIT_ORDER = 1, 2
+TUNE_BIN_ROWS, TUNE_DEPOSIT_ORDER, TUNE_SORT_WINDOW = 1, 2, 3
+WINDOW_DIRECT, WINDOW_BOX, WINDOW_LINEAR, WINDOW_WIDE, WINDOW_TWOLEVEL = 0, 1, 2, 3, 4
+SORT_ONEPASS, SORT_WIDE, SORT_TWOLEVEL = 0, 1, 2     # values of TUNE_SORT_WINDOW (-1 adaptive)
 
 STAGES = ['ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit', 'ma_atomic', 'pk_prep', 'pk_fft',
-          'pk_bin', 'ma_final']
+          'pk_bin', 'ma_final', 'ma_bucket']
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -96,6 +98,7 @@ EXPORTS = {
     'pyl_graph_launch': (ctypes.c_int, [ctypes.c_void_p]),
     'pyl_graph_destroy': (ctypes.c_int, [ctypes.c_void_p]),
     'pyl_ctx_set_tuning': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    'pyl_ctx_last_sort_window': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     'NGP': (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
                    ctypes.c_int, ctypes.c_float, ctypes.c_int]),
     'CIC': (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,

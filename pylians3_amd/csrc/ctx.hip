@@ -489,6 +489,12 @@ int pyl_graph_destroy(pyl_graph *g) {
     return PYL_OK;
 }
 
+int pyl_ctx_last_sort_window(pyl_ctx *ctx, int *window) {
+    PYL_CHECK(ctx != nullptr && window != nullptr, PYL_EINVAL, "NULL argument");
+    *window = ctx->c.last_window;
+    return PYL_OK;
+}
+
 int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value) {
     PYL_CHECK(ctx != nullptr, PYL_EINVAL, "NULL context");
     Ctx *c = &ctx->c;
@@ -500,6 +506,10 @@ int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value) {
         case PYL_TUNE_DEPOSIT_ORDER:
             PYL_CHECK(value >= -1 && value <= 1, PYL_EINVAL, "deposit order must be -1, 0 or 1");
             c->tune_order = value;
+            return PYL_OK;
+        case PYL_TUNE_SORT_WINDOW:
+            PYL_CHECK(value >= -1 && value <= 2, PYL_EINVAL, "sort window must be -1, 0, 1 or 2");
+            c->tune_window = value;
             return PYL_OK;
     }
     set_error("unknown tuning knob %d", knob);

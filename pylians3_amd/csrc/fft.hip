@@ -171,7 +171,8 @@ fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, 
 template <int N, int R>
 __global__ void __launch_bounds__(kFftBlock)
 fft_r2c_rows_kernel(const float *__restrict__ in, float2 *__restrict__ out, long long nrows,
-                    const float2 *__restrict__ tw_half, const float2 *__restrict__ tw_full) {
+                    const float2 *__restrict__ tw_half, const float2 *__restrict__ tw_full,
+                    long long zs) {
     constexpr int H = N / 2;
     constexpr int RP = R + 1;                      // padded LDS stride: rows land in distinct banks
     constexpr int PER = H * R / kFftBlock;
@@ -196,7 +197,7 @@ fft_r2c_rows_kernel(const float *__restrict__ in, float2 *__restrict__ out, long
     }
     __syncthreads();
     fft_columns<H, R, RP, kFftBlock>(buf, tw_half);
-    float2 *dst = out + row0 * (H + 1);
+    float2 *dst = out + row0 * zs;                        // output rows zs >= H+1 apart
     for (int t = threadIdx.x; t < (H + 1) * R; t += kFftBlock) {
         const int r = t / (H + 1), k = t % (H + 1);
         if (r >= rw) continue;
@@ -206,7 +207,7 @@ fft_r2c_rows_kernel(const float *__restrict__ in, float2 *__restrict__ out, long
         // O = -i/2 (a - conj(b)) = ((a.y + b.y)/2, -(a.x - b.x)/2)
         const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
         const float2 w = tw_full[k];                      // k <= N/2 < N
-        dst[(long long)r * (H + 1) + k] = cadd(e, cmul(w, o));
+        dst[(long long)r * zs + k] = cadd(e, cmul(w, o));
     }
 }
 
@@ -241,24 +242,24 @@ int cols(Ctx *c, int n, float2 *data, long long nstride, long long ostride, long
 
 template <int N>
 int launch_rows(Ctx *c, const float *in, float2 *out, long long nrows, const float2 *twh,
-                const float2 *twf) {
+                const float2 *twf, long long zs) {
     constexpr int R = (N >= 512) ? 8 : 32;
     const long long nb = (nrows + R - 1) / R;
-    fft_r2c_rows_kernel<N, R><<<(unsigned)nb, kFftBlock, 0, c->stream>>>(in, out, nrows, twh, twf);
+    fft_r2c_rows_kernel<N, R><<<(unsigned)nb, kFftBlock, 0, c->stream>>>(in, out, nrows, twh, twf, zs);
     PYL_HIP(hipGetLastError());
     return PYL_OK;
 }
 
 int rows(Ctx *c, int n, const float *in, float2 *out, long long nrows, const float2 *twh,
-         const float2 *twf) {
+         const float2 *twf, long long zs) {
     switch (n) {
-        case 16: return launch_rows<16>(c, in, out, nrows, twh, twf);
-        case 32: return launch_rows<32>(c, in, out, nrows, twh, twf);
-        case 64: return launch_rows<64>(c, in, out, nrows, twh, twf);
-        case 128: return launch_rows<128>(c, in, out, nrows, twh, twf);
-        case 256: return launch_rows<256>(c, in, out, nrows, twh, twf);
-        case 512: return launch_rows<512>(c, in, out, nrows, twh, twf);
-        case 1024: return launch_rows<1024>(c, in, out, nrows, twh, twf);
+        case 16: return launch_rows<16>(c, in, out, nrows, twh, twf, zs);
+        case 32: return launch_rows<32>(c, in, out, nrows, twh, twf, zs);
+        case 64: return launch_rows<64>(c, in, out, nrows, twh, twf, zs);
+        case 128: return launch_rows<128>(c, in, out, nrows, twh, twf, zs);
+        case 256: return launch_rows<256>(c, in, out, nrows, twh, twf, zs);
+        case 512: return launch_rows<512>(c, in, out, nrows, twh, twf, zs);
+        case 1024: return launch_rows<1024>(c, in, out, nrows, twh, twf, zs);
     }
     set_error("no row FFT of length %d", n);
     return PYL_EINVAL;
@@ -289,10 +290,21 @@ int get_twiddles(Ctx *c, int n, const float2 **out) {
     return PYL_OK;
 }
 
-// out = r2c FFT of `batch` contiguous (N,N,N) float cubes, (N,N,N/2+1) each.
-// `in` may be overwritten.
-int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch) {
+// Row stride (complex elements) of the half-complex output the library keeps for
+// itself: N/2+1 rounded up to 16 (128 bytes) for the own passes, so that every
+// 16-column group of the column passes is one aligned 128-byte segment per row;
+// the dense N/2+1 otherwise.
+long long fft_zstride(int dims) {
+    const long long z = dims / 2 + 1;
+    return fft_own_size(dims) ? (z + 15) / 16 * 16 : z;
+}
+
+// out = r2c FFT of `batch` contiguous (N,N,N) float cubes, rows of N/2+1 modes zs
+// apart (zs = 0: dense, N/2+1), N*N*zs per cube.  `in` may be overwritten.
+int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs) {
+    if (zs <= 0) zs = dims / 2 + 1;
     if (!fft_own_size(dims)) {
+        PYL_CHECK(zs == dims / 2 + 1, PYL_EINVAL, "padded rows need the own FFT passes");
         PYL_CHECK(!c->capturing, PYL_EINVAL,
                   "graph capture takes the own FFT passes only (power-of-two sides 16..1024)");
         FFTPlan *plan = nullptr;
@@ -303,12 +315,12 @@ int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch) {
     const float2 *tw = nullptr, *twh = nullptr;
     PYL_TRY(get_twiddles(c, dims, &tw));
     PYL_TRY(get_twiddles(c, dims / 2, &twh));
-    // z: real rows -> half-complex rows
-    PYL_TRY(rows(c, dims, in, out, N * N * batch, twh, tw));
-    // y: n = y (stride Z), other = x (stride N*Z)
-    PYL_TRY(cols(c, dims, out, Z, N * Z, N * N * Z, (int)Z, dims, batch, tw));
-    // x: n = x (stride N*Z), other = y (stride Z)
-    PYL_TRY(cols(c, dims, out, N * Z, Z, N * N * Z, (int)Z, dims, batch, tw));
+    // z: real rows -> half-complex rows (zs apart)
+    PYL_TRY(rows(c, dims, in, out, N * N * batch, twh, tw, zs));
+    // y: n = y (stride zs), other = x (stride N*zs); Z columns of each row
+    PYL_TRY(cols(c, dims, out, zs, N * zs, N * N * zs, (int)Z, dims, batch, tw));
+    // x: n = x (stride N*zs), other = y (stride zs)
+    PYL_TRY(cols(c, dims, out, N * zs, zs, N * N * zs, (int)Z, dims, batch, tw));
     return PYL_OK;
 }
 

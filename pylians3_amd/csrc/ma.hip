@@ -229,9 +229,9 @@ constexpr bool kUseWide = true;
 struct Chunk {
     long long c0, c1;   // particle range
     int r;              // realisation
-    int wbase;          // window placement (count pass): w0 (linear) or wx * nt + wy (box)
+    int wbase;          // window placement (count pass): w0 (linear) or wx * nt + wy (box, block)
     int smin, smax;     // first and last non-empty window slot (count pass; smin > smax: none)
-    int pad0, pad1;
+    int wz, pad1;       // block window: first tile along z
 };
 
 // Key windows of a sort chunk (TileGeom::wmode, chosen on the host):
@@ -245,9 +245,14 @@ struct Chunk {
 //              ID-ordered chunk spans a few tiles in x and y whatever the grid
 //              size, so its displaced particles stay inside;
 //  kWinLinear  (key - rbase - w0) mod ntiles < kWin, centred on the middle
-//              particle's key (grids too thin or too deep for the box).
-constexpr int kWinDirect = 0, kWinBox = 1, kWinLinear = 2;
+//              particle's key (grids too thin or too deep for the box);
+//  kWinBlock   the kBkt x kBkt x kBktZ tiles of one bucket (second pass of the two-level
+//              sort: every particle of the chunk has its home tile there).
+constexpr int kWinDirect = 0, kWinBox = 1, kWinLinear = 2, kWinBlock = 3;
 constexpr int kBoxW = 8;
+constexpr int kBkt = 8;                         // tiles per bucket along x and y
+constexpr int kBktZ = 4;                        // and along z
+constexpr int kBktSlots = kBkt * kBkt * kBktZ;  // block window slots
 
 // Home tile along one axis; POW2: every tile has extent 2^shift (or nt == 1).
 template <int MAS, bool POW2>
@@ -297,7 +302,7 @@ __device__ __forceinline__ int3 dist_tiles(const float3 &d, const TileGeom &g) {
     return t;
 }
 
-__device__ __forceinline__ unsigned tiles_key(const int3 &t, const TileGeom &g, long long rbase) {
+__host__ __device__ __forceinline__ unsigned tiles_key(const int3 &t, const TileGeom &g, long long rbase) {
     return (unsigned)(rbase + ((long long)t.x * g.nt + t.y) * g.ntz + t.z);
 }
 
@@ -306,14 +311,19 @@ __device__ __forceinline__ unsigned dist_key(const float3 &d, const TileGeom &g,
     return tiles_key(dist_tiles<MAS, AXES, POW2>(d, g), g, rbase);
 }
 
-// Chunk window: mode, placement (w0 or wx * nt + wy) and number of slots.
+// Chunk window: mode, placement (w0, or first tile along x, y, z) and number of slots.
 struct Win {
-    int mode, a, b, slots;
+    int mode, a, b, c, slots;
 };
 
-__device__ __forceinline__ Win make_win(const TileGeom &g, const int3 &t0, long long k0, long long rbase) {
-    Win w{g.wmode, 0, 0, (int)g.ntiles};
-    if (g.wmode == kWinBox) {
+__host__ __device__ __forceinline__ Win make_win(const TileGeom &g, const int3 &t0, long long k0, long long rbase) {
+    Win w{g.wmode, 0, 0, 0, (int)g.ntiles};
+    if (g.wmode == kWinBlock) {
+        w.a = t0.x / kBkt * kBkt;
+        w.b = t0.y / kBkt * kBkt;
+        w.c = t0.z / kBktZ * kBktZ;
+        w.slots = kBktSlots;
+    } else if (g.wmode == kWinBox) {
         w.a = t0.x - kBoxW / 2;
         if (w.a < 0) w.a += g.nt;
         w.b = t0.y - kBoxW / 2;
@@ -330,8 +340,13 @@ __device__ __forceinline__ Win make_win(const TileGeom &g, const int3 &t0, long 
 }
 
 // Window slot of a particle's tiles (>= w.slots: outside, counted globally).
-__device__ __forceinline__ unsigned win_slot(const int3 &t, unsigned key, const Win &w,
+__host__ __device__ __forceinline__ unsigned win_slot(const int3 &t, unsigned key, const Win &w,
                                              const TileGeom &g, long long rbase) {
+    if (w.mode == kWinBlock) {
+        const unsigned dx = (unsigned)(t.x - w.a), dy = (unsigned)(t.y - w.b), dz = (unsigned)(t.z - w.c);
+        if ((dx | dy) >= (unsigned)kBkt || dz >= (unsigned)kBktZ) return 0xffffffffu;
+        return (dx * kBkt + dy) * kBktZ + dz;
+    }
     if (w.mode == kWinBox) {
         int dx = t.x - w.a, dy = t.y - w.b;
         if (dx < 0) dx += g.nt;
@@ -344,8 +359,12 @@ __device__ __forceinline__ unsigned win_slot(const int3 &t, unsigned key, const 
     return (unsigned)s;
 }
 
-__device__ __forceinline__ long long win_key(int s, const Win &w, const TileGeom &g,
+__host__ __device__ __forceinline__ long long win_key(int s, const Win &w, const TileGeom &g,
                                              long long rbase) {
+    if (w.mode == kWinBlock) {
+        const int dz = s % kBktZ, dy = (s / kBktZ) % kBkt, dx = s / (kBkt * kBktZ);
+        return rbase + ((long long)(w.a + dx) * g.nt + (w.b + dy)) * g.ntz + (w.c + dz);
+    }
     if (w.mode == kWinBox) {
         const int tz = s % g.ntz, c = s / g.ntz;
         int tx = w.a + c / kBoxW, ty = w.b + c % kBoxW;
@@ -378,7 +397,10 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
     if (ch.c1 > ch.c0)
         t0 = dist_tiles<MAS, AXES, POW2>(load_dist<AXES>(pos, rsd, ch.c0 + (ch.c1 - ch.c0) / 2, inv), g);
     const Win w = make_win(g, t0, tiles_key(t0, g, rbase), rbase);
-    if (threadIdx.x == 0) chunks[blockIdx.x].wbase = (w.mode == kWinBox) ? w.a * g.nt + w.b : w.a;
+    if (threadIdx.x == 0) {
+        chunks[blockIdx.x].wbase = (w.mode == kWinBox || w.mode == kWinBlock) ? w.a * g.nt + w.b : w.a;
+        chunks[blockIdx.x].wz = w.c;
+    }
     // particles the 8 x 8 tile-column box around t0 would miss (box_miss: the
     // host's choice between the box and the wide window for the next call)
     TileGeom gb = g;
@@ -403,8 +425,8 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
                 if (s < (unsigned)w.slots) atomicAdd(&hist[s], 1u);
                 else atomicAdd(&ocounts[key], 1u);
                 // in the box window itself the miss is the overflow just taken
-                if (box_miss) miss += WS == kWinWide ? win_slot(t, key, wb, gb, rbase) >= (unsigned)wb.slots
-                                                     : s >= (unsigned)w.slots;
+                if (box_miss) miss += w.mode == kWinBox ? s >= (unsigned)w.slots
+                                                        : win_slot(t, key, wb, gb, rbase) >= (unsigned)wb.slots;
             }
         }
     }
@@ -425,22 +447,27 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
         smax = max(smax, __shfl_xor(smax, off));
         if (box_miss) miss += (unsigned)__shfl_xor((int)miss, off);
     }
-    if (box_miss && (threadIdx.x & 63) == 0 && miss) atomicAdd(box_miss, miss);
     __syncthreads();                           // hist is reused for the wave results
     int *red = (int *)hist;
+    const int nw = (int)(blockDim.x >> 6);
     if ((threadIdx.x & 63) == 0) {
         red[2 * (threadIdx.x >> 6)] = smin;
         red[2 * (threadIdx.x >> 6) + 1] = smax;
+        red[2 * nw + (threadIdx.x >> 6)] = (int)miss;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int a = WS, b = -1;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        unsigned m = 0;
+        for (int k = 0; k < nw; ++k) {
             a = min(a, red[2 * k]);
             b = max(b, red[2 * k + 1]);
+            m += (unsigned)red[2 * nw + k];
         }
         chunks[blockIdx.x].smin = a;
         chunks[blockIdx.x].smax = b;
+        // one atomic per workgroup on the single tally word
+        if (box_miss && m) atomicAdd(box_miss, m);
     }
 }
 
@@ -460,11 +487,12 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
     const Chunk ch = chunks[blockIdx.x];
     const long long rbase = (long long)ch.r * g.ntiles;
     // the count pass's window of this chunk
-    Win win{g.wmode, 0, 0, (int)g.ntiles};
-    if (g.wmode == kWinBox) {
+    Win win{g.wmode, 0, 0, 0, (int)g.ntiles};
+    if (g.wmode == kWinBox || g.wmode == kWinBlock) {
         win.a = ch.wbase / g.nt;
         win.b = ch.wbase - win.a * g.nt;
-        win.slots = kBoxW * kBoxW * g.ntz;
+        win.c = ch.wz;
+        win.slots = g.wmode == kWinBox ? kBoxW * kBoxW * g.ntz : kBktSlots;
     } else if (g.wmode == kWinLinear) {
         win.a = ch.wbase;
         win.slots = kWin;
@@ -610,6 +638,344 @@ scan_final_kernel(const unsigned *__restrict__ in, const unsigned *__restrict__ 
         ex += v[k];
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) out[n] = ex;
+}
+
+// ---- two-level sort (3D, particles in any order) -------------------------------
+// A chunk of particles in random order spreads over every tile of a large grid, so
+// the one-pass scatter writes each record alone into its own cache line.  The
+// two-level sort first moves every particle, as its (dist_x, dist_y, dist_z) record
+// and weight, into its bucket of kBkt x kBkt x kBktZ tiles, then sorts each bucket by tile
+// with the block window (512 LDS slots per chunk).  Both passes count, then scatter
+// through LDS in slot order (staged_scatter_kernel).  The second pass reads the
+// bucketed records with inv = 1 (dist * 1 = dist exactly) and no redshift shift,
+// so every record and every tile sum is the same as in one pass.
+constexpr int kMaxBuckets = kScanTile;           // the bucket plan is one scan workgroup
+
+// Buckets along x and y (nbt) and z (nbtz) of a grid of nt x nt x ntz tiles.
+__host__ __device__ __forceinline__ int bucket_counts_xy(int nt) { return (nt + kBkt - 1) / kBkt; }
+__host__ __device__ __forceinline__ int bucket_counts_z(int ntz) { return (ntz + kBktZ - 1) / kBktZ; }
+
+__host__ __device__ __forceinline__ int tile_bucket(const int3 &t, int nbt, int nbtz) {
+    return ((t.x / kBkt) * nbt + t.y / kBkt) * nbtz + t.z / kBktZ;
+}
+
+// Pass 1, count: particles per bucket in LDS, written as the chunk's column of the
+// bucket-major count matrix bmat[b * nchunk + chunk] (plain stores: every chunk
+// has particles in every bucket when the input is out of order, so returning
+// atomics on the bucket totals would queue nchunk deep on each address).  With few
+// buckets the lanes of a wave often hit the same LDS counter, and LDS atomics to
+// one address in one instruction are serialised: each bucket has `copies`
+// counters, lane l adding to copy l mod copies, summed at the flush.  Tallies the
+// particles the 8 x 8 tile-column box around the chunk's middle particle misses,
+// as the one-pass count does (one global atomic per workgroup).
+constexpr int kBucketHist = 16384;               // counters of the count pass (64 KB)
+
+template <int MAS, bool POW2>
+__global__ void __launch_bounds__(kSortBlock)
+bucket_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g, int nbt,
+                    const Chunk *__restrict__ chunks, unsigned *__restrict__ bmat,
+                    unsigned *__restrict__ box_miss) {
+    __shared__ unsigned hist[kBucketHist];
+    __shared__ unsigned wmiss[kSortBlock / 64];
+    const int nbtz = bucket_counts_z(g.ntz);
+    const int nb = nbt * nbt * nbtz;
+    int copies = 1;                                  // a power of two, nb * copies <= kBucketHist
+    while (copies < 16 && nb * copies * 2 <= kBucketHist) copies *= 2;
+    const int cmask = copies - 1;
+    const Chunk ch = chunks[blockIdx.x];
+    int3 t0 = make_int3(0, 0, 0);
+    if (ch.c1 > ch.c0)
+        t0 = dist_tiles<MAS, 3, POW2>(load_dist<3>(pos, rsd, ch.c0 + (ch.c1 - ch.c0) / 2, inv), g);
+    TileGeom gb = g;
+    gb.wmode = kWinBox;
+    const Win wb = make_win(gb, t0, 0, 0);
+    unsigned miss = 0;
+    for (int s = threadIdx.x; s < nb * copies; s += blockDim.x) hist[s] = 0u;
+    __syncthreads();
+    const int mycopy = (int)threadIdx.x & cmask;
+    const long long step = (long long)blockDim.x * kUnroll;
+    const long long last = ch.c1 - 1;
+    for (long long base = ch.c0 + threadIdx.x; base < ch.c1; base += step) {
+        float3 d[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+            d[u] = load_dist<3>(pos, rsd, min(base + (long long)u * blockDim.x, last), inv);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (base + (long long)u * blockDim.x < ch.c1) {
+                const int3 t = dist_tiles<MAS, 3, POW2>(d[u], g);
+                atomicAdd(&hist[tile_bucket(t, nbt, nbtz) * copies + mycopy], 1u);
+                if (box_miss) miss += win_slot(t, 0u, wb, gb, 0) >= (unsigned)wb.slots;
+            }
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < nb; s += blockDim.x) {
+        unsigned v = 0;
+        for (int k = 0; k < copies; ++k) v += hist[s * copies + k];
+        bmat[(long long)s * gridDim.x + blockIdx.x] = v;
+    }
+    if (!box_miss) return;
+    for (int off = 32; off > 0; off >>= 1) miss += (unsigned)__shfl_xor((int)miss, off);
+    if ((threadIdx.x & 63) == 0) wmiss[threadIdx.x >> 6] = miss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m += wmiss[k];
+        if (m) atomicAdd(box_miss, m);
+    }
+}
+
+// Column scan of the count matrix, one workgroup per bucket: the chunk's base
+// inside the bucket (bdump[chunk * nb + b], the layout the scatter reads) and the
+// bucket's total (bcounts[b]).
+__global__ void __launch_bounds__(kScanBlock)
+bucket_scan_kernel(const unsigned *__restrict__ bmat, int nchunk, int nb,
+                   unsigned *__restrict__ bdump, unsigned *__restrict__ bcounts) {
+    __shared__ unsigned long long sh[kScanBlock];
+    const int b = blockIdx.x;
+    const unsigned *col = bmat + (long long)b * nchunk;
+    unsigned long long carry = 0;
+    for (int c0 = 0; c0 < nchunk; c0 += kScanTile) {
+        const int i0 = c0 + threadIdx.x * kScanItems;
+        unsigned v[kScanItems];
+        unsigned long long s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = i0 + k < nchunk ? col[i0 + k] : 0u;
+            s += v[k];
+        }
+        unsigned long long tot;
+        unsigned long long ex = block_exclusive_scan(s, sh, tot) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < nchunk) bdump[(long long)(i0 + k) * nb + b] = (unsigned)ex;
+            ex += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) bcounts[b] = (unsigned)carry;
+}
+
+// Bucket offsets (exclusive scan of the counts; boffsets[nb] = total) and the chunk
+// table of pass 2: bucket b is cut into ceil(len_b / target) chunks of at most
+// target records, so no chunk straddles two buckets; the table has room for
+// n / target + nb chunks and the entries past the last one are empty.
+__global__ void __launch_bounds__(kScanBlock)
+bucket_plan_kernel(const unsigned *__restrict__ bcounts, int nb, long long target,
+                   unsigned long long *__restrict__ boffsets, Chunk *__restrict__ ch, int nchunk) {
+    __shared__ unsigned long long sh[kScanBlock];
+    const int b0 = threadIdx.x * kScanItems;
+    unsigned long long len[kScanItems], s = 0, nc = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        len[k] = b0 + k < nb ? bcounts[b0 + k] : 0ull;
+        s += len[k];
+        nc += (len[k] + target - 1) / target;
+    }
+    unsigned long long tot, ctot;
+    unsigned long long off = block_exclusive_scan(s, sh, tot);
+    unsigned long long ci = block_exclusive_scan(nc, sh, ctot);
+    for (int k = 0; k < kScanItems; ++k) {
+        if (b0 + k >= nb) break;
+        boffsets[b0 + k] = off;
+        const unsigned long long e = off + len[k];
+        for (unsigned long long c0 = off; c0 < e; c0 += target, ++ci)
+            if (ci < (unsigned long long)nchunk)
+                ch[ci] = Chunk{(long long)c0, (long long)(c0 + target < e ? c0 + target : e), 0, 0, 0, 0, 0, 0};
+        off = e;
+    }
+    if (threadIdx.x == 0) boffsets[nb] = tot;
+    for (long long i = (long long)ctot + threadIdx.x; i < nchunk; i += blockDim.x)
+        ch[i] = Chunk{0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+// LDS-staged scatter of the two-level sort.  Pass 1 (MODE 0) moves particles into
+// their buckets, pass 2 (MODE 1) the bucketed records into their tiles (block
+// window slots).  A chunk is taken STAGE particles at a time: they are counted
+// per slot in LDS, placed in LDS in slot order, and written out by consecutive
+// lanes to consecutive records of each slot's run, so a run of about STAGE /
+// slots records leaves in whole cache lines instead of one record per lane to
+// scattered lines.  The chunk's run bases come from its count pass (dump) as in
+// the one-pass scatter; the runs of a chunk's sub-chunks follow one another.
+constexpr int kStage1 = 4096;      // particles per sub-chunk, pass 1 (buckets)
+constexpr int kStage2 = 8192;      // and pass 2 (tiles)
+
+// Exclusive scan of cnt[0..ns) into ofs[0..ns) over the workgroup (kSortBlock
+// threads, ns <= NS); returns the total.  wsum: kSortBlock / 64 + 1 words of LDS.
+template <int NS>
+__device__ __forceinline__ unsigned stage_scan(const unsigned *cnt, unsigned *ofs, int ns, unsigned *wsum) {
+    constexpr int E = (NS + kSortBlock - 1) / kSortBlock;
+    constexpr int NW = kSortBlock / 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned v[E], s = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = t * E + e;
+        v[e] = i < ns ? cnt[i] : 0u;
+        s += v[e];
+    }
+    unsigned x = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = (unsigned)__shfl_up((int)x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (t == 0) {
+        unsigned a = 0;
+        for (int k = 0; k < NW; ++k) {
+            const unsigned b = wsum[k];
+            wsum[k] = a;
+            a += b;
+        }
+        wsum[NW] = a;
+    }
+    __syncthreads();
+    unsigned ex = x - s + wsum[wave];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = t * E + e;
+        if (i < ns) ofs[i] = ex;
+        ex += v[e];
+    }
+    const unsigned total = wsum[NW];
+    __syncthreads();
+    return total;
+}
+
+template <int MAS, bool POW2, bool WT, int MODE, int NS, int STAGE>
+__global__ void __launch_bounds__(kSortBlock)
+staged_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restrict__ W, float inv,
+                      TileGeom g, int nbt, const Chunk *__restrict__ chunks,
+                      const unsigned *__restrict__ dump, const unsigned long long *__restrict__ base,
+                      const unsigned *__restrict__ counts, unsigned *__restrict__ ocursors,
+                      float *__restrict__ out, float *__restrict__ out_w,
+                      unsigned *__restrict__ wmax_bits, unsigned long long rcap) {
+    constexpr int kStagePer = STAGE / kSortBlock;
+    __shared__ float sx[STAGE], sy[STAGE], sz[STAGE];
+    __shared__ float sw[WT ? STAGE : 1];
+    __shared__ unsigned short sslot[STAGE];
+    __shared__ unsigned hist[NS], lofs[NS];
+    __shared__ unsigned long long gcur[NS];
+    __shared__ unsigned wsum[kSortBlock / 64 + 1];
+    const Chunk ch = chunks[blockIdx.x];
+    const long long rbase = (long long)ch.r * g.ntiles;
+    const int t = threadIdx.x;
+    const int nbtz = bucket_counts_z(g.ntz);
+    int ns = nbt * nbt * nbtz;
+    Win win{kWinBlock, 0, 0, 0, kBktSlots};
+    if (MODE == 1) {
+        win.a = ch.wbase / g.nt;
+        win.b = ch.wbase - win.a * g.nt;
+        win.c = ch.wz;
+        ns = kBktSlots;
+    }
+    // run bases: pass 1, bucket offset + the chunk's base in the bucket; pass 2, tile
+    // offset + the chunk's base in the tile (only the slots the count found filled
+    // have a base; in pass 2 they lie in [smin, smax], and a slot of the block past
+    // the grid's last tile is never filled)
+    const unsigned *dmp = dump + (long long)blockIdx.x * (MODE == 0 ? ns : NS);
+    if (MODE == 0) {
+        for (int s = t; s < ns; s += blockDim.x) gcur[s] = base[s] + dmp[s];
+    } else {
+        for (int s = ch.smin + t; s <= ch.smax; s += blockDim.x) {
+            const int dz = s % kBktZ, dy = (s / kBktZ) % kBkt, dx = s / (kBkt * kBktZ);
+            if (win.a + dx < g.nt && win.b + dy < g.nt && win.c + dz < g.ntz)
+                gcur[s] = base[win_key(s, win, g, rbase)] + dmp[s];
+        }
+    }
+    unsigned m = 0;
+    for (long long s0 = ch.c0; s0 < ch.c1; s0 += STAGE) {
+        const int cnt = (int)min((long long)STAGE, ch.c1 - s0);
+        for (int s = t; s < ns; s += blockDim.x) hist[s] = 0u;
+        __syncthreads();
+        float3 d[kStagePer];
+        float w[kStagePer];
+        int sl[kStagePer];
+        unsigned rk[kStagePer];
+#pragma unroll
+        for (int k = 0; k < kStagePer; ++k) {
+            const long long i = s0 + min(t + k * kSortBlock, cnt - 1);
+            d[k] = load_dist<3>(pos, rsd, i, inv);
+            w[k] = WT ? W[i] : 1.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kStagePer; ++k) {
+            sl[k] = -1;
+            if (t + k * kSortBlock < cnt) {
+                const int3 tt = dist_tiles<MAS, 3, POW2>(d[k], g);
+                if (MODE == 0) {
+                    sl[k] = tile_bucket(tt, nbt, nbtz);
+                } else {
+                    const unsigned key = tiles_key(tt, g, rbase);
+                    const unsigned s = win_slot(tt, key, win, g, rbase);
+                    if (s < (unsigned)kBktSlots) {
+                        sl[k] = (int)s;
+                    } else {
+                        // outside the bucket's block (not produced by pass 1): the
+                        // overflow slots of the count pass
+                        const unsigned long long at =
+                            base[key] + counts[key] + atomicAdd(&ocursors[key], 1u);
+                        if (at < rcap) {
+                            if (WT) {
+                                reinterpret_cast<float4 *>(out)[at] = make_float4(d[k].x, d[k].y, d[k].z, w[k]);
+                            } else {
+                                out[3 * at] = d[k].x;
+                                out[3 * at + 1] = d[k].y;
+                                out[3 * at + 2] = d[k].z;
+                            }
+                        }
+                    }
+                }
+                if (sl[k] >= 0) rk[k] = atomicAdd(&hist[sl[k]], 1u);
+                if (MODE == 1 && WT) {
+                    const unsigned b = __float_as_uint(fabsf(w[k]));
+                    if (b < 0x7f800000u) m = max(m, b);
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned nst = stage_scan<NS>(hist, lofs, ns, wsum);
+#pragma unroll
+        for (int k = 0; k < kStagePer; ++k) {
+            if (sl[k] >= 0) {
+                const unsigned p = lofs[sl[k]] + rk[k];
+                sx[p] = d[k].x;
+                sy[p] = d[k].y;
+                sz[p] = d[k].z;
+                if (WT) sw[p] = w[k];
+                sslot[p] = (unsigned short)sl[k];
+            }
+        }
+        __syncthreads();
+        for (int j = t; j < (int)nst; j += blockDim.x) {
+            const int s = sslot[j];
+            const unsigned long long at = gcur[s] + (unsigned long long)(j - (int)lofs[s]);
+            if (at >= rcap) continue;                 // only after a miscount (reported)
+            if (MODE == 1 && WT) {
+                reinterpret_cast<float4 *>(out)[at] = make_float4(sx[j], sy[j], sz[j], sw[j]);
+            } else {
+                out[3 * at] = sx[j];
+                out[3 * at + 1] = sy[j];
+                out[3 * at + 2] = sz[j];
+                if (MODE == 0 && WT) out_w[at] = sw[j];
+            }
+        }
+        __syncthreads();
+        for (int s = t; s < ns; s += blockDim.x) gcur[s] += hist[s];
+        __syncthreads();
+    }
+    if (MODE == 1 && WT) {
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+        if ((t & 63) == 0) wsum[t >> 6] = m;
+        __syncthreads();
+        if (t == 0) {
+            unsigned mm = 0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mm = max(mm, wsum[k]);
+            atomicMax(&wmax_bits[ch.r], mm);
+        }
+    }
 }
 
 // ---- deposit -----------------------------------------------------------------
@@ -1694,24 +2060,35 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const int nscan_blocks = (int)((nt + kScanTile - 1) / kScanTile);
     PYL_CHECK(nscan_blocks <= kScanTile, PYL_EINVAL, "too many tiles (%lld)", nt);
     PYL_CHECK(nt < (1ll << 31), PYL_EINVAL, "too many tiles (%lld)", nt);
-    // Wide window: a 3D realisation of kWin < ntiles <= kWinWide tiles is counted
-    // over all its keys in LDS, with chunks of 2-16 particles per tile, so that
-    // particles in any order (not only lattice or ID order) are counted and
-    // placed without per-particle global atomics.
-    // Taken when the previous eligible call on this context found more than a
-    // quarter of its particles outside their chunk's box (read back without
-    // waiting: until the copy has landed the previous choice stands).  Both
-    // windows give the same bits (integer tile sums).
+    // Sort forms for particles out of order.  Coherent input (a lattice, a snapshot
+    // in ID order, a sorted catalogue) keeps each sort chunk inside a few tiles, so
+    // the one-pass sort writes its records in runs.  Particles in random order
+    // spread each chunk over the whole grid:
+    //  * the wide window (3D realisations of kWin < ntiles <= kWinWide tiles) counts
+    //    every key in LDS with chunks of 2-16 particles per tile, so no particle
+    //    takes a global atomic, but each record is still written alone;
+    //  * the two-level sort (one 3D grid of at least 16 tiles per axis) moves the
+    //    particles into buckets of kBkt^2 x kBktZ tiles first and then sorts each bucket, both
+    //    passes writing in runs.
+    // The count pass tallies the particles its chunks' 8 x 8 tile-column boxes miss
+    // (read back without waiting: until the copy has landed the previous choice
+    // stands); when more than a quarter missed, the next call with the same (dims,
+    // MAS) on this context takes the two-level sort.  Every form gives the same
+    // bits (integer tile sums).
     // particle ranges in device memory (pyl_ma_batch_dev): chunks cut on the device
     const bool devt = c->dev_offs != nullptr;
-    // (the box-miss readback needs the host between calls: not with device ranges
-    // or under graph capture)
-    const bool eligible = kUseWide && axes == 3 && g.ntiles > kWin && g.ntiles <= kWinWide &&
-                          !devt && !c->capturing;
+    // (the readback needs the host between calls: not with device ranges or under
+    // graph capture; the tally is a 32-bit counter: not for 2^32 particles or more)
+    const bool tally = axes == 3 && g.nt >= 16 && !devt && !c->capturing && n < (1ll << 32);
+    const bool wide_ok = kUseWide && axes == 3 && g.ntiles > kWin && g.ntiles <= kWinWide;
+    const int nbt = bucket_counts_xy(g.nt);
+    const int nbk = nbt * nbt * bucket_counts_z(g.ntz);
+    const bool two_ok = axes == 3 && g.nt >= 16 && nreal == 1 && !devt && !c->capturing &&
+                        !c->arena && nbk <= kMaxBuckets && n > 0 && n < (1ll << 32);
     if (c->box_miss_n > 0 && !c->capturing) {
         const hipError_t q = hipEventQuery(c->ev_total);
         if (q == hipSuccess) {
-            c->wide_hint = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
+            c->shuffled_hint[c->box_miss_key] = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
             c->box_miss_n = -1;
         } else if (q == hipErrorNotReady) {
             (void)hipGetLastError();          // pending, not an error: keep it out of later checks
@@ -1719,9 +2096,18 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
             PYL_HIP(q);
         }
     }
-    const bool wide = eligible && c->wide_hint;
+    const std::pair<int, int> hkey(dims, MAS);
+    bool wide = false, two = false;
+    if (c->tune_window == 1) wide = wide_ok;
+    else if (c->tune_window == 2) two = two_ok;
+    else if (c->tune_window < 0 && tally && c->shuffled_hint[hkey]) {
+        two = two_ok;
+        wide = !two && wide_ok;          // batches of host ranges, fixed scratch
+    }
     if (wide) g.wmode = kWinDirect;
-    const int ws = wide ? kWinWide : kWin;
+    // kWinDirect/Box/Linear == PYL_WINDOW_DIRECT/BOX/LINEAR
+    c->last_window = two ? PYL_WINDOW_TWOLEVEL : wide ? PYL_WINDOW_WIDE : g.wmode;
+    const int ws = two ? kBktSlots : wide ? kWinWide : kWin;
     // Chunks of consecutive particles, never straddling two realisations.
     const long long target = wide ? std::max(2 * g.ntiles, std::min(16 * g.ntiles, (n + 2047) / 2048))
                                   : std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
@@ -1734,16 +2120,28 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         }
     PYL_CHECK(nchunk_ll < (1 << 30), PYL_EINVAL, "too many particle chunks");
     const int nchunk = (int)nchunk_ll;
-    // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | wmax[nreal] | pad |
-    //              offsets[nt+1] | block sums | chunk table | per-chunk window bases
-    const size_t nwords = (size_t)nt * 3 + (size_t)nreal + 1;     // + box_miss
+    // two-level: pass-2 chunks of up to target2 records, none straddling two buckets
+    const long long target2 = 32768;
+    const long long nchunk2_ll = two ? (n + target2 - 1) / target2 + nbk : 0;
+    PYL_CHECK(nchunk2_ll < (1 << 30), PYL_EINVAL, "too many particle chunks");
+    const int nchunk2 = (int)nchunk2_ll;
+    // chunks of the tile sort (pass 2 of the two-level sort, or the only pass)
+    const int nsort = two ? nchunk2 : nchunk;
+    // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | wmax[nreal] | box_miss |
+    //              (two-level) bucket counts[nbk] | pad | offsets[nt+1] | block sums |
+    //              chunk table | tile origins | tile order | per-chunk window bases |
+    //              (two-level) bucket offsets[nbk+1] | pass-2 chunk table | bucket bases
+    const size_t nwords = (size_t)nt * 3 + (size_t)nreal + 1 + (two ? (size_t)nbk : 0);
     const size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t off_bytes = (((size_t)nt + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255;
     const size_t bs_bytes = ((size_t)nscan_blocks * sizeof(unsigned long long) + 255) & ~(size_t)255;
     const size_t ch_bytes = ((size_t)std::max(nchunk, 1) * sizeof(Chunk) + 255) & ~(size_t)255;
     const size_t org_bytes = ((size_t)(g.nt + 1) * sizeof(int) + 255) & ~(size_t)255;
     const size_t ord_bytes = ((size_t)nt * sizeof(unsigned) + 255) & ~(size_t)255;
-    const size_t dump_bytes = (size_t)std::max(nchunk, 1) * ws * sizeof(unsigned);
+    const size_t dump_bytes = ((size_t)std::max(nsort, 1) * ws * sizeof(unsigned) + 255) & ~(size_t)255;
+    const size_t boff_bytes = two ? (((size_t)nbk + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255 : 0;
+    const size_t ch2_bytes = two ? ((size_t)nchunk2 * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
+    const size_t bdump_bytes = two ? ((size_t)std::max(nchunk, 1) * nbk * sizeof(unsigned) + 255) & ~(size_t)255 : 0;
     // Few tiles with many records each (a small grid and many particles): each
     // tile's records are split over nsplit workgroups, which add their boxes into
     // per-tile int64 boxes, and a second pass writes the grid.  Otherwise, largest
@@ -1759,11 +2157,14 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const size_t nbox = (size_t)(T + MAS - 1) * (T + MAS - 1) * (axes == 3 ? T + MAS - 1 : 1);
     const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
     // all scratch of the call is taken before the first launch
-    void *tbv = nullptr, *pv = nullptr, *rv = nullptr, *bv = nullptr;
+    void *tbv = nullptr, *pv = nullptr, *rv = nullptr, *bv = nullptr, *kv = nullptr;
     PYL_TRY(take(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
-                                      ord_bytes + dump_bytes, &tbv));
+                                      ord_bytes + dump_bytes + boff_bytes + ch2_bytes + 2 * bdump_bytes, &tbv));
     PYL_TRY(take(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float), &pv));
     PYL_TRY(take(c, BUF_RECORDS, (size_t)(n ? n : 1) * rec_bytes, &rv));
+    // bucketed records: n (dist_x, dist_y, dist_z) | n weights, the weights 256-byte aligned
+    const size_t bxyz_bytes = ((size_t)n * 3 * sizeof(float) + 255) & ~(size_t)255;
+    if (two) PYL_TRY(take(c, BUF_BUCKETED, bxyz_bytes + (W ? (size_t)n * sizeof(float) : 0), &kv));
     if (nsplit > 1)
         PYL_TRY(take(c, BUF_TILE_BOX, (size_t)nt * nbox * sizeof(unsigned long long), &bv));
     char *tb = (char *)tbv;
@@ -1772,12 +2173,20 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     unsigned *ocursors = ocounts + nt;
     unsigned *wmax = ocursors + nt;
     unsigned *box_miss = wmax + nreal;
+    unsigned *bcounts = box_miss + 1;
     unsigned long long *offsets = (unsigned long long *)(tb + cnt_bytes);
     unsigned long long *bsums = (unsigned long long *)(tb + cnt_bytes + off_bytes);
     Chunk *dchunks = (Chunk *)(tb + cnt_bytes + off_bytes + bs_bytes);
     int *dorig = (int *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes);
     unsigned *order = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes);
     unsigned *dump = (unsigned *)(tb + cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes + ord_bytes);
+    char *tb2 = (char *)dump + dump_bytes;
+    unsigned long long *boffsets = (unsigned long long *)tb2;
+    Chunk *dchunks2 = (Chunk *)(tb2 + boff_bytes);
+    unsigned *bdump = (unsigned *)(tb2 + boff_bytes + ch2_bytes);
+    unsigned *bmat = (unsigned *)(tb2 + boff_bytes + ch2_bytes + bdump_bytes);
+    float *bxyz = (float *)kv;
+    float *bw = (two && W) ? (float *)((char *)kv + bxyz_bytes) : nullptr;
     g.orig = dorig;
     float *partials = (float *)pv;
     float *records = (float *)rv;
@@ -1803,7 +2212,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                                                                            dorig, g.nt, dims);
         PYL_HIP(hipGetLastError());
     }
-    bool same = !c->arena && !devt && tt.gen == c->bufs[BUF_TILE_CNT].gen && tt.n == n && tt.nchunk == nchunk &&
+    bool same = !c->arena && !devt && tt.gen == c->bufs[BUF_TILE_CNT].gen && tt.at == (const void *)dchunks &&
+                tt.n == n && tt.nchunk == nchunk &&
                 tt.target == target &&
                 tt.nreal == nreal && tt.dims == dims && tt.axes == axes && tt.mas == MAS &&
                 tt.offs.size() == (h_offs ? (size_t)nreal + 1 : 0);
@@ -1837,18 +2247,56 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         tt.axes = axes;
         tt.mas = MAS;
         tt.offs.assign(h_offs ? h_offs : nullptr, h_offs ? h_offs + nreal + 1 : nullptr);
+        tt.at = dchunks;
         tt.gen = c->bufs[BUF_TILE_CNT].gen;
     }
-    if (nchunk > 0) {
+    // inputs of the tile sort: the particles, or the bucketed records of pass 1
+    const float *spos = pos, *sW = W;
+    RSD srsd = rsd;
+    float sinv = inv;
+    Chunk *schunks = dchunks;
+    if (two) {
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_BUCKET));
+        auto kb = pow2 ? bucket_count_kernel<MAS, true> : bucket_count_kernel<MAS, false>;
+        kb<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, inv, g, nbt, dchunks, bmat,
+                                                 tally ? box_miss : nullptr);
+        bucket_scan_kernel<<<nbk, kScanBlock, 0, c->stream>>>(bmat, nchunk, nbk, bdump, bcounts);
+        bucket_plan_kernel<<<1, kScanBlock, 0, c->stream>>>(bcounts, nbk, target2, boffsets, dchunks2,
+                                                            nchunk2);
+        // (bucket tables of kBktSlots entries when they suffice: less LDS, more workgroups per CU)
+        auto kbs = nbk <= kBktSlots
+            ? (W ? (pow2 ? staged_scatter_kernel<MAS, true, true, 0, kBktSlots, kStage1>
+                         : staged_scatter_kernel<MAS, false, true, 0, kBktSlots, kStage1>)
+                 : (pow2 ? staged_scatter_kernel<MAS, true, false, 0, kBktSlots, kStage1>
+                         : staged_scatter_kernel<MAS, false, false, 0, kBktSlots, kStage1>))
+            : (W ? (pow2 ? staged_scatter_kernel<MAS, true, true, 0, kMaxBuckets, kStage1>
+                         : staged_scatter_kernel<MAS, false, true, 0, kMaxBuckets, kStage1>)
+                 : (pow2 ? staged_scatter_kernel<MAS, true, false, 0, kMaxBuckets, kStage1>
+                         : staged_scatter_kernel<MAS, false, false, 0, kMaxBuckets, kStage1>));
+        kbs<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, W, inv, g, nbt, dchunks, bdump, boffsets,
+                                                  counts, ocursors, bxyz, bw, nullptr,
+                                                  (unsigned long long)n);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_BUCKET));
+        spos = bxyz;
+        sW = bw;
+        srsd = RSD{bxyz, 0.0f, 0.0f, 0, 3, 0};
+        sinv = 1.0f;
+        schunks = dchunks2;
+        g.wmode = kWinBlock;
+    }
+    if (nsort > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
-        auto kc = wide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
+        auto kc = two ? (pow2 ? tile_count_kernel<MAS, 3, true, kBktSlots>
+                              : tile_count_kernel<MAS, 3, false, kBktSlots>)
+                : wide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
                                : tile_count_kernel<MAS, 3, false, kWinWide>)
                 : (axes == 3) ? (pow2 ? tile_count_kernel<MAS, 3, true, kWin>
                                       : tile_count_kernel<MAS, 3, false, kWin>)
                               : (pow2 ? tile_count_kernel<MAS, 2, true, kWin>
                                       : tile_count_kernel<MAS, 2, false, kWin>);
-        kc<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, inv, g, dchunks, counts, ocounts, dump,
-                                                 eligible ? box_miss : nullptr);
+        kc<<<nsort, kSortBlock, 0, c->stream>>>(spos, srsd, sinv, g, schunks, counts, ocounts, dump,
+                                                (tally && !two) ? box_miss : nullptr);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
     }
@@ -1888,16 +2336,28 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
             PYL_HIP(hipMemcpyAsync(c->h_total + 2, c->dev_offs + nreal, sizeof(unsigned long long),
                                    hipMemcpyDeviceToHost, c->stream));
         c->box_miss_n = -1;
-        if (eligible && nchunk > 0) {
+        if (tally && nchunk > 0) {
+            c->h_total[1] = 0;                 // the copy fills the low word
             PYL_HIP(hipMemcpyAsync(c->h_total + 1, box_miss, sizeof(unsigned), hipMemcpyDeviceToHost,
                                    c->stream));
             c->box_miss_n = n;
+            c->box_miss_key = hkey;
         }
         PYL_HIP(hipEventRecord(c->ev_total, c->stream));
         c->total_expect = devt ? -2 : n;      // -2: against the device offsets' total
     }
     const unsigned long long rcap = (unsigned long long)n;
-    if (nchunk > 0) {
+    if (nsort > 0 && two) {
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
+        auto ks = W ? (pow2 ? staged_scatter_kernel<MAS, true, true, 1, kBktSlots, kStage2>
+                            : staged_scatter_kernel<MAS, false, true, 1, kBktSlots, kStage2>)
+                    : (pow2 ? staged_scatter_kernel<MAS, true, false, 1, kBktSlots, kStage2>
+                            : staged_scatter_kernel<MAS, false, false, 1, kBktSlots, kStage2>);
+        ks<<<nsort, kSortBlock, 0, c->stream>>>(spos, srsd, sW, sinv, g, nbt, schunks, dump, offsets,
+                                                counts, ocursors, records, nullptr, wmax, rcap);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
+    } else if (nsort > 0) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
         auto ks = wide
             ? (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWinWide>
@@ -1912,8 +2372,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                                        : tile_scatter_kernel<MAS, 3, false, false, kWin>)
                                : (pow2 ? tile_scatter_kernel<MAS, 2, true, false, kWin>
                                        : tile_scatter_kernel<MAS, 2, false, false, kWin>));
-        ks<<<nchunk, kSortBlock, 0, c->stream>>>(pos, rsd, W, inv, g, dchunks, dump, offsets, counts,
-                                                 ocursors, wmax, records, rcap);
+        ks<<<nsort, kSortBlock, 0, c->stream>>>(spos, srsd, sW, sinv, g, schunks, dump, offsets, counts,
+                                                ocursors, wmax, records, rcap);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
     }

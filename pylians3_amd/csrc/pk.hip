@@ -254,6 +254,7 @@ struct BinArgs {
     const int4 *items;
     double *partials;        // [nreal][nitems][P]
     long long fstride;       // elements between realisations in dk[] (blockIdx.y = realisation)
+    long long zs;            // elements between rows (kx, ky) of dk[] (>= zlen)
     int nitems;
     int dims, m, zlen, zpad, even, kmax, L2, L1;
     int P, off2, off1;       // partial layout: acc3 at 0, line2 at off2, line1 at off1
@@ -347,7 +348,7 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
             if (r >= nrows) break;
         }
         const int kx = rowi[3 * r + 0], ky = rowi[3 * r + 1];
-        const long long rowoff = (long long)blockIdx.y * a.fstride + (long long)rowi[3 * r + 2] * a.zlen;
+        const long long rowoff = (long long)blockIdx.y * a.fstride + (long long)rowi[3 * r + 2] * a.zs;
         const int kxy2 = kx * kx + ky * ky;
         const bool kx_neg = kx < 0;
         const bool kx_edge = (kx == 0) || (a.even && kx == a.m);
@@ -671,7 +672,7 @@ void bin_host_part(const PkDims &p, const double *hp, size_t out3, size_t out2, 
 
 template <int AXIS, int NFL, bool PHASE>
 int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
-               long long fstride, int nreal, const double *const *corr,
+               long long fstride, long long zs, int nreal, const double *const *corr,
                const AccsPtr &accs, const int *spec_out) {
     constexpr int S = Spec<NFL>::S;
     constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);
@@ -706,6 +707,7 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     a.items = bp->d_items;
     a.partials = partials;
     a.fstride = fstride;
+    a.zs = zs;
     a.nitems = bp->nitems;
     a.dims = p.dims; a.m = p.m; a.zlen = p.zlen; a.zpad = p.zpad; a.even = p.even;
     a.kmax = p.kmax; a.L2 = L2; a.L1 = L1; a.P = P; a.off2 = off2; a.off1 = off1; a.rmax = rmax;
@@ -766,11 +768,11 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
 
 template <int NFL, bool PHASE>
 int launch_bin_axis(Ctx *c, int axis, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
-                    long long fstride, int nreal, const double *const *corr,
+                    long long fstride, long long zs, int nreal, const double *const *corr,
                     const AccsPtr &accs, const int *spec_out) {
-    if (axis == 0) return launch_bin<0, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
-    if (axis == 1) return launch_bin<1, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
-    return launch_bin<2, NFL, PHASE>(c, bp, p, dk, fstride, nreal, corr, accs, spec_out);
+    if (axis == 0) return launch_bin<0, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, accs, spec_out);
+    if (axis == 1) return launch_bin<1, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, accs, spec_out);
+    return launch_bin<2, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, accs, spec_out);
 }
 
 // Normalisation, Pk_library.pyx:384-418 (and :735-791 for XPk).
@@ -852,9 +854,9 @@ void finalize_values(const ModePlan *mp, const PkDims &p, double BoxSize, const 
 
 }  // namespace
 
-int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch) {
+int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs = 0) {
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_FFT));
-    PYL_TRY(fft_r2c_3d(c, in, out, dims, batch));
+    PYL_TRY(fft_r2c_3d(c, in, out, dims, batch, zs));
     PYL_TRY(stage_end(c, PYL_STAGE_PK_FFT));
     return PYL_OK;
 }
@@ -883,8 +885,10 @@ int get_mas_tab(Ctx *c, int dims, int p, const double **out) {
     return PYL_OK;
 }
 
-int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal, int F,
+// Rows of each field are zs elements apart (zs = 0: dense, dims/2+1).
+int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, long long zs, int nreal, int F,
                   const int *mas, int dims, double BoxSize, int axis, int flags, pyl_pk_out *outs) {
+    if (zs <= 0) zs = dims / 2 + 1;
     const PkDims p = pk_dims(dims);
     const ModePlan *mp = get_mode_plan(c, dims, axis);
     BinPlan *bp = nullptr;
@@ -918,8 +922,8 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
         const float2 *f0[1] = {dk[0]};
         const double *c0[1] = {tab[0]};
         int so[1] = {0};
-        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, nreal, c0, accs_p, so)));
-        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, nreal, c0, accs_p, so)));
+        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, zs, nreal, c0, accs_p, so)));
+        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, zs, nreal, c0, accs_p, so)));
     } else {
         // pairs i<j in lexicographic order; each launch yields auto i, auto j, cross ij
         std::vector<bool> have(F, false);
@@ -930,7 +934,7 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, int nreal,
                 const double *cp[2] = {tab[i], tab[j]};
                 int so[3] = {have[i] ? -1 : i, have[j] ? -1 : j, F + xi};
                 have[i] = have[j] = true;
-                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, nreal, cp, accs_p, so)));
+                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, zs, nreal, cp, accs_p, so)));
             }
     }
     if (c->capturing) {
@@ -1025,7 +1029,9 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
     PYL_HIP(hipSetDevice(c->device));
     reset_stages(c);
     const long long n = (long long)dims * dims * dims;
-    const long long nk = (long long)dims * dims * (dims / 2 + 1);
+    // the library's own half-complex buffer: rows padded to 128 bytes (fft_zstride)
+    const long long zs = fft_zstride(dims);
+    const long long nk = (long long)dims * dims * zs;
     const bool apply = !(scale == 1.0f && shift == 0.0f);
     const bool dev = (flags & PYL_PTR_DEVICE) != 0;
     const bool destroy = dev && (flags & PYL_PK_DESTROY_INPUT);
@@ -1043,7 +1049,7 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
                 PYL_HIP(hipGetLastError());
                 PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
             }
-            PYL_TRY(fft_device(c, x, dk + (size_t)f * nk, dims, 1));
+            PYL_TRY(fft_device(c, x, dk + (size_t)f * nk, dims, 1, zs));
             dks[f] = dk + (size_t)f * nk;
         }
     } else {
@@ -1071,10 +1077,10 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
             }
         }
         PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
-        PYL_TRY(fft_device(c, in, dk, dims, nfields));
+        PYL_TRY(fft_device(c, in, dk, dims, nfields, zs));
         for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nk;
     }
-    PYL_TRY(pk_bin_device(c, dks.data(), 0, 1, nfields, mas, dims, BoxSize, axis, flags, out));
+    PYL_TRY(pk_bin_device(c, dks.data(), 0, zs, 1, nfields, mas, dims, BoxSize, axis, flags, out));
     return finish(c, flags);
 }
 
@@ -1094,7 +1100,8 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfield
     PYL_HIP(hipSetDevice(c->device));
     reset_stages(c);
     const long long n = (long long)dims * dims * dims;
-    const long long nk = (long long)dims * dims * (dims / 2 + 1);
+    const long long zs = fft_zstride(dims);              // padded rows (fft_zstride)
+    const long long nk = (long long)dims * dims * zs;
     const int ncube = nreal * nfields;                 // cube (f, r) at index f*nreal + r
     const bool dev = (flags & PYL_PTR_DEVICE) != 0;
     const bool destroy = dev && (flags & PYL_PK_DESTROY_INPUT);
@@ -1151,13 +1158,13 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfield
     PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
     if (destroy) {
         for (int f = 0; f < nfields; ++f)
-            PYL_TRY(fft_device(c, (float *)deltas[f], dk + (size_t)f * nreal * nk, dims, nreal));
+            PYL_TRY(fft_device(c, (float *)deltas[f], dk + (size_t)f * nreal * nk, dims, nreal, zs));
     } else {
-        PYL_TRY(fft_device(c, in, dk, dims, ncube));
+        PYL_TRY(fft_device(c, in, dk, dims, ncube, zs));
     }
     std::vector<const float2 *> dks(nfields);
     for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nreal * nk;
-    PYL_TRY(pk_bin_device(c, dks.data(), nk, nreal, nfields, mas, dims, BoxSize, axis, flags, outs));
+    PYL_TRY(pk_bin_device(c, dks.data(), nk, zs, nreal, nfields, mas, dims, BoxSize, axis, flags, outs));
     return finish(c, flags);
 }
 
@@ -1183,7 +1190,7 @@ int pyl_pk_from_fourier(pyl_ctx *ctx, const float *const *delta_k, int nfields, 
             dks[f] = dk + (size_t)f * nk;
         }
     }
-    PYL_TRY(pk_bin_device(c, dks.data(), 0, 1, nfields, mas, dims, BoxSize, axis, flags, out));
+    PYL_TRY(pk_bin_device(c, dks.data(), 0, 0, 1, nfields, mas, dims, BoxSize, axis, flags, out));
     return finish(c, flags);
 }
 

@@ -115,6 +115,7 @@ enum BufId {
     BUF_VEL,
     BUF_PARTIAL,     // per-tile halo partial sums of the tiled deposit
     BUF_TILE_BOX,    // per-tile int64 boxes of a split deposit
+    BUF_BUCKETED,    // records of the two-level sort's first pass
     BUF_COUNT_
 };
 
@@ -135,6 +136,8 @@ struct Ctx {
     // were built from: a call with the same particle ranges and grid reuses them.
     struct TileTables {
         unsigned long long gen = 0;          // BUF_TILE_CNT allocation they live in
+        const void *at = nullptr;            // the chunk table's address in it (the layout
+                                             // before it depends on the sort form)
         long long n = -1, nchunk = -1, target = -1;
         int nreal = 0, dims = 0, axes = 0, mas = 0;
         std::vector<long long> offs;         // realisation offsets (empty: one realisation)
@@ -147,7 +150,10 @@ struct Ctx {
     // pinned: [0] record total, [1] box misses, [2] particle total of device offsets
     unsigned long long *h_total = nullptr;
     long long box_miss_n = -1;       // particles of the call whose box misses h_total[1] will hold
-    bool wide_hint = false;          // next eligible MA takes the wide sort window
+    std::pair<int, int> box_miss_key{0, 0};           // its (dims, MAS)
+    std::map<std::pair<int, int>, bool> shuffled_hint; // per (dims, MAS): the latest call's particles
+                                                        // were out of order (two-level sort next)
+    int last_window = -1;            // sort window of the latest tiled MA (PYL_WINDOW_*)
     long long total_expect = -1;
     hipEvent_t ev_total = nullptr;
     // Recorded on the outgoing stream when pyl_ctx_set_stream changes streams; the
@@ -173,6 +179,7 @@ struct Ctx {
     // Launch parameters a tuner may set (0: the built-in choice).
     int tune_bin_rows = 0;           // binning rows per work item
     int tune_order = -1;             // largest-first deposit order: -1 auto, 0 off, 1 on
+    int tune_window = -1;            // MA sort: -1 adaptive, 0 one-pass, 1 wide, 2 two-level
     // Stage timing: a ring of event pairs per stage, so that timing many calls
     // needs no host synchronisation between them.  A pair is read back when its
     // ring slot comes round again (kEvRing calls later) or on request.
@@ -207,7 +214,10 @@ int check_total(Ctx *c);                 // reports a tile sort that lost partic
 enum FFTKind { FFT_R2C_3D = 0, FFT_C2R_3D = 1, FFT_R2C_2D = 2, FFT_R2C_1D = 3 };
 int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out, int kind = FFT_R2C_3D);
 // Forward r2c of `batch` contiguous cubes (own passes for power-of-two sizes).
-int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch);
+// Rows of N/2+1 modes zs apart (zs = 0: dense); fft_zstride(dims): the padded
+// stride of the library's own half-complex buffers.
+int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs = 0);
+long long fft_zstride(int dims);
 bool fft_own_size(int dims);
 int run_fft(Ctx *c, FFTPlan *p, void *in, void *out);
 

@@ -465,8 +465,61 @@ static int check_fix64() {
     return bad;
 }
 
+// Two-level sort: every tile's bucket is in range; the block window made from any
+// tile of a bucket is the bucket's kBkt x kBkt x kBktZ tiles, gives each of its tiles its own slot
+// below kBktSlots and maps the slot back to the tile's key; tiles of the
+// neighbouring buckets fall outside; the window survives the chunk round trip
+// (wbase = a * nt + b, wz = c) the scatter pass decodes.
+static int check_block_window() {
+    int bad = 0;
+    for (int dims : {256, 272, 400, 480, 512, 1000, 1024, 2048}) {
+        TileGeom g = make_geom(dims, 3, PYL_TSC);
+        g.wmode = kWinBlock;
+        const int nbt = bucket_counts_xy(g.nt), nbtz = bucket_counts_z(g.ntz);
+        const long long nbk = (long long)nbt * nbt * nbtz;
+        if (nbk > kMaxBuckets) continue;          // the host keeps the one-pass sort
+        std::vector<std::vector<int3>> tiles(nbk);
+        for (int x = 0; x < g.nt; ++x)
+            for (int y = 0; y < g.nt; ++y)
+                for (int z = 0; z < g.nt; ++z) {
+                    const int3 t = make_int3(x, y, z);
+                    const int b = tile_bucket(t, nbt, nbtz);
+                    if (b < 0 || b >= nbk) { ++bad; continue; }
+                    tiles[b].push_back(t);
+                }
+        for (long long b = 0; b < nbk; ++b) {
+            if (tiles[b].empty()) { ++bad; continue; }
+            const int3 t0 = tiles[b][tiles[b].size() / 2];
+            const Win w = make_win(g, t0, 0, 0);
+            if (w.slots != kBktSlots || w.a % kBkt || w.b % kBkt || w.c % kBktZ) ++bad;
+            const int wbase = w.a * g.nt + w.b;
+            if (wbase / g.nt != w.a || wbase % g.nt != w.b) ++bad;
+            std::vector<char> used(kBktSlots, 0);
+            for (const int3 &t : tiles[b]) {
+                const unsigned key = tiles_key(t, g, 0);
+                const unsigned sl = win_slot(t, key, w, g, 0);
+                if (sl >= (unsigned)kBktSlots || used[sl]) { ++bad; continue; }
+                used[sl] = 1;
+                if (win_key((int)sl, w, g, 0) != (long long)key) ++bad;
+            }
+            // tiles one step outside the block along each axis (when they exist)
+            for (int a = 0; a < 3; ++a)
+                for (int side : {-1, a == 2 ? kBktZ : kBkt}) {
+                    int3 t = make_int3(w.a, w.b, w.c);
+                    int *c = a == 0 ? &t.x : a == 1 ? &t.y : &t.z;
+                    *c += side;
+                    if (*c < 0 || *c >= g.nt) continue;
+                    if (win_slot(t, tiles_key(t, g, 0), w, g, 0) < (unsigned)kBktSlots) ++bad;
+                }
+        }
+    }
+    printf("block window: %s\n", bad ? "FAIL" : "ok");
+    return bad;
+}
+
 int main() {
     int bad = check_dealing<4>(11) + check_dealing<2>(12) + check_count_bucket() + check_fix64();
+    bad += check_block_window();
     bad += check_uni_final<PYL_NGP>() + check_uni_final<PYL_CIC>() + check_uni_final<PYL_TSC>() +
            check_uni_final<PYL_PCS>();
     bad += check_box_uni<PYL_CIC>(21) + check_box_uni<PYL_TSC>(22) + check_box_uni<PYL_PCS>(23);

@@ -226,15 +226,32 @@ def test_ma_mass_weighted_sparse_vs_oracle(oracle, mas):
         assert np.array_equal(g, _gpu_ma(pos, dims, box, mas, w))
 
 
+def _forced_ma(ctx, force, pos, n, w, g, dims, box, masid):
+    """pyl_ma on ctx with the sort form forced; returns the form taken."""
+    import ctypes
+    from pylians3_amd import _native as N
+    L = N.lib()
+    N.check(L.pyl_ctx_set_tuning(ctx.handle, N.TUNE_SORT_WINDOW, force))
+    N.check(L.pyl_ma(ctx.handle, pos.ctypes.data, n, 3, None if w is None else w.ctypes.data,
+                     g.ctypes.data, dims, ctypes.c_float(box), masid, 0))
+    got = ctypes.c_int()
+    N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+    return got.value
+
+
 @pytest.mark.parametrize('mas', ['CIC', 'PCS'])
 @pytest.mark.parametrize('dims', [512, 480])
 def test_ma_wide_window_any_order(oracle, mas, dims):
-    # 3D grids of 16384 < tiles <= 32768 (512^3: 32^3 tiles; 480^3: 30^3 unequal
-    # tiles): a call whose chunks' tile boxes miss over a quarter of the particles
-    # (random order) makes the next call on the context count every tile in LDS (the
-    # wide window), and a cell-sorted call switches it back, so the calls below
-    # alternate between the two windows.  Every cell matches the oracle, with and
-    # without weights, and the same particles sorted by cell give the same bits
+    # 3D grids of 16384 < tiles <= 32768 (512^3: 32^3 tiles; 480^3: 30^3 tiles)
+    # can be sorted in one pass with the 8 x 8 tile-column box window or the wide
+    # window (every tile in LDS), or with the two-level sort (buckets of 8 x 8 x 4
+    # tiles, then tiles).  Each form is forced in turn on a context of its own and
+    # the form taken is read back; every cell matches the oracle, with and without
+    # weights, in random order and sorted by cell, and all forms give the same bits.
+    # Then the adaptive choice: random order moves the next call to the two-level
+    # sort, and tile-ordered input back to one pass.
+    import ctypes
+    from pylians3_amd import MAS_library as MASL, _native as N
     print('case wide window', mas, dims, flush=True)
     box, n = 1000.0, 1 << 21
     rng = np.random.default_rng(dims + 3)
@@ -244,14 +261,134 @@ def test_ma_wide_window_any_order(oracle, mas, dims):
     nadds = contrib_counts(pos, dims, box, mas)
     cell = np.minimum((pos.astype(np.float64) * (dims / box)).astype(np.int64), dims - 1)
     srt = np.argsort((cell[:, 0] * dims + cell[:, 1]) * dims + cell[:, 2], kind='stable')
+    L = N.lib()
+    masid = MASL._MAS_IDS[mas]
+    firsts = []
     for w in (None, W):
         ref = np.zeros((dims,) * 3, np.float32)
         oracle.ma_f32(pos, ref, box, mas, w)
-        g = _gpu_ma(pos, dims, box, mas, w)
-        assert_cells(g, ref, what='wide %s %d %s' % (mas, dims, w is not None), nadds=nadds)
-        gs = _gpu_ma(np.ascontiguousarray(pos[srt]), dims, box, mas,
-                     None if w is None else np.ascontiguousarray(w[srt]))
-        assert np.array_equal(g, gs)
+        first = None
+        for force, expect in ((N.SORT_ONEPASS, N.WINDOW_BOX), (N.SORT_WIDE, N.WINDOW_WIDE),
+                              (N.SORT_TWOLEVEL, N.WINDOW_TWOLEVEL)):
+            ctx = N.Context(0)
+            try:
+                for order in (None, srt):
+                    p = pos if order is None else np.ascontiguousarray(pos[order])
+                    ww = None if w is None else (w if order is None else np.ascontiguousarray(w[order]))
+                    g = np.zeros((dims,) * 3, np.float32)
+                    got = _forced_ma(ctx, force, p, n, ww, g, dims, box, masid)
+                    assert got == expect, (force, got)
+                    if first is None:
+                        assert_cells(g, ref, what='window %d %s %d %s' % (force, mas, dims, w is not None),
+                                     nadds=nadds)
+                        first = g
+                    else:
+                        assert np.array_equal(g, first), ('window', force, order is None)
+            finally:
+                ctx.close()
+        firsts.append(first)
+    first = firsts[0]                        # unweighted
+    # adaptive: a random-order call tallies over a quarter of its particles outside
+    # the box, so the following call on the context takes the two-level sort
+    ctx = N.Context(0)
+    try:
+        got = ctypes.c_int()
+        for k in range(3):
+            g = np.zeros((dims,) * 3, np.float32)
+            N.check(L.pyl_ma(ctx.handle, pos.ctypes.data, n, 3, None, g.ctypes.data, dims,
+                             ctypes.c_float(box), masid, 0))
+            N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+            if k == 0:
+                assert got.value == N.WINDOW_BOX
+            else:
+                assert np.array_equal(g, first)
+        assert got.value == N.WINDOW_TWOLEVEL
+        # input ordered by 16^3 tile on the same context tallies few misses (a sort
+        # chunk spans a few neighbouring tile columns): back to one pass
+        tl = cell // 16
+        nt = -(-dims // 16)
+        tsrt = np.argsort((tl[:, 0] * nt + tl[:, 1]) * nt + tl[:, 2], kind='stable')
+        p = np.ascontiguousarray(pos[tsrt])
+        for k in range(3):
+            g = np.zeros((dims,) * 3, np.float32)
+            N.check(L.pyl_ma(ctx.handle, p.ctypes.data, n, 3, None, g.ctypes.data, dims,
+                             ctypes.c_float(box), masid, 0))
+            assert np.array_equal(g, first)
+            N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+        assert got.value == N.WINDOW_BOX
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize('dims,mas,n,weights', [(256, 'TSC', 1 << 21, (False, True)),
+                                                (400, 'PCS', 1 << 21, (False, True)),
+                                                (1040, 'CIC', 1 << 22, (True,)),
+                                                (1024, 'NGP', 1 << 22, (False,))])
+def test_ma_two_level_sort_vs_oracle(oracle, dims, mas, n, weights):
+    # The two-level sort (buckets of 8 x 8 x 4 tiles) on grids of other shapes:
+    # 256^3 (16^3 tiles, every tile in the one-pass LDS window; 16 buckets), 400^3
+    # (25^3 unequal tiles, partial buckets at the upper edges), 1040^3 (65^3 tiles,
+    # 9 x 9 x 17 buckets, the last a single tile thick), 1024^3 (64^3 tiles, 1024
+    # buckets; one pass would send random-order particles to the global overflow
+    # counters).  Random order, with
+    # and without weights (one of the two on the 1024-class grids): every cell
+    # matches the oracle and equals the one-pass sort bitwise.
+    from pylians3_amd import MAS_library as MASL, _native as N
+    print('case two-level', dims, mas, n, flush=True)
+    box = 700.0
+    rng = np.random.default_rng(dims)
+    pos = (rng.random((n, 3)) * box).astype(np.float32)
+    pos[pos >= box] = 0.0
+    pos[:3] = [[0.0, 0.0, 0.0], [box * 0.999999, 0.0, box * 0.5], [box * 0.5, box * 0.999999, 0.0]]
+    W = rng.lognormal(0.0, 1.0, n).astype(np.float32)
+    masid = MASL._MAS_IDS[mas]
+    nadds = contrib_counts(pos, dims, box, mas)
+    for w in [W if wt else None for wt in weights]:
+        ref = np.zeros((dims,) * 3, np.float32)
+        oracle.ma_f32(pos, ref, box, mas, w)
+        grids = []
+        for force, expect in ((N.SORT_TWOLEVEL, N.WINDOW_TWOLEVEL), (N.SORT_ONEPASS, None)):
+            ctx = N.Context(0)
+            try:
+                g = np.zeros((dims,) * 3, np.float32)
+                got = _forced_ma(ctx, force, pos, n, w, g, dims, box, masid)
+                assert expect is None or got == expect, (force, got)
+                grids.append(g)
+            finally:
+                ctx.close()
+        assert_cells(grids[0], ref, what='two-level %d %s %s' % (dims, mas, w is not None), nadds=nadds)
+        assert np.array_equal(grids[0], grids[1])
+        del ref, grids
+
+
+def test_ma_rsd_two_level_equals_one_pass():
+    # the redshift-space shift is applied once, in the bucket pass
+    import ctypes
+    from pylians3_amd import _native as N
+    print('case two-level rsd', flush=True)
+    box, dims, n = 500.0, 512, 1 << 21
+    rng = np.random.default_rng(77)
+    pos = (rng.random((n, 3)) * box).astype(np.float32)
+    vel = rng.normal(0.0, 2000.0, (n, 3)).astype(np.float32)
+    W = rng.random(n).astype(np.float32)
+    L = N.lib()
+    out = []
+    for force in (N.SORT_TWOLEVEL, N.SORT_ONEPASS):
+        ctx = N.Context(0)
+        try:
+            N.check(L.pyl_ctx_set_tuning(ctx.handle, N.TUNE_SORT_WINDOW, force))
+            g = np.zeros((dims,) * 3, np.float32)
+            N.check(L.pyl_ma_rsd(ctx.handle, pos.ctypes.data, vel.ctypes.data, n, W.ctypes.data,
+                                 g.ctypes.data, dims, ctypes.c_float(box), N.TSC, ctypes.c_float(100.0),
+                                 ctypes.c_float(0.5), 1, ctypes.c_float(1.0), ctypes.c_float(0.0), 0))
+            got = ctypes.c_int()
+            N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+            assert (got.value == N.WINDOW_TWOLEVEL) == (force == N.SORT_TWOLEVEL)
+            out.append(g)
+        finally:
+            ctx.close()
+    assert np.array_equal(out[0], out[1])
+    assert abs(out[0].astype(np.float64).sum() - W.astype(np.float64).sum()) <= 1e-5 * W.sum()
 
 
 @pytest.mark.parametrize('mas', SCHEMES)

@@ -40,5 +40,6 @@ def test_tiled_index_logic(tmp_path):
     assert out.count(": ok") >= 380
     assert "dealing KR 4" in out and "dealing KR 2" in out and "count buckets: ok" in out
     assert "fix64: ok" in out
+    assert "block window: ok" in out
     checked = [l for l in out.splitlines() if l.startswith('uniform finalize cells checked')]
     assert checked and int(checked[0].split(':')[1]) > 0

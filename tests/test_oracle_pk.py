@@ -241,3 +241,55 @@ def test_threaded_loop_equals_serial(oracle, F):
                 np.testing.assert_allclose(b[k], a[k], rtol=1e-13, atol=1e-13 * np.abs(a[k]).max())
             for k in ('N1D', 'N2D', 'N3D', 'k1D'):
                 np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_same_delta_criterion_accepts_float32_ffts_and_rejects_mistakes(oracle):
+    """The per-bin criterion the GPU spectra are held to on large clustered fields
+    (conftest.assert_bins_same_delta): a second, independent float32 transform (a
+    direct DFT by complex64 matrix products) of a steep-spectrum field passes it; a
+    bin off by 1e-4, a bin swapped with its neighbour and a dropped Nyquist plane
+    each fail it."""
+    from conftest import assert_bins_same_delta
+    d, box = 64, 500.0
+    rng = np.random.default_rng(5)
+    # a steep (k^-3) Gaussian field, exponentiated: bins span many decades of power
+    kx = np.fft.fftfreq(d)[:, None, None]
+    ky = np.fft.fftfreq(d)[None, :, None]
+    kz = np.fft.rfftfreq(d)[None, None, :]
+    k = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    amp = np.where(k > 0, k, 1.0) ** -1.5
+    amp[0, 0, 0] = 0.0
+    g = np.fft.irfftn(amp * (rng.standard_normal(amp.shape) + 1j * rng.standard_normal(amp.shape)),
+                      s=(d, d, d), axes=(0, 1, 2))
+    g = np.exp(1.5 * g / g.std())
+    delta = (g / g.mean() - 1.0).astype(np.float32)
+    p32 = oracle.pk(delta, box, axis=0, MAS='CIC')
+    p64 = oracle.pk(delta, box, axis=0, MAS='CIC', f64=True)
+    n = np.arange(d)
+    Wz = np.exp(-2j * np.pi * np.outer(n, np.arange(d // 2 + 1)) / d).astype(np.complex64)
+    W = np.exp(-2j * np.pi * np.outer(n, n) / d).astype(np.complex64)
+    dk_t = delta.astype(np.complex64) @ Wz
+    dk_t = np.tensordot(W, dk_t, axes=(1, 0)).astype(np.complex64)
+    dk_t = np.tensordot(W, dk_t, axes=(1, 1)).transpose(1, 0, 2).astype(np.complex64)
+    assert dk_t.dtype == np.complex64
+    np.testing.assert_allclose(dk_t, oracle.fft_f32(delta), rtol=0, atol=1e-4 * np.abs(dk_t).max())
+    alt = oracle.pk(delta, box, axis=0, MAS='CIC', dk=dk_t)
+    fl2 = np.full_like(p32.Pk2D, np.median(np.abs(p32.Pk2D)))
+    cases = [(alt.Pk[:, l], p32.Pk[:, l], p64.Pk[:, l], p32.Pk[:, 0]) for l in range(3)]
+    cases += [(alt.Pk2D, p32.Pk2D, p64.Pk2D, fl2), (alt.Pk1D, p32.Pk1D, p64.Pk1D, p32.Pk1D)]
+    for got, a, b, fl in cases:
+        assert_bins_same_delta(got, a, b, fl)
+    P0 = alt.Pk[:, 0]
+    off = P0.copy()
+    off[3] *= 1 + 1e-4
+    with pytest.raises(AssertionError):
+        assert_bins_same_delta(off, p32.Pk[:, 0], p64.Pk[:, 0], p32.Pk[:, 0])
+    swp = P0.copy()
+    swp[[5, 6]] = swp[[6, 5]]
+    with pytest.raises(AssertionError):
+        assert_bins_same_delta(swp, p32.Pk[:, 0], p64.Pk[:, 0], p32.Pk[:, 0])
+    dk_cut = dk_t.copy()
+    dk_cut[d // 2] = 0
+    cut = oracle.pk(delta, box, axis=0, MAS='CIC', dk=dk_cut)
+    with pytest.raises(AssertionError):
+        assert_bins_same_delta(cut.Pk2D, p32.Pk2D, p64.Pk2D, fl2)
