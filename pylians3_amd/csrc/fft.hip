@@ -118,10 +118,25 @@ __device__ __forceinline__ void fft_columns(float2 *buf, const float2 *__restric
 // In-place FFT of length N along one axis of a batch of complex arrays:
 // element n of column (o, z) of batch b at data[b*bstride + o*ostride + n*nstride + z],
 // z in [0, zlen).  Grid: (ceil(zlen/C), nother, nbatch).
+// amax (optional, the last pass): the largest |Re| or |Im| of the transform of cube
+// b, as float bits, atomically maxed into one of 64 words amax[b*64 + (group & 63)]
+// (an integer max, so the words do not depend on the order of the atomics).
+__device__ __forceinline__ void block_absmax(float m, unsigned *__restrict__ amax_words, unsigned slot) {
+    __shared__ unsigned wmax[16];
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = __float_as_uint(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned mm = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mm = max(mm, wmax[k]);
+        if (mm) atomicMax(&amax_words[slot], mm);
+    }
+}
+
 template <int N, int C, int BLK>
 __global__ void __launch_bounds__(BLK)
 fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, long long bstride,
-               int zlen, int nother, const float2 *__restrict__ tw) {
+               int zlen, int nother, const float2 *__restrict__ tw, unsigned *__restrict__ amax) {
     constexpr int PER = N * C / BLK;
     static_assert(PER * BLK == N * C, "block shape");
     __shared__ float2 buf[N * C];
@@ -155,12 +170,36 @@ fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, 
     }
     __syncthreads();
     fft_columns<N, C, C, BLK>(buf, tw);
+    float m = 0.0f;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int t = threadIdx.x + p * BLK;
         const int c = t % C, n = t / C;
-        if (c < cw) base[(long long)n * nstride + c] = buf[t];
+        if (c < cw) {
+            const float2 x = buf[t];
+            base[(long long)n * nstride + c] = x;
+            m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+        }
     }
+    if (amax) block_absmax(m, amax, b * 64u + (L & 63u));
+}
+
+// The same maximum over nb cubes of half-complex rows (zlen modes, zs apart, cube
+// stride cstride), for transforms not made by the passes above.
+__global__ void __launch_bounds__(256)
+absmax_kernel(const float2 *__restrict__ dk, long long nrows, int zlen, long long zs,
+              long long cstride, unsigned *__restrict__ amax) {
+    const long long cube = blockIdx.y;
+    const float2 *d = dk + cube * cstride;
+    const long long n = nrows * zlen;
+    float m = 0.0f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long r = i / zlen, k = i - r * zlen;
+        const float2 x = d[r * zs + k];
+        m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+    }
+    block_absmax(m, amax, (unsigned)cube * 64u + (blockIdx.x & 63u));
 }
 
 // Real-to-complex FFT of rows of N floats (contiguous), R rows per workgroup:
@@ -213,28 +252,28 @@ fft_r2c_rows_kernel(const float *__restrict__ in, float2 *__restrict__ out, long
 
 template <int N>
 int launch_cols(Ctx *c, float2 *data, long long nstride, long long ostride, long long bstride,
-                int zlen, int nother, int nbatch, const float2 *tw) {
+                int zlen, int nother, int nbatch, const float2 *tw, unsigned *amax) {
     // 16 complex (128 B) contiguous per point; 64 KB of LDS at most
     constexpr int C = (N >= 1024) ? 8 : 16;
     constexpr int BLK = (N >= 512) ? 512 : 256;
     const long long nwg = (long long)ceil_div(zlen, C) * nother * nbatch;
     PYL_CHECK(nwg < (1ll << 31), PYL_EINVAL, "FFT column launch too large (%lld workgroups)", nwg);
     fft_col_kernel<N, C, BLK><<<(unsigned)nwg, BLK, 0, c->stream>>>(data, nstride, ostride, bstride,
-                                                                     zlen, nother, tw);
+                                                                     zlen, nother, tw, amax);
     PYL_HIP(hipGetLastError());
     return PYL_OK;
 }
 
 int cols(Ctx *c, int n, float2 *data, long long nstride, long long ostride, long long bstride,
-         int zlen, int nother, int nbatch, const float2 *tw) {
+         int zlen, int nother, int nbatch, const float2 *tw, unsigned *amax = nullptr) {
     switch (n) {
-        case 16: return launch_cols<16>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
-        case 32: return launch_cols<32>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
-        case 64: return launch_cols<64>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
-        case 128: return launch_cols<128>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
-        case 256: return launch_cols<256>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
-        case 512: return launch_cols<512>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
-        case 1024: return launch_cols<1024>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw);
+        case 16: return launch_cols<16>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw, amax);
+        case 32: return launch_cols<32>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw, amax);
+        case 64: return launch_cols<64>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw, amax);
+        case 128: return launch_cols<128>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw, amax);
+        case 256: return launch_cols<256>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw, amax);
+        case 512: return launch_cols<512>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw, amax);
+        case 1024: return launch_cols<1024>(c, data, nstride, ostride, bstride, zlen, nother, nbatch, tw, amax);
     }
     set_error("no column FFT of length %d", n);
     return PYL_EINVAL;
@@ -299,9 +338,21 @@ long long fft_zstride(int dims) {
     return fft_own_size(dims) ? (z + 15) / 16 * 16 : z;
 }
 
+int complex_absmax(Ctx *c, const float2 *dk, int dims, long long zs, int ncube, long long cstride,
+                   unsigned *amax) {
+    if (zs <= 0) zs = dims / 2 + 1;
+    const long long nrows = (long long)dims * dims;
+    const int nbx = (int)std::max<long long>(64, std::min<long long>(1024, nrows * (dims / 2 + 1) / 4096));
+    absmax_kernel<<<dim3(nbx, ncube), 256, 0, c->stream>>>(dk, nrows, dims / 2 + 1, zs, cstride, amax);
+    PYL_HIP(hipGetLastError());
+    return PYL_OK;
+}
+
 // out = r2c FFT of `batch` contiguous (N,N,N) float cubes, rows of N/2+1 modes zs
-// apart (zs = 0: dense, N/2+1), N*N*zs per cube.  `in` may be overwritten.
-int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs) {
+// apart (zs = 0: dense, N/2+1), N*N*zs per cube.  `in` may be overwritten.  amax
+// (optional, 64 zeroed words per cube): the largest |Re| or |Im| of each cube's
+// transform, float bits (see fft_col_kernel).
+int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs, unsigned *amax) {
     if (zs <= 0) zs = dims / 2 + 1;
     if (!fft_own_size(dims)) {
         PYL_CHECK(zs == dims / 2 + 1, PYL_EINVAL, "padded rows need the own FFT passes");
@@ -309,7 +360,10 @@ int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs
                   "graph capture takes the own FFT passes only (power-of-two sides 16..1024)");
         FFTPlan *plan = nullptr;
         PYL_TRY(get_fft_plan(c, dims, batch, &plan, FFT_R2C_3D));
-        return run_fft(c, plan, in, out);
+        PYL_TRY(run_fft(c, plan, in, out));
+        if (amax)
+            PYL_TRY(complex_absmax(c, out, dims, zs, batch, (long long)dims * dims * zs, amax));
+        return PYL_OK;
     }
     const long long N = dims, Z = dims / 2 + 1;
     const float2 *tw = nullptr, *twh = nullptr;
@@ -320,7 +374,7 @@ int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs
     // y: n = y (stride zs), other = x (stride N*zs); Z columns of each row
     PYL_TRY(cols(c, dims, out, zs, N * zs, N * N * zs, (int)Z, dims, batch, tw));
     // x: n = x (stride N*zs), other = y (stride zs)
-    PYL_TRY(cols(c, dims, out, N * zs, zs, N * N * zs, (int)Z, dims, batch, tw));
+    PYL_TRY(cols(c, dims, out, N * zs, zs, N * N * zs, (int)Z, dims, batch, tw, amax));
     return PYL_OK;
 }
 

@@ -22,6 +22,7 @@
 // Mode counts and k sums do not depend on the data: they are computed exactly
 // on the host once per (dims, axis) from Hermitian-pair counting and cached.
 #include "pyl_internal.h"
+#include "fixsum.h"
 
 #include <array>
 
@@ -183,8 +184,12 @@ int get_bin_plan(Ctx *c, int dims, int axis, int nreal, BinPlan **out) {
     // so that the partials stay small
     const long long nrows_total = (long long)dims * dims;
     const long long target = nreal > 1 ? std::max(1LL, 16384LL / nreal) : 2048LL;
-    const int R = c->tune_bin_rows > 0 ? std::min(256, c->tune_bin_rows)
-                                       : (int)std::max(4LL, std::min(256LL, nrows_total / target));
+    int R = c->tune_bin_rows > 0 ? std::min(256, c->tune_bin_rows)
+                                 : (int)std::max(4LL, std::min(256LL, nrows_total / target));
+    // an exact bin sum's low word takes at most kFixMaxAdds additions: one per run of
+    // kRun modes of the item's rows
+    const int runs_per_row = (pk_dims(dims).zpad) / kRun;
+    R = std::max(1, std::min(R, kFixMaxAdds / runs_per_row));
     auto key = std::make_tuple(dims, axis, R);
     auto it = c->bin_plans.find(key);
     if (it != c->bin_plans.end()) {
@@ -250,9 +255,14 @@ template <int NFL> struct Spec { static constexpr int S = NFL == 1 ? 1 : 3; };
 struct BinArgs {
     const float2 *dk[2];     // half-complex fields
     const double *corr[2];   // MAS correction tables [dims] per field
+    float cmax[2];           // largest product of three MAS corrections per field
+    const unsigned *amax[2]; // per field [realisation][64]: largest |Re|, |Im| (float bits)
+    int *escale;             // [realisation][2]: E of the power and the phase sums
+    int nreal;
     const int *rows;
     const int4 *items;
-    double *partials;        // [nreal][nitems][P]
+    unsigned long long *plo; // [nreal][nitems][P] low words of the item sums
+    int *phi;                // [nreal][nitems][P] high words
     long long fstride;       // elements between realisations in dk[] (blockIdx.y = realisation)
     long long zs;            // elements between rows (kx, ky) of dk[] (>= zlen)
     int nitems;
@@ -298,22 +308,45 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
     constexpr int S = Spec<NFL>::S;
     constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);   // per 3D bin: P0,P2,P4 per spectrum [+phase]
     extern __shared__ double lds[];
-    // acc3 is [NA3][kmax+1] in LDS (consecutive bins in consecutive banks: lanes
-    // flushing neighbouring bins do not collide), [kmax+1][NA3] in the partials
-    double *acc3 = lds;                               // [NA3*(kmax+1)]
+    // The P accumulators, fixed point: low words alo[P], high words ahi[P].  acc3
+    // (indices [0, n3)) is [NA3][kmax+1] in LDS (consecutive bins in consecutive
+    // banks: lanes flushing neighbouring bins do not collide), [kmax+1][NA3] in the
+    // partials; the 2D line at off2, the 1D line at off1.
+    unsigned long long *alo = (unsigned long long *)lds;
+    int *ahi = (int *)(alo + a.P);
     const int K1 = a.kmax + 1;
-    double *line2 = acc3 + (a.kmax + 1) * NA3;        // [L2*S]
-    double *line1 = line2 + a.L2 * S;                 // [L1*S]
-    double *corrz = line1 + a.L1 * S;                 // [zlen*NFL]
+    double *corrz = (double *)(ahi + ((a.P + 1) & ~1));  // [zlen*NFL]
     double *rowc = corrz + a.zlen * NFL;              // [rmax*NFL]
     int *rowi = (int *)(rowc + a.rmax * NFL);         // [rmax*3]: kx, ky, linear row
+    __shared__ int sE[2];
 
     const int4 item = a.items[blockIdx.x];
     const int nrows = item.y;
     const int key = item.z;
     const int tid = threadIdx.x;
     const int n3 = (a.kmax + 1) * NA3;
-    for (int i = tid; i < n3 + (a.L2 + a.L1) * S; i += blockDim.x) lds[i] = 0.0;
+    for (int i = tid; i < a.P; i += blockDim.x) {
+        alo[i] = 0ull;
+        ahi[i] = 0;
+    }
+    if (tid == 0) {
+        // the realisation's scale: the same in every work item
+        const double nm = (double)a.dims * a.dims * a.zlen;
+        double bound = 0.0;
+        for (int f = 0; f < NFL; ++f) {
+            const unsigned *w = a.amax[f] + (long long)blockIdx.y * 64;
+            float m = 0.0f;
+            for (int j = 0; j < 64; ++j) m = fmaxf(m, __uint_as_float(w[j]));
+            const double am = (double)m * (double)a.cmax[f];
+            bound = fmax(bound, 2.0 * nm * am * am);
+        }
+        sE[0] = fix_exponent(bound * 1.01);
+        sE[1] = fix_exponent(nm);
+        if (blockIdx.x == 0) {
+            a.escale[2 * blockIdx.y] = sE[0];
+            a.escale[2 * blockIdx.y + 1] = sE[1];
+        }
+    }
     for (int i = tid; i < a.zlen * NFL; i += blockDim.x) {
         int f = i / a.zlen, kz = i - f * a.zlen;
         corrz[i] = a.corr[f][kz];
@@ -329,11 +362,19 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
         for (int f = 0; f < NFL; ++f) rowc[r * NFL + f] = a.corr[f][kxx] * a.corr[f][kyy];
     }
     __syncthreads();
+    const int Ep = sE[0], Eph = sE[1];
 
     const int mm = a.m * a.m;
-    double p1[S];                                      // 1D accumulator (axes 0/1: k_par = key)
+    // 1D accumulator (axes 0/1: k_par = key): each run's sum in fixed point, added
+    // up per lane in integers, so the result does not depend on which lane took which
+    // run (nor on how the rows were cut into work items)
+    unsigned long long p1lo[S];
+    long long p1hi[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) p1[s] = 0.0;
+    for (int s = 0; s < S; ++s) {
+        p1lo[s] = 0ull;
+        p1hi[s] = 0;
+    }
 
     // Lane runs start at flat index tid*kRun and advance by blockDim*kRun over the
     // item's rows x zpad (zpad is a multiple of kRun, so a run never crosses a row).
@@ -365,11 +406,14 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
                 v[f][s] = (kz0 + s < a.zlen) ? a.dk[f][rowoff + kz0 + s] : make_float2(0.f, 0.f);
 
         int cur3 = -1, cur2 = -1;
-        double r3[NA3], r2[S];
+        double r3[NA3], r2[S], p1[S];
 #pragma unroll
         for (int i = 0; i < NA3; ++i) r3[i] = 0.0;
 #pragma unroll
-        for (int s = 0; s < S; ++s) r2[s] = 0.0;
+        for (int s = 0; s < S; ++s) {
+            r2[s] = 0.0;
+            p1[s] = 0.0;
+        }
 
         // |k|^2 and k_per^2 of the run's first mode; along the run kz grows by one,
         // so each square grows by 2kz-1 <= 2*isqrt+1 and each integer root by at
@@ -423,7 +467,8 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
             if (kidx != cur3) {
                 if (cur3 >= 0) {
 #pragma unroll
-                    for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[i * K1 + cur3], r3[i]);
+                    for (int i = 0; i < NA3; ++i)
+                        fix_add(alo, ahi, i * K1 + cur3, r3[i], (PHASE && i == NA3 - 1) ? Eph : Ep);
                 }
                 cur3 = kidx;
 #pragma unroll
@@ -453,12 +498,12 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
             const int li = (AXIS == 2) ? kz : kper;
             if (AXIS == 2) {
 #pragma unroll
-                for (int sp = 0; sp < S; ++sp) atomicAdd(&line2[li * S + sp], d2[sp]);
+                for (int sp = 0; sp < S; ++sp) fix_add(alo, ahi, a.off2 + li * S + sp, d2[sp], Ep);
             } else {
                 if (li != cur2) {
                     if (cur2 >= 0) {
 #pragma unroll
-                        for (int sp = 0; sp < S; ++sp) atomicAdd(&line2[cur2 * S + sp], r2[sp]);
+                        for (int sp = 0; sp < S; ++sp) fix_add(alo, ahi, a.off2 + cur2 * S + sp, r2[sp], Ep);
                     }
                     cur2 = li;
 #pragma unroll
@@ -471,7 +516,7 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
             if (k2 <= mm) {
                 if (AXIS == 2) {
 #pragma unroll
-                    for (int sp = 0; sp < S; ++sp) atomicAdd(&line1[kz * S + sp], d2[sp]);
+                    for (int sp = 0; sp < S; ++sp) fix_add(alo, ahi, a.off1 + kz * S + sp, d2[sp], Ep);
                 } else {
 #pragma unroll
                     for (int sp = 0; sp < S; ++sp) p1[sp] += d2[sp];
@@ -480,66 +525,104 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
         }
         if (cur3 >= 0) {
 #pragma unroll
-            for (int i = 0; i < NA3; ++i) atomicAdd(&acc3[i * K1 + cur3], r3[i]);
+            for (int i = 0; i < NA3; ++i)
+                fix_add(alo, ahi, i * K1 + cur3, r3[i], (PHASE && i == NA3 - 1) ? Eph : Ep);
         }
         if (AXIS != 2 && cur2 >= 0) {
 #pragma unroll
-            for (int sp = 0; sp < S; ++sp) atomicAdd(&line2[cur2 * S + sp], r2[sp]);
+            for (int sp = 0; sp < S; ++sp) fix_add(alo, ahi, a.off2 + cur2 * S + sp, r2[sp], Ep);
+        }
+        if (AXIS != 2) {
+#pragma unroll
+            for (int sp = 0; sp < S; ++sp) {
+                if (p1[sp] == 0.0) continue;
+                unsigned long long lo;
+                int hi;
+                fix_split(p1[sp], Ep, lo, hi);
+                fix_acc(p1lo[sp], p1hi[sp], lo, hi);
+            }
         }
     }
     if (AXIS != 2) {
 #pragma unroll
         for (int sp = 0; sp < S; ++sp) {
-            double x = p1[sp];
-            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-            if ((tid & 63) == 0) atomicAdd(&line1[sp], x);
+            // exact integer sum over the wave, then one addition per wave
+            unsigned long long lo = p1lo[sp];
+            long long hi = p1hi[sp];
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long olo = (unsigned long long)__shfl_xor((long long)lo, off);
+                const long long ohi = __shfl_xor(hi, off);
+                fix_acc(lo, hi, olo, ohi);
+            }
+            if ((tid & 63) == 0) fix_add_parts(alo, ahi, a.off1 + sp, lo, (int)hi);
         }
     }
     __syncthreads();
-    double *out = a.partials + ((long long)blockIdx.y * a.nitems + blockIdx.x) * a.P;
-    for (int i = tid; i < n3; i += blockDim.x) out[i] = acc3[(i % NA3) * K1 + i / NA3];
-    for (int i = tid; i < a.L2 * S; i += blockDim.x) out[a.off2 + i] = line2[i];
-    for (int i = tid; i < a.L1 * S; i += blockDim.x) out[a.off1 + i] = line1[i];
+    const long long ob = ((long long)blockIdx.y * a.nitems + blockIdx.x) * a.P;
+    unsigned long long *olo = a.plo + ob;
+    int *ohi = a.phi + ob;
+    for (int i = tid; i < n3; i += blockDim.x) {
+        const int j = (i % NA3) * K1 + i / NA3;
+        olo[i] = alo[j];
+        ohi[i] = ahi[j];
+    }
+    for (int i = n3 + tid; i < a.P; i += blockDim.x) {
+        olo[i] = alo[i];
+        ohi[i] = ahi[i];
+    }
 }
 
-// Sum item partials in item order (deterministic).  blockIdx.y = realisation.
-// Output layout per realisation: out3[(kmax+1)*NA3], out2[nb2*S] (reference 2D
-// index), out1[(m+1)*S].
+// Sum the item partials (exact integer sums, so the order does not matter) and
+// convert each bin to double with its realisation's scale.  blockIdx.y =
+// realisation.  Output layout per realisation: out3[(kmax+1)*NA3], out2[nb2*S]
+// (reference 2D index), out1[(m+1)*S].
 struct ReduceArgs {
-    const double *partials;
+    const unsigned long long *plo;
+    const int *phi;
     const int *key_items;
+    const int *escale;
     double *out3, *out2, *out1;
     int nitems, nkeys, n3, L2, L1, S, P, off2, off1, axis, kmax_par;
     long long nb2, n1;       // out2 / out1 elements per realisation (nb2*S, (m+1)*S)
 };
 
 // Columns summed over every item (3D bins; the 1D line for axis 2) are reduced
-// in two fixed stages: chunks of kRedChunk items, then the chunk sums in order.
+// in two stages: chunks of kRedChunk items, then the chunk sums.
 constexpr int kRedChunk = 64;
 
-// tmp[r][ch][c] = sum over items of chunk ch of partials[r][item][col0 + c]
-__global__ void pk_reduce_chunks_kernel(const double *__restrict__ partials, int nitems, int P,
-                                        int col0, int ncol, double *__restrict__ tmp) {
+// (tlo, thi)[r][ch][c] = sum over items of chunk ch of the partials' column col0 + c
+__global__ void pk_reduce_chunks_kernel(const unsigned long long *__restrict__ plo,
+                                        const int *__restrict__ phi, int nitems, int P,
+                                        int col0, int ncol, unsigned long long *__restrict__ tlo,
+                                        long long *__restrict__ thi) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncol) return;
     const int ch = blockIdx.y, nch = gridDim.y;
-    const double *part = partials + (long long)blockIdx.z * nitems * P + col0 + c;
+    const long long base = (long long)blockIdx.z * nitems * P + col0 + c;
     const int i0 = ch * kRedChunk, i1 = min(nitems, i0 + kRedChunk);
-    double s = 0.0;
+    unsigned long long slo = 0;
+    long long shi = 0;
 #pragma unroll 8
-    for (int it = i0; it < i1; ++it) s += part[(long long)it * P];
-    tmp[((long long)blockIdx.z * nch + ch) * ncol + c] = s;
+    for (int it = i0; it < i1; ++it) fix_acc(slo, shi, plo[base + (long long)it * P], phi[base + (long long)it * P]);
+    const long long o = ((long long)blockIdx.z * nch + ch) * ncol + c;
+    tlo[o] = slo;
+    thi[o] = shi;
 }
 
-// out[r*ostride + c] = sum over chunks (in order) of tmp[r][ch][c]
-__global__ void pk_reduce_final_kernel(const double *__restrict__ tmp, int nch, int ncol,
-                                       double *__restrict__ out, long long ostride) {
+// out[r*ostride + c] = sum over chunks of (tlo, thi)[r][ch][c], as a double; with
+// na3 > 0, column c of the 3D block is a phase sum when c % na3 == na3 - 1
+__global__ void pk_reduce_final_kernel(const unsigned long long *__restrict__ tlo,
+                                       const long long *__restrict__ thi, int nch, int ncol,
+                                       double *__restrict__ out, long long ostride,
+                                       const int *__restrict__ escale, int na3) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncol) return;
-    const double *t = tmp + (long long)blockIdx.y * nch * ncol + c;
-    double s = 0.0;
-    for (int ch = 0; ch < nch; ++ch) s += t[(long long)ch * ncol];
-    out[(long long)blockIdx.y * ostride + c] = s;
+    const long long t0 = (long long)blockIdx.y * nch * ncol + c;
+    unsigned long long slo = 0;
+    long long shi = 0;
+    for (int ch = 0; ch < nch; ++ch) fix_acc(slo, shi, tlo[t0 + (long long)ch * ncol], thi[t0 + (long long)ch * ncol]);
+    const int E = escale[2 * blockIdx.y + ((na3 > 0 && c % na3 == na3 - 1) ? 1 : 0)];
+    out[(long long)blockIdx.y * ostride + c] = fix_value(slo, shi, E);
 }
 
 // One thread per (key, line index, spectrum): sums the key's items.
@@ -547,17 +630,20 @@ __global__ void pk_reduce2_kernel(ReduceArgs a) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long n = (long long)a.nkeys * a.L2 * a.S;
     if (i >= n) return;
-    const double *part = a.partials + (long long)blockIdx.y * a.nitems * a.P;
+    const long long base = (long long)blockIdx.y * a.nitems * a.P;
     int sp = (int)(i % a.S);
     int li = (int)((i / a.S) % a.L2);
     int key = (int)(i / ((long long)a.S * a.L2));
-    double s = 0.0;
-    for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
-        s += part[(long long)it * a.P + a.off2 + li * a.S + sp];
+    unsigned long long slo = 0;
+    long long shi = 0;
+    for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it) {
+        const long long j = base + (long long)it * a.P + a.off2 + li * a.S + sp;
+        fix_acc(slo, shi, a.plo[j], a.phi[j]);
+    }
     // reference 2D index: (kmax_par+1)*k_per + k_par
     long long idx2 = (a.axis == 2) ? (long long)(a.kmax_par + 1) * key + li
                                    : (long long)(a.kmax_par + 1) * li + key;
-    a.out2[(long long)blockIdx.y * a.nb2 + idx2 * a.S + sp] = s;
+    a.out2[(long long)blockIdx.y * a.nb2 + idx2 * a.S + sp] = fix_value(slo, shi, a.escale[2 * blockIdx.y]);
 }
 
 // Axes 0/1: 1D bin = key = k_par; sums the key's items.  (Axis 2 keeps one 1D line
@@ -565,13 +651,16 @@ __global__ void pk_reduce2_kernel(ReduceArgs a) {
 __global__ void pk_reduce1_kernel(ReduceArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nkeys * a.S) return;
-    const double *part = a.partials + (long long)blockIdx.y * a.nitems * a.P;
+    const long long base = (long long)blockIdx.y * a.nitems * a.P;
     double *out1 = a.out1 + (long long)blockIdx.y * a.n1;
     int key = i / a.S, sp = i % a.S;
-    double s = 0.0;
-    for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it)
-        s += part[(long long)it * a.P + a.off1 + sp];
-    out1[key * a.S + sp] = s;
+    unsigned long long slo = 0;
+    long long shi = 0;
+    for (int it = a.key_items[key]; it < a.key_items[key + 1]; ++it) {
+        const long long j = base + (long long)it * a.P + a.off1 + sp;
+        fix_acc(slo, shi, a.plo[j], a.phi[j]);
+    }
+    out1[key * a.S + sp] = fix_value(slo, shi, a.escale[2 * blockIdx.y]);
 }
 
 // FFT prologue: out = (in / scale) - shift in float32 (the reference caller's
@@ -673,7 +762,8 @@ void bin_host_part(const PkDims &p, const double *hp, size_t out3, size_t out2, 
 template <int AXIS, int NFL, bool PHASE>
 int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
                long long fstride, long long zs, int nreal, const double *const *corr,
-               const AccsPtr &accs, const int *spec_out) {
+               const float *cmax, const unsigned *const *amax, const AccsPtr &accs,
+               const int *spec_out) {
     constexpr int S = Spec<NFL>::S;
     constexpr int NA3 = 3 * S + (PHASE ? 1 : 0);
     const int L2 = (AXIS == 2) ? p.zlen : p.kmax_per + 1;
@@ -682,30 +772,44 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     const int P = n3 + (L2 + L1) * S;
     const int off2 = n3, off1 = n3 + L2 * S;
     const int rmax = bp->R;
-    size_t lds = sizeof(double) * ((size_t)P + (size_t)p.zlen * NFL + (size_t)rmax * NFL) +
+    // LDS: P low words (8 B) and P high words (4 B, padded to an even count), the
+    // kz corrections, the row corrections and the row table
+    size_t lds = sizeof(unsigned long long) * (size_t)P + sizeof(int) * (size_t)((P + 1) & ~1) +
+                 sizeof(double) * ((size_t)p.zlen * NFL + (size_t)rmax * NFL) +
                  sizeof(int) * 3 * (size_t)rmax;
     PYL_CHECK(lds <= 160 * 1024, PYL_EINVAL, "grid too large for the binning kernel (%zu B LDS)", lds);
-    // scratch: partials + reduced outputs, per realisation
+    // scratch: partials (low words | high words), reduced outputs, chunk sums, scales
     const size_t part_elems = (size_t)bp->nitems * P * nreal;
     const size_t out3 = (size_t)n3, out2 = (size_t)p.nb2 * S, out1 = (size_t)(p.m + 1) * S;
     const int nch = ceil_div(bp->nitems, kRedChunk);
     const int ncol_all = (AXIS == 2) ? std::max(n3, L1 * S) : n3;
     const size_t tmp_elems = (size_t)nch * ncol_all * nreal;
-    PYL_TRY(ensure(c, BUF_BINS, sizeof(double) * (part_elems + (out3 + out2 + out1) * nreal +
-                                                 tmp_elems)));
-    double *partials = (double *)c->bufs[BUF_BINS].ptr;
-    double *d3 = partials + part_elems;
+    const size_t phi_bytes = (sizeof(int) * part_elems + 7) & ~(size_t)7;
+    PYL_TRY(ensure(c, BUF_BINS, sizeof(unsigned long long) * part_elems + phi_bytes +
+                                    sizeof(double) * (out3 + out2 + out1) * nreal +
+                                    2 * sizeof(long long) * tmp_elems + sizeof(int) * 2 * (size_t)nreal));
+    unsigned long long *plo = (unsigned long long *)c->bufs[BUF_BINS].ptr;
+    int *phi = (int *)(plo + part_elems);
+    double *d3 = (double *)((char *)phi + phi_bytes);
     double *d2 = d3 + out3 * nreal;
     double *d1 = d2 + out2 * nreal;
+    unsigned long long *tlo = (unsigned long long *)(d1 + out1 * nreal);
+    long long *thi = (long long *)(tlo + tmp_elems);
+    int *escale = (int *)(thi + tmp_elems);
 
     BinArgs a{};
     for (int f = 0; f < NFL; ++f) {
         a.dk[f] = dk[f];
         a.corr[f] = corr[f];
+        a.cmax[f] = cmax[f];
+        a.amax[f] = amax[f];
     }
+    a.escale = escale;
+    a.nreal = nreal;
     a.rows = bp->d_rows;
     a.items = bp->d_items;
-    a.partials = partials;
+    a.plo = plo;
+    a.phi = phi;
     a.fstride = fstride;
     a.zs = zs;
     a.nitems = bp->nitems;
@@ -717,23 +821,23 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     kern<<<dim3(bp->nitems, nreal), kBinBlock, lds, c->stream>>>(a);
     PYL_HIP(hipGetLastError());
     ReduceArgs r{};
-    r.partials = partials; r.key_items = bp->d_key_items; r.out3 = d3; r.out2 = d2; r.out1 = d1;
+    r.plo = plo; r.phi = phi; r.escale = escale;
+    r.key_items = bp->d_key_items; r.out3 = d3; r.out2 = d2; r.out1 = d1;
     r.nitems = bp->nitems; r.nkeys = bp->nkeys; r.n3 = n3; r.L2 = L2; r.L1 = L1; r.S = S;
     r.P = P; r.off2 = off2; r.off1 = off1; r.axis = AXIS; r.kmax_par = p.kmax_par;
     r.nb2 = (long long)out2; r.n1 = (long long)out1;
-    double *tmp = d1 + out1 * nreal;
     // 3D bins: every item contributes
     pk_reduce_chunks_kernel<<<dim3(ceil_div(n3, 256), nch, nreal), 256, 0, c->stream>>>(
-        partials, bp->nitems, P, 0, n3, tmp);
+        plo, phi, bp->nitems, P, 0, n3, tlo, thi);
     pk_reduce_final_kernel<<<dim3(ceil_div(n3, 256), nreal), 256, 0, c->stream>>>(
-        tmp, nch, n3, d3, (long long)out3);
+        tlo, thi, nch, n3, d3, (long long)out3, escale, PHASE ? NA3 : 0);
     long long n2 = (long long)bp->nkeys * L2 * S;
     pk_reduce2_kernel<<<dim3((unsigned)((n2 + 255) / 256), nreal), 256, 0, c->stream>>>(r);
     if (AXIS == 2) {   // 1D bins are k_par = kz: every item contributes
         pk_reduce_chunks_kernel<<<dim3(ceil_div(L1 * S, 256), nch, nreal), 256, 0, c->stream>>>(
-            partials, bp->nitems, P, off1, L1 * S, tmp);
+            plo, phi, bp->nitems, P, off1, L1 * S, tlo, thi);
         pk_reduce_final_kernel<<<dim3(ceil_div(L1 * S, 256), nreal), 256, 0, c->stream>>>(
-            tmp, nch, L1 * S, d1, (long long)out1);
+            tlo, thi, nch, L1 * S, d1, (long long)out1, escale, 0);
     } else {
         pk_reduce1_kernel<<<dim3(ceil_div(bp->nkeys * S, 256), nreal), 256, 0, c->stream>>>(r);
     }
@@ -769,10 +873,13 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
 template <int NFL, bool PHASE>
 int launch_bin_axis(Ctx *c, int axis, const BinPlan *bp, const PkDims &p, const float2 *const *dk,
                     long long fstride, long long zs, int nreal, const double *const *corr,
-                    const AccsPtr &accs, const int *spec_out) {
-    if (axis == 0) return launch_bin<0, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, accs, spec_out);
-    if (axis == 1) return launch_bin<1, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, accs, spec_out);
-    return launch_bin<2, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, accs, spec_out);
+                    const float *cmax, const unsigned *const *amax, const AccsPtr &accs,
+                    const int *spec_out) {
+    if (axis == 0)
+        return launch_bin<0, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, cmax, amax, accs, spec_out);
+    if (axis == 1)
+        return launch_bin<1, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, cmax, amax, accs, spec_out);
+    return launch_bin<2, NFL, PHASE>(c, bp, p, dk, fstride, zs, nreal, corr, cmax, amax, accs, spec_out);
 }
 
 // Normalisation, Pk_library.pyx:384-418 (and :735-791 for XPk).
@@ -854,9 +961,27 @@ void finalize_values(const ModePlan *mp, const PkDims &p, double BoxSize, const 
 
 }  // namespace
 
-int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs = 0) {
+__global__ void zero_u32_kernel(unsigned *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// 64 zeroed words per cube for the transforms' largest |Re|, |Im| (a fill kernel,
+// not a memset node, so that a captured call clears them on every replay).
+int pk_amax_words(Ctx *c, int ncube, unsigned **out) {
+    const int n = 64 * ncube;
+    PYL_TRY(ensure(c, BUF_PK_AMAX, sizeof(unsigned) * (size_t)n));
+    unsigned *a = (unsigned *)c->bufs[BUF_PK_AMAX].ptr;
+    zero_u32_kernel<<<ceil_div(n, 256), 256, 0, c->stream>>>(a, n);
+    PYL_HIP(hipGetLastError());
+    *out = a;
+    return PYL_OK;
+}
+
+int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs = 0,
+               unsigned *amax = nullptr) {
     PYL_TRY(stage_begin(c, PYL_STAGE_PK_FFT));
-    PYL_TRY(fft_r2c_3d(c, in, out, dims, batch, zs));
+    PYL_TRY(fft_r2c_3d(c, in, out, dims, batch, zs, amax));
     PYL_TRY(stage_end(c, PYL_STAGE_PK_FFT));
     return PYL_OK;
 }
@@ -885,16 +1010,28 @@ int get_mas_tab(Ctx *c, int dims, int p, const double **out) {
     return PYL_OK;
 }
 
-// Rows of each field are zs elements apart (zs = 0: dense, dims/2+1).
+// Rows of each field are zs elements apart (zs = 0: dense, dims/2+1).  amax: per
+// field, realisation r's 64 words of the transform's largest |Re|, |Im| at r*64
+// (fft_r2c_3d, complex_absmax).
 int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, long long zs, int nreal, int F,
-                  const int *mas, int dims, double BoxSize, int axis, int flags, pyl_pk_out *outs) {
+                  const int *mas, int dims, double BoxSize, int axis, int flags, pyl_pk_out *outs,
+                  const unsigned *const *amax) {
     if (zs <= 0) zs = dims / 2 + 1;
     const PkDims p = pk_dims(dims);
     const ModePlan *mp = get_mode_plan(c, dims, axis);
     BinPlan *bp = nullptr;
     PYL_TRY(get_bin_plan(c, dims, axis, nreal, &bp));
     std::vector<const double *> tab(F);
-    for (int f = 0; f < F; ++f) PYL_TRY(get_mas_tab(c, dims, mas[f], &tab[f]));
+    std::vector<float> cmax(F);
+    for (int f = 0; f < F; ++f) {
+        PYL_TRY(get_mas_tab(c, dims, mas[f], &tab[f]));
+        // the largest product of three axis corrections, rounded up
+        std::vector<double> h(dims);
+        mas_table(dims, mas[f], h.data());
+        double mx = 1.0;
+        for (double v : h) mx = std::max(mx, v);
+        cmax[f] = (float)(mx * mx * mx * 1.001);
+    }
     // the mode-plan outputs are written while the FFT already queued runs
     for (int r = 0; r < nreal; ++r) PYL_TRY(finalize_modes(mp, p, BoxSize, &outs[r]));
     const int S = F + F * (F - 1) / 2;
@@ -921,9 +1058,13 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, long long 
     if (F == 1) {
         const float2 *f0[1] = {dk[0]};
         const double *c0[1] = {tab[0]};
+        const float m0[1] = {cmax[0]};
+        const unsigned *a0[1] = {amax[0]};
         int so[1] = {0};
-        if (phase) PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, zs, nreal, c0, accs_p, so)));
-        else PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, zs, nreal, c0, accs_p, so)));
+        if (phase)
+            PYL_TRY((launch_bin_axis<1, true>(c, axis, bp, p, f0, fstride, zs, nreal, c0, m0, a0, accs_p, so)));
+        else
+            PYL_TRY((launch_bin_axis<1, false>(c, axis, bp, p, f0, fstride, zs, nreal, c0, m0, a0, accs_p, so)));
     } else {
         // pairs i<j in lexicographic order; each launch yields auto i, auto j, cross ij
         std::vector<bool> have(F, false);
@@ -932,9 +1073,12 @@ int pk_bin_device(Ctx *c, const float2 *const *dk, long long fstride, long long 
             for (int j = i + 1; j < F; ++j, ++xi) {
                 const float2 *fp[2] = {dk[i], dk[j]};
                 const double *cp[2] = {tab[i], tab[j]};
+                const float mp2[2] = {cmax[i], cmax[j]};
+                const unsigned *ap[2] = {amax[i], amax[j]};
                 int so[3] = {have[i] ? -1 : i, have[j] ? -1 : j, F + xi};
                 have[i] = have[j] = true;
-                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, zs, nreal, cp, accs_p, so)));
+                PYL_TRY((launch_bin_axis<2, false>(c, axis, bp, p, fp, fstride, zs, nreal, cp, mp2, ap,
+                                                   accs_p, so)));
             }
     }
     if (c->capturing) {
@@ -1037,6 +1181,10 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
     const bool destroy = dev && (flags & PYL_PK_DESTROY_INPUT);
     PYL_TRY(ensure(c, BUF_FFT_OUT, (size_t)nk * nfields * sizeof(float2)));
     float2 *dk = (float2 *)c->bufs[BUF_FFT_OUT].ptr;
+    unsigned *amax = nullptr;
+    PYL_TRY(pk_amax_words(c, nfields, &amax));
+    std::vector<const unsigned *> amf(nfields);
+    for (int f = 0; f < nfields; ++f) amf[f] = amax + 64 * f;
     std::vector<const float2 *> dks(nfields);
     const int pb = 256;
     const int nb = (int)std::min<long long>((n + pb - 1) / pb, 8192);
@@ -1049,7 +1197,7 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
                 PYL_HIP(hipGetLastError());
                 PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
             }
-            PYL_TRY(fft_device(c, x, dk + (size_t)f * nk, dims, 1, zs));
+            PYL_TRY(fft_device(c, x, dk + (size_t)f * nk, dims, 1, zs, amax + 64 * f));
             dks[f] = dk + (size_t)f * nk;
         }
     } else {
@@ -1077,10 +1225,11 @@ int pyl_pk(pyl_ctx *ctx, const float *const *deltas, int nfields, const int *mas
             }
         }
         PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
-        PYL_TRY(fft_device(c, in, dk, dims, nfields, zs));
+        PYL_TRY(fft_device(c, in, dk, dims, nfields, zs, amax));
         for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nk;
     }
-    PYL_TRY(pk_bin_device(c, dks.data(), 0, zs, 1, nfields, mas, dims, BoxSize, axis, flags, out));
+    PYL_TRY(pk_bin_device(c, dks.data(), 0, zs, 1, nfields, mas, dims, BoxSize, axis, flags, out,
+                          amf.data()));
     return finish(c, flags);
 }
 
@@ -1108,6 +1257,10 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfield
     const bool apply = scales != nullptr || shifts != nullptr;
     PYL_TRY(ensure(c, BUF_FFT_OUT, (size_t)nk * ncube * sizeof(float2)));
     float2 *dk = (float2 *)c->bufs[BUF_FFT_OUT].ptr;
+    unsigned *amax = nullptr;                          // cube (f, r) at (f*nreal + r)*64
+    PYL_TRY(pk_amax_words(c, ncube, &amax));
+    std::vector<const unsigned *> amf(nfields);
+    for (int f = 0; f < nfields; ++f) amf[f] = amax + (size_t)64 * nreal * f;
     float *in = nullptr;
     if (!destroy) {
         PYL_TRY(ensure(c, BUF_FFT_IN, (size_t)n * ncube * sizeof(float)));
@@ -1158,13 +1311,15 @@ int pyl_pk_batch(pyl_ctx *ctx, const float *const *deltas, int nreal, int nfield
     PYL_TRY(stage_end(c, PYL_STAGE_PK_PREP));
     if (destroy) {
         for (int f = 0; f < nfields; ++f)
-            PYL_TRY(fft_device(c, (float *)deltas[f], dk + (size_t)f * nreal * nk, dims, nreal, zs));
+            PYL_TRY(fft_device(c, (float *)deltas[f], dk + (size_t)f * nreal * nk, dims, nreal, zs,
+                               amax + (size_t)64 * nreal * f));
     } else {
-        PYL_TRY(fft_device(c, in, dk, dims, ncube, zs));
+        PYL_TRY(fft_device(c, in, dk, dims, ncube, zs, amax));
     }
     std::vector<const float2 *> dks(nfields);
     for (int f = 0; f < nfields; ++f) dks[f] = dk + (size_t)f * nreal * nk;
-    PYL_TRY(pk_bin_device(c, dks.data(), nk, zs, nreal, nfields, mas, dims, BoxSize, axis, flags, outs));
+    PYL_TRY(pk_bin_device(c, dks.data(), nk, zs, nreal, nfields, mas, dims, BoxSize, axis, flags, outs,
+                          amf.data()));
     return finish(c, flags);
 }
 
@@ -1190,7 +1345,15 @@ int pyl_pk_from_fourier(pyl_ctx *ctx, const float *const *delta_k, int nfields, 
             dks[f] = dk + (size_t)f * nk;
         }
     }
-    PYL_TRY(pk_bin_device(c, dks.data(), 0, 0, 1, nfields, mas, dims, BoxSize, axis, flags, out));
+    unsigned *amax = nullptr;
+    PYL_TRY(pk_amax_words(c, nfields, &amax));
+    std::vector<const unsigned *> amf(nfields);
+    for (int f = 0; f < nfields; ++f) {
+        PYL_TRY(complex_absmax(c, dks[f], dims, 0, 1, nk, amax + 64 * f));
+        amf[f] = amax + 64 * f;
+    }
+    PYL_TRY(pk_bin_device(c, dks.data(), 0, 0, 1, nfields, mas, dims, BoxSize, axis, flags, out,
+                          amf.data()));
     return finish(c, flags);
 }
 

@@ -116,6 +116,7 @@ enum BufId {
     BUF_PARTIAL,     // per-tile halo partial sums of the tiled deposit
     BUF_TILE_BOX,    // per-tile int64 boxes of a split deposit
     BUF_BUCKETED,    // records of the two-level sort's first pass
+    BUF_PK_AMAX,     // per-cube largest |Re|, |Im| of the transforms being binned
     BUF_COUNT_
 };
 
@@ -216,7 +217,12 @@ int get_fft_plan(Ctx *c, int dims, int batch, FFTPlan **out, int kind = FFT_R2C_
 // Forward r2c of `batch` contiguous cubes (own passes for power-of-two sizes).
 // Rows of N/2+1 modes zs apart (zs = 0: dense); fft_zstride(dims): the padded
 // stride of the library's own half-complex buffers.
-int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs = 0);
+int fft_r2c_3d(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs = 0,
+               unsigned *amax = nullptr);
+// Largest |Re| or |Im| of ncube half-complex cubes (rows zs apart, cube stride
+// cstride) as float bits, maxed into 64 words per cube (zeroed by the caller).
+int complex_absmax(Ctx *c, const float2 *dk, int dims, long long zs, int ncube, long long cstride,
+                   unsigned *amax);
 long long fft_zstride(int dims);
 bool fft_own_size(int dims);
 int run_fft(Ctx *c, FFTPlan *p, void *in, void *out);
