@@ -684,6 +684,20 @@ def main():
     ms = time_steps(pipe, args.steps, args.warmup, stages=[dom])
     dom_ms = pipe.stage_avg()[dom]
     log('step %.3f ms  %s %.3f ms' % (ms, dom, dom_ms))
+    # the whole step (MA, FFT, binning) is the same bits on every run: hash every
+    # output of the last timed step and of three more steps
+    import hashlib
+
+    def _outputs_hash():
+        return hashlib.sha1(b''.join(np.ascontiguousarray(v).tobytes()
+                                     for _, v in sorted(pipe.arrs.items()))).hexdigest()
+    hashes = [_outputs_hash()]
+    for _ in range(3):
+        pipe.step()
+        torch.cuda.synchronize()
+        hashes.append(_outputs_hash())
+    reproducible = {'steps_compared': len(hashes), 'bitwise_identical': len(set(hashes)) == 1,
+                    'outputs_sha1': hashes[0][:16]}
     grid_host = None
     if not args.no_cpu:
         # one more MA to keep the density grid for the CPU leg
@@ -801,6 +815,7 @@ def main():
         'ma_mparticles_per_s': round(n / ma_ms / 1e3, 2) if ma_ms > 0 else None,
         'grid_cells_per_s': round(dims ** 3 / (ms * 1e-3), 1),
         'errors_vs_fp64': errors,
+        'step_reproducible': reproducible,
         'precision_1024': precision,
         'kernels_ms': kernels,                 # from the instrumented steps (events on every stage)
         'instrumented_ms_per_step': round(ms_all, 4),
