@@ -264,8 +264,8 @@ __host__ __device__ __forceinline__ int home_tile(float dist, const TileGeom &g)
 
 // Optional redshift-space shift applied to one coordinate as the position is
 // read (redshift_space_library.pyx:29-46 fused into the MA load; pos itself is
-// not modified).  When off, vel points at pos with vstride = axes and the shift
-// is skipped, so the extra load stays in bounds and hits the same cache lines.
+// not modified).  When off, vel points at pos with vstride = axes (a valid
+// address) and neither the velocity load nor the shift is made.
 struct RSD {
     const float *vel;
     float factor;      // (1 + z) / H, rounded to float as in the reference
@@ -280,8 +280,8 @@ __device__ __forceinline__ float3 load_dist(const float *__restrict__ pos, const
                                             long long i, float inv) {
     const float *p = pos + (long long)AXES * i;
     float x = p[0], y = p[1], z = (AXES == 3) ? p[2] : 0.0f;
-    const float v = rsd.vel[(long long)rsd.vstride * i + rsd.axis];
-    if (rsd.on) {
+    if (rsd.on) {                                // uniform across the launch
+        const float v = rsd.vel[(long long)rsd.vstride * i + rsd.axis];
         if (rsd.axis == 0) x = rsd_shift(x, v, rsd.factor, rsd.box);
         else if (rsd.axis == 1) y = rsd_shift(y, v, rsd.factor, rsd.box);
         else z = rsd_shift(z, v, rsd.factor, rsd.box);
