@@ -2050,6 +2050,13 @@ TileGeom make_geom(int dims, int axes, int mas) {
     return g;
 }
 
+// The latest order finding per (device, dims, MAS) over every context of the
+// process: a context that has not sorted such a grid yet starts from it (requests
+// from one source -- a server's slots, a pipeline's threads -- tend to share their
+// particle order).
+std::mutex g_hint_mu;
+std::map<std::tuple<int, int, int>, bool> g_shuffled_hint;
+
 template <int MAS>
 int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long long *h_offs,
               int nreal, int axes, const float *W, float *grid, int dims, float inv,
@@ -2088,7 +2095,13 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     if (c->box_miss_n > 0 && !c->capturing) {
         const hipError_t q = hipEventQuery(c->ev_total);
         if (q == hipSuccess) {
-            c->shuffled_hint[c->box_miss_key] = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
+            const bool shuffled = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
+            c->shuffled_hint[c->box_miss_key] = shuffled;
+            {
+                std::lock_guard<std::mutex> lk(g_hint_mu);
+                g_shuffled_hint[std::make_tuple(c->device, c->box_miss_key.first, c->box_miss_key.second)] =
+                    shuffled;
+            }
             c->box_miss_n = -1;
         } else if (q == hipErrorNotReady) {
             (void)hipGetLastError();          // pending, not an error: keep it out of later checks
@@ -2097,10 +2110,22 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         }
     }
     const std::pair<int, int> hkey(dims, MAS);
+    // the context's own finding for this (dims, MAS), else the process's latest
+    bool shuffled = false;
+    if (tally) {
+        const auto it = c->shuffled_hint.find(hkey);
+        if (it != c->shuffled_hint.end()) {
+            shuffled = it->second;
+        } else {
+            std::lock_guard<std::mutex> lk(g_hint_mu);
+            const auto gt = g_shuffled_hint.find(std::make_tuple(c->device, dims, MAS));
+            shuffled = gt != g_shuffled_hint.end() && gt->second;
+        }
+    }
     bool wide = false, two = false;
     if (c->tune_window == 1) wide = wide_ok;
     else if (c->tune_window == 2) two = two_ok;
-    else if (c->tune_window < 0 && tally && c->shuffled_hint[hkey]) {
+    else if (c->tune_window < 0 && tally && shuffled) {
         two = two_ok;
         wide = !two && wide_ok;          // batches of host ranges, fixed scratch
     }

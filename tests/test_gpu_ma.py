@@ -249,7 +249,8 @@ def test_ma_wide_window_any_order(oracle, mas, dims):
     # the form taken is read back; every cell matches the oracle, with and without
     # weights, in random order and sorted by cell, and all forms give the same bits.
     # Then the adaptive choice: random order moves the next call to the two-level
-    # sort, and tile-ordered input back to one pass.
+    # sort, tile-ordered input back to one pass, and a fresh context starts from the
+    # process's latest finding for this grid and scheme.
     import ctypes
     from pylians3_amd import MAS_library as MASL, _native as N
     print('case wide window', mas, dims, flush=True)
@@ -288,8 +289,10 @@ def test_ma_wide_window_any_order(oracle, mas, dims):
                 ctx.close()
         firsts.append(first)
     first = firsts[0]                        # unweighted
-    # adaptive: a random-order call tallies over a quarter of its particles outside
-    # the box, so the following call on the context takes the two-level sort
+    # adaptive: a call tallies the particles outside their chunk's box; over a quarter
+    # moves the next call with this (dims, MAS) to the two-level sort.  A context
+    # that has not sorted such a grid yet follows the process's latest finding: the
+    # calls above found random order, so a fresh context starts with two-level
     ctx = N.Context(0)
     try:
         got = ctypes.c_int()
@@ -298,11 +301,8 @@ def test_ma_wide_window_any_order(oracle, mas, dims):
             N.check(L.pyl_ma(ctx.handle, pos.ctypes.data, n, 3, None, g.ctypes.data, dims,
                              ctypes.c_float(box), masid, 0))
             N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
-            if k == 0:
-                assert got.value == N.WINDOW_BOX
-            else:
-                assert np.array_equal(g, first)
-        assert got.value == N.WINDOW_TWOLEVEL
+            assert got.value == N.WINDOW_TWOLEVEL, (k, got.value)
+            assert np.array_equal(g, first)
         # input ordered by 16^3 tile on the same context tallies few misses (a sort
         # chunk spans a few neighbouring tile columns): back to one pass
         tl = cell // 16
@@ -316,6 +316,22 @@ def test_ma_wide_window_any_order(oracle, mas, dims):
             assert np.array_equal(g, first)
             N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
         assert got.value == N.WINDOW_BOX
+    finally:
+        ctx.close()
+    # the process's latest finding is now ordered input: a fresh context starts with
+    # one pass, and random input moves it to two-level from its second call
+    ctx = N.Context(0)
+    try:
+        got = ctypes.c_int()
+        for k in range(3):
+            g = np.zeros((dims,) * 3, np.float32)
+            N.check(L.pyl_ma(ctx.handle, pos.ctypes.data, n, 3, None, g.ctypes.data, dims,
+                             ctypes.c_float(box), masid, 0))
+            N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+            if k == 0:
+                assert got.value == N.WINDOW_BOX
+            assert np.array_equal(g, first)
+        assert got.value == N.WINDOW_TWOLEVEL
     finally:
         ctx.close()
 
