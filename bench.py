@@ -722,7 +722,7 @@ def main():
         random_order = random_order_leg(N, pos, dims, box, mas)
 
     ma_ms = sum(st.get(k, 0.0) for k in ('ma_bucket', 'ma_count', 'ma_scan', 'ma_scatter',
-                                          'ma_deposit', 'ma_final'))
+                                          'ma_spill', 'ma_deposit', 'ma_final'))
     pk_ms = sum(st.get(k, 0.0) for k in ('pk_prep', 'pk_fft', 'pk_bin'))
     nk = dims * dims * (dims // 2 + 1)
     B_ma = 12.0 * n + 4.0 * dims ** 3            # positions once + grid written once

@@ -95,7 +95,8 @@ int pyl_ctx_memory(pyl_ctx *ctx, size_t *bytes_out);
 #define PYL_STAGE_PK_BIN     7
 #define PYL_STAGE_MA_FINAL   8   /* halo finalize of the tiled MA */
 #define PYL_STAGE_MA_BUCKET  9   /* first pass of the two-level sort (particles into buckets) */
-#define PYL_STAGE_COUNT      10
+#define PYL_STAGE_MA_SPILL   10  /* capacity sort: ordering of the particles its tile slots missed */
+#define PYL_STAGE_COUNT      11
 int pyl_ctx_enable_timing(pyl_ctx *ctx, int enable);   /* all stages; also clears the totals */
 /* Times only the stages whose bit (1u << PYL_STAGE_*) is set: each timed stage adds
  * two HIP events to the stream, so a caller timing one kernel records only its own. */
@@ -314,11 +315,19 @@ int pyl_graph_destroy(pyl_graph *graph);
                                       default window; 1 the wide window (every tile in LDS;
                                       3D grids of 16385..32768 tiles); 2 the two-level sort
                                       (buckets of 8x8x4 tiles, then tiles; one 3D grid of
-                                      16 to 96 tiles of 16 nodes per axis, 256^3 to 1536^3); -1 the adaptive choice: per
+                                      16 to 96 tiles of 16 nodes per axis, 256^3 to 1536^3);
+                                      3 the capacity sort (one read of the particles into
+                                      per-tile slots of 3x the mean occupancy, the rest
+                                      sorted after; one 3D grid whose sort window is every
+                                      tile or the 8x8-column box); -1 the adaptive choice: per
                                       (dims, MAS), when the previous call found more than a
                                       quarter of its particles outside their sort chunk's
-                                      8x8-column box, the two-level sort, else the default.
-                                      A form that does not apply falls back to the default. */
+                                      8x8-column box, the two-level sort; else the capacity
+                                      sort when the tiles hold 256 particles or more on
+                                      average and no capacity call of the context on such a
+                                      grid spilled more than 1/64 of its particles; else the
+                                      default.  A form that does not apply falls back to the
+                                      default. */
 int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value);
 /* Sort window of the context's latest tiled MA (-1 none yet). */
 #define PYL_WINDOW_DIRECT 0        /* every tile of the realisation counted in LDS */
@@ -326,6 +335,7 @@ int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value);
 #define PYL_WINDOW_LINEAR 2        /* 16384 consecutive keys around the chunk */
 #define PYL_WINDOW_WIDE   3        /* every tile (up to 32768) in 128 KB of LDS */
 #define PYL_WINDOW_TWOLEVEL 4      /* buckets of 8x8x4 tiles, then the tiles of each bucket */
+#define PYL_WINDOW_CAPACITY 5      /* per-tile slots filled in one pass, then the spill sorted */
 int pyl_ctx_last_sort_window(pyl_ctx *ctx, int *window);
 
 #ifdef __cplusplus

@@ -560,6 +560,157 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
     }
 }
 
+// ---- capacity sort: one read of the particles ----------------------------------
+// For coherent input and one 3D grid.  Tile t owns `cap` record slots at t*cap in
+// crec (a few times the mean occupancy).  Each sub-chunk of kCapSub particles per
+// thread ranks its particles per tile in LDS (the chunk's window: every tile, or
+// the 8 x 8 tile columns around its middle particle), claims the run of slots it
+// needs in each tile it touches with one global atomic on the tile's cursor, and
+// writes its records there.  Particles outside the window, and those past a full
+// tile's capacity, go to the spill list (dist_x, dist_y, dist_z | W), which the
+// regular count / scan / scatter then order by tile into a second record segment.
+// No count pass: the positions are read once.  cursor[t] ends as the number of
+// particles that claimed a slot of t (min(cursor, cap) were placed).
+constexpr int kCapSub = 8;          // particles per thread per sub-chunk
+
+template <int MAS, bool POW2, bool WT>
+__global__ void __launch_bounds__(kSortBlock)
+cap_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restrict__ W, float inv,
+                   TileGeom g, const Chunk *__restrict__ chunks, unsigned *__restrict__ cursor,
+                   unsigned cap, float *__restrict__ crec, float *__restrict__ sxyz,
+                   float *__restrict__ sw, unsigned long long *__restrict__ nspill,
+                   unsigned long long *__restrict__ placed, unsigned *__restrict__ wmax_bits,
+                   unsigned *__restrict__ box_miss) {
+    extern __shared__ unsigned cap_lds[];                   // hist[slots] | claimed base[slots]
+    __shared__ unsigned red[3 * (kSortBlock / 64)];
+    const Chunk ch = chunks[blockIdx.x];
+    const int t = threadIdx.x, lane = t & 63;
+    int3 t0 = make_int3(0, 0, 0);
+    if (ch.c1 > ch.c0)
+        t0 = dist_tiles<MAS, 3, POW2>(load_dist<3>(pos, rsd, ch.c0 + (ch.c1 - ch.c0) / 2, inv), g);
+    const Win w = make_win(g, t0, tiles_key(t0, g, 0), 0);
+    unsigned *hist = cap_lds, *gbase = cap_lds + w.slots;
+    // misses of the 8 x 8 tile-column box (the host's order finding), as in the count pass
+    TileGeom gb = g;
+    gb.wmode = kWinBox;
+    const Win wb = make_win(gb, t0, 0, 0);
+    unsigned m = 0, miss = 0, nplaced = 0;
+    constexpr int SUB = kSortBlock * kCapSub;
+    for (long long s0 = ch.c0; s0 < ch.c1; s0 += SUB) {
+        const int cnt = (int)min((long long)SUB, ch.c1 - s0);
+        for (int s = t; s < w.slots; s += kSortBlock) hist[s] = 0u;
+        __syncthreads();
+        float3 d[kCapSub];
+        float wt[kCapSub];
+        unsigned key[kCapSub], rk[kCapSub];
+        int sl[kCapSub];                                    // window slot; -1 none, -2 outside
+#pragma unroll
+        for (int k = 0; k < kCapSub; ++k) {
+            const long long i = s0 + min(t + k * kSortBlock, cnt - 1);
+            d[k] = load_dist<3>(pos, rsd, i, inv);
+            wt[k] = WT ? W[i] : 1.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kCapSub; ++k) {
+            sl[k] = -1;
+            key[k] = 0u;
+            rk[k] = 0u;
+            if (t + k * kSortBlock < cnt) {
+                const int3 tt = dist_tiles<MAS, 3, POW2>(d[k], g);
+                key[k] = tiles_key(tt, g, 0);
+                const unsigned s = win_slot(tt, key[k], w, g, 0);
+                if (s < (unsigned)w.slots) {
+                    sl[k] = (int)s;
+                    rk[k] = atomicAdd(&hist[s], 1u);
+                } else {
+                    sl[k] = -2;
+                }
+                if (box_miss) miss += w.mode == kWinBox ? s >= (unsigned)w.slots
+                                                        : win_slot(tt, key[k], wb, gb, 0) >= (unsigned)wb.slots;
+                if (WT) {
+                    const unsigned b = __float_as_uint(fabsf(wt[k]));
+                    if (b < 0x7f800000u) m = max(m, b);
+                }
+            }
+        }
+        __syncthreads();
+        // one claim per tile the sub-chunk touches
+        for (int s = t; s < w.slots; s += kSortBlock) {
+            const unsigned h = hist[s];
+            if (h) gbase[s] = atomicAdd(&cursor[win_key(s, w, g, 0)], h);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kCapSub; ++k) {
+            bool spill = sl[k] == -2;
+            if (sl[k] >= 0) {
+                const unsigned p = gbase[sl[k]] + rk[k];
+                if (p < cap) {
+                    const unsigned long long at = (unsigned long long)key[k] * cap + p;
+                    if (WT) {
+                        reinterpret_cast<float4 *>(crec)[at] = make_float4(d[k].x, d[k].y, d[k].z, wt[k]);
+                    } else {
+                        float *o = crec + 3 * at;
+                        o[0] = d[k].x;
+                        o[1] = d[k].y;
+                        o[2] = d[k].z;
+                    }
+                    ++nplaced;
+                } else {
+                    spill = true;
+                }
+            }
+            // spill-list slots: one atomic per wave
+            const unsigned long long bal = __ballot(spill);
+            if (bal) {
+                const int first = __ffsll((long long)bal) - 1;
+                unsigned long long b0 = 0;
+                if (lane == first) b0 = atomicAdd(nspill, (unsigned long long)__popcll(bal));
+                b0 = (unsigned long long)__shfl((long long)b0, first);
+                if (spill) {
+                    const unsigned long long at = b0 + (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
+                    sxyz[3 * at] = d[k].x;
+                    sxyz[3 * at + 1] = d[k].y;
+                    sxyz[3 * at + 2] = d[k].z;
+                    if (WT) sw[at] = wt[k];
+                }
+            }
+        }
+        // (the next sub-chunk clears hist only after every thread has passed the
+        // claims, and rewrites gbase only after every thread has written its records)
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m = max(m, (unsigned)__shfl_xor((int)m, off));
+        miss += (unsigned)__shfl_xor((int)miss, off);
+        nplaced += (unsigned)__shfl_xor((int)nplaced, off);
+    }
+    const int nw = kSortBlock / 64;
+    if (lane == 0) {
+        red[t >> 6] = m;
+        red[nw + (t >> 6)] = miss;
+        red[2 * nw + (t >> 6)] = nplaced;
+    }
+    __syncthreads();
+    if (t == 0) {
+        unsigned mm = 0, ms = 0;
+        unsigned long long pl = 0;
+        for (int k = 0; k < nw; ++k) {
+            mm = max(mm, red[k]);
+            ms += red[nw + k];
+            pl += red[2 * nw + k];
+        }
+        if (WT) atomicMax(&wmax_bits[0], mm);
+        if (box_miss && ms) atomicAdd(box_miss, ms);
+        if (pl) atomicAdd(placed, pl);
+    }
+}
+
+// out[0] = a[0] + b[0] (the record total of a capacity sort: sorted spill + placed).
+__global__ void add_u64_kernel(const unsigned long long *__restrict__ a,
+                               const unsigned long long *__restrict__ b, unsigned long long *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = a[0] + b[0];
+}
+
 // Exclusive scan of per-tile counts into 64-bit offsets (offsets[ntiles] = total).
 constexpr int kScanBlock = 1024;
 constexpr int kScanItems = 4;
@@ -979,6 +1130,20 @@ staged_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__res
 }
 
 // ---- deposit -----------------------------------------------------------------
+// Records of the capacity sort (cap_scatter_kernel): tile `key` has min(count[key],
+// cap) records at rec + key*cap; rec == nullptr: none (the sorted records alone).
+// nrec >= 0: the realisation's record total (otherwise taken from the offsets).
+struct CapRecs {
+    const float *rec;
+    const unsigned *count;
+    unsigned cap;
+    long long nrec;
+};
+
+__device__ __forceinline__ unsigned long long cap_count(const CapRecs &cr, long long key) {
+    return cr.rec ? (unsigned long long)min(cr.count[key], cr.cap) : 0ull;
+}
+
 // Output transform of a final node value v: overwrite ? v/scale - shift : grid + v.
 struct OutMode {
     int overwrite;
@@ -1339,7 +1504,7 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
                     float weight_factor, const unsigned *__restrict__ wmax_bits,
                     float *__restrict__ grid, float *__restrict__ partials, OutMode om,
                     const unsigned *__restrict__ order, int nsplit,
-                    unsigned long long *__restrict__ gbox) {
+                    unsigned long long *__restrict__ gbox, CapRecs cr) {
     constexpr int F = MAS;
     constexpr int T = (AXES == 3) ? kT3 : kT2;
     constexpr int BY = T + F - 1;
@@ -1370,10 +1535,20 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
         r1 = min(r0 + part, r1);
         if (r0 == r1) return;                     // no records in this part
     }
+    // Record segments: the tile's capacity slots (capacity sort), then its run of
+    // the sorted records.  recs, s0, s1: the segment being deposited.
+    unsigned long long c0 = 0, c1 = 0;
+    if (cr.rec) {
+        c0 = (unsigned long long)key * cr.cap;
+        c1 = c0 + min(cr.count[key], cr.cap);
+    }
+    const bool two_seg = c1 > c0 && r1 > r0;
+    const float *recs = c1 > c0 ? cr.rec : records;
+    unsigned long long s0 = c1 > c0 ? c0 : r0, s1 = c1 > c0 ? c1 : r1;
     // UNI: every tile has extent T (dims % T == 0, two tiles or more per axis)
     const int ex = UNI ? T : tb.ex, ey = UNI ? T : tb.ey, ez = UNI ? (AXES == 3 ? T : 1) : tb.ez;
 
-    if (r0 == r1) {
+    if (s0 == s1) {
         // Empty tile: its interior is zero; the finalize reads no halo from it.
         if (om.overwrite) {
             const int n = ex * ey * ez;
@@ -1394,128 +1569,143 @@ tile_deposit_kernel(const float *__restrict__ records, unsigned long long rcap,
     // records b .. b+WB-1 of a set, clamped to the tile's last record, through a
     // wave-uniform pointer and 32-bit lane offsets
     auto load_set = [&](unsigned long long b, float4 *dst) {
-        const unsigned long long sb = min(b, r1 - 1);
-        const float *set = records + (unsigned long long)RS * sb;
-        const unsigned lim = (unsigned)min(r1 - 1 - sb, (unsigned long long)WB);
+        const unsigned long long sb = min(b, s1 - 1);
+        const float *set = recs + (unsigned long long)RS * sb;
+        const unsigned lim = (unsigned)min(s1 - 1 - sb, (unsigned long long)WB);
 #pragma unroll
         for (int k = 0; k < KR; ++k) dst[k] = load_set_record<RW>(set, min((unsigned)(ln + 64 * k), lim));
     };
     float4 nxt[KR];
-    if constexpr (BAL) load_set(r0 + (unsigned long long)wv * WB, nxt);
+    if constexpr (BAL) load_set(s0 + (unsigned long long)wv * WB, nxt);
     for (int c = threadIdx.x; c < NB; c += blockDim.x) acc[c] = 0ull;
 
     float to_q;
     double from_q;
     // the realisation's record count, so that a batch and single calls agree bitwise
-    const unsigned long long nrec = offsets[(long long)(real + 1) * g.ntiles] - offsets[(long long)real * g.ntiles];
+    const unsigned long long nrec = cr.nrec >= 0 ? (unsigned long long)cr.nrec
+        : offsets[(long long)(real + 1) * g.ntiles] - offsets[(long long)real * g.ntiles];
     box_quantum<WT>(wmax_bits, real, weight_factor, nrec, to_q, from_q);
     __syncthreads();
 
     if constexpr (!BAL) {
-        // the next record is loaded while the current one is deposited
-        unsigned long long r = r0 + threadIdx.x;
-        float4 nxt = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (r < r1) nxt = load_record<RW>(records, r);
-        for (; r < r1; r += blockDim.x) {
-            const float4 rec = nxt;
-            if (r + blockDim.x < r1) nxt = load_record<RW>(records, r + blockDim.x);
-            deposit_record<MAS, AXES, WT, RW>(acc + box_base<MAS, AXES>(rec, g, tb), rec, to_q,
-                                              weight_factor);
+        for (int seg = 0;; ++seg) {
+            // the next record is loaded while the current one is deposited
+            unsigned long long r = s0 + threadIdx.x;
+            float4 nxt = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < s1) nxt = load_record<RW>(recs, r);
+            for (; r < s1; r += blockDim.x) {
+                const float4 rec = nxt;
+                if (r + blockDim.x < s1) nxt = load_record<RW>(recs, r + blockDim.x);
+                deposit_record<MAS, AXES, WT, RW>(acc + box_base<MAS, AXES>(rec, g, tb), rec, to_q,
+                                                  weight_factor);
+            }
+            if (seg > 0 || !two_seg) break;
+            recs = records;
+            s0 = r0;
+            s1 = r1;
         }
     } else {
         unsigned *wh = whist + wv * 16;
         unsigned short *si = sidx + wv * WB;
         unsigned char *ht = htab + wv * WB;
         constexpr unsigned G = 4 * KR;                 // layers of 16 lanes
-        // wave wv takes the round sets starting at r0 + (wv + NW*j)*WB; the next
-        // set's records are loaded (clamped into the tile) while this one runs
-        unsigned long long b0 = r0 + (unsigned long long)wv * WB;
-        for (; b0 < r1; b0 += (unsigned long long)NW * WB) {
-            float4 rec[KR];
+        // wave wv takes the round sets starting at s0 + (wv + NW*j)*WB; the next
+        // set's records are loaded (clamped into the segment) while this one runs
+        for (int seg = 0;; ++seg) {
+            unsigned long long b0 = s0 + (unsigned long long)wv * WB;
+            for (; b0 < s1; b0 += (unsigned long long)NW * WB) {
+                float4 rec[KR];
 #pragma unroll
-            for (int k = 0; k < KR; ++k) rec[k] = nxt[k];
-            const int nb = (int)min((unsigned long long)WB, r1 - b0);   // records in this set
-            // rank within class (LDS ops of one wave run in issue order)
-            if (ln < 16) wh[ln] = 0u;
-            __builtin_amdgcn_wave_barrier();
-            unsigned cls[KR], rank[KR];
+                for (int k = 0; k < KR; ++k) rec[k] = nxt[k];
+                const int nb = (int)min((unsigned long long)WB, s1 - b0);   // records in this set
+                // rank within class (LDS ops of one wave run in issue order)
+                if (ln < 16) wh[ln] = 0u;
+                __builtin_amdgcn_wave_barrier();
+                unsigned cls[KR], rank[KR];
 #pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                cls[k] = bank_class<MAS, AXES>(rec[k]);
-                rank[k] = (ln + 64 * k < nb) ? atomicAdd(&wh[cls[k]], 1u) : 0u;
-            }
-            __builtin_amdgcn_wave_barrier();
-            // lanes 0..15 hold class j = ln: c_j records, G - c_j holes or c_j - G
-            // excess copies.  Exclusive prefixes of both (excess in the high half)
-            // by an inclusive DPP scan over row 0 (row_shr 1, 2, 4, 8).
-            const unsigned cj = (ln < 16) ? wh[ln] : 0u;
-            const unsigned v = (ln < 16) ? ((cj > G ? cj - G : 0u) << 16) | (cj < G ? G - cj : 0u) : 0u;
-            unsigned incl = v;
-            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x111, 0xf, 0xf, false);
-            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x112, 0xf, 0xf, false);
-            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x114, 0xf, 0xf, false);
-            incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x118, 0xf, 0xf, false);
-            const unsigned excl = incl - v;
-            const unsigned E = (unsigned)__builtin_amdgcn_readlane((int)incl, 15) >> 16;
-            __builtin_amdgcn_wave_barrier();
-            // class j's count, first hole and first excess index (9 bits each)
-            if (ln < 16) wh[ln] = cj | (excl & 0xffffu) << 9 | (excl >> 16) << 18;
-            __builtin_amdgcn_wave_barrier();
-            {
-                // the G*16 (layer, lane) slots, KR per lane: lane ln takes class ln % 16
-                const unsigned cw = wh[ln & 15];
-#pragma unroll
-                for (int k = 0; k < KR; ++k)
-                    deal_hole<KR>((unsigned)(ln >> 4) + 4u * k, (unsigned)(ln & 15), cw & 511u,
-                                  (cw >> 9) & 511u, E, ht, si);
-            }
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                if (ln + 64 * k < nb) {
-                    const unsigned X = rank[k] < G ? 0u : wh[cls[k]] >> 18;
-                    si[deal_record_slot<KR>(cls[k], rank[k], X, ht)] = (unsigned short)(ln + 64 * k);
+                for (int k = 0; k < KR; ++k) {
+                    cls[k] = bank_class<MAS, AXES>(rec[k]);
+                    rank[k] = (ln + 64 * k < nb) ? atomicAdd(&wh[cls[k]], 1u) : 0u;
                 }
-            }
-            __builtin_amdgcn_wave_barrier();
-            // the dealt records of all rounds are reloaded (from cache) before the
-            // first atomic, and the next set is loaded only after them: loads
-            // complete in issue order, so the atomics wait for the reloads alone
-            // and the next set's memory latency overlaps this set's atomics.  The
-            // next-set loads are issued unconditionally (past the tile's end they
-            // all read its last record) so that the number of loads in flight is
-            // the same on every path and the waits before the atomics stay exact.
-            unsigned short di[KR];
-            float4 dealt[KR];
+                __builtin_amdgcn_wave_barrier();
+                // lanes 0..15 hold class j = ln: c_j records, G - c_j holes or c_j - G
+                // excess copies.  Exclusive prefixes of both (excess in the high half)
+                // by an inclusive DPP scan over row 0 (row_shr 1, 2, 4, 8).
+                const unsigned cj = (ln < 16) ? wh[ln] : 0u;
+                const unsigned v = (ln < 16) ? ((cj > G ? cj - G : 0u) << 16) | (cj < G ? G - cj : 0u) : 0u;
+                unsigned incl = v;
+                incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x111, 0xf, 0xf, false);
+                incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x112, 0xf, 0xf, false);
+                incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x114, 0xf, 0xf, false);
+                incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x118, 0xf, 0xf, false);
+                const unsigned excl = incl - v;
+                const unsigned E = (unsigned)__builtin_amdgcn_readlane((int)incl, 15) >> 16;
+                __builtin_amdgcn_wave_barrier();
+                // class j's count, first hole and first excess index (9 bits each)
+                if (ln < 16) wh[ln] = cj | (excl & 0xffffu) << 9 | (excl >> 16) << 18;
+                __builtin_amdgcn_wave_barrier();
+                {
+                    // the G*16 (layer, lane) slots, KR per lane: lane ln takes class ln % 16
+                    const unsigned cw = wh[ln & 15];
 #pragma unroll
-            for (int r = 0; r < KR; ++r) di[r] = si[r * 64 + ln];
-            const float *set = records + (unsigned long long)RS * b0;
-#pragma unroll
-            for (int r = 0; r < KR; ++r)
-                dealt[r] = load_set_record<RW>(set, di[r] != kNoRecord ? di[r] : 0u);
-            __builtin_amdgcn_sched_barrier(0);         // keep the reloads ahead of the next set
-            load_set(b0 + (unsigned long long)NW * WB, nxt);
-            __builtin_amdgcn_sched_barrier(0);
-            // every reloaded record is consumed here, on all lanes: the reloads
-            // are not sunk into the per-round branches below (where they would be
-            // issued after the next set's loads), and one wait covers them
-#pragma unroll
-            for (int r = 0; r < KR; ++r) {
-                if constexpr (RW)
-                    asm volatile("" ::"v"(dealt[r].x), "v"(dealt[r].y), "v"(dealt[r].z), "v"(dealt[r].w));
-                else
-                    asm volatile("" ::"v"(dealt[r].x), "v"(dealt[r].y), "v"(dealt[r].z));
-            }
-#pragma unroll
-            for (int r = 0; r < KR; ++r) {
-                if (di[r] != kNoRecord) {
-                    int bb;
-                    if constexpr (UNI && AXES == 3) bb = box_base_uni<MAS>(dealt[r]);
-                    else bb = box_base<MAS, AXES>(dealt[r], g, tb);
-                    deposit_record<MAS, AXES, WT, RW>(acc + bb, dealt[r], to_q, weight_factor);
+                    for (int k = 0; k < KR; ++k)
+                        deal_hole<KR>((unsigned)(ln >> 4) + 4u * k, (unsigned)(ln & 15), cw & 511u,
+                                      (cw >> 9) & 511u, E, ht, si);
                 }
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int k = 0; k < KR; ++k) {
+                    if (ln + 64 * k < nb) {
+                        const unsigned X = rank[k] < G ? 0u : wh[cls[k]] >> 18;
+                        si[deal_record_slot<KR>(cls[k], rank[k], X, ht)] = (unsigned short)(ln + 64 * k);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                // the dealt records of all rounds are reloaded (from cache) before the
+                // first atomic, and the next set is loaded only after them: loads
+                // complete in issue order, so the atomics wait for the reloads alone
+                // and the next set's memory latency overlaps this set's atomics.  The
+                // next-set loads are issued unconditionally (past the tile's end they
+                // all read its last record) so that the number of loads in flight is
+                // the same on every path and the waits before the atomics stay exact.
+                unsigned short di[KR];
+                float4 dealt[KR];
+#pragma unroll
+                for (int r = 0; r < KR; ++r) di[r] = si[r * 64 + ln];
+                const float *set = recs + (unsigned long long)RS * b0;
+#pragma unroll
+                for (int r = 0; r < KR; ++r)
+                    dealt[r] = load_set_record<RW>(set, di[r] != kNoRecord ? di[r] : 0u);
+                __builtin_amdgcn_sched_barrier(0);         // keep the reloads ahead of the next set
+                load_set(b0 + (unsigned long long)NW * WB, nxt);
+                __builtin_amdgcn_sched_barrier(0);
+                // every reloaded record is consumed here, on all lanes: the reloads
+                // are not sunk into the per-round branches below (where they would be
+                // issued after the next set's loads), and one wait covers them
+#pragma unroll
+                for (int r = 0; r < KR; ++r) {
+                    if constexpr (RW)
+                        asm volatile("" ::"v"(dealt[r].x), "v"(dealt[r].y), "v"(dealt[r].z), "v"(dealt[r].w));
+                    else
+                        asm volatile("" ::"v"(dealt[r].x), "v"(dealt[r].y), "v"(dealt[r].z));
+                }
+#pragma unroll
+                for (int r = 0; r < KR; ++r) {
+                    if (di[r] != kNoRecord) {
+                        int bb;
+                        if constexpr (UNI && AXES == 3) bb = box_base_uni<MAS>(dealt[r]);
+                        else bb = box_base<MAS, AXES>(dealt[r], g, tb);
+                        deposit_record<MAS, AXES, WT, RW>(acc + bb, dealt[r], to_q, weight_factor);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
+            if (seg > 0 || !two_seg) break;
+            // the tile's sorted records: the first set is loaded before the loop, as above
+            recs = records;
+            s0 = r0;
+            s1 = r1;
+            load_set(s0 + (unsigned long long)wv * WB, nxt);
         }
     }
     __syncthreads();
@@ -1568,6 +1758,14 @@ constexpr long long kOrderMinRecords = 512;      // mean records per key
 constexpr long long kSplitMaxKeys = 2048;
 constexpr long long kSplitRecords = 16384;
 constexpr long long kSplitMax = 256;
+// Capacity sort: tile slots kCapFactor times the mean occupancy; adaptive use from
+// kCapMinMean particles per tile on average; a spill above 1/kCapSpillDen of the
+// particles turns it off for the (dims, MAS); the spill list is sorted in
+// kSpillChunks chunks.
+constexpr long long kCapFactor = 3;
+constexpr long long kCapMinMean = 256;
+constexpr unsigned long long kCapSpillDen = 64;
+constexpr int kSpillChunks = 512;
 
 // 16 buckets per octave of the record count (0 alone in bucket 0); distinct
 // buckets also spread the LDS histogram's atomics over more addresses
@@ -1579,7 +1777,8 @@ __host__ __device__ __forceinline__ int count_bucket(unsigned long long n) {
 }
 
 __global__ void __launch_bounds__(kOrderBlock)
-tile_order_kernel(const unsigned long long *__restrict__ offsets, long long nt, unsigned *__restrict__ order) {
+tile_order_kernel(const unsigned long long *__restrict__ offsets, long long nt, unsigned *__restrict__ order,
+                  CapRecs cr) {
     constexpr int U = 8;                                // keys per thread in flight
     __shared__ unsigned hist[kOrderBuckets];
     for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) hist[b] = 0u;
@@ -1596,7 +1795,7 @@ tile_order_kernel(const unsigned long long *__restrict__ offsets, long long nt, 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const long long k = k0 + threadIdx.x + (long long)u * blockDim.x;
-            if (k < nt) atomicAdd(&hist[count_bucket(o1[u] - o0[u])], 1u);
+            if (k < nt) atomicAdd(&hist[count_bucket(o1[u] - o0[u] + cap_count(cr, k))], 1u);
         }
     }
     __syncthreads();
@@ -1628,7 +1827,7 @@ tile_order_kernel(const unsigned long long *__restrict__ offsets, long long nt, 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const long long k = k0 + threadIdx.x + (long long)u * blockDim.x;
-            if (k < nt) order[atomicAdd(&hist[count_bucket(o1[u] - o0[u])], 1u)] = (unsigned)k;
+            if (k < nt) order[atomicAdd(&hist[count_bucket(o1[u] - o0[u] + cap_count(cr, k))], 1u)] = (unsigned)k;
         }
     }
 }
@@ -1679,7 +1878,8 @@ __device__ __forceinline__ int xcd_contiguous_block() {
 template <int MAS, int AXES>
 __global__ void __launch_bounds__(kBlock)
 tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
-                     const float *__restrict__ partials, float *__restrict__ grid, OutMode om) {
+                     const float *__restrict__ partials, float *__restrict__ grid, OutMode om,
+                     CapRecs cr) {
     constexpr int F = MAS;
     __shared__ long long s_pbase[64];   // partial base of image combination, -1 if empty
     const int key = xcd_contiguous_block();
@@ -1697,7 +1897,8 @@ tile_finalize_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
     if ((int)threadIdx.x < ncomb) {
         const int c = threadIdx.x % nz, b = (threadIdx.x / nz) % ny, a = threadIdx.x / (nz * ny);
         const long long vkey = kbase + ((long long)itx[a] * g.nt + ity[b]) * g.ntz + itz[c];
-        s_pbase[threadIdx.x] = (offsets[vkey] == offsets[vkey + 1]) ? -1 : vkey * g.pslot;
+        s_pbase[threadIdx.x] =
+            (offsets[vkey] == offsets[vkey + 1] && cap_count(cr, vkey) == 0) ? -1 : vkey * g.pslot;
     }
     __syncthreads();
     const int ex = tb.ex, ey = tb.ey, ez = tb.ez;
@@ -1785,7 +1986,7 @@ template <int MAS>
 __global__ void __launch_bounds__(kBlock)
 tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
                              const float *__restrict__ partials, float *__restrict__ grid,
-                             OutMode om) {
+                             OutMode om, CapRecs cr) {
     constexpr int F = MAS, T = kT3, L = F - 1;
     constexpr int NA = L * T * T;                       // lx < L
     constexpr int NB = (T - L) * L * T;                 // lx >= L, ly < L
@@ -1803,7 +2004,8 @@ tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, Til
         const int iy = b ? (ty == 0 ? g.nt - 1 : ty - 1) : ty;
         const int iz = c ? (tz == 0 ? g.nt - 1 : tz - 1) : tz;
         const long long vkey = kbase + ((long long)ix * g.nt + iy) * g.nt + iz;
-        s_pbase[threadIdx.x] = (offsets[vkey] == offsets[vkey + 1]) ? -1 : vkey * g.pslot;
+        s_pbase[threadIdx.x] =
+            (offsets[vkey] == offsets[vkey + 1] && cap_count(cr, vkey) == 0) ? -1 : vkey * g.pslot;
     }
     __syncthreads();
     grid += real * g.ncell;
@@ -2092,17 +2294,22 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const int nbk = nbt * nbt * bucket_counts_z(g.ntz);
     const bool two_ok = axes == 3 && g.nt >= 16 && nreal == 1 && !devt && !c->capturing &&
                         !c->arena && nbk <= kMaxBuckets && n > 0 && n < (1ll << 32);
-    if (c->box_miss_n > 0 && !c->capturing) {
+    if ((c->box_miss_n > 0 || c->cap_spill_n > 0) && !c->capturing) {
         const hipError_t q = hipEventQuery(c->ev_total);
         if (q == hipSuccess) {
-            const bool shuffled = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
-            c->shuffled_hint[c->box_miss_key] = shuffled;
-            {
+            if (c->box_miss_n > 0) {
+                const bool shuffled = c->h_total[1] * 4 > (unsigned long long)c->box_miss_n;
+                c->shuffled_hint[c->box_miss_key] = shuffled;
                 std::lock_guard<std::mutex> lk(g_hint_mu);
                 g_shuffled_hint[std::make_tuple(c->device, c->box_miss_key.first, c->box_miss_key.second)] =
                     shuffled;
             }
+            // a capacity sort that spilled more than 1/64 of its particles: the one-pass
+            // sort from now on for this (dims, MAS)
+            if (c->cap_spill_n > 0 && c->h_total[3] * kCapSpillDen > (unsigned long long)c->cap_spill_n)
+                c->cap_bad[c->cap_key] = true;
             c->box_miss_n = -1;
+            c->cap_spill_n = -1;
         } else if (q == hipErrorNotReady) {
             (void)hipGetLastError();          // pending, not an error: keep it out of later checks
         } else {
@@ -2122,17 +2329,55 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
             shuffled = gt != g_shuffled_hint.end() && gt->second;
         }
     }
-    bool wide = false, two = false;
+    // Few tiles with many records each (a small grid and many particles): each
+    // tile's records are split over nsplit workgroups, which add their boxes into
+    // per-tile int64 boxes, and a second pass writes the grid.
+    int nsplit = 1;
+    if (nt < kSplitMaxKeys && n > kSplitRecords * nt)
+        nsplit = (int)std::min<long long>(kSplitMax, (n + kSplitRecords * nt - 1) / (kSplitRecords * nt));
+    // The capacity sort (cap_scatter_kernel): coherent particles of one 3D grid whose
+    // sort chunks have every tile or the 8 x 8 tile-column box in their window;
+    // adaptively when the tiles hold kCapMinMean particles or more on average and no
+    // earlier capacity call of this context on such a grid spilled too much.  Its
+    // slots take about kCapFactor times the records' bytes.
+    const bool cap_ok = axes == 3 && nreal == 1 && !devt && !c->arena && !c->capturing && nsplit == 1 &&
+                        (g.wmode == kWinDirect || g.wmode == kWinBox) && n > 0 && n < (1ll << 32);
+    const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
+    bool wide = false, two = false, capm = false;
     if (c->tune_window == 1) wide = wide_ok;
     else if (c->tune_window == 2) two = two_ok;
+    else if (c->tune_window == 3) capm = cap_ok;
     else if (c->tune_window < 0 && tally && shuffled) {
         two = two_ok;
         wide = !two && wide_ok;          // batches of host ranges, fixed scratch
+    } else if (c->tune_window < 0 && tally && cap_ok && n >= kCapMinMean * g.ntiles) {
+        const auto it = c->cap_bad.find(hkey);
+        capm = it == c->cap_bad.end() || !it->second;
     }
+    // tile capacity: kCapFactor times the mean, in whole 64-record runs
+    const unsigned long long cap64 =
+        capm ? ((unsigned long long)(kCapFactor * n + g.ntiles - 1) / g.ntiles + 63) / 64 * 64 : 0;
+    void *cv = nullptr;
+    if (capm) {
+        const size_t bytes = (size_t)g.ntiles * cap64 * rec_bytes;
+        // within the context's budget, else the one-pass sort
+        capm = cap64 < (1ull << 31) &&
+               held_bytes(c) - c->bufs[BUF_CAPREC].cap + bytes + (bytes >> 4) <= c->budget &&
+               take(c, BUF_CAPREC, bytes, &cv) == PYL_OK;
+    }
+    const unsigned cap = (unsigned)cap64;
     if (wide) g.wmode = kWinDirect;
+    // the spill sort of the capacity form: every tile in LDS when they fit
+    TileGeom gs = g;
+    int sws = kWin;
+    if (g.ntiles <= kWin) gs.wmode = kWinDirect;
+    else if (kUseWide && g.ntiles <= kWinWide) {
+        gs.wmode = kWinDirect;
+        sws = kWinWide;
+    }
     // kWinDirect/Box/Linear == PYL_WINDOW_DIRECT/BOX/LINEAR
-    c->last_window = two ? PYL_WINDOW_TWOLEVEL : wide ? PYL_WINDOW_WIDE : g.wmode;
-    const int ws = two ? kBktSlots : wide ? kWinWide : kWin;
+    c->last_window = capm ? PYL_WINDOW_CAPACITY : two ? PYL_WINDOW_TWOLEVEL : wide ? PYL_WINDOW_WIDE : g.wmode;
+    const int ws = capm ? sws : two ? kBktSlots : wide ? kWinWide : kWin;
     // Chunks of consecutive particles, never straddling two realisations.
     const long long target = wide ? std::max(2 * g.ntiles, std::min(16 * g.ntiles, (n + 2047) / 2048))
                                   : std::max(8192LL, std::min(65536LL, (n + 4095) / 4096));
@@ -2150,13 +2395,19 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const long long nchunk2_ll = two ? (n + target2 - 1) / target2 + nbk : 0;
     PYL_CHECK(nchunk2_ll < (1 << 30), PYL_EINVAL, "too many particle chunks");
     const int nchunk2 = (int)nchunk2_ll;
-    // chunks of the tile sort (pass 2 of the two-level sort, or the only pass)
-    const int nsort = two ? nchunk2 : nchunk;
+    // chunks of the tile sort (pass 2 of the two-level sort, the capacity form's spill
+    // sort, or the only pass)
+    const int nsort = two ? nchunk2 : capm ? kSpillChunks : nchunk;
     // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | wmax[nreal] | box_miss |
     //              (two-level) bucket counts[nbk] | pad | offsets[nt+1] | block sums |
     //              chunk table | tile origins | tile order | per-chunk window bases |
     //              (two-level) bucket offsets[nbk+1] | pass-2 chunk table | bucket bases
-    const size_t nwords = (size_t)nt * 3 + (size_t)nreal + 1 + (two ? (size_t)nbk : 0);
+    //              (capacity) tile cursors[nt] | pad | spill range {0, count}, placed, total (u64)
+    size_t nwords = (size_t)nt * 3 + (size_t)nreal + 1 + (two ? (size_t)nbk : 0);
+    const size_t cur_at = nwords;
+    if (capm) nwords += (size_t)nt;
+    const size_t capw_at = (nwords + 1) & ~(size_t)1;     // 8-byte aligned
+    if (capm) nwords = capw_at + 8;
     const size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t off_bytes = (((size_t)nt + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255;
     const size_t bs_bytes = ((size_t)nscan_blocks * sizeof(unsigned long long) + 255) & ~(size_t)255;
@@ -2165,31 +2416,25 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const size_t ord_bytes = ((size_t)nt * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t dump_bytes = ((size_t)std::max(nsort, 1) * ws * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t boff_bytes = two ? (((size_t)nbk + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255 : 0;
-    const size_t ch2_bytes = two ? ((size_t)nchunk2 * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
+    const size_t ch2_bytes = (two || capm) ? ((size_t)(two ? nchunk2 : kSpillChunks) * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
     const size_t bdump_bytes = two ? ((size_t)std::max(nchunk, 1) * nbk * sizeof(unsigned) + 255) & ~(size_t)255 : 0;
-    // Few tiles with many records each (a small grid and many particles): each
-    // tile's records are split over nsplit workgroups, which add their boxes into
-    // per-tile int64 boxes, and a second pass writes the grid.  Otherwise, largest
-    // tiles first, when there are enough tiles, and records per tile, for the
+    // Largest tiles first, when there are enough tiles, and records per tile, for the
     // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys).
-    int nsplit = 1;
-    if (nt < kSplitMaxKeys && n > kSplitRecords * nt)
-        nsplit = (int)std::min<long long>(kSplitMax, (n + kSplitRecords * nt - 1) / (kSplitRecords * nt));
     const bool ordered = c->tune_order >= 0
         ? (nsplit == 1 && c->tune_order == 1)
         : (nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt);
     const int T = axes == 3 ? kT3 : kT2;
     const size_t nbox = (size_t)(T + MAS - 1) * (T + MAS - 1) * (axes == 3 ? T + MAS - 1 : 1);
-    const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
     // all scratch of the call is taken before the first launch
     void *tbv = nullptr, *pv = nullptr, *rv = nullptr, *bv = nullptr, *kv = nullptr;
     PYL_TRY(take(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
                                       ord_bytes + dump_bytes + boff_bytes + ch2_bytes + 2 * bdump_bytes, &tbv));
     PYL_TRY(take(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float), &pv));
     PYL_TRY(take(c, BUF_RECORDS, (size_t)(n ? n : 1) * rec_bytes, &rv));
-    // bucketed records: n (dist_x, dist_y, dist_z) | n weights, the weights 256-byte aligned
+    // bucketed records (or the capacity form's spill list): n (dist_x, dist_y, dist_z) |
+    // n weights, the weights 256-byte aligned
     const size_t bxyz_bytes = ((size_t)n * 3 * sizeof(float) + 255) & ~(size_t)255;
-    if (two) PYL_TRY(take(c, BUF_BUCKETED, bxyz_bytes + (W ? (size_t)n * sizeof(float) : 0), &kv));
+    if (two || capm) PYL_TRY(take(c, BUF_BUCKETED, bxyz_bytes + (W ? (size_t)n * sizeof(float) : 0), &kv));
     if (nsplit > 1)
         PYL_TRY(take(c, BUF_TILE_BOX, (size_t)nt * nbox * sizeof(unsigned long long), &bv));
     char *tb = (char *)tbv;
@@ -2211,7 +2456,10 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     unsigned *bdump = (unsigned *)(tb2 + boff_bytes + ch2_bytes);
     unsigned *bmat = (unsigned *)(tb2 + boff_bytes + ch2_bytes + bdump_bytes);
     float *bxyz = (float *)kv;
-    float *bw = (two && W) ? (float *)((char *)kv + bxyz_bytes) : nullptr;
+    float *bw = ((two || capm) && W) ? (float *)((char *)kv + bxyz_bytes) : nullptr;
+    unsigned *cursor = counts + cur_at;
+    unsigned long long *capw = (unsigned long long *)(counts + capw_at);
+    float *crec = (float *)cv;
     g.orig = dorig;
     float *partials = (float *)pv;
     float *records = (float *)rv;
@@ -2280,6 +2528,55 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     RSD srsd = rsd;
     float sinv = inv;
     Chunk *schunks = dchunks;
+    if (capm) {
+        // one read of the particles into the tile slots; the rest into the spill list
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
+        auto kcs = W ? (pow2 ? cap_scatter_kernel<MAS, true, true> : cap_scatter_kernel<MAS, false, true>)
+                     : (pow2 ? cap_scatter_kernel<MAS, true, false> : cap_scatter_kernel<MAS, false, false>);
+        const int slots = g.wmode == kWinBox ? kBoxW * kBoxW * g.ntz : (int)g.ntiles;
+        const size_t lds = 2 * (size_t)slots * sizeof(unsigned);
+        if (lds > 65536)
+            PYL_HIP(hipFuncSetAttribute((const void *)kcs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        kcs<<<nchunk, kSortBlock, lds, c->stream>>>(pos, rsd, W, inv, g, dchunks, cursor, cap, crec, bxyz, bw,
+                                                    capw + 1, capw + 2, wmax, tally ? box_miss : nullptr);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
+        // the spill list (capw[0..1] = {0, count}) sorted by the regular passes, in
+        // kSpillChunks chunks cut on the device (empty ones when nothing spilled)
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SPILL));
+        chunk_table_dev_kernel<<<ceil_div(std::max(kSpillChunks, g.nt + 1), kBlock), kBlock, 0, c->stream>>>(
+            dchunks2, (const long long *)capw, 1, kSpillChunks, dorig, g.nt, dims);
+        const RSD off{bxyz, 0.0f, 0.0f, 0, 3, 0};
+        auto kc = sws == kWinWide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
+                                          : tile_count_kernel<MAS, 3, false, kWinWide>)
+                                  : (pow2 ? tile_count_kernel<MAS, 3, true, kWin>
+                                          : tile_count_kernel<MAS, 3, false, kWin>);
+        kc<<<kSpillChunks, kSortBlock, 0, c->stream>>>(bxyz, off, 1.0f, gs, dchunks2, counts, ocounts, dump,
+                                                       nullptr);
+        scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums);
+        scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
+        scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums, offsets);
+        auto ks = sws == kWinWide
+            ? (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWinWide>
+                         : tile_scatter_kernel<MAS, 3, false, true, kWinWide>)
+                 : (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWinWide>
+                         : tile_scatter_kernel<MAS, 3, false, false, kWinWide>))
+            : (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWin>
+                         : tile_scatter_kernel<MAS, 3, false, true, kWin>)
+                 : (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWin>
+                         : tile_scatter_kernel<MAS, 3, false, false, kWin>));
+        ks<<<kSpillChunks, kSortBlock, 0, c->stream>>>(bxyz, off, bw, 1.0f, gs, dchunks2, dump, offsets, counts,
+                                                       ocursors, wmax, records, (unsigned long long)n);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SPILL));
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
+        if (ordered)
+            tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order,
+                                                                CapRecs{crec, cursor, cap, (long long)n});
+        add_u64_kernel<<<1, 64, 0, c->stream>>>(offsets + nt, capw + 2, capw + 3);   // record total
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
+    }
     if (two) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_BUCKET));
         auto kb = pow2 ? bucket_count_kernel<MAS, true> : bucket_count_kernel<MAS, false>;
@@ -2310,7 +2607,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         schunks = dchunks2;
         g.wmode = kWinBlock;
     }
-    if (nsort > 0) {
+    if (nsort > 0 && !capm) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));
         auto kc = two ? (pow2 ? tile_count_kernel<MAS, 3, true, kBktSlots>
                               : tile_count_kernel<MAS, 3, false, kBktSlots>)
@@ -2325,18 +2622,24 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_COUNT));
     }
-    PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
-    scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums);
-    scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
-    scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
-                                                                   offsets);
-    if (ordered) tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order);
-    PYL_HIP(hipGetLastError());
-    PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
-    // offsets[nt] must equal n; it is read back asynchronously and checked at the
-    // next synchronising call on this context.  Until then the scatter and the
-    // deposit keep every record index below n, so a bad count cannot write or
-    // read outside the record buffer.
+    if (!capm) {
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
+        scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums);
+        scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
+        scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
+                                                                       offsets);
+        if (ordered)
+            tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order,
+                                                                CapRecs{nullptr, nullptr, 0, -1});
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
+    }
+    // offsets[nt] (capacity form: the sorted spill plus the placed records) must
+    // equal n; it is read back asynchronously and checked at the next synchronising
+    // call on this context.  Until then the scatter and the deposit keep every
+    // record index below n (capacity slots: below the tile's capacity), so a bad
+    // count cannot write or read outside the record buffers.
+    const unsigned long long *total_word = capm ? capw + 3 : offsets + nt;
     PYL_TRY(check_total(c));                    // the previous call's check, if still pending
     if (c->capturing) {
         // each captured call reads its totals into pinned words of its own, compared
@@ -2345,7 +2648,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         PYL_HIP(hipHostMalloc((void **)&ht, 2 * sizeof(unsigned long long), hipHostMallocDefault));
         c->cap_host.push_back(ht);
         ht[1] = (unsigned long long)n;
-        PYL_HIP(hipMemcpyAsync(ht, offsets + nt, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+        PYL_HIP(hipMemcpyAsync(ht, total_word, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                c->stream));
         if (devt)
             PYL_HIP(hipMemcpyAsync(ht + 1, c->dev_offs + nreal, sizeof(unsigned long long),
@@ -2355,7 +2658,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
             return PYL_OK;
         });
     } else {
-        PYL_HIP(hipMemcpyAsync(c->h_total, offsets + nt, sizeof(unsigned long long),
+        PYL_HIP(hipMemcpyAsync(c->h_total, total_word, sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, c->stream));
         if (devt)
             PYL_HIP(hipMemcpyAsync(c->h_total + 2, c->dev_offs + nreal, sizeof(unsigned long long),
@@ -2368,11 +2671,20 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
             c->box_miss_n = n;
             c->box_miss_key = hkey;
         }
+        c->cap_spill_n = -1;
+        if (capm) {
+            PYL_HIP(hipMemcpyAsync(c->h_total + 3, capw + 1, sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, c->stream));
+            c->cap_spill_n = n;
+            c->cap_key = hkey;
+        }
         PYL_HIP(hipEventRecord(c->ev_total, c->stream));
         c->total_expect = devt ? -2 : n;      // -2: against the device offsets' total
     }
     const unsigned long long rcap = (unsigned long long)n;
-    if (nsort > 0 && two) {
+    if (capm) {
+        // sorted above
+    } else if (nsort > 0 && two) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
         auto ks = W ? (pow2 ? staged_scatter_kernel<MAS, true, true, 1, kBktSlots, kStage2>
                             : staged_scatter_kernel<MAS, false, true, 1, kBktSlots, kStage2>)
@@ -2419,9 +2731,10 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                        : (W ? tile_deposit_kernel<MAS, 2, true, true, false>
                             : tile_deposit_kernel<MAS, 2, true, false, false>)));
     if (nsplit > 1) PYL_TRY(zero_bytes(c, gbox, (size_t)nt * nbox * sizeof(unsigned long long)));
+    const CapRecs cr = capm ? CapRecs{crec, cursor, cap, (long long)n} : CapRecs{nullptr, nullptr, 0, -1};
     kd<<<(unsigned)(nt * nsplit), kDepBlock, 0, c->stream>>>(records, rcap, offsets, g, weight_factor,
                                                             W ? wmax : nullptr, grid, partials, om,
-                                                            ordered ? order : nullptr, nsplit, gbox);
+                                                            ordered ? order : nullptr, nsplit, gbox, cr);
     PYL_HIP(hipGetLastError());
     if (nsplit > 1) {
         auto kf = (axes == 3)
@@ -2438,13 +2751,13 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_FINAL));
         if (axes == 3 && g.nt >= 2 && dims % kT3 == 0)
             tile_finalize_uniform_kernel<MAS><<<(unsigned)nt, kBlock, 0, c->stream>>>(
-                offsets, g, partials, grid, om);
+                offsets, g, partials, grid, om, cr);
         else if (axes == 3)
             tile_finalize_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
-                                                                                grid, om);
+                                                                                grid, om, cr);
         else
             tile_finalize_kernel<MAS, 2><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
-                                                                                grid, om);
+                                                                                grid, om, cr);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_FINAL));
     }

@@ -117,6 +117,7 @@ enum BufId {
     BUF_TILE_BOX,    // per-tile int64 boxes of a split deposit
     BUF_BUCKETED,    // records of the two-level sort's first pass
     BUF_PK_AMAX,     // per-cube largest |Re|, |Im| of the transforms being binned
+    BUF_CAPREC,      // per-tile record slots of the capacity sort
     BUF_COUNT_
 };
 
@@ -148,12 +149,17 @@ struct Ctx {
     PinnedBuf pin_scales;                                    // Pk batch scale/shift upload
     // Tile-sort consistency check of the latest tiled MA, read back without a
     // synchronisation of its own: checked at the next synchronising call.
-    // pinned: [0] record total, [1] box misses, [2] particle total of device offsets
+    // pinned: [0] record total, [1] box misses, [2] particle total of device offsets,
+    // [3] particles the capacity sort spilled
     unsigned long long *h_total = nullptr;
     long long box_miss_n = -1;       // particles of the call whose box misses h_total[1] will hold
     std::pair<int, int> box_miss_key{0, 0};           // its (dims, MAS)
     std::map<std::pair<int, int>, bool> shuffled_hint; // per (dims, MAS): the latest call's particles
                                                         // were out of order (two-level sort next)
+    long long cap_spill_n = -1;      // particles of the capacity call whose spill h_total[3] will hold
+    std::pair<int, int> cap_key{0, 0};                // its (dims, MAS)
+    std::map<std::pair<int, int>, bool> cap_bad;      // per (dims, MAS): a capacity call spilled
+                                                      // too much (the one-pass sort instead)
     int last_window = -1;            // sort window of the latest tiled MA (PYL_WINDOW_*)
     long long total_expect = -1;
     hipEvent_t ev_total = nullptr;
@@ -180,7 +186,7 @@ struct Ctx {
     // Launch parameters a tuner may set (0: the built-in choice).
     int tune_bin_rows = 0;           // binning rows per work item
     int tune_order = -1;             // largest-first deposit order: -1 auto, 0 off, 1 on
-    int tune_window = -1;            // MA sort: -1 adaptive, 0 one-pass, 1 wide, 2 two-level
+    int tune_window = -1;            // MA sort: -1 adaptive, 0 one-pass, 1 wide, 2 two-level, 3 capacity
     // Stage timing: a ring of event pairs per stage, so that timing many calls
     // needs no host synchronisation between them.  A pair is read back when its
     // ring slot comes round again (kEvRing calls later) or on request.

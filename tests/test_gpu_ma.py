@@ -407,6 +407,130 @@ def test_ma_rsd_two_level_equals_one_pass():
     assert abs(out[0].astype(np.float64).sum() - W.astype(np.float64).sum()) <= 1e-5 * W.sum()
 
 
+def _clumped_lattice(d, box, seed, frac=0.06, disp=3.0):
+    """A displaced d^3 lattice in lattice order with a fraction of its particles moved
+    into one sphere of 4 cells' radius (a few tiles hold many times the mean)."""
+    p = _lattice(d, box, seed, disp)
+    rng = np.random.default_rng(seed + 1)
+    k = int(frac * p.shape[0])
+    idx = np.sort(rng.choice(p.shape[0], k, replace=False))
+    c = np.float32(0.37 * box)
+    r = rng.normal(0.0, 4.0 * box / d / 3.0, (k, 3))
+    p[idx] = ((c + r) % box).astype(np.float32) % np.float32(box)
+    return p
+
+
+@pytest.mark.parametrize('dims,d,mas,weighted,order', [
+    (256, 256, 'TSC', False, 'lattice'),       # every tile in the window, 4096 per tile
+    (256, 192, 'CIC', True, 'clumped'),        # clumped: tiles past their capacity spill
+    (512, 256, 'PCS', False, 'lattice'),       # 8 x 8 tile-column box window, 512 per tile
+    (512, 200, 'NGP', True, 'clumped'),
+    (432, 160, 'TSC', True, 'shuffled'),       # every particle misses its chunk's box
+    (200, 96, 'PCS', False, 'lattice'),        # unequal tiles (13 per axis)
+])
+def test_ma_capacity_sort_vs_oracle(oracle, dims, d, mas, weighted, order):
+    # The capacity sort: one read of the particles into per-tile slots (3x the mean
+    # occupancy); particles outside their chunk's window or past a full tile go to the
+    # spill list, which the regular passes sort into a second segment.  Forced on a
+    # context of its own: every cell matches the oracle and equals the one-pass sort
+    # bitwise.
+    from pylians3_amd import MAS_library as MASL, _native as N
+    print('case capacity', dims, d, mas, weighted, order, flush=True)
+    box = 600.0
+    pos = _clumped_lattice(d, box, 40 + d) if order == 'clumped' else _lattice(d, box, 40 + d)
+    if order == 'shuffled':
+        pos = pos[np.random.default_rng(3).permutation(pos.shape[0])]
+    pos = np.ascontiguousarray(pos)
+    n = pos.shape[0]
+    W = np.random.default_rng(d).lognormal(0.0, 1.0, n).astype(np.float32) if weighted else None
+    masid = MASL._MAS_IDS[mas]
+    ref = np.zeros((dims,) * 3, np.float32)
+    oracle.ma_f32(pos, ref, box, mas, W)
+    grids = []
+    for force, expect in ((N.SORT_CAPACITY, N.WINDOW_CAPACITY), (N.SORT_ONEPASS, None)):
+        ctx = N.Context(0)
+        try:
+            g = np.zeros((dims,) * 3, np.float32)
+            got = _forced_ma(ctx, force, pos, n, W, g, dims, box, masid)
+            assert expect is None or got == expect, (force, got)
+            grids.append(g)
+        finally:
+            ctx.close()
+    assert_cells(grids[0], ref, what='capacity %d %s' % (dims, mas),
+                 nadds=contrib_counts(pos, dims, box, mas))
+    assert np.array_equal(grids[0], grids[1])
+
+
+def test_ma_capacity_adaptive_and_spill_fallback():
+    # Adaptive choice on a 256^3 grid with 256^3 lattice-ordered particles (4096 per
+    # tile): the capacity sort, once the context has seen ordered input.  A call that
+    # spills more than 1/64 of its particles (most particles in one clump, in cell
+    # order) turns it off for that grid and scheme on the context: the next call takes
+    # the one-pass sort.  All grids of the same input are bitwise equal.
+    import ctypes
+    from pylians3_amd import _native as N
+    print('case capacity adaptive', flush=True)
+    box, dims = 500.0, 256
+    lat = _lattice(dims, box, 8)
+    rng = np.random.default_rng(9)
+    clump = ((np.float32(0.5 * box) + rng.normal(0.0, 3.0, (1 << 21, 3))) % box).astype(np.float32)
+    clump = clump[np.lexsort((clump[:, 2], clump[:, 1], clump[:, 0]))]
+    L = N.lib()
+    ctx = N.Context(0)
+    got = ctypes.c_int()
+
+    def run(p):
+        g = np.zeros((dims,) * 3, np.float32)
+        N.check(L.pyl_ma(ctx.handle, p.ctypes.data, p.shape[0], 3, None, g.ctypes.data, dims,
+                         ctypes.c_float(box), N.TSC, 0))
+        N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+        return g, got.value
+
+    try:
+        first, _ = run(lat)
+        for _ in range(2):
+            g, w = run(lat)
+            assert np.array_equal(g, first)
+        assert w == N.WINDOW_CAPACITY
+        c0, w0 = run(clump)
+        assert w0 == N.WINDOW_CAPACITY
+        c1, w1 = run(clump)
+        assert w1 != N.WINDOW_CAPACITY
+        assert np.array_equal(c0, c1)
+        assert abs(c0.astype(np.float64).sum() - clump.shape[0]) <= 1e-6 * clump.shape[0]
+    finally:
+        ctx.close()
+
+
+def test_ma_rsd_capacity_equals_one_pass():
+    # the redshift-space shift is applied once, as the capacity pass reads the particles
+    import ctypes
+    from pylians3_amd import _native as N
+    print('case capacity rsd', flush=True)
+    box, dims, d = 500.0, 256, 200
+    pos = _lattice(d, box, 78)
+    vel = np.random.default_rng(79).normal(0.0, 300.0, pos.shape).astype(np.float32)
+    W = np.random.default_rng(80).random(pos.shape[0]).astype(np.float32)
+    L = N.lib()
+    out = []
+    for force in (N.SORT_CAPACITY, N.SORT_ONEPASS):
+        ctx = N.Context(0)
+        try:
+            N.check(L.pyl_ctx_set_tuning(ctx.handle, N.TUNE_SORT_WINDOW, force))
+            g = np.zeros((dims,) * 3, np.float32)
+            N.check(L.pyl_ma_rsd(ctx.handle, pos.ctypes.data, vel.ctypes.data, pos.shape[0], W.ctypes.data,
+                                 g.ctypes.data, dims, ctypes.c_float(box), N.TSC, ctypes.c_float(100.0),
+                                 ctypes.c_float(0.5), 0, ctypes.c_float(1.0), ctypes.c_float(0.0), 0))
+            got = ctypes.c_int()
+            N.check(L.pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+            assert (got.value == N.WINDOW_CAPACITY) == (force == N.SORT_CAPACITY)
+            out.append(g)
+        finally:
+            ctx.close()
+    assert np.array_equal(out[0], out[1])
+    assert abs(out[0].astype(np.float64).sum() - W.astype(np.float64).sum()) <= 1e-5 * W.sum()
+
+
 @pytest.mark.parametrize('mas', SCHEMES)
 @pytest.mark.parametrize('dims,coord', [(32, 3), (40, 3), (128, 2)])
 def test_ma_split_tiles_vs_oracle(oracle, mas, dims, coord):
