@@ -559,8 +559,8 @@ def serving_leg(N, nreq=120, seed=5):
     1e5-1.3e8, MAS and axis drawn uniformly) against the continuous-batching server.
     Plain form: one request at a time, one slot, arrival order, no fusion; its
     service latency alone, then under Poisson arrivals at 0.9x its capacity.
-    Served form (4 slots, shortest job first with aging, fusion of small
-    requests): the same arrival schedule, then a saturated burst."""
+    Served form (4 slots, admission in arrival order, fusion of small requests
+    waiting together): the same arrival schedule, then a saturated burst."""
     import numpy as np
     import torch
     from pylians3_amd import workloads
@@ -639,8 +639,8 @@ def serving_leg(N, nreq=120, seed=5):
                 'forms': {'plain': '1 slot, arrival order, no fusion, tiled scatter',
                           'plain_atomic': '1 slot, arrival order, no fusion, per-particle '
                                           'atomic scatter (the plain per-call GPU form)',
-                          'served': '4 slots, shortest job first with aging, fusion of '
-                                    'small requests, tiled scatter'}})
+                          'served': '4 slots, admission in arrival order, fusion of '
+                                    'small requests waiting together, tiled scatter'}})
     return out
 
 

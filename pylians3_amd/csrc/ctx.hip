@@ -17,6 +17,11 @@ void set_error(const char *fmt, ...) {
     g_err = buf;
 }
 
+SharedTables &shared_tables() {
+    static SharedTables *t = new SharedTables();   // never destroyed: used until exit
+    return *t;
+}
+
 size_t held_bytes(const Ctx *c) {
     size_t s = 0;
     for (int i = 0; i < BUF_COUNT_; ++i) s += c->bufs[i].cap;
@@ -276,17 +281,6 @@ int pyl_ctx_trim(pyl_ctx *ctx) {
     }
     for (auto &kv : c->fft_plans) destroy_plan(kv.second);
     c->fft_plans.clear();
-    c->mode_plans.clear();
-    for (auto &kv : c->bin_plans) {
-        (void)hipFree(kv.second.d_rows);
-        (void)hipFree(kv.second.d_items);
-        (void)hipFree(kv.second.d_key_items);
-    }
-    c->bin_plans.clear();
-    for (auto &kv : c->twiddles) (void)hipFree(kv.second);
-    c->twiddles.clear();
-    for (auto &kv : c->mas_tabs) (void)hipFree(kv.second);
-    c->mas_tabs.clear();
     for (PinnedBuf *b : {&c->pin_up, &c->pin_down, &c->pin_scales}) {
         if (b->ptr) (void)hipHostFree(b->ptr);
         b->ptr = nullptr;

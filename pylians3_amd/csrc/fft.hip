@@ -310,10 +310,13 @@ bool fft_own_size(int dims) {
     return dims >= 16 && dims <= 1024 && (dims & (dims - 1)) == 0;
 }
 
-// Twiddle table exp(-2 pi i k/N), k < N, resident per N.
+// Twiddle table exp(-2 pi i k/N), k < N, resident per (device, N) for the process.
 int get_twiddles(Ctx *c, int n, const float2 **out) {
-    auto it = c->twiddles.find(n);
-    if (it == c->twiddles.end()) {
+    SharedTables &st = shared_tables();
+    std::lock_guard<std::mutex> lk(st.mu);
+    const auto key = std::make_pair(c->device, n);
+    auto it = st.twiddles.find(key);
+    if (it == st.twiddles.end()) {
         PYL_CHECK(!c->capturing, PYL_EINVAL, "FFT twiddles created while a graph is captured");
         std::vector<float2> h(n);
         for (int k = 0; k < n; ++k) {
@@ -322,8 +325,13 @@ int get_twiddles(Ctx *c, int n, const float2 **out) {
         }
         void *d = nullptr;
         PYL_HIP(hipMalloc(&d, sizeof(float2) * n));
-        PYL_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-        it = c->twiddles.emplace(n, d).first;
+        const hipError_t e = hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            set_error("FFT twiddle upload failed: %s", hipGetErrorString(e));
+            return PYL_EHIP;
+        }
+        it = st.twiddles.emplace(key, d).first;
     }
     *out = (const float2 *)it->second;
     return PYL_OK;

@@ -133,13 +133,15 @@ void compute_mode_plan(int dims, int axis, ModePlan *mp) {
     }
 }
 
-const ModePlan *get_mode_plan(Ctx *c, int dims, int axis) {
-    auto key = std::make_pair(dims, axis);
-    auto it = c->mode_plans.find(key);
-    if (it != c->mode_plans.end()) return &it->second;
+const ModePlan *get_mode_plan(Ctx *, int dims, int axis) {
+    SharedTables &st = shared_tables();
+    std::lock_guard<std::mutex> lk(st.mu);
+    const auto key = std::make_pair(dims, axis);
+    auto it = st.mode.find(key);
+    if (it != st.mode.end()) return &it->second;
     ModePlan mp;
     compute_mode_plan(dims, axis, &mp);
-    return &c->mode_plans.emplace(key, std::move(mp)).first->second;
+    return &st.mode.emplace(key, std::move(mp)).first->second;
 }
 
 namespace {
@@ -190,9 +192,11 @@ int get_bin_plan(Ctx *c, int dims, int axis, int nreal, BinPlan **out) {
     // kRun modes of the item's rows
     const int runs_per_row = (pk_dims(dims).zpad) / kRun;
     R = std::max(1, std::min(R, kFixMaxAdds / runs_per_row));
-    auto key = std::make_tuple(dims, axis, R);
-    auto it = c->bin_plans.find(key);
-    if (it != c->bin_plans.end()) {
+    SharedTables &st = shared_tables();
+    std::lock_guard<std::mutex> lk(st.mu);
+    const auto key = std::make_tuple(c->device, dims, axis, R);
+    auto it = st.bin.find(key);
+    if (it != st.bin.end()) {
         *out = &it->second;
         return PYL_OK;
     }
@@ -244,7 +248,7 @@ int get_bin_plan(Ctx *c, int dims, int axis, int nreal, BinPlan **out) {
     if ((e = hipMemcpy(bp.d_key_items, key_items.data(), key_items.size() * sizeof(int),
                        hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e);
-    *out = &c->bin_plans.emplace(key, bp).first->second;
+    *out = &st.bin.emplace(key, bp).first->second;
     return PYL_OK;
 }
 
@@ -988,11 +992,14 @@ int fft_device(Ctx *c, float *in, float2 *out, int dims, int batch, long long zs
 
 // Binning of F device half-complex fields of nreal realisations -> finalized outputs.
 // Field f of realisation r is dk[f] + r*fstride.
-// Device-resident MAS correction table for (dims, p), built once with the host's libm.
+// Device-resident MAS correction table for (dims, p), built once per device with the
+// host's libm.
 int get_mas_tab(Ctx *c, int dims, int p, const double **out) {
-    auto key = std::make_pair(dims, p);
-    auto it = c->mas_tabs.find(key);
-    if (it == c->mas_tabs.end()) {
+    SharedTables &st = shared_tables();
+    std::lock_guard<std::mutex> lk(st.mu);
+    const auto key = std::make_tuple(c->device, dims, p);
+    auto it = st.mas.find(key);
+    if (it == st.mas.end()) {
         PYL_CHECK(!c->capturing, PYL_EINVAL, "MAS table created while a graph is captured");
         std::vector<double> tab(dims);
         mas_table(dims, p, tab.data());
@@ -1004,7 +1011,7 @@ int get_mas_tab(Ctx *c, int dims, int p, const double **out) {
             set_error("MAS table upload failed: %s", hipGetErrorString(e));
             return PYL_EHIP;
         }
-        it = c->mas_tabs.emplace(key, d).first;
+        it = st.mas.emplace(key, d).first;
     }
     *out = (const double *)it->second;
     return PYL_OK;

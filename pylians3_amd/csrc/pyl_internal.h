@@ -129,10 +129,6 @@ struct Ctx {
     size_t budget = 0;
     DevBuf bufs[BUF_COUNT_];
     std::map<std::tuple<int, int, int>, FFTPlan> fft_plans; // (dims, batch, kind)
-    std::map<std::pair<int, int>, ModePlan> mode_plans;    // (dims, axis)
-    std::map<std::tuple<int, int, int>, BinPlan> bin_plans; // (dims, axis, rows per item)
-    std::map<int, void *> twiddles;                          // exp(-2 pi i k/N), per N
-    std::map<std::pair<int, int>, void *> mas_tabs;          // MAS correction table per (dims, p)
     unsigned long long alloc_gen = 0;                        // device allocations so far
     // The chunk table and tile origins last uploaded into BUF_TILE_CNT, and what they
     // were built from: a call with the same particle ranges and grid reuses them.
@@ -201,6 +197,21 @@ struct Ctx {
     long long ev_count[PYL_STAGE_COUNT] = {};
     bool ev_ready = false;
 };
+
+// Tables that depend only on the grid, built once per process and shared by every
+// context (the device tables once per device), so that the slots of a server or the
+// threads of a pipeline do not each rebuild them: mode counts per (dims, axis),
+// binning plans per (device, dims, axis, rows per item), MAS corrections per (device,
+// dims, p) and FFT twiddles exp(-2 pi i k/N) per (device, N).  Built under `mu` and
+// never changed or freed afterwards (they live until the process ends).
+struct SharedTables {
+    std::mutex mu;
+    std::map<std::pair<int, int>, ModePlan> mode;
+    std::map<std::tuple<int, int, int, int>, BinPlan> bin;
+    std::map<std::tuple<int, int, int>, void *> mas;
+    std::map<std::pair<int, int>, void *> twiddles;
+};
+SharedTables &shared_tables();
 
 size_t held_bytes(const Ctx *c);
 // Looks up this device's address of the fixed MA scratch array (once per device,

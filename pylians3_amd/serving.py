@@ -10,10 +10,13 @@ resident FFT plans and its scratch memory.  A dispatcher admits queued requests
 while a slot is free and the estimated HBM need of all in-flight work stays
 within the budget; requests that arrive while others run are admitted as
 slots and memory free up, and their kernels overlap on the device (one stream
-per slot).  The next request is the shortest waiting job (policy='sjf', work =
-particles + grid nodes) unless one has waited longer than max_wait_s, which
-then goes first; policy='fifo' serves in arrival order.  Small requests with the same (dims, MAS, axis, kind) that are
-waiting together are fused into one batched launch sequence
+per slot).  Requests are admitted in arrival order (policy='fifo', the default:
+at moderate load it gives the lowest tail latency, since a large request then
+starts as soon as a slot frees instead of waiting behind smaller ones); 'sjf'
+takes the shortest waiting job first (work = particles + grid nodes) unless one
+has waited longer than max_wait_s, and 'hrrn' the highest response ratio
+(wait + service) / service.  Small requests with the same (dims, MAS, axis,
+kind) that are waiting together are fused into one batched launch sequence
 (pyl_ma_batch + pyl_pk_batch).  Each result is bitwise the same whether the
 request runs alone, in a batch or alongside other requests: the scatter and
 the binning are order-independent/deterministic.
@@ -60,12 +63,19 @@ class Request:
         """Relative cost: particles deposited plus grid nodes transformed and binned."""
         return self.particles() + 4 * self.nfields * self.dims ** 3
 
+    def service_s(self):
+        """Rough GPU seconds alone: a fixed cost per request, about 30 ps per particle
+        deposited and 10 ps per grid node transformed and binned (MI355X)."""
+        return 3e-4 + 3e-11 * self.particles() + 1e-11 * self.nfields * self.dims ** 3
+
     def hbm_bytes(self):
-        """Estimated device bytes while running (scratch + FFT buffers)."""
+        """Estimated device bytes while running (scratch + FFT buffers): per particle
+        the sorted record (16 B with weights), the bucketed or spill copy (16 B) and the
+        capacity sort's tile slots (3 records)."""
         d = self.dims
         n = self.particles()
         cells = d ** 3
-        return int(48 * n + self.nfields * (4 * cells + 8 * d * d * (d // 2 + 1)) * 2 + 4 * cells
+        return int(80 * n + self.nfields * (4 * cells + 8 * d * d * (d // 2 + 1)) * 2 + 4 * cells
                    + (64 << 20))
 
 
@@ -78,14 +88,14 @@ def _is_small(req):
 
 
 class Server:
-    def __init__(self, hbm_budget=None, slots=4, max_batch=32, device=0, policy='sjf',
+    def __init__(self, hbm_budget=None, slots=4, max_batch=32, device=0, policy='fifo',
                  max_wait_s=2.0, atomic=False):
         import torch
         self.device = device
         # atomic=True: per-particle atomic scatter (the plain GPU form, for comparison)
         self.ma_flags = N.MA_ATOMIC if atomic else 0
-        if policy not in ('sjf', 'fifo'):
-            raise ValueError('policy must be "sjf" or "fifo"')
+        if policy not in ('sjf', 'fifo', 'hrrn'):
+            raise ValueError('policy must be "sjf", "hrrn" or "fifo"')
         self.policy = policy
         self.max_wait_s = float(max_wait_s)
         if hbm_budget is None:
@@ -155,12 +165,17 @@ class Server:
 
     # ------------------------------------------------------------- scheduling
     def _pick(self):
-        """Index of the next request: shortest job first (work ~ particles + grid
-        nodes), except that a request waiting longer than `max_wait_s` goes first
-        (oldest first), so that large requests are not starved."""
+        """Index of the next request: the oldest ('fifo'); the highest response ratio
+        ('hrrn'); or the shortest job (work ~ particles + grid nodes, 'sjf'), except
+        that a request waiting longer than `max_wait_s` goes first (oldest first), so
+        that large requests are not starved."""
         now = time.perf_counter()
         if self.policy == 'fifo':
             return 0
+        if self.policy == 'hrrn':
+            # highest response ratio (wait + service) / service first
+            return max(range(len(self.queue)),
+                       key=lambda i: (now - self.queue[i][0].t_submit) / self.queue[i][0].service_s())
         oldest = min(range(len(self.queue)), key=lambda i: self.queue[i][0].t_submit)
         if now - self.queue[oldest][0].t_submit > self.max_wait_s:
             return oldest

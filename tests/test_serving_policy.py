@@ -1,7 +1,9 @@
 """Dispatch order of the serving loop (host logic only, no GPU).
 
-Shortest job first by work = particles + 4 * fields * dims^3; a request that has
-waited longer than max_wait_s goes first (oldest first); 'fifo' keeps arrival order.
+'fifo' (the default) keeps arrival order; 'sjf' takes the shortest job first by
+work = particles + 4 * fields * dims^3, except that a request that has waited longer
+than max_wait_s goes first (oldest first); 'hrrn' takes the highest ratio
+(wait + service) / service.
 """
 import collections
 import time
@@ -41,3 +43,22 @@ def test_fifo_policy_keeps_arrival_order():
     now = time.perf_counter()
     s.queue = collections.deque([_item(1_000_000, 256, 0.0, now), _item(10, 16, 0.0, now)])
     assert Server._pick(s) == 0
+
+
+def test_hrrn_policy_ratio():
+    s = _Srv()
+    s.policy, s.max_wait_s = 'hrrn', 2.0
+    now = time.perf_counter()
+    big, small = _item(10_000_000, 512, 0.0, now), _item(1000, 64, 0.0, now)
+    s.queue = collections.deque([big, small])
+    # equal waits: the short job has the higher ratio
+    big[0].t_submit = small[0].t_submit = now - 0.001
+    assert Server._pick(s) == 1
+    # the long job has waited many times its own service estimate
+    big[0].t_submit = now - 20 * big[0].service_s()
+    assert Server._pick(s) == 0
+
+
+def test_default_policy_is_arrival_order():
+    import inspect
+    assert inspect.signature(Server.__init__).parameters['policy'].default == 'fifo'
