@@ -285,15 +285,13 @@ __device__ __forceinline__ int isqrt_i(int v) {
 }
 
 // atan2(a, b) on the half-plane b >= |a| (the phase of Pk_library.pyx:358,
-// atan2(real, |delta|)), 0 when b == 0.  t = a/b lies in [-1, 1] and atan(t)
-// is t*P(t^2), P of degree 7 fitted to atan(t)/t on [0, 1] by iteratively
-// reweighted least squares on Chebyshev nodes (max error 1.7e-7 in float,
-// 2.3e-7 relative).
+// atan2(real, |delta|)) for b in [0.5, 2): the caller scales both components by a
+// power of two so that the larger has magnitude in [0.5, 1).  t = a/b (hardware
+// reciprocal, 1 ulp) lies in [-1, 1] and atan(t) is t*P(t^2), P of degree 7 fitted
+// to atan(t)/t on [0, 1] by iteratively reweighted least squares on Chebyshev nodes
+// (max error 1.7e-7 in float, 2.3e-7 relative).
 __device__ __forceinline__ float atan_halfplane(float a, float b) {
-    // hardware reciprocal (1 ulp) for b in the normal range; exact division below
-    float t = 0.0f;
-    if (b >= 0x1p-100f) t = a * __builtin_amdgcn_rcpf(b);
-    else if (b > 0.0f) t = a / b;
+    float t = a * __builtin_amdgcn_rcpf(b);
     t = fminf(fmaxf(t, -1.0f), 1.0f);       // |a| <= b: only rounding can leave [-1, 1]
     const float u = t * t;
     float p = -0.00478044f;
@@ -448,7 +446,8 @@ __global__ void __launch_bounds__(kBinBlock) pk_bin_kernel(BinArgs a) {
             if (AXIS == 0) kpar = kx < 0 ? -kx : kx;
             else if (AXIS == 1) kpar = ky < 0 ? -ky : ky;
             else kpar = kz;
-            const float mu2 = (k2 == 0) ? 0.0f : __fdividef((float)(kpar * kpar), (float)k2);
+            // mu^2 = k_par^2 / k^2 in float (hardware reciprocal, within 2 ulp)
+            const float mu2 = (k2 == 0) ? 0.0f : (float)(kpar * kpar) * __builtin_amdgcn_rcpf((float)k2);
             const float L2 = 1.5f * mu2 - 0.5f;
             const float L4 = (35.0f * mu2 * mu2 - 30.0f * mu2 + 3.0f) * 0.125f;
 
