@@ -99,6 +99,91 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw
     __syncthreads();
 }
 
+// First and last stage of a column FFT straight from and to global memory: the
+// first stage (span 1, radix R, no twiddles) reads its R inputs of butterfly b from
+// base[n*nstride + c] (c clamped to the cw columns present; the copies of the last
+// column in the others are never stored) and writes the LDS block; the last stage
+// reads the LDS block and stores its outputs of the columns c < cw, returning the
+// largest |Re|, |Im| it stored.  Same butterflies and twiddles as stage<>, so the
+// same bits as staging the whole block through LDS.
+template <int N, int C, int CP, int R, int BLK>
+__device__ __forceinline__ void stage_first(const float2 *base, long long nstride, int cw, float2 *buf) {
+    constexpr int NB = (N / R) * C;
+    constexpr int PER = (NB + BLK - 1) / BLK;
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = min((int)threadIdx.x + p * BLK, NB - 1);
+        const int c = b % C, j = b / C;
+#pragma unroll
+        for (int m = 0; m < R; ++m) v[p][m] = base[(long long)(j + m * (N / R)) * nstride + min(c, cw - 1)];
+    }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = threadIdx.x + p * BLK;
+        if (b < NB) {
+            const int c = b % C, j = b / C;
+            dft<R>(v[p]);
+#pragma unroll
+            for (int m = 0; m < R; ++m) buf[(j * R + m) * CP + c] = v[p][m];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int C, int CP, int R, int BLK>
+__device__ __forceinline__ float stage_last(const float2 *buf, const float2 *__restrict__ tw, int Ns,
+                                            float2 *base, long long nstride, int cw) {
+    constexpr int NB = (N / R) * C;
+    constexpr int PER = (NB + BLK - 1) / BLK;
+    float mx = 0.0f;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = threadIdx.x + p * BLK;
+        if (b < NB) {
+            const int c = b % C, j = b / C;
+            const int k = j % Ns;
+            const int tstep = (N / (Ns * R)) * k;
+            float2 v[R];
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                float2 x = buf[(j + m * (N / R)) * CP + c];
+                if (m > 0) x = cmul(x, tw[(m * tstep) & (N - 1)]);
+                v[m] = x;
+            }
+            dft<R>(v);
+            if (c < cw) {
+                const int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int m = 0; m < R; ++m) {
+                    base[(long long)(d + m * Ns) * nstride + c] = v[m];
+                    mx = fmaxf(mx, fmaxf(fabsf(v[m].x), fabsf(v[m].y)));
+                }
+            }
+        }
+    }
+    return mx;
+}
+
+// Column FFTs with the first stage read from and the last stage written to global
+// memory (stage_first, stage_last); radix-8 stages in between, the last stage of
+// radix 8, 4 or 2.
+template <int N, int C, int BLK>
+__device__ __forceinline__ float fft_columns_global(float2 *base, long long nstride, int cw, float2 *buf,
+                                                    const float2 *__restrict__ tw) {
+    constexpr int RL = (N == 16 || N == 128 || N == 1024) ? 2 : (N == 32 || N == 256) ? 4 : 8;
+    stage_first<N, C, C, 8, BLK>(base, nstride, cw, buf);
+    int Ns = 8;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        if (Ns * 8 * RL <= N) {
+            stage<N, C, C, 8, BLK>(buf, tw, Ns);
+            Ns *= 8;
+        }
+    }
+    return stage_last<N, C, C, RL, BLK>(buf, tw, Ns, base, nstride, cw);
+}
+
 template <int N, int C, int CP, int BLK>
 __device__ __forceinline__ void fft_columns(float2 *buf, const float2 *__restrict__ tw) {
     int Ns = 1;
@@ -137,8 +222,7 @@ template <int N, int C, int BLK>
 __global__ void __launch_bounds__(BLK)
 fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, long long bstride,
                int zlen, int nother, const float2 *__restrict__ tw, unsigned *__restrict__ amax) {
-    constexpr int PER = N * C / BLK;
-    static_assert(PER * BLK == N * C, "block shape");
+    static_assert((N * C) % BLK == 0, "block shape");
     __shared__ float2 buf[N * C];
     // Workgroups are dealt to the 8 XCDs round-robin by launch index, and each
     // XCD has its own L2.  Rows are 8*zlen bytes long, so a group's 128-byte
@@ -155,32 +239,7 @@ fft_col_kernel(float2 *__restrict__ data, long long nstride, long long ostride, 
     const int z0 = (int)zg * C;
     const int cw = min(C, zlen - z0);
     float2 *base = data + (long long)b * bstride + (long long)line * ostride + z0;
-    // all loads issued before any is used (clamped column index: always in range)
-    float2 v[PER];
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-        const int t = threadIdx.x + p * BLK;
-        const int c = t % C, n = t / C;
-        v[p] = base[(long long)n * nstride + min(c, cw - 1)];
-    }
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-        const int t = threadIdx.x + p * BLK;
-        buf[t] = (t % C < cw) ? v[p] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_columns<N, C, C, BLK>(buf, tw);
-    float m = 0.0f;
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-        const int t = threadIdx.x + p * BLK;
-        const int c = t % C, n = t / C;
-        if (c < cw) {
-            const float2 x = buf[t];
-            base[(long long)n * nstride + c] = x;
-            m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
-        }
-    }
+    const float m = fft_columns_global<N, C, BLK>(base, nstride, cw, buf, tw);
     if (amax) block_absmax(m, amax, b * 64u + (L & 63u));
 }
 
