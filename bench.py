@@ -297,9 +297,9 @@ def cpu_check_errors(pos_host, grid_gpu, pk_gpu, dims, box, mas_name, axis, mean
 def random_order_leg(N, pos, dims, box, mas, reps=5):
     """The timed step's particles in random order (a seeded permutation): one MA per
     sort form, forced on a context of its own -- the one-pass sort with the grid's
-    default window, the wide window, the two-level sort -- then the adaptive
-    choice (the first call takes one pass and tallies its box misses; the next calls
-    take the two-level sort).  Median of `reps` timed MAs after one warm-up (HIP
+    default window, the wide window, the two-level sort, the two-level sort with the
+    capacity form as its second pass -- then the adaptive choice (the first call takes
+    one pass and tallies its box misses; the next calls take the two-level sort).  Median of `reps` timed MAs after one warm-up (HIP
     events around each stage), and whether every form gave the same bits."""
     import numpy as np
     import torch
@@ -316,7 +316,8 @@ def random_order_leg(N, pos, dims, box, mas, reps=5):
            'forms': {}}
     same = True
     for name, force in (('one_pass', N.SORT_ONEPASS), ('wide_window', N.SORT_WIDE),
-                        ('two_level', N.SORT_TWOLEVEL), ('adaptive', -1)):
+                        ('two_level', N.SORT_TWOLEVEL),
+                        ('two_level_capacity', N.SORT_TWOLEVEL_CAPACITY), ('adaptive', -1)):
         ctx = N.Context(0)
         N.check(L.pyl_ctx_set_tuning(ctx.handle, N.TUNE_SORT_WINDOW, force))
 
@@ -343,7 +344,8 @@ def random_order_leg(N, pos, dims, box, mas, reps=5):
         else:
             same = same and bool(torch.equal(grid, ref))
         names = {N.WINDOW_DIRECT: 'direct', N.WINDOW_BOX: 'box', N.WINDOW_LINEAR: 'linear',
-                 N.WINDOW_WIDE: 'wide', N.WINDOW_TWOLEVEL: 'two_level'}
+                 N.WINDOW_WIDE: 'wide', N.WINDOW_TWOLEVEL: 'two_level', N.WINDOW_CAPACITY: 'capacity',
+                 N.WINDOW_TWOLEVEL_CAPACITY: 'two_level_capacity'}
         out['forms'][name] = {
             'ma_ms_median': round(1e3 * float(np.median(t)), 3),
             'form_taken': names.get(form.value, form.value),
@@ -356,6 +358,8 @@ def random_order_leg(N, pos, dims, box, mas, reps=5):
                                                  f['two_level']['ma_ms_median'], 2)
     out['two_level_speedup_vs_wide'] = round(f['wide_window']['ma_ms_median'] /
                                              f['two_level']['ma_ms_median'], 2)
+    out['two_level_capacity_speedup_vs_two_level'] = round(f['two_level']['ma_ms_median'] /
+                                                           f['two_level_capacity']['ma_ms_median'], 2)
     del sp, grid, ref
     torch.cuda.empty_cache()
     return out

@@ -319,15 +319,16 @@ int pyl_graph_destroy(pyl_graph *graph);
                                       3 the capacity sort (one read of the particles into
                                       per-tile slots of 3x the mean occupancy, the rest
                                       sorted after; one 3D grid whose sort window is every
-                                      tile or the 8x8-column box); -1 the adaptive choice: per
-                                      (dims, MAS), when the previous call found more than a
-                                      quarter of its particles outside their sort chunk's
-                                      8x8-column box, the two-level sort; else the capacity
-                                      sort when the tiles hold 256 particles or more on
-                                      average and no capacity call of the context on such a
-                                      grid spilled more than 1/64 of its particles; else the
-                                      default.  A form that does not apply falls back to the
-                                      default. */
+                                      tile or the 8x8-column box); 4 the two-level sort with
+                                      the capacity form as its second pass; -1 the adaptive
+                                      choice: per (dims, MAS), when the previous call found
+                                      more than a quarter of its particles outside their
+                                      sort chunk's 8x8-column box, the two-level sort, else
+                                      the one-pass sort; either with the capacity form when
+                                      the tiles hold 256 particles or more on average and no
+                                      capacity call of the context on such a grid spilled
+                                      more than 1/64 of its particles.  A form that does not
+                                      apply falls back to the default. */
 int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value);
 /* Sort window of the context's latest tiled MA (-1 none yet). */
 #define PYL_WINDOW_DIRECT 0        /* every tile of the realisation counted in LDS */
@@ -336,6 +337,7 @@ int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value);
 #define PYL_WINDOW_WIDE   3        /* every tile (up to 32768) in 128 KB of LDS */
 #define PYL_WINDOW_TWOLEVEL 4      /* buckets of 8x8x4 tiles, then the tiles of each bucket */
 #define PYL_WINDOW_CAPACITY 5      /* per-tile slots filled in one pass, then the spill sorted */
+#define PYL_WINDOW_TWOLEVEL_CAPACITY 6  /* buckets, then the capacity form within each bucket */
 int pyl_ctx_last_sort_window(pyl_ctx *ctx, int *window);
 
 #ifdef __cplusplus

@@ -25,8 +25,10 @@ PK_DESTROY_INPUT = 0x0040
 PK_NO_PHASE = 0x0080
 PK_SCALES_DEVICE = 0x0100
 TUNE_BIN_ROWS, TUNE_DEPOSIT_ORDER, TUNE_SORT_WINDOW = 1, 2, 3
-WINDOW_DIRECT, WINDOW_BOX, WINDOW_LINEAR, WINDOW_WIDE, WINDOW_TWOLEVEL, WINDOW_CAPACITY = 0, 1, 2, 3, 4, 5
-SORT_ONEPASS, SORT_WIDE, SORT_TWOLEVEL, SORT_CAPACITY = 0, 1, 2, 3   # TUNE_SORT_WINDOW (-1 adaptive)
+WINDOW_DIRECT, WINDOW_BOX, WINDOW_LINEAR, WINDOW_WIDE, WINDOW_TWOLEVEL = 0, 1, 2, 3, 4
+WINDOW_CAPACITY, WINDOW_TWOLEVEL_CAPACITY = 5, 6
+# values of TUNE_SORT_WINDOW (-1 adaptive)
+SORT_ONEPASS, SORT_WIDE, SORT_TWOLEVEL, SORT_CAPACITY, SORT_TWOLEVEL_CAPACITY = 0, 1, 2, 3, 4
 
 STAGES = ['ma_count', 'ma_scan', 'ma_scatter', 'ma_deposit', 'ma_atomic', 'pk_prep', 'pk_fft',
           'pk_bin', 'ma_final', 'ma_bucket', 'ma_spill']

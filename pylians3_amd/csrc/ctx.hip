@@ -502,7 +502,7 @@ int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value) {
             c->tune_order = value;
             return PYL_OK;
         case PYL_TUNE_SORT_WINDOW:
-            PYL_CHECK(value >= -1 && value <= 3, PYL_EINVAL, "sort window must be -1, 0, 1, 2 or 3");
+            PYL_CHECK(value >= -1 && value <= 4, PYL_EINVAL, "sort window must be -1 .. 4");
             c->tune_window = value;
             return PYL_OK;
     }

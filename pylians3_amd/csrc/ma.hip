@@ -1129,6 +1129,144 @@ staged_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__res
     }
 }
 
+// Capacity form of the two-level sort's second pass.  Every chunk lies in one
+// bucket (the block window), but its records are in random tile order, so they are
+// staged like staged_scatter_kernel's: a sub-chunk of STAGE records is ranked per
+// tile in LDS, placed in LDS in tile order, claims the run of tile slots it needs in
+// each tile with one global atomic on the tile's cursor, and is written by
+// consecutive lanes to consecutive slots.  Records past a full tile's capacity go to
+// the spill list (one atomic per wave); there are no misses (pass 1 put every record
+// in its chunk's bucket).  Records are read as (dist_x, dist_y, dist_z) | w.
+template <int MAS, bool POW2, bool WT, int STAGE>
+__global__ void __launch_bounds__(kSortBlock)
+staged_cap_scatter_kernel(const float *__restrict__ bxyz, const float *__restrict__ bw, TileGeom g,
+                          const Chunk *__restrict__ chunks, unsigned *__restrict__ cursor, unsigned cap,
+                          float *__restrict__ crec, float *__restrict__ sxyz, float *__restrict__ sw,
+                          unsigned long long *__restrict__ nspill, unsigned long long *__restrict__ placed,
+                          unsigned *__restrict__ wmax_bits) {
+    constexpr int NS = kBktSlots;
+    constexpr int kPer = STAGE / kSortBlock;
+    __shared__ float sx[STAGE], sy[STAGE], sz[STAGE];
+    __shared__ float ssw[WT ? STAGE : 1];
+    __shared__ unsigned short sslot[STAGE];
+    __shared__ unsigned hist[NS], lofs[NS], grem[NS];
+    __shared__ unsigned long long gbase[NS];
+    __shared__ unsigned wsum[kSortBlock / 64 + 1];
+    const Chunk ch = chunks[blockIdx.x];
+    const int t = threadIdx.x, lane = t & 63;
+    const RSD off{bxyz, 0.0f, 0.0f, 0, 3, 0};
+    int3 t0 = make_int3(0, 0, 0);
+    if (ch.c1 > ch.c0)
+        t0 = dist_tiles<MAS, 3, POW2>(load_dist<3>(bxyz, off, ch.c0 + (ch.c1 - ch.c0) / 2, 1.0f), g);
+    const Win win = make_win(g, t0, 0, 0);                      // g.wmode == kWinBlock
+    unsigned m = 0, np = 0;
+    for (long long s0 = ch.c0; s0 < ch.c1; s0 += STAGE) {
+        const int cnt = (int)min((long long)STAGE, ch.c1 - s0);
+        for (int s = t; s < NS; s += kSortBlock) hist[s] = 0u;
+        __syncthreads();
+        float3 d[kPer];
+        float w[kPer];
+        int sl[kPer];
+        unsigned rk[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const long long i = s0 + min(t + k * kSortBlock, cnt - 1);
+            d[k] = load_dist<3>(bxyz, off, i, 1.0f);
+            w[k] = WT ? bw[i] : 1.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            sl[k] = -1;
+            rk[k] = 0u;
+            if (t + k * kSortBlock < cnt) {
+                const int3 tt = dist_tiles<MAS, 3, POW2>(d[k], g);
+                const unsigned s = win_slot(tt, tiles_key(tt, g, 0), win, g, 0);
+                if (s < (unsigned)NS) {
+                    sl[k] = (int)s;
+                    rk[k] = atomicAdd(&hist[s], 1u);
+                }
+                if (WT) {
+                    const unsigned b = __float_as_uint(fabsf(w[k]));
+                    if (b < 0x7f800000u) m = max(m, b);
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned nst = stage_scan<NS>(hist, lofs, NS, wsum);
+        for (int s = t; s < NS; s += kSortBlock) {
+            const unsigned h = hist[s];
+            if (h) {
+                const unsigned long long key = (unsigned long long)win_key(s, win, g, 0);
+                const unsigned claim = atomicAdd(&cursor[key], h);
+                gbase[s] = key * cap + claim;
+                grem[s] = claim < cap ? cap - claim : 0u;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (sl[k] >= 0) {
+                const unsigned q = lofs[sl[k]] + rk[k];
+                sx[q] = d[k].x;
+                sy[q] = d[k].y;
+                sz[q] = d[k].z;
+                if (WT) ssw[q] = w[k];
+                sslot[q] = (unsigned short)sl[k];
+            }
+        }
+        __syncthreads();
+        for (int j = t; j < (int)nst; j += kSortBlock) {
+            const int s = sslot[j];
+            const unsigned r = (unsigned)j - lofs[s];
+            const bool spill = r >= grem[s];
+            if (!spill) {
+                const unsigned long long at = gbase[s] + r;
+                if (WT) {
+                    reinterpret_cast<float4 *>(crec)[at] = make_float4(sx[j], sy[j], sz[j], ssw[j]);
+                } else {
+                    crec[3 * at] = sx[j];
+                    crec[3 * at + 1] = sy[j];
+                    crec[3 * at + 2] = sz[j];
+                }
+                ++np;
+            }
+            const unsigned long long bal = __ballot(spill);
+            if (bal) {
+                const int first = __ffsll((long long)bal) - 1;
+                unsigned long long b0 = 0;
+                if (lane == first) b0 = atomicAdd(nspill, (unsigned long long)__popcll(bal));
+                b0 = (unsigned long long)__shfl((long long)b0, first);
+                if (spill) {
+                    const unsigned long long at = b0 + (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
+                    sxyz[3 * at] = sx[j];
+                    sxyz[3 * at + 1] = sy[j];
+                    sxyz[3 * at + 2] = sz[j];
+                    if (WT) sw[at] = ssw[j];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m = max(m, (unsigned)__shfl_xor((int)m, o));
+        np += (unsigned)__shfl_xor((int)np, o);
+    }
+    if (lane == 0) {
+        wsum[t >> 6] = m;
+        lofs[t >> 6] = np;
+    }
+    __syncthreads();
+    if (t == 0) {
+        unsigned mm = 0;
+        unsigned long long pl = 0;
+        for (int k = 0; k < kSortBlock / 64; ++k) {
+            mm = max(mm, wsum[k]);
+            pl += lofs[k];
+        }
+        if (WT) atomicMax(&wmax_bits[0], mm);
+        if (pl) atomicAdd(placed, pl);
+    }
+}
+
 // ---- deposit -----------------------------------------------------------------
 // Records of the capacity sort (cap_scatter_kernel): tile `key` has min(count[key],
 // cap) records at rec + key*cap; rec == nullptr: none (the sorted records alone).
@@ -2343,27 +2481,42 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const bool cap_ok = axes == 3 && nreal == 1 && !devt && !c->arena && !c->capturing && nsplit == 1 &&
                         (g.wmode == kWinDirect || g.wmode == kWinBox) && n > 0 && n < (1ll << 32);
     const size_t rec_bytes = W ? sizeof(float4) : 3 * sizeof(float);
+    // (the capacity form also serves the two-level sort's second pass: its chunks lie
+    // in one bucket, so only full tiles spill)
+    const bool dense = n >= kCapMinMean * g.ntiles;
+    const auto cap_good = [&]() {
+        const auto it = c->cap_bad.find(hkey);
+        return it == c->cap_bad.end() || !it->second;
+    };
     bool wide = false, two = false, capm = false;
     if (c->tune_window == 1) wide = wide_ok;
     else if (c->tune_window == 2) two = two_ok;
     else if (c->tune_window == 3) capm = cap_ok;
+    else if (c->tune_window == 4) capm = two = two_ok;
     else if (c->tune_window < 0 && tally && shuffled) {
         two = two_ok;
         wide = !two && wide_ok;          // batches of host ranges, fixed scratch
-    } else if (c->tune_window < 0 && tally && cap_ok && n >= kCapMinMean * g.ntiles) {
-        const auto it = c->cap_bad.find(hkey);
-        capm = it == c->cap_bad.end() || !it->second;
+        capm = two && dense && cap_good();
+    } else if (c->tune_window < 0 && tally && cap_ok && dense) {
+        capm = cap_good();
     }
     // tile capacity: kCapFactor times the mean, in whole 64-record runs
     const unsigned long long cap64 =
         capm ? ((unsigned long long)(kCapFactor * n + g.ntiles - 1) / g.ntiles + 63) / 64 * 64 : 0;
-    void *cv = nullptr;
+    void *cv = nullptr, *sv = nullptr;
+    // the spill list: n (dist_x, dist_y, dist_z) | n weights, the weights 256-byte
+    // aligned (in the bucketed-record buffer, or a buffer of its own when that holds
+    // the two-level sort's first pass)
+    const size_t pxyz_bytes = ((size_t)n * 3 * sizeof(float) + 255) & ~(size_t)255;
+    const size_t spill_bytes = pxyz_bytes + (W ? (size_t)n * sizeof(float) : 0);
     if (capm) {
         const size_t bytes = (size_t)g.ntiles * cap64 * rec_bytes;
-        // within the context's budget, else the one-pass sort
+        const size_t extra = two ? spill_bytes - std::min(spill_bytes, (size_t)c->bufs[BUF_SPILL].cap) : 0;
+        // within the context's budget, else the form without it
         capm = cap64 < (1ull << 31) &&
-               held_bytes(c) - c->bufs[BUF_CAPREC].cap + bytes + (bytes >> 4) <= c->budget &&
-               take(c, BUF_CAPREC, bytes, &cv) == PYL_OK;
+               held_bytes(c) - c->bufs[BUF_CAPREC].cap + bytes + (bytes >> 4) + extra <= c->budget &&
+               take(c, BUF_CAPREC, bytes, &cv) == PYL_OK &&
+               (!two || take(c, BUF_SPILL, spill_bytes, &sv) == PYL_OK);
     }
     const unsigned cap = (unsigned)cap64;
     if (wide) g.wmode = kWinDirect;
@@ -2376,7 +2529,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         sws = kWinWide;
     }
     // kWinDirect/Box/Linear == PYL_WINDOW_DIRECT/BOX/LINEAR
-    c->last_window = capm ? PYL_WINDOW_CAPACITY : two ? PYL_WINDOW_TWOLEVEL : wide ? PYL_WINDOW_WIDE : g.wmode;
+    c->last_window = (capm && two) ? PYL_WINDOW_TWOLEVEL_CAPACITY
+                   : capm ? PYL_WINDOW_CAPACITY : two ? PYL_WINDOW_TWOLEVEL : wide ? PYL_WINDOW_WIDE : g.wmode;
     const int ws = capm ? sws : two ? kBktSlots : wide ? kWinWide : kWin;
     // Chunks of consecutive particles, never straddling two realisations.
     const long long target = wide ? std::max(2 * g.ntiles, std::min(16 * g.ntiles, (n + 2047) / 2048))
@@ -2397,7 +2551,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const int nchunk2 = (int)nchunk2_ll;
     // chunks of the tile sort (pass 2 of the two-level sort, the capacity form's spill
     // sort, or the only pass)
-    const int nsort = two ? nchunk2 : capm ? kSpillChunks : nchunk;
+    const int nsort = capm ? kSpillChunks : two ? nchunk2 : nchunk;
     // tile buffer: counts[nt] | ocounts[nt] | ocursors[nt] | wmax[nreal] | box_miss |
     //              (two-level) bucket counts[nbk] | pad | offsets[nt+1] | block sums |
     //              chunk table | tile origins | tile order | per-chunk window bases |
@@ -2416,7 +2570,8 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const size_t ord_bytes = ((size_t)nt * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t dump_bytes = ((size_t)std::max(nsort, 1) * ws * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t boff_bytes = two ? (((size_t)nbk + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255 : 0;
-    const size_t ch2_bytes = (two || capm) ? ((size_t)(two ? nchunk2 : kSpillChunks) * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
+    const size_t ch2_bytes = two ? ((size_t)nchunk2 * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
+    const size_t chs_bytes = capm ? ((size_t)kSpillChunks * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
     const size_t bdump_bytes = two ? ((size_t)std::max(nchunk, 1) * nbk * sizeof(unsigned) + 255) & ~(size_t)255 : 0;
     // Largest tiles first, when there are enough tiles, and records per tile, for the
     // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys).
@@ -2428,12 +2583,13 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     // all scratch of the call is taken before the first launch
     void *tbv = nullptr, *pv = nullptr, *rv = nullptr, *bv = nullptr, *kv = nullptr;
     PYL_TRY(take(c, BUF_TILE_CNT, cnt_bytes + off_bytes + bs_bytes + ch_bytes + org_bytes +
-                                      ord_bytes + dump_bytes + boff_bytes + ch2_bytes + 2 * bdump_bytes, &tbv));
+                                      ord_bytes + dump_bytes + boff_bytes + ch2_bytes + 2 * bdump_bytes +
+                                      chs_bytes, &tbv));
     PYL_TRY(take(c, BUF_PARTIAL, (size_t)nt * g.pslot * sizeof(float), &pv));
     PYL_TRY(take(c, BUF_RECORDS, (size_t)(n ? n : 1) * rec_bytes, &rv));
-    // bucketed records (or the capacity form's spill list): n (dist_x, dist_y, dist_z) |
-    // n weights, the weights 256-byte aligned
-    const size_t bxyz_bytes = ((size_t)n * 3 * sizeof(float) + 255) & ~(size_t)255;
+    // bucketed records (or the one-pass capacity form's spill list): n (dist_x, dist_y,
+    // dist_z) | n weights, the weights 256-byte aligned
+    const size_t bxyz_bytes = pxyz_bytes;
     if (two || capm) PYL_TRY(take(c, BUF_BUCKETED, bxyz_bytes + (W ? (size_t)n * sizeof(float) : 0), &kv));
     if (nsplit > 1)
         PYL_TRY(take(c, BUF_TILE_BOX, (size_t)nt * nbox * sizeof(unsigned long long), &bv));
@@ -2455,11 +2611,15 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     Chunk *dchunks2 = (Chunk *)(tb2 + boff_bytes);
     unsigned *bdump = (unsigned *)(tb2 + boff_bytes + ch2_bytes);
     unsigned *bmat = (unsigned *)(tb2 + boff_bytes + ch2_bytes + bdump_bytes);
+    Chunk *dchunksS = (Chunk *)(tb2 + boff_bytes + ch2_bytes + 2 * bdump_bytes);   // spill chunks
     float *bxyz = (float *)kv;
     float *bw = ((two || capm) && W) ? (float *)((char *)kv + bxyz_bytes) : nullptr;
     unsigned *cursor = counts + cur_at;
     unsigned long long *capw = (unsigned long long *)(counts + capw_at);
     float *crec = (float *)cv;
+    float *pxyz = two ? (float *)sv : bxyz;
+    float *pw = W ? (two ? (float *)((char *)sv + pxyz_bytes) : bw) : nullptr;
+    const int nsrc = two ? nchunk2 : nchunk;          // chunks the capacity scatter reads
     g.orig = dorig;
     float *partials = (float *)pv;
     float *records = (float *)rv;
@@ -2528,55 +2688,6 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     RSD srsd = rsd;
     float sinv = inv;
     Chunk *schunks = dchunks;
-    if (capm) {
-        // one read of the particles into the tile slots; the rest into the spill list
-        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
-        auto kcs = W ? (pow2 ? cap_scatter_kernel<MAS, true, true> : cap_scatter_kernel<MAS, false, true>)
-                     : (pow2 ? cap_scatter_kernel<MAS, true, false> : cap_scatter_kernel<MAS, false, false>);
-        const int slots = g.wmode == kWinBox ? kBoxW * kBoxW * g.ntz : (int)g.ntiles;
-        const size_t lds = 2 * (size_t)slots * sizeof(unsigned);
-        if (lds > 65536)
-            PYL_HIP(hipFuncSetAttribute((const void *)kcs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        kcs<<<nchunk, kSortBlock, lds, c->stream>>>(pos, rsd, W, inv, g, dchunks, cursor, cap, crec, bxyz, bw,
-                                                    capw + 1, capw + 2, wmax, tally ? box_miss : nullptr);
-        PYL_HIP(hipGetLastError());
-        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
-        // the spill list (capw[0..1] = {0, count}) sorted by the regular passes, in
-        // kSpillChunks chunks cut on the device (empty ones when nothing spilled)
-        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SPILL));
-        chunk_table_dev_kernel<<<ceil_div(std::max(kSpillChunks, g.nt + 1), kBlock), kBlock, 0, c->stream>>>(
-            dchunks2, (const long long *)capw, 1, kSpillChunks, dorig, g.nt, dims);
-        const RSD off{bxyz, 0.0f, 0.0f, 0, 3, 0};
-        auto kc = sws == kWinWide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
-                                          : tile_count_kernel<MAS, 3, false, kWinWide>)
-                                  : (pow2 ? tile_count_kernel<MAS, 3, true, kWin>
-                                          : tile_count_kernel<MAS, 3, false, kWin>);
-        kc<<<kSpillChunks, kSortBlock, 0, c->stream>>>(bxyz, off, 1.0f, gs, dchunks2, counts, ocounts, dump,
-                                                       nullptr);
-        scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums);
-        scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
-        scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums, offsets);
-        auto ks = sws == kWinWide
-            ? (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWinWide>
-                         : tile_scatter_kernel<MAS, 3, false, true, kWinWide>)
-                 : (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWinWide>
-                         : tile_scatter_kernel<MAS, 3, false, false, kWinWide>))
-            : (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWin>
-                         : tile_scatter_kernel<MAS, 3, false, true, kWin>)
-                 : (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWin>
-                         : tile_scatter_kernel<MAS, 3, false, false, kWin>));
-        ks<<<kSpillChunks, kSortBlock, 0, c->stream>>>(bxyz, off, bw, 1.0f, gs, dchunks2, dump, offsets, counts,
-                                                       ocursors, wmax, records, (unsigned long long)n);
-        PYL_HIP(hipGetLastError());
-        PYL_TRY(stage_end(c, PYL_STAGE_MA_SPILL));
-        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
-        if (ordered)
-            tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order,
-                                                                CapRecs{crec, cursor, cap, (long long)n});
-        add_u64_kernel<<<1, 64, 0, c->stream>>>(offsets + nt, capw + 2, capw + 3);   // record total
-        PYL_HIP(hipGetLastError());
-        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
-    }
     if (two) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_BUCKET));
         auto kb = pow2 ? bucket_count_kernel<MAS, true> : bucket_count_kernel<MAS, false>;
@@ -2606,6 +2717,68 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         sinv = 1.0f;
         schunks = dchunks2;
         g.wmode = kWinBlock;
+    }
+    if (capm) {
+        // one read of the particles (or, after the two-level sort's first pass, of the
+        // bucketed records) into the tile slots; the rest into the spill list
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCATTER));
+        auto kcs = W ? (pow2 ? cap_scatter_kernel<MAS, true, true> : cap_scatter_kernel<MAS, false, true>)
+                     : (pow2 ? cap_scatter_kernel<MAS, true, false> : cap_scatter_kernel<MAS, false, false>);
+        const int slots = g.wmode == kWinBlock ? kBktSlots
+                        : g.wmode == kWinBox   ? kBoxW * kBoxW * g.ntz : (int)g.ntiles;
+        const size_t lds = 2 * (size_t)slots * sizeof(unsigned);
+        if (lds > 65536)
+            PYL_HIP(hipFuncSetAttribute((const void *)kcs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (two) {
+            // records of one bucket per chunk, in random tile order: staged writes
+            auto kss = W ? (pow2 ? staged_cap_scatter_kernel<MAS, true, true, kStage2>
+                                 : staged_cap_scatter_kernel<MAS, false, true, kStage2>)
+                         : (pow2 ? staged_cap_scatter_kernel<MAS, true, false, kStage2>
+                                 : staged_cap_scatter_kernel<MAS, false, false, kStage2>);
+            if (nsrc > 0)
+                kss<<<nsrc, kSortBlock, 0, c->stream>>>(bxyz, bw, g, schunks, cursor, cap, crec, pxyz, pw,
+                                                        capw + 1, capw + 2, wmax);
+        } else if (nsrc > 0) {
+            kcs<<<nsrc, kSortBlock, lds, c->stream>>>(spos, srsd, sW, sinv, g, schunks, cursor, cap, crec, pxyz,
+                                                      pw, capw + 1, capw + 2, wmax, tally ? box_miss : nullptr);
+        }
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
+        // the spill list (capw[0..1] = {0, count}) sorted by the regular passes, in
+        // kSpillChunks chunks cut on the device (empty ones when nothing spilled)
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SPILL));
+        chunk_table_dev_kernel<<<ceil_div(std::max(kSpillChunks, g.nt + 1), kBlock), kBlock, 0, c->stream>>>(
+            dchunksS, (const long long *)capw, 1, kSpillChunks, dorig, g.nt, dims);
+        const RSD off{pxyz, 0.0f, 0.0f, 0, 3, 0};
+        auto kc = sws == kWinWide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
+                                          : tile_count_kernel<MAS, 3, false, kWinWide>)
+                                  : (pow2 ? tile_count_kernel<MAS, 3, true, kWin>
+                                          : tile_count_kernel<MAS, 3, false, kWin>);
+        kc<<<kSpillChunks, kSortBlock, 0, c->stream>>>(pxyz, off, 1.0f, gs, dchunksS, counts, ocounts, dump,
+                                                       nullptr);
+        scan_partial_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums);
+        scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
+        scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums, offsets);
+        auto ks = sws == kWinWide
+            ? (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWinWide>
+                         : tile_scatter_kernel<MAS, 3, false, true, kWinWide>)
+                 : (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWinWide>
+                         : tile_scatter_kernel<MAS, 3, false, false, kWinWide>))
+            : (W ? (pow2 ? tile_scatter_kernel<MAS, 3, true, true, kWin>
+                         : tile_scatter_kernel<MAS, 3, false, true, kWin>)
+                 : (pow2 ? tile_scatter_kernel<MAS, 3, true, false, kWin>
+                         : tile_scatter_kernel<MAS, 3, false, false, kWin>));
+        ks<<<kSpillChunks, kSortBlock, 0, c->stream>>>(pxyz, off, pw, 1.0f, gs, dchunksS, dump, offsets, counts,
+                                                       ocursors, wmax, records, (unsigned long long)n);
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SPILL));
+        PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
+        if (ordered)
+            tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order,
+                                                                CapRecs{crec, cursor, cap, (long long)n});
+        add_u64_kernel<<<1, 64, 0, c->stream>>>(offsets + nt, capw + 2, capw + 3);   // record total
+        PYL_HIP(hipGetLastError());
+        PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
     }
     if (nsort > 0 && !capm) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_COUNT));

@@ -118,6 +118,7 @@ enum BufId {
     BUF_BUCKETED,    // records of the two-level sort's first pass
     BUF_PK_AMAX,     // per-cube largest |Re|, |Im| of the transforms being binned
     BUF_CAPREC,      // per-tile record slots of the capacity sort
+    BUF_SPILL,       // its spill list, when the bucketed records hold the input
     BUF_COUNT_
 };
 
@@ -182,7 +183,8 @@ struct Ctx {
     // Launch parameters a tuner may set (0: the built-in choice).
     int tune_bin_rows = 0;           // binning rows per work item
     int tune_order = -1;             // largest-first deposit order: -1 auto, 0 off, 1 on
-    int tune_window = -1;            // MA sort: -1 adaptive, 0 one-pass, 1 wide, 2 two-level, 3 capacity
+    int tune_window = -1;            // MA sort: -1 adaptive, 0 one-pass, 1 wide, 2 two-level, 3 capacity,
+                                     // 4 two-level with a capacity second pass
     // Stage timing: a ring of event pairs per stage, so that timing many calls
     // needs no host synchronisation between them.  A pair is read back when its
     // ring slot comes round again (kEvRing calls later) or on request.

@@ -461,6 +461,58 @@ def test_ma_capacity_sort_vs_oracle(oracle, dims, d, mas, weighted, order):
     assert np.array_equal(grids[0], grids[1])
 
 
+@pytest.mark.parametrize('dims,d,mas,weighted,clump', [
+    (256, 256, 'TSC', False, False),           # 16 buckets, 4096 particles per tile
+    (512, 256, 'CIC', True, True),             # 1024 buckets; clumped tiles overflow
+    (400, 192, 'PCS', False, True),            # unequal tiles, partial buckets at the edges
+])
+def test_ma_two_level_capacity_vs_oracle(oracle, dims, d, mas, weighted, clump):
+    # The two-level sort with the capacity form as its second pass: random-order
+    # particles go into buckets of 8 x 8 x 4 tiles, then each bucket's records into
+    # per-tile slots (only full tiles spill).  Forced on a context of its own: every
+    # cell matches the oracle and equals the one-pass sort bitwise.  Then the adaptive
+    # choice on such input (at least 256 particles per tile): two-level with capacity,
+    # or, once a clumped input has spilled more than 1/64 of its particles, without.
+    import ctypes
+    from pylians3_amd import MAS_library as MASL, _native as N
+    print('case two-level capacity', dims, d, mas, weighted, clump, flush=True)
+    box = 600.0
+    pos = _clumped_lattice(d, box, 70 + d) if clump else _lattice(d, box, 70 + d)
+    pos = np.ascontiguousarray(pos[np.random.default_rng(4).permutation(pos.shape[0])])
+    n = pos.shape[0]
+    W = np.random.default_rng(d + 1).lognormal(0.0, 1.0, n).astype(np.float32) if weighted else None
+    masid = MASL._MAS_IDS[mas]
+    ref = np.zeros((dims,) * 3, np.float32)
+    oracle.ma_f32(pos, ref, box, mas, W)
+    grids = []
+    for force, expect in ((N.SORT_TWOLEVEL_CAPACITY, N.WINDOW_TWOLEVEL_CAPACITY), (N.SORT_ONEPASS, None)):
+        ctx = N.Context(0)
+        try:
+            g = np.zeros((dims,) * 3, np.float32)
+            got = _forced_ma(ctx, force, pos, n, W, g, dims, box, masid)
+            assert expect is None or got == expect, (force, got)
+            grids.append(g)
+        finally:
+            ctx.close()
+    assert_cells(grids[0], ref, what='two-level capacity %d %s' % (dims, mas),
+                 nadds=contrib_counts(pos, dims, box, mas))
+    assert np.array_equal(grids[0], grids[1])
+    if n < 256 * (-(-dims // 16)) ** 3:
+        return
+    ctx = N.Context(0)
+    try:
+        got = ctypes.c_int()
+        for _ in range(3):
+            g = np.zeros((dims,) * 3, np.float32)
+            N.check(N.lib().pyl_ma(ctx.handle, pos.ctypes.data, n, 3, None if W is None else W.ctypes.data,
+                                   g.ctypes.data, dims, ctypes.c_float(box), masid, 0))
+            N.check(N.lib().pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(got)))
+            assert np.array_equal(g, grids[0])
+        assert got.value == (N.WINDOW_TWOLEVEL if clump else N.WINDOW_TWOLEVEL_CAPACITY)
+    finally:
+        ctx.close()
+
+
 def test_ma_capacity_adaptive_and_spill_fallback():
     # Adaptive choice on a 256^3 grid with 256^3 lattice-ordered particles (4096 per
     # tile): the capacity sort, once the context has seen ordered input.  A call that
