@@ -96,6 +96,15 @@ class Pipeline:
         return {k: ms / n for k, (ms, n) in self.totals.items() if n > 0 and ms > 0}
 
 
+def sort_form(N, ctx):
+    """Name of the tile-sort form the context's latest tiled MA took."""
+    w = ctypes.c_int()
+    N.check(N.lib().pyl_ctx_last_sort_window(ctx.handle, ctypes.byref(w)))
+    return {N.WINDOW_DIRECT: 'one_pass_direct', N.WINDOW_BOX: 'one_pass_box', N.WINDOW_LINEAR: 'one_pass_linear',
+            N.WINDOW_WIDE: 'wide', N.WINDOW_TWOLEVEL: 'two_level', N.WINDOW_CAPACITY: 'capacity',
+            N.WINDOW_TWOLEVEL_CAPACITY: 'two_level_capacity'}.get(w.value, w.value)
+
+
 def algorithmic_bytes(n, dims):
     """Algorithmic HBM bytes per launch of each timed kernel of the MA + Pk step
     (n particles, dims^3 grid, TSC, unweighted): what the kernel must move at
@@ -308,6 +317,8 @@ def random_order_leg(N, pos, dims, box, mas, reps=5):
     perm = torch.randperm(pos.shape[0], generator=g, device='cuda')
     sp = pos[perm].contiguous()
     del perm
+    # the library's stream does not wait for torch's: the copy must be complete first
+    torch.cuda.synchronize()
     n = sp.shape[0]
     grid = torch.empty((dims,) * 3, device='cuda')
     ref = None
@@ -376,6 +387,7 @@ def precision_leg(N, reps=10, steps=5):
     from pylians3_amd import workloads
     dims, box, mas = 1024, 2000.0, N.PCS
     pos = workloads.config_c3()
+    torch.cuda.synchronize()            # generated on torch's stream; read on the library's
     n = pos.shape[0]
     ctx = N.Context(0)
     L = N.lib()
@@ -385,6 +397,7 @@ def precision_leg(N, reps=10, steps=5):
     ms = time_steps(pipe, steps, 2)
     st = pipe.stage_avg()
     out['ms_per_step'] = round(ms, 3)
+    out['sort_form'] = sort_form(N, ctx)
     out['mparticles_per_s'] = round(n / ms / 1e3, 1)
     out['grid_cells_per_s'] = round(dims ** 3 / (ms * 1e-3), 1)
     out['kernels_ms'] = {k: round(v, 4) for k, v in st.items()}
@@ -564,7 +577,8 @@ def serving_leg(N, nreq=120, seed=5):
     Plain form: one request at a time, one slot, arrival order, no fusion; its
     service latency alone, then under Poisson arrivals at 0.9x its capacity.
     Served form (4 slots, admission in arrival order, fusion of small requests
-    waiting together): the same arrival schedule, then a saturated burst."""
+    waiting together): the same arrival schedule, then a saturated burst.  Each
+    server first runs the largest request once on every slot (Server.warm)."""
     import numpy as np
     import torch
     from pylians3_amd import workloads
@@ -589,14 +603,18 @@ def serving_leg(N, nreq=120, seed=5):
             o2 = int(rng.integers(0, nmax - n2 + 1))
             reqs.append(Request(pool[o:o + n], d, box, mas, axis, pos2=pool[o2:o2 + n2],
                                 W2=poolw[o2:o2 + n2]))
+    # every server is warmed with the largest request on each of its slots first
+    biggest = max(reqs, key=lambda r: r.hbm_bytes())
     # plain forms: one request after the other, one slot, no fusion; the tiled
     # scatter, and the per-particle atomic scatter (the plain GPU form)
     with Server(slots=1, max_batch=1, policy='fifo') as srv:
+        srv.warm(biggest)
         srv.submit(reqs[0]).result()
         t0 = time.perf_counter()
         plain_lat = np.array([srv.submit(r).result().latency_s for r in reqs])
         t_plain = time.perf_counter() - t0
     with Server(slots=1, max_batch=1, policy='fifo', atomic=True) as srv:
+        srv.warm(biggest)
         srv.submit(reqs[0]).result()
         t0 = time.perf_counter()
         atomic_lat = np.array([srv.submit(r).result().latency_s for r in reqs])
@@ -622,10 +640,12 @@ def serving_leg(N, nreq=120, seed=5):
         return np.array([f.result().latency_s for f in futs]), time.perf_counter() - t0
 
     with Server(slots=1, max_batch=1, policy='fifo') as srv:
+        srv.warm(biggest)
         lat_p, _ = loaded(srv)
     out.update({'plain_loaded_p50_ms': round(1e3 * np.percentile(lat_p, 50), 3),
                 'plain_loaded_p99_ms': round(1e3 * np.percentile(lat_p, 99), 3)})
     with Server(slots=4, max_batch=32) as srv:
+        srv.warm(biggest)
         lat, t_arr = loaded(srv)
         srv.completed.clear()
         t0 = time.perf_counter()
@@ -702,6 +722,7 @@ def main():
         hashes.append(_outputs_hash())
     reproducible = {'steps_compared': len(hashes), 'bitwise_identical': len(set(hashes)) == 1,
                     'outputs_sha1': hashes[0][:16]}
+    main_form = sort_form(N, ctx)
     grid_host = None
     if not args.no_cpu:
         # one more MA to keep the density grid for the CPU leg
@@ -814,6 +835,7 @@ def main():
                                '(MA + delta + Pk per step)' % (dims, dims),
                    'particles': n, 'grid': dims, 'box_mpc_h': box, 'mas': 'TSC', 'axis': axis},
         'ma_ms': round(ma_ms, 4),
+        'sort_form': main_form,
         'pk_ms': round(pk_ms, 4),
         'kernel_GBps': per_kernel_gbps,
         'ma_mparticles_per_s': round(n / ma_ms / 1e3, 2) if ma_ms > 0 else None,

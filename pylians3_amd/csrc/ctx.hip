@@ -408,6 +408,7 @@ int pyl_graph_begin(pyl_ctx *ctx) {
     PYL_TRY(check_total(c));
     for (PinnedBuf *b : {&c->pin_up, &c->pin_down, &c->pin_scales}) b->pending = false;
     c->box_miss_n = -1;
+    c->cap_spill_n = -1;
     c->timing = false;
     drop_capture_state(c);
     PYL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));

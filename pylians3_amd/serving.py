@@ -141,6 +141,19 @@ class Server:
             self.cv.notify_all()
         return fut
 
+    def warm(self, req):
+        """Runs `req` once on every slot, one slot after the other, so that each slot's
+        context has grown its scratch buffers and built its plans for requests up to
+        that size before serving starts (a slot's first large request otherwise pays
+        the device allocations).  Call before submitting requests: it uses the slot
+        contexts directly and waits for each run."""
+        with self.cv:
+            busy = len(self.free_slots) != len(self.ctxs) or bool(self.queue)
+        if busy:
+            raise RuntimeError('warm() needs an idle server')
+        for slot in range(len(self.ctxs)):
+            self._run_batch(self.ctxs[slot], self.streams[slot], [req])
+
     def close(self):
         with self.cv:
             self.stop = True

@@ -88,3 +88,17 @@ def test_budget_admission_and_rejection():
         big = Request(torch.zeros((10, 3), device='cuda'), 1024, 1.0)
         with pytest.raises(N.BudgetError):
             srv.submit(big)
+
+
+def test_warm_runs_every_slot_and_keeps_results():
+    # warm() runs a request once on every slot (growing each context's scratch); the
+    # results served afterwards are the same bits as on an unwarmed server
+    from pylians3_amd.serving import Server
+    reqs = _requests(3, n=4)
+    with Server(slots=1, max_batch=1) as srv:
+        cold = [srv.submit(r).result(timeout=120) for r in reqs]
+    with Server(slots=3, max_batch=1) as srv:
+        srv.warm(max(reqs, key=lambda r: r.hbm_bytes()))
+        warm = [srv.submit(r).result(timeout=120) for r in reqs]
+    for a, b in zip(cold, warm):
+        _same(a, b)

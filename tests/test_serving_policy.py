@@ -62,3 +62,19 @@ def test_hrrn_policy_ratio():
 def test_default_policy_is_arrival_order():
     import inspect
     assert inspect.signature(Server.__init__).parameters['policy'].default == 'fifo'
+
+
+def test_warm_refuses_a_busy_server():
+    import threading
+    import pytest
+    s = _Srv()
+    s.cv = threading.Condition()
+    s.ctxs = [None, None]
+    s.free_slots = [0]                       # one slot running
+    s.queue = collections.deque()
+    with pytest.raises(RuntimeError):
+        Server.warm(s, None)
+    s.free_slots = [0, 1]
+    s.queue = collections.deque([_item(10, 16, 0.0, time.perf_counter())])
+    with pytest.raises(RuntimeError):
+        Server.warm(s, None)
