@@ -325,10 +325,11 @@ int pyl_graph_destroy(pyl_graph *graph);
                                       more than a quarter of its particles outside their
                                       sort chunk's 8x8-column box, the two-level sort, else
                                       the one-pass sort; either with the capacity form when
-                                      the tiles hold 256 particles or more on average and no
-                                      capacity call of the context on such a grid spilled
-                                      more than 1/64 of its particles.  A form that does not
-                                      apply falls back to the default. */
+                                      the tiles hold 256 particles or more on average, except
+                                      for the 16 such calls that follow a capacity call of the
+                                      context on that grid which spilled more than 1/64 of its
+                                      particles.  A form that does not apply falls back to the
+                                      default. */
 int pyl_ctx_set_tuning(pyl_ctx *ctx, int knob, int value);
 /* Sort window of the context's latest tiled MA (-1 none yet). */
 #define PYL_WINDOW_DIRECT 0        /* every tile of the realisation counted in LDS */

@@ -1904,6 +1904,7 @@ constexpr long long kCapFactor = 3;
 constexpr long long kCapMinMean = 256;
 constexpr unsigned long long kCapSpillDen = 64;
 constexpr int kSpillChunks = 512;
+constexpr int kCapRetry = 16;                   // calls without the capacity form after a heavy spill
 
 // 16 buckets per octave of the record count (0 alone in bucket 0); distinct
 // buckets also spread the LDS histogram's atomics over more addresses
@@ -2442,10 +2443,10 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                 g_shuffled_hint[std::make_tuple(c->device, c->box_miss_key.first, c->box_miss_key.second)] =
                     shuffled;
             }
-            // a capacity sort that spilled more than 1/64 of its particles: the one-pass
-            // sort from now on for this (dims, MAS)
+            // a capacity sort that spilled more than 1/64 of its particles: the next
+            // kCapRetry calls with this (dims, MAS) that would take it sort without it
             if (c->cap_spill_n > 0 && c->h_total[3] * kCapSpillDen > (unsigned long long)c->cap_spill_n)
-                c->cap_bad[c->cap_key] = true;
+                c->cap_bad[c->cap_key] = kCapRetry;
             c->box_miss_n = -1;
             c->cap_spill_n = -1;
         } else if (q == hipErrorNotReady) {
@@ -2484,9 +2485,11 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     // (the capacity form also serves the two-level sort's second pass: its chunks lie
     // in one bucket, so only full tiles spill)
     const bool dense = n >= kCapMinMean * g.ntiles;
-    const auto cap_good = [&]() {
+    const auto cap_good = [&]() {                  // counts the calls it turns down
         const auto it = c->cap_bad.find(hkey);
-        return it == c->cap_bad.end() || !it->second;
+        if (it == c->cap_bad.end() || it->second <= 0) return true;
+        --it->second;
+        return false;
     };
     bool wide = false, two = false, capm = false;
     if (c->tune_window == 1) wide = wide_ok;

@@ -155,8 +155,8 @@ struct Ctx {
                                                         // were out of order (two-level sort next)
     long long cap_spill_n = -1;      // particles of the capacity call whose spill h_total[3] will hold
     std::pair<int, int> cap_key{0, 0};                // its (dims, MAS)
-    std::map<std::pair<int, int>, bool> cap_bad;      // per (dims, MAS): a capacity call spilled
-                                                      // too much (the one-pass sort instead)
+    std::map<std::pair<int, int>, int> cap_bad;       // per (dims, MAS): calls left without the
+                                                      // capacity form after one spilled too much
     int last_window = -1;            // sort window of the latest tiled MA (PYL_WINDOW_*)
     long long total_expect = -1;
     hipEvent_t ev_total = nullptr;
