@@ -2121,8 +2121,10 @@ __host__ __device__ __forceinline__ void band_cell(int t, int &lx, int &ly, int 
 // partial slots take the closed forms of band_cell.  The up to 8 loads of a
 // cell are issued before any is summed; the sum runs in the image order of the
 // general kernel (a, b, c nested), so both kernels give the same bits.
+constexpr int kFinBlock = 256;      // uniform finalize workgroup (one tile)
+
 template <int MAS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kFinBlock)
 tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, TileGeom g,
                              const float *__restrict__ partials, float *__restrict__ grid,
                              OutMode om, CapRecs cr) {
@@ -2159,7 +2161,7 @@ tile_finalize_uniform_kernel(const unsigned long long *__restrict__ offsets, Til
     const unsigned long long node0 =
         (((unsigned long long)tx * T * g.dims + (unsigned long long)ty * T) * g.dims) + (unsigned long long)tz * T;
     const unsigned dd = (unsigned)g.dims * (unsigned)g.dims;
-    for (int t = threadIdx.x; t < NBAND; t += kBlock) {
+    for (int t = threadIdx.x; t < NBAND; t += kFinBlock) {
         int lx, ly, lz, base, dA, dB, dC;
         band_cell<MAS>(t, lx, ly, lz, base, dA, dB, dC);
         float x[8];
@@ -2926,7 +2928,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     if (MAS > 1) {
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_FINAL));
         if (axes == 3 && g.nt >= 2 && dims % kT3 == 0)
-            tile_finalize_uniform_kernel<MAS><<<(unsigned)nt, kBlock, 0, c->stream>>>(
+            tile_finalize_uniform_kernel<MAS><<<(unsigned)nt, kFinBlock, 0, c->stream>>>(
                 offsets, g, partials, grid, om, cr);
         else if (axes == 3)
             tile_finalize_kernel<MAS, 3><<<(unsigned)nt, kBlock, 0, c->stream>>>(offsets, g, partials,
