@@ -5,7 +5,8 @@ own inputs, keeps the fastest, and caches the choice per device and batch shape.
     best = tune.tune_batch(fields, nreal, dims, BoxSize, axis)   # {'chunks_per_real': ..}
     g = BatchGraph(fields, nreal, dims, BoxSize, axis, tuning=best)
 
-Parameters (none changes a result beyond the rounding of float64 bin sums):
+Parameters (none changes any bit of a result: the tile and bin sums are exact
+integers, whatever their grouping):
   chunks_per_real  sort chunks per realisation: the counting sort's granularity
                    (fewer, longer chunks: fewer tile-count flushes; more: more
                    workgroups in flight for small batches)

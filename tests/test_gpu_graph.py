@@ -146,3 +146,17 @@ def test_tuner_caches_its_choice(tmp_path, monkeypatch):
         assert len(res) == nreal and np.all(np.isfinite(res[0].Pk))
     finally:
         g.close()
+    # the tuned parameters change no bit: tile and bin sums are exact integers
+    outs = []
+    for tuning in (dict(chunks_per_real=2, bin_rows=0, deposit_order=-1),
+                   dict(chunks_per_real=8, bin_rows=8, deposit_order=0)):
+        g = BatchGraph(fields, nreal, dims, box, 0, tuning=tuning)
+        try:
+            outs.append(g.run())
+        finally:
+            g.close()
+    for r in range(nreal):
+        for a, b in ((outs[0][r], outs[1][r]), (outs[0][r], res[r])):
+            np.testing.assert_array_equal(a.Pk, b.Pk)
+            np.testing.assert_array_equal(a.Pk2D, b.Pk2D)
+            np.testing.assert_array_equal(a.Pk1D, b.Pk1D)
