@@ -400,7 +400,12 @@ tile_count_kernel(const float *__restrict__ pos, RSD rsd, float inv, TileGeom g,
     if (threadIdx.x == 0) {
         chunks[blockIdx.x].wbase = (w.mode == kWinBox || w.mode == kWinBlock) ? w.a * g.nt + w.b : w.a;
         chunks[blockIdx.x].wz = w.c;
+        if (ch.c1 <= ch.c0) {                  // empty chunk: no slot for the scatter
+            chunks[blockIdx.x].smin = WS;
+            chunks[blockIdx.x].smax = -1;
+        }
     }
+    if (ch.c1 <= ch.c0) return;                // (uniform over the workgroup)
     // particles the 8 x 8 tile-column box around t0 would miss (box_miss: the
     // host's choice between the box and the wide window for the next call)
     TileGeom gb = g;
@@ -485,6 +490,7 @@ tile_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restr
                     unsigned long long rcap) {
     __shared__ unsigned cur[WS];
     const Chunk ch = chunks[blockIdx.x];
+    if (ch.c1 <= ch.c0) return;                // empty chunk (uniform over the workgroup)
     const long long rbase = (long long)ch.r * g.ntiles;
     // the count pass's window of this chunk
     Win win{g.wmode, 0, 0, 0, (int)g.ntiles};
@@ -709,6 +715,98 @@ cap_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restri
 __global__ void add_u64_kernel(const unsigned long long *__restrict__ a,
                                const unsigned long long *__restrict__ b, unsigned long long *__restrict__ out) {
     if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = a[0] + b[0];
+}
+
+// ---- small spill lists ----------------------------------------------------------
+// A capacity sort that spilled at most kSpillAtomic particles orders them by tile
+// without the windowed count and scatter, whose per-chunk LDS histograms of every
+// tile (or tile box) cost more than the particles of a short list: each wave counts
+// its particles per tile with one global atomic per distinct tile among its lanes,
+// then writes every record at its tile's offset plus a rank from a second such
+// atomic.  The order of the records within a tile is whatever the atomics give; the
+// deposit's integer sums do not depend on it.  Longer spill lists take the windowed
+// passes (spill_table_kernel gives them the chunks; these kernels then return at
+// once).  512^3 TSC (0.1% spilled): 0.068 -> 0.049 ms; 1024^3 PCS: 0.20 -> 0.08 ms.
+constexpr unsigned long long kSpillAtomic = 1ull << 22;
+constexpr int kSpillBlocks = 2048;        // workgroups of the atomic spill kernels (grid-stride)
+
+// Adds, for each distinct key among the active lanes of the wave, the number of
+// lanes holding it to ctr[key] (one atomic, by the lowest such lane), and returns
+// to every active lane the old value plus its rank among those lanes.
+__device__ __forceinline__ unsigned wave_group_add(unsigned *__restrict__ ctr, unsigned key, bool active) {
+    const int lane = (int)(threadIdx.x & 63u);
+    unsigned long long todo = __ballot(active);
+    unsigned mine = 0;
+    while (todo) {                                   // wave-uniform
+        const int leader = __ffsll((long long)todo) - 1;
+        const unsigned lk = (unsigned)__shfl((int)key, leader);
+        const bool in = active && key == lk;
+        const unsigned long long grp = __ballot(in);
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lk], (unsigned)__popcll(grp));
+        base = (unsigned)__shfl((int)base, leader);
+        if (in) mine = base + (unsigned)__popcll(grp & ((1ull << lane) - 1ull));
+        todo &= ~grp;
+    }
+    return mine;
+}
+
+// Spill records per tile into ocounts (the scan adds counts + ocounts; counts stay
+// zero here).  sxyz holds (dist_x, dist_y, dist_z) per spilled particle.
+template <int MAS, bool POW2>
+__global__ void __launch_bounds__(kBlock)
+spill_count_kernel(const float *__restrict__ sxyz, const unsigned long long *__restrict__ nspill,
+                   TileGeom g, unsigned *__restrict__ ocounts) {
+    const unsigned long long ns = *nspill;
+    if (ns > kSpillAtomic) return;
+    const RSD off{sxyz, 0.0f, 0.0f, 0, 3, 0};
+    const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long b = (unsigned long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < ns;
+         b += step) {
+        const unsigned long long i = b + (threadIdx.x & 63u);
+        const bool active = i < ns;
+        unsigned key = 0;
+        if (active) key = dist_key<MAS, 3, POW2>(load_dist<3>(sxyz, off, (long long)i, 1.0f), g, 0);
+        (void)wave_group_add(ocounts, key, active);
+    }
+}
+
+// One record per spilled particle at offsets[key] + its rank (ocursors zeroed).
+template <int MAS, bool POW2, bool WT>
+__global__ void __launch_bounds__(kBlock)
+spill_scatter_kernel(const float *__restrict__ sxyz, const float *__restrict__ sw,
+                     const unsigned long long *__restrict__ nspill, TileGeom g,
+                     const unsigned long long *__restrict__ offsets, unsigned *__restrict__ ocursors,
+                     float *__restrict__ records, unsigned long long rcap) {
+    const unsigned long long ns = *nspill;
+    if (ns > kSpillAtomic) return;
+    const RSD off{sxyz, 0.0f, 0.0f, 0, 3, 0};
+    const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long b = (unsigned long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < ns;
+         b += step) {
+        const unsigned long long i = b + (threadIdx.x & 63u);
+        const bool active = i < ns;
+        float3 d = make_float3(0.0f, 0.0f, 0.0f);
+        float w = 1.0f;
+        unsigned key = 0;
+        if (active) {
+            d = load_dist<3>(sxyz, off, (long long)i, 1.0f);
+            key = dist_key<MAS, 3, POW2>(d, g, 0);
+            if (WT) w = sw[i];
+        }
+        const unsigned r = wave_group_add(ocursors, key, active);
+        if (!active) continue;
+        const unsigned long long at = offsets[key] + r;
+        if (at >= rcap) continue;                        // only after a miscount (reported)
+        if (WT) {
+            reinterpret_cast<float4 *>(records)[at] = make_float4(d.x, d.y, d.z, w);
+        } else {
+            float *o = records + 3 * at;
+            o[0] = d.x;
+            o[1] = d.y;
+            o[2] = d.z;
+        }
+    }
 }
 
 // Exclusive scan of per-tile counts into 64-bit offsets (offsets[ntiles] = total).
@@ -2346,6 +2444,20 @@ __global__ void chunk_table_dev_kernel(Chunk *__restrict__ ch, const long long *
     if (i <= nt) orig[i] = tile_origin(i, nt, dims);
 }
 
+// The capacity sort's spill list [0, nspill) in cpr chunks for the windowed count
+// and scatter, or cpr empty chunks when the atomic spill kernels take it
+// (nspill <= kSpillAtomic); also the tile origins.
+__global__ void spill_table_kernel(Chunk *__restrict__ ch, const unsigned long long *__restrict__ nspill,
+                                   int cpr, int *__restrict__ orig, int nt, int dims) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cpr) {
+        const unsigned long long ns = *nspill;
+        const long long len = ns > kSpillAtomic ? (long long)ns : 0;
+        ch[i] = Chunk{len * i / cpr, len * (i + 1) / cpr, 0, 0, 0, 0, 0, 0};
+    }
+    if (i <= nt) orig[i] = tile_origin(i, nt, dims);
+}
+
 // Scratch piece `id` of a call: the context's grow-only buffer, or, while
 // pyl_ma_noalloc runs (c->arena set), the next 256-byte-aligned piece of its
 // fixed device array.
@@ -2752,9 +2864,12 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         // the spill list (capw[0..1] = {0, count}) sorted by the regular passes, in
         // kSpillChunks chunks cut on the device (empty ones when nothing spilled)
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SPILL));
-        chunk_table_dev_kernel<<<ceil_div(std::max(kSpillChunks, g.nt + 1), kBlock), kBlock, 0, c->stream>>>(
-            dchunksS, (const long long *)capw, 1, kSpillChunks, dorig, g.nt, dims);
+        spill_table_kernel<<<ceil_div(std::max(kSpillChunks, g.nt + 1), kBlock), kBlock, 0, c->stream>>>(
+            dchunksS, capw + 1, kSpillChunks, dorig, g.nt, dims);
         const RSD off{pxyz, 0.0f, 0.0f, 0, 3, 0};
+        // short lists: one pass of global atomics (the windowed passes see empty chunks)
+        (pow2 ? spill_count_kernel<MAS, true> : spill_count_kernel<MAS, false>)
+            <<<kSpillBlocks, kBlock, 0, c->stream>>>(pxyz, capw + 1, gs, ocounts);
         auto kc = sws == kWinWide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
                                           : tile_count_kernel<MAS, 3, false, kWinWide>)
                                   : (pow2 ? tile_count_kernel<MAS, 3, true, kWin>
@@ -2775,6 +2890,10 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
                          : tile_scatter_kernel<MAS, 3, false, false, kWin>));
         ks<<<kSpillChunks, kSortBlock, 0, c->stream>>>(pxyz, off, pw, 1.0f, gs, dchunksS, dump, offsets, counts,
                                                        ocursors, wmax, records, (unsigned long long)n);
+        auto kss2 = W ? (pow2 ? spill_scatter_kernel<MAS, true, true> : spill_scatter_kernel<MAS, false, true>)
+                      : (pow2 ? spill_scatter_kernel<MAS, true, false> : spill_scatter_kernel<MAS, false, false>);
+        kss2<<<kSpillBlocks, kBlock, 0, c->stream>>>(pxyz, pw, capw + 1, gs, offsets, ocursors, records,
+                                                     (unsigned long long)n);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SPILL));
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));

@@ -425,7 +425,9 @@ def _clumped_lattice(d, box, seed, frac=0.06, disp=3.0):
     (256, 192, 'CIC', True, 'clumped'),        # clumped: tiles past their capacity spill
     (512, 256, 'PCS', False, 'lattice'),       # 8 x 8 tile-column box window, 512 per tile
     (512, 200, 'NGP', True, 'clumped'),
-    (432, 160, 'TSC', True, 'shuffled'),       # every particle misses its chunk's box
+    (432, 160, 'TSC', True, 'shuffled'),       # every particle misses its chunk's box:
+                                               # 4.1 M spilled, the atomic spill sort
+    (432, 176, 'CIC', True, 'shuffled'),       # 5.5 M spilled (> 2^22): the windowed spill sort
     (200, 96, 'PCS', False, 'lattice'),        # unequal tiles (13 per axis)
 ])
 def test_ma_capacity_sort_vs_oracle(oracle, dims, d, mas, weighted, order):
