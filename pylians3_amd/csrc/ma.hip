@@ -1731,7 +1731,7 @@ __device__ __forceinline__ void flush_box(const unsigned long long *acc, int key
 // class (LDS counters), deals them to rounds as described at deal_slot, stages
 // the dealt record indices (2 B each) in LDS and reloads the records (from
 // cache).  The sums are integers, so the order of the atomics does not change
-// any bit of the result.  With `order` (tile_order_kernel), workgroup i takes
+// any bit of the result.  With `order` (order_place_kernel), workgroup i takes
 // key order[i] instead of key i.
 template <int MAS, int AXES, bool WT, bool RW, bool UNI>
 __global__ void __launch_bounds__(kDepBlock)
@@ -1979,11 +1979,11 @@ tile_box_flush_kernel(const unsigned long long *__restrict__ gbox, TileGeom g, f
                               from_q, ex, ey, ez);
 }
 
-// Deposit order: keys by decreasing record count (quarter-octave buckets), so
+// Deposit order: keys by decreasing record count (16 buckets per octave), so
 // that the largest tiles start first and the smallest fill the end; in key order
 // the big tiles of a dense region can start last and leave most CUs idle while
-// they finish.  One workgroup; the order within a bucket is arbitrary, and no
-// result depends on it (every tile's sums are its own integers).
+// they finish.  The order within a bucket is arbitrary, and no result depends on
+// it (every tile's sums are its own integers).
 constexpr int kOrderBlock = 1024;
 constexpr int kOrderBuckets = 512;
 constexpr long long kOrderMinKeys = 4096;
@@ -2013,35 +2013,58 @@ __host__ __device__ __forceinline__ int count_bucket(unsigned long long n) {
     return min(kOrderBuckets - 1, 1 + 16 * lg + q);
 }
 
+// Records of key k: its run of the sorted records plus its capacity slots.
+__device__ __forceinline__ unsigned long long key_records(const unsigned long long *__restrict__ offsets,
+                                                          long long k, const CapRecs &cr) {
+    return offsets[k + 1] - offsets[k] + cap_count(cr, k);
+}
+
+// The order is built in two launches of order_wgs(nt) workgroups, each thread
+// taking up to kOrderPer keys (k = its global index + j * threads): bucket sizes
+// (order_hist_kernel), then the keys placed at their bucket's start plus a run
+// claimed per workgroup (order_place_kernel).  Counters sit in LDS per workgroup
+// and only their totals go to the global ones (ghist, gcur: zeroed by the call).
+constexpr int kOrderPer = 8;
+constexpr int kOrderMinWGs = 32;
+inline int order_wgs(long long nt) {
+    return (int)std::max<long long>(kOrderMinWGs, (nt + (long long)kOrderBlock * kOrderPer - 1) /
+                                                      ((long long)kOrderBlock * kOrderPer));
+}
+
 __global__ void __launch_bounds__(kOrderBlock)
-tile_order_kernel(const unsigned long long *__restrict__ offsets, long long nt, unsigned *__restrict__ order,
-                  CapRecs cr) {
-    constexpr int U = 8;                                // keys per thread in flight
+order_hist_kernel(const unsigned long long *__restrict__ offsets, long long nt, CapRecs cr,
+                  unsigned *__restrict__ ghist) {
     __shared__ unsigned hist[kOrderBuckets];
     for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) hist[b] = 0u;
     __syncthreads();
-    const long long step = (long long)blockDim.x * U;
-    for (long long k0 = 0; k0 < nt; k0 += step) {
-        unsigned long long o0[U], o1[U];
+    const long long k0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    unsigned long long c[kOrderPer];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long k = min(k0 + threadIdx.x + (long long)u * blockDim.x, nt - 1);
-            o0[u] = offsets[k];
-            o1[u] = offsets[k + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long k = k0 + threadIdx.x + (long long)u * blockDim.x;
-            if (k < nt) atomicAdd(&hist[count_bucket(o1[u] - o0[u] + cap_count(cr, k))], 1u);
-        }
+    for (int j = 0; j < kOrderPer; ++j) {        // all loads before the first atomic
+        const long long k = k0 + j * step;
+        c[j] = k < nt ? key_records(offsets, k, cr) : 0ull;
     }
+#pragma unroll
+    for (int j = 0; j < kOrderPer; ++j)
+        if (k0 + j * step < nt) atomicAdd(&hist[count_bucket(c[j])], 1u);
     __syncthreads();
-    if (threadIdx.x < 64) {                             // starts, largest bucket first
-        constexpr int PB = kOrderBuckets / 64;          // buckets per lane
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+        if (hist[b]) atomicAdd(&ghist[b], hist[b]);
+}
+
+__global__ void __launch_bounds__(kOrderBlock)
+order_place_kernel(const unsigned long long *__restrict__ offsets, long long nt, CapRecs cr,
+                   const unsigned *__restrict__ ghist, unsigned *__restrict__ gcur,
+                   unsigned *__restrict__ order) {
+    __shared__ unsigned start[kOrderBuckets];   // first slot of each bucket, largest first
+    __shared__ unsigned lrun[kOrderBuckets];    // this workgroup's keys per bucket, then its cursor
+    if (threadIdx.x < 64) {                      // exclusive scan of ghist from the top bucket down
+        constexpr int PB = kOrderBuckets / 64;   // buckets per lane
         const int top = kOrderBuckets - 1 - (int)threadIdx.x * PB;
         unsigned h[PB], sum = 0;
 #pragma unroll
-        for (int j = 0; j < PB; ++j) { h[j] = hist[top - j]; sum += h[j]; }
+        for (int j = 0; j < PB; ++j) { h[j] = ghist[top - j]; sum += h[j]; }
         unsigned incl = sum;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -2050,22 +2073,30 @@ tile_order_kernel(const unsigned long long *__restrict__ offsets, long long nt, 
         }
         unsigned run = incl - sum;
 #pragma unroll
-        for (int j = 0; j < PB; ++j) { hist[top - j] = run; run += h[j]; }
+        for (int j = 0; j < PB; ++j) { start[top - j] = run; run += h[j]; }
     }
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) lrun[b] = 0u;
     __syncthreads();
-    for (long long k0 = 0; k0 < nt; k0 += step) {
-        unsigned long long o0[U], o1[U];
+    const long long k0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    int bk[kOrderPer];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long k = min(k0 + threadIdx.x + (long long)u * blockDim.x, nt - 1);
-            o0[u] = offsets[k];
-            o1[u] = offsets[k + 1];
-        }
+    for (int j = 0; j < kOrderPer; ++j) {
+        const long long k = k0 + j * step;
+        bk[j] = k < nt ? count_bucket(key_records(offsets, k, cr)) : 0;
+    }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long k = k0 + threadIdx.x + (long long)u * blockDim.x;
-            if (k < nt) order[atomicAdd(&hist[count_bucket(o1[u] - o0[u] + cap_count(cr, k))], 1u)] = (unsigned)k;
-        }
+    for (int j = 0; j < kOrderPer; ++j)
+        if (k0 + j * step < nt) atomicAdd(&lrun[bk[j]], 1u);
+    __syncthreads();
+    // one claim per bucket this workgroup holds keys of
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+        if (lrun[b]) lrun[b] = start[b] + atomicAdd(&gcur[b], lrun[b]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kOrderPer; ++j) {
+        const long long k = k0 + j * step;
+        if (k < nt) order[atomicAdd(&lrun[bk[j]], 1u)] = (unsigned)k;
     }
 }
 
@@ -2674,11 +2705,19 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     //              chunk table | tile origins | tile order | per-chunk window bases |
     //              (two-level) bucket offsets[nbk+1] | pass-2 chunk table | bucket bases
     //              (capacity) tile cursors[nt] | pad | spill range {0, count}, placed, total (u64)
+    //              (ordered) order bucket sizes[512] | bucket cursors[512]
+    // Largest tiles first, when there are enough tiles, and records per tile, for the
+    // dispatch tail to outweigh the ordering passes.
+    const bool ordered = c->tune_order >= 0
+        ? (nsplit == 1 && c->tune_order == 1)
+        : (nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt);
     size_t nwords = (size_t)nt * 3 + (size_t)nreal + 1 + (two ? (size_t)nbk : 0);
     const size_t cur_at = nwords;
     if (capm) nwords += (size_t)nt;
     const size_t capw_at = (nwords + 1) & ~(size_t)1;     // 8-byte aligned
     if (capm) nwords = capw_at + 8;
+    const size_t ohist_at = nwords;                        // (ordered) bucket sizes | cursors
+    if (ordered) nwords += 2 * (size_t)kOrderBuckets;
     const size_t cnt_bytes = (nwords * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t off_bytes = (((size_t)nt + 1) * sizeof(unsigned long long) + 255) & ~(size_t)255;
     const size_t bs_bytes = ((size_t)nscan_blocks * sizeof(unsigned long long) + 255) & ~(size_t)255;
@@ -2690,11 +2729,6 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     const size_t ch2_bytes = two ? ((size_t)nchunk2 * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
     const size_t chs_bytes = capm ? ((size_t)kSpillChunks * sizeof(Chunk) + 255) & ~(size_t)255 : 0;
     const size_t bdump_bytes = two ? ((size_t)std::max(nchunk, 1) * nbk * sizeof(unsigned) + 255) & ~(size_t)255 : 0;
-    // Largest tiles first, when there are enough tiles, and records per tile, for the
-    // dispatch tail to outweigh the ordering pass (about 25 us per 32K keys).
-    const bool ordered = c->tune_order >= 0
-        ? (nsplit == 1 && c->tune_order == 1)
-        : (nsplit == 1 && nt >= kOrderMinKeys && n >= kOrderMinRecords * nt);
     const int T = axes == 3 ? kT3 : kT2;
     const size_t nbox = (size_t)(T + MAS - 1) * (T + MAS - 1) * (axes == 3 ? T + MAS - 1 : 1);
     // all scratch of the call is taken before the first launch
@@ -2732,6 +2766,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
     float *bxyz = (float *)kv;
     float *bw = ((two || capm) && W) ? (float *)((char *)kv + bxyz_bytes) : nullptr;
     unsigned *cursor = counts + cur_at;
+    unsigned *ohist = counts + ohist_at;
     unsigned long long *capw = (unsigned long long *)(counts + capw_at);
     float *crec = (float *)cv;
     float *pxyz = two ? (float *)sv : bxyz;
@@ -2897,9 +2932,12 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SPILL));
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
-        if (ordered)
-            tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order,
-                                                                CapRecs{crec, cursor, cap, (long long)n});
+        if (ordered) {
+            const CapRecs ocr{crec, cursor, cap, (long long)n};
+            order_hist_kernel<<<order_wgs(nt), kOrderBlock, 0, c->stream>>>(offsets, nt, ocr, ohist);
+            order_place_kernel<<<order_wgs(nt), kOrderBlock, 0, c->stream>>>(offsets, nt, ocr, ohist,
+                                                                              ohist + kOrderBuckets, order);
+        }
         add_u64_kernel<<<1, 64, 0, c->stream>>>(offsets + nt, capw + 2, capw + 3);   // record total
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
@@ -2925,9 +2963,12 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         scan_blocksums_kernel<<<1, kScanBlock, 0, c->stream>>>(bsums, nscan_blocks);
         scan_final_kernel<<<nscan_blocks, kScanBlock, 0, c->stream>>>(counts, ocounts, nt, bsums,
                                                                        offsets);
-        if (ordered)
-            tile_order_kernel<<<1, kOrderBlock, 0, c->stream>>>(offsets, nt, order,
-                                                                CapRecs{nullptr, nullptr, 0, -1});
+        if (ordered) {
+            const CapRecs ocr{nullptr, nullptr, 0, -1};
+            order_hist_kernel<<<order_wgs(nt), kOrderBlock, 0, c->stream>>>(offsets, nt, ocr, ohist);
+            order_place_kernel<<<order_wgs(nt), kOrderBlock, 0, c->stream>>>(offsets, nt, ocr, ohist,
+                                                                              ohist + kOrderBuckets, order);
+        }
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
     }
