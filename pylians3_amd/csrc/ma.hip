@@ -711,12 +711,6 @@ cap_scatter_kernel(const float *__restrict__ pos, RSD rsd, const float *__restri
     }
 }
 
-// out[0] = a[0] + b[0] (the record total of a capacity sort: sorted spill + placed).
-__global__ void add_u64_kernel(const unsigned long long *__restrict__ a,
-                               const unsigned long long *__restrict__ b, unsigned long long *__restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = a[0] + b[0];
-}
-
 // ---- small spill lists ----------------------------------------------------------
 // A capacity sort that spilled at most kSpillAtomic particles orders them by tile
 // without the windowed count and scatter, whose per-chunk LDS histograms of every
@@ -725,7 +719,7 @@ __global__ void add_u64_kernel(const unsigned long long *__restrict__ a,
 // then writes every record at its tile's offset plus a rank from a second such
 // atomic.  The order of the records within a tile is whatever the atomics give; the
 // deposit's integer sums do not depend on it.  Longer spill lists take the windowed
-// passes (spill_table_kernel gives them the chunks; these kernels then return at
+// passes (spill_count_kernel writes their chunks; these kernels then return at
 // once).  512^3 TSC (0.1% spilled): 0.068 -> 0.049 ms; 1024^3 PCS: 0.20 -> 0.08 ms.
 constexpr unsigned long long kSpillAtomic = 1ull << 22;
 constexpr int kSpillBlocks = 2048;        // workgroups of the atomic spill kernels (grid-stride)
@@ -752,12 +746,21 @@ __device__ __forceinline__ unsigned wave_group_add(unsigned *__restrict__ ctr, u
 }
 
 // Spill records per tile into ocounts (the scan adds counts + ocounts; counts stay
-// zero here).  sxyz holds (dist_x, dist_y, dist_z) per spilled particle.
+// zero here).  sxyz holds (dist_x, dist_y, dist_z) per spilled particle.  The first
+// threads also write the windowed passes' cpr chunks of the list (empty when this
+// kernel takes it) and the g.nt + 1 tile origins.
 template <int MAS, bool POW2>
 __global__ void __launch_bounds__(kBlock)
 spill_count_kernel(const float *__restrict__ sxyz, const unsigned long long *__restrict__ nspill,
-                   TileGeom g, unsigned *__restrict__ ocounts) {
+                   TileGeom g, unsigned *__restrict__ ocounts, Chunk *__restrict__ ch, int cpr,
+                   int *__restrict__ orig) {
     const unsigned long long ns = *nspill;
+    const int gi = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (gi < cpr) {
+        const long long len = ns > kSpillAtomic ? (long long)ns : 0;
+        ch[gi] = Chunk{len * gi / cpr, len * (gi + 1) / cpr, 0, 0, 0, 0, 0, 0};
+    }
+    if (gi <= g.nt) orig[gi] = tile_origin(gi, g.nt, g.dims);
     if (ns > kSpillAtomic) return;
     const RSD off{sxyz, 0.0f, 0.0f, 0, 3, 0};
     const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
@@ -771,13 +774,16 @@ spill_count_kernel(const float *__restrict__ sxyz, const unsigned long long *__r
     }
 }
 
-// One record per spilled particle at offsets[key] + its rank (ocursors zeroed).
+// One record per spilled particle at offsets[key] + its rank (ocursors zeroed).  Also
+// the capacity sort's record total: total = sorted spill (offsets[ntiles]) + placed.
 template <int MAS, bool POW2, bool WT>
 __global__ void __launch_bounds__(kBlock)
 spill_scatter_kernel(const float *__restrict__ sxyz, const float *__restrict__ sw,
                      const unsigned long long *__restrict__ nspill, TileGeom g,
                      const unsigned long long *__restrict__ offsets, unsigned *__restrict__ ocursors,
-                     float *__restrict__ records, unsigned long long rcap) {
+                     float *__restrict__ records, unsigned long long rcap,
+                     const unsigned long long *__restrict__ placed, unsigned long long *__restrict__ total) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) total[0] = offsets[g.ntiles] + placed[0];
     const unsigned long long ns = *nspill;
     if (ns > kSpillAtomic) return;
     const RSD off{sxyz, 0.0f, 0.0f, 0, 3, 0};
@@ -2475,20 +2481,6 @@ __global__ void chunk_table_dev_kernel(Chunk *__restrict__ ch, const long long *
     if (i <= nt) orig[i] = tile_origin(i, nt, dims);
 }
 
-// The capacity sort's spill list [0, nspill) in cpr chunks for the windowed count
-// and scatter, or cpr empty chunks when the atomic spill kernels take it
-// (nspill <= kSpillAtomic); also the tile origins.
-__global__ void spill_table_kernel(Chunk *__restrict__ ch, const unsigned long long *__restrict__ nspill,
-                                   int cpr, int *__restrict__ orig, int nt, int dims) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cpr) {
-        const unsigned long long ns = *nspill;
-        const long long len = ns > kSpillAtomic ? (long long)ns : 0;
-        ch[i] = Chunk{len * i / cpr, len * (i + 1) / cpr, 0, 0, 0, 0, 0, 0};
-    }
-    if (i <= nt) orig[i] = tile_origin(i, nt, dims);
-}
-
 // Scratch piece `id` of a call: the context's grow-only buffer, or, while
 // pyl_ma_noalloc runs (c->arena set), the next 256-byte-aligned piece of its
 // fixed device array.
@@ -2896,15 +2888,16 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         }
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCATTER));
-        // the spill list (capw[0..1] = {0, count}) sorted by the regular passes, in
-        // kSpillChunks chunks cut on the device (empty ones when nothing spilled)
+        // the spill list (capw[0..1] = {0, count}) ordered by tile: up to kSpillAtomic
+        // particles by the atomic kernels (the windowed passes then see empty chunks),
+        // more by the windowed count and scatter in kSpillChunks chunks cut on the device
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SPILL));
-        spill_table_kernel<<<ceil_div(std::max(kSpillChunks, g.nt + 1), kBlock), kBlock, 0, c->stream>>>(
-            dchunksS, capw + 1, kSpillChunks, dorig, g.nt, dims);
-        const RSD off{pxyz, 0.0f, 0.0f, 0, 3, 0};
-        // short lists: one pass of global atomics (the windowed passes see empty chunks)
+        static_assert((long long)kSpillBlocks * kBlock >= kSpillChunks, "spill chunk table");
+        PYL_CHECK((long long)g.nt + 1 <= (long long)kSpillBlocks * kBlock, PYL_EINVAL, "grid too large");
         (pow2 ? spill_count_kernel<MAS, true> : spill_count_kernel<MAS, false>)
-            <<<kSpillBlocks, kBlock, 0, c->stream>>>(pxyz, capw + 1, gs, ocounts);
+            <<<kSpillBlocks, kBlock, 0, c->stream>>>(pxyz, capw + 1, gs, ocounts, dchunksS, kSpillChunks,
+                                                     dorig);
+        const RSD off{pxyz, 0.0f, 0.0f, 0, 3, 0};
         auto kc = sws == kWinWide ? (pow2 ? tile_count_kernel<MAS, 3, true, kWinWide>
                                           : tile_count_kernel<MAS, 3, false, kWinWide>)
                                   : (pow2 ? tile_count_kernel<MAS, 3, true, kWin>
@@ -2928,7 +2921,7 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
         auto kss2 = W ? (pow2 ? spill_scatter_kernel<MAS, true, true> : spill_scatter_kernel<MAS, false, true>)
                       : (pow2 ? spill_scatter_kernel<MAS, true, false> : spill_scatter_kernel<MAS, false, false>);
         kss2<<<kSpillBlocks, kBlock, 0, c->stream>>>(pxyz, pw, capw + 1, gs, offsets, ocursors, records,
-                                                     (unsigned long long)n);
+                                                     (unsigned long long)n, capw + 2, capw + 3);
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SPILL));
         PYL_TRY(stage_begin(c, PYL_STAGE_MA_SCAN));
@@ -2938,7 +2931,6 @@ int run_tiled(Ctx *c, const float *pos, const RSD &rsd, long long n, const long 
             order_place_kernel<<<order_wgs(nt), kOrderBlock, 0, c->stream>>>(offsets, nt, ocr, ohist,
                                                                               ohist + kOrderBuckets, order);
         }
-        add_u64_kernel<<<1, 64, 0, c->stream>>>(offsets + nt, capw + 2, capw + 3);   // record total
         PYL_HIP(hipGetLastError());
         PYL_TRY(stage_end(c, PYL_STAGE_MA_SCAN));
     }
