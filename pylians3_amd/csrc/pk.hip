@@ -585,7 +585,7 @@ struct ReduceArgs {
     const int *key_items;
     const int *escale;
     double *out3, *out2, *out1;
-    int nitems, nkeys, n3, L2, L1, S, P, off2, off1, axis, kmax_par;
+    int nitems, nkeys, n3, L2, L1, S, P, off2, off1, axis, kmax_par, nreal;
     long long nb2, n1;       // out2 / out1 elements per realisation (nb2*S, (m+1)*S)
 };
 
@@ -593,23 +593,30 @@ struct ReduceArgs {
 // in two stages: chunks of kRedChunk items, then the chunk sums.
 constexpr int kRedChunk = 64;
 
-// (tlo, thi)[r][ch][c] = sum over items of chunk ch of the partials' column col0 + c
+// (tlo, thi)[r][ch][c] = sum over items of chunk ch (of nch) of the partials' column
+// col0 + c, realisation r
+__device__ __forceinline__ void reduce_chunk_col(const unsigned long long *__restrict__ plo,
+                                                 const int *__restrict__ phi, int nitems, int P, int col0,
+                                                 int ncol, unsigned long long *__restrict__ tlo,
+                                                 long long *__restrict__ thi, int c, int ch, int nch, int r) {
+    const long long base = (long long)r * nitems * P + col0 + c;
+    const int i0 = ch * kRedChunk, i1 = min(nitems, i0 + kRedChunk);
+    unsigned long long slo = 0;
+    long long shi = 0;
+#pragma unroll 8
+    for (int it = i0; it < i1; ++it) fix_acc(slo, shi, plo[base + (long long)it * P], phi[base + (long long)it * P]);
+    const long long o = ((long long)r * nch + ch) * ncol + c;
+    tlo[o] = slo;
+    thi[o] = shi;
+}
+
 __global__ void pk_reduce_chunks_kernel(const unsigned long long *__restrict__ plo,
                                         const int *__restrict__ phi, int nitems, int P,
                                         int col0, int ncol, unsigned long long *__restrict__ tlo,
                                         long long *__restrict__ thi) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncol) return;
-    const int ch = blockIdx.y, nch = gridDim.y;
-    const long long base = (long long)blockIdx.z * nitems * P + col0 + c;
-    const int i0 = ch * kRedChunk, i1 = min(nitems, i0 + kRedChunk);
-    unsigned long long slo = 0;
-    long long shi = 0;
-#pragma unroll 8
-    for (int it = i0; it < i1; ++it) fix_acc(slo, shi, plo[base + (long long)it * P], phi[base + (long long)it * P]);
-    const long long o = ((long long)blockIdx.z * nch + ch) * ncol + c;
-    tlo[o] = slo;
-    thi[o] = shi;
+    reduce_chunk_col(plo, phi, nitems, P, col0, ncol, tlo, thi, c, blockIdx.y, gridDim.y, blockIdx.z);
 }
 
 // out[r*ostride + c] = sum over chunks of (tlo, thi)[r][ch][c], as a double; with
@@ -628,12 +635,9 @@ __global__ void pk_reduce_final_kernel(const unsigned long long *__restrict__ tl
     out[(long long)blockIdx.y * ostride + c] = fix_value(slo, shi, E);
 }
 
-// One thread per (key, line index, spectrum): sums the key's items.
-__global__ void pk_reduce2_kernel(ReduceArgs a) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long n = (long long)a.nkeys * a.L2 * a.S;
-    if (i >= n) return;
-    const long long base = (long long)blockIdx.y * a.nitems * a.P;
+// 2D line element i = (key, line index, spectrum) of realisation r: sums the key's items.
+__device__ __forceinline__ void reduce2_elem(const ReduceArgs &a, long long i, int r) {
+    const long long base = (long long)r * a.nitems * a.P;
     int sp = (int)(i % a.S);
     int li = (int)((i / a.S) % a.L2);
     int key = (int)(i / ((long long)a.S * a.L2));
@@ -646,16 +650,14 @@ __global__ void pk_reduce2_kernel(ReduceArgs a) {
     // reference 2D index: (kmax_par+1)*k_per + k_par
     long long idx2 = (a.axis == 2) ? (long long)(a.kmax_par + 1) * key + li
                                    : (long long)(a.kmax_par + 1) * li + key;
-    a.out2[(long long)blockIdx.y * a.nb2 + idx2 * a.S + sp] = fix_value(slo, shi, a.escale[2 * blockIdx.y]);
+    a.out2[(long long)r * a.nb2 + idx2 * a.S + sp] = fix_value(slo, shi, a.escale[2 * r]);
 }
 
-// Axes 0/1: 1D bin = key = k_par; sums the key's items.  (Axis 2 keeps one 1D line
-// per item and goes through the chunked reduction.)
-__global__ void pk_reduce1_kernel(ReduceArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nkeys * a.S) return;
-    const long long base = (long long)blockIdx.y * a.nitems * a.P;
-    double *out1 = a.out1 + (long long)blockIdx.y * a.n1;
+// Axes 0/1: 1D bin = key = k_par; element i = (key, spectrum) sums the key's items.
+// (Axis 2 keeps one 1D line per item and goes through the chunked reduction.)
+__device__ __forceinline__ void reduce1_elem(const ReduceArgs &a, int i, int r) {
+    const long long base = (long long)r * a.nitems * a.P;
+    double *out1 = a.out1 + (long long)r * a.n1;
     int key = i / a.S, sp = i % a.S;
     unsigned long long slo = 0;
     long long shi = 0;
@@ -663,7 +665,29 @@ __global__ void pk_reduce1_kernel(ReduceArgs a) {
         const long long j = base + (long long)it * a.P + a.off1 + sp;
         fix_acc(slo, shi, a.plo[j], a.phi[j]);
     }
-    out1[key * a.S + sp] = fix_value(slo, shi, a.escale[2 * blockIdx.y]);
+    out1[key * a.S + sp] = fix_value(slo, shi, a.escale[2 * r]);
+}
+
+// The independent first reductions in one launch of 256-thread blocks, by block
+// range: the 3D columns' chunk sums (g3 = ceil(n3/256) * nch * nreal blocks), then
+// the 2D line (g2 blocks), then for axes 0/1 the 1D line (g1 blocks).
+__global__ void __launch_bounds__(256)
+pk_reduce_first_kernel(ReduceArgs a, unsigned long long *__restrict__ tlo, long long *__restrict__ thi,
+                       int nch, int g3, int g2) {
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (b < g3) {
+        const int xb = (a.n3 + 255) / 256;
+        const int c = (b % xb) * 256 + t, rest = b / xb;
+        if (c < a.n3) reduce_chunk_col(a.plo, a.phi, a.nitems, a.P, 0, a.n3, tlo, thi, c, rest % nch, nch, rest / nch);
+    } else if (b < g3 + g2) {
+        const int xb = g2 / a.nreal, bb = b - g3;
+        const long long i = (long long)(bb % xb) * 256 + t;
+        if (i < (long long)a.nkeys * a.L2 * a.S) reduce2_elem(a, i, bb / xb);
+    } else {
+        const int bb = b - g3 - g2, xb = (a.nkeys * a.S + 255) / 256;
+        const int i = (bb % xb) * 256 + t;
+        if (i < a.nkeys * a.S) reduce1_elem(a, i, bb / xb);
+    }
 }
 
 // FFT prologue: out = (in / scale) - shift in float32 (the reference caller's
@@ -828,21 +852,23 @@ int launch_bin(Ctx *c, const BinPlan *bp, const PkDims &p, const float2 *const *
     r.key_items = bp->d_key_items; r.out3 = d3; r.out2 = d2; r.out1 = d1;
     r.nitems = bp->nitems; r.nkeys = bp->nkeys; r.n3 = n3; r.L2 = L2; r.L1 = L1; r.S = S;
     r.P = P; r.off2 = off2; r.off1 = off1; r.axis = AXIS; r.kmax_par = p.kmax_par;
-    r.nb2 = (long long)out2; r.n1 = (long long)out1;
-    // 3D bins: every item contributes
-    pk_reduce_chunks_kernel<<<dim3(ceil_div(n3, 256), nch, nreal), 256, 0, c->stream>>>(
-        plo, phi, bp->nitems, P, 0, n3, tlo, thi);
+    r.nb2 = (long long)out2; r.n1 = (long long)out1; r.nreal = nreal;
+    // 3D bins (every item contributes: chunk sums, then their sum), the 2D line and,
+    // for axes 0/1, the 1D line (each bin from its key's items): the first stages in
+    // one launch
+    const long long n2 = (long long)bp->nkeys * L2 * S;
+    const long long g3 = (long long)ceil_div(n3, 256) * nch * nreal;
+    const long long g2 = (n2 + 255) / 256 * nreal;
+    const long long g1 = AXIS == 2 ? 0 : (long long)ceil_div(bp->nkeys * S, 256) * nreal;
+    PYL_CHECK(g3 + g2 + g1 < (1ll << 31), PYL_EINVAL, "binning reduction too large");
+    pk_reduce_first_kernel<<<(unsigned)(g3 + g2 + g1), 256, 0, c->stream>>>(r, tlo, thi, nch, (int)g3, (int)g2);
     pk_reduce_final_kernel<<<dim3(ceil_div(n3, 256), nreal), 256, 0, c->stream>>>(
         tlo, thi, nch, n3, d3, (long long)out3, escale, PHASE ? NA3 : 0);
-    long long n2 = (long long)bp->nkeys * L2 * S;
-    pk_reduce2_kernel<<<dim3((unsigned)((n2 + 255) / 256), nreal), 256, 0, c->stream>>>(r);
     if (AXIS == 2) {   // 1D bins are k_par = kz: every item contributes
         pk_reduce_chunks_kernel<<<dim3(ceil_div(L1 * S, 256), nch, nreal), 256, 0, c->stream>>>(
             plo, phi, bp->nitems, P, off1, L1 * S, tlo, thi);
         pk_reduce_final_kernel<<<dim3(ceil_div(L1 * S, 256), nreal), 256, 0, c->stream>>>(
             tlo, thi, nch, L1 * S, d1, (long long)out1, escale, 0);
-    } else {
-        pk_reduce1_kernel<<<dim3(ceil_div(bp->nkeys * S, 256), nreal), 256, 0, c->stream>>>(r);
     }
     PYL_HIP(hipGetLastError());
     PYL_TRY(stage_end(c, PYL_STAGE_PK_BIN));
